@@ -1,0 +1,79 @@
+// Host symbolic analysis for the supernodal Cholesky of A = sum_e w_e^2 D_e^T D_e
+// over the free vertices (the matrix the reference hands to SimplicialLDLT with AMD,
+// deps/mcloptlib/include/MCL/LinearSolver.hpp:279-298, assembled by
+// src/TetElements.cpp:483-510).
+//
+// Output layout, shared with the device kernels:
+//   * the permutation perm (new -> old) from geometric nested dissection, postordered;
+//   * supernodes s with columns [col0[s], col0[s+1]) and a front of f_s rows: its own
+//     ns_s columns followed by the m_s rows of its off-diagonal structure (sorted);
+//   * per supernode a dense column-major block G_s of f_s x ns_s doubles holding
+//       G_s = [ L11^{-1} ; -L21 L11^{-1} ]
+//     so the forward solve is   [y_J ; u] = G_s c_s   and the backward solve is
+//     x_J = G_s^T [y_J ; x_R]  (both single dense products, no in-block recursion);
+//   * forward assembly lists: for every front row, the child update entries that
+//     add into it (deterministic gather, no atomics);
+//   * a schedule: small supernodes grouped into leaf subtrees (one workgroup walks a
+//     whole subtree), the remaining supernodes by level.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace wrapd {
+
+struct CholSymbolic {
+    int n = 0;
+    std::vector<int> perm;        // new -> old
+    std::vector<int> iperm;       // old -> new
+    int nsuper = 0;
+    std::vector<int> col0;        // [nsuper+1]
+    std::vector<int> fsize;       // front rows f_s
+    std::vector<int64_t> rowptr;  // [nsuper+1] into rows
+    std::vector<int> rows;        // front rows (new indices), first ns are the columns
+    std::vector<int> parent;      // supernodal etree
+    std::vector<int64_t> goff;    // [nsuper+1] offsets of G_s (f_s * ns_s doubles each)
+    std::vector<int64_t> uoff;    // [nsuper+1] offsets of u_s in rows (m_s rows each)
+    // forward assembly: for global front row index p (rowptr[s] + r), the u-rows adding into it
+    std::vector<int64_t> asm_ptr; // [rows.size()+1]
+    std::vector<int64_t> asm_idx; // u-row indices
+    // schedule
+    std::vector<int> leaf_ptr, leaf_sn;    // leaf subtrees, supernodes in postorder
+    std::vector<int> level_ptr, level_sn;  // top supernodes grouped by level (bottom-up)
+    int max_front = 0, max_ns = 0;
+    double nnz_L = 0, flops_factor = 0, g_entries = 0;
+
+    // A (upper triangle CSR over old free indices) -> lower entries in new numbering:
+    // for each A slot the supernode column position, used to scatter A into fronts.
+    std::vector<int> a_sn;        // supernode owning the slot's column
+    std::vector<int64_t> a_pos;   // column-major offset inside that supernode's front (f_s x f_s)
+    // update extend-add maps: child c's front rows ns_c.. map to parent positions
+    std::vector<int64_t> relptr;  // [nsuper+1] into rel (m_s entries each)
+    std::vector<int> rel;         // position of each off-diagonal row of s inside parent's front
+};
+
+struct SymbolicOptions {
+    int leaf_size = 48;          // nested-dissection leaf size
+    int ns_max = 1024;           // widest supernode after splitting
+    int relax_small = 16;        // always merge a child into its parent below this width
+    double relax_frac = 0.06;    // allowed fraction of explicit zeros for larger merges
+    int relax_max = 192;         // no relaxed merge beyond this width
+    int small_front = 256;       // leaf-subtree supernodes must have f_s <= this
+    int leaf_tasks = 2048;       // target number of leaf subtrees
+};
+
+// Ap/Aj: upper-triangular CSR (row i holds columns j >= i) over n free vertices;
+// xyz: (n,3) coordinates used for the dissection.
+void chol_analyze(int n, const std::vector<int>& Ap, const std::vector<int>& Aj, const double* xyz,
+                  const SymbolicOptions& opt, CholSymbolic& out);
+
+// Host reference numeric factorization into the G layout (multifrontal, dense
+// kernels in plain loops).  Ax: values aligned with Aj.  Returns false if a pivot is
+// not positive.
+bool chol_numeric_host(const CholSymbolic& S, const std::vector<int>& Ap, const std::vector<int>& Aj,
+                       const std::vector<double>& Ax, std::vector<double>& G);
+
+// Host reference solve with the G layout (b and x are (n,3) row-major, old numbering).
+void chol_solve_host(const CholSymbolic& S, const std::vector<double>& G, const double* b, double* x);
+
+}  // namespace wrapd
