@@ -1,0 +1,168 @@
+/* wrapd_gpu.h - C ABI of the MI355X WRAPD kernels (one GPU, one HIP stream per context).
+ *
+ * Every call returns a status code (WRAPD_OK = 0); nothing throws or exits across this
+ * boundary.  A context owns all device memory of one solve and is confined to the
+ * host thread that created it.  Arrays are plain host pointers:
+ *   positions (n,3) row-major doubles, tets (n_e,4) int32, 3x3 matrices row-major.
+ * Free/fixed vertex numbering follows samples/utils/AddMeshes.hpp:89-110: pins are
+ * sorted, free vertices are numbered in vertex order, fixed ones in pin order.
+ */
+#ifndef WRAPD_GPU_H_
+#define WRAPD_GPU_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    WRAPD_OK = 0,
+    WRAPD_ERR_ARG = 1,            /* bad argument / size                                   */
+    WRAPD_ERR_HIP = 2,            /* HIP runtime error                                      */
+    WRAPD_ERR_INVERTED_REST = 3,  /* rest element with negative volume (src/TetElements.cpp:99-101) */
+    WRAPD_ERR_NOT_SPD = 4,        /* non-positive pivot in the factorization (singular A)   */
+    WRAPD_ERR_NAN = 5,            /* NaN L-BFGS decrement (SplitLBFGS.hpp:181-184)          */
+    WRAPD_ERR_NEG_SIGMA = 6,      /* negative prox singular value (src/TetElements.cpp:320-323) */
+    WRAPD_ERR_STATE = 7,          /* call out of order                                      */
+    WRAPD_ERR_NO_DEVICE = 8,      /* no HIP device                                          */
+    WRAPD_ERR_LINESEARCH = 9      /* line search failure (SplitLBFGS.hpp:200-203)            */
+};
+
+typedef struct wrapd_ctx wrapd_ctx;
+
+typedef struct {
+    int model;                  /* 0 = ARAP, 1 = Neo-Hookean (src/ModelSettings.hpp:43-46)   */
+    double mu, lambda;          /* Lame parameters used when no per-element arrays are given */
+    double beta_static;         /* w^2 = beta_static k vol (src/TetElements.cpp:139)          */
+    double beta_min, beta_max;  /* reweighting clamp (src/TetElements.cpp:399-402)            */
+} wrapd_material;
+
+enum { WRAPD_FP64 = 0, WRAPD_FP32 = 1 };                               /* local-step precision */
+enum { WRAPD_PROX_LBFGS = 0, WRAPD_PROX_NEWTON = 1, WRAPD_PROX_LEARNED = 2 };
+
+typedef struct {
+    int precision;              /* WRAPD_FP64 | WRAPD_FP32 (FP32 with per-element FP64 fallback) */
+    int prox_mode;              /* LBFGS = reference solver (src/TetElements.cpp:302-318)        */
+    int prox_iters;             /* L-BFGS cap (reference: 20) or Newton cap                      */
+    int update_cand;            /* compute candidate weights (src/TetElements.cpp:351-353)       */
+    int cand_use_hessian;       /* reweighting counter past the delay (src/TetElements.cpp:405)  */
+    double newton_rtol;         /* Newton tolerance, relative to (mu + w^2/vol) max(1,|s|)       */
+    double fallback_sigma;      /* FP32: elements with s_hat[2] below this redo the step in FP64 */
+} wrapd_local_args;
+
+typedef struct {
+    double delta_auglag;        /* aft - bef + pe - pe_prev (src/TetElements.cpp:355)        */
+    double bef, aft, pe;
+    double max_wsq_ratio;       /* Elements::max_wsq_ratio (src/Elements.hpp:56-69)          */
+    double prox_iters_total;    /* prox iterations summed over elements                      */
+    double fallback_count;      /* FP32 elements redone in FP64                               */
+    double neg_sigma_count;     /* elements whose prox returned a negative singular value    */
+    double learned_accepted;    /* learned warm starts accepted without a Newton iteration   */
+    double evals_total;         /* prox objective evaluations (line-search trials / Newton steps) */
+    double evals_max;           /* largest per-element count                                 */
+} wrapd_local_out;
+
+typedef struct {
+    int memory;                 /* history M (reference: 6, SplitLBFGS.hpp:41)               */
+    int max_iters;              /* -global_it (src/Settings.hpp:124)                          */
+    double kappa;               /* -kappa                                                     */
+    int linesearch;             /* 0 none, 1 Armijo backtracking (-global_ls)                 */
+} wrapd_lbfgs_cfg;
+
+typedef struct {
+    int iters;                  /* value returned by SplitLBFGS::minimize                     */
+    double obj_bef, obj_aft;    /* GlobalProblem::value before/after (GlobalStepBehavior.hpp:247,256) */
+    double energy;              /* potential_energy_x at the final x (src/TetElements.cpp:598-635) */
+    int flips;                  /* inversion_count (src/TetElements.cpp:444-468)              */
+    int exit_reason;            /* 0 loop end, 1 kappa early exit, 2 zero denominator, 3 no change */
+} wrapd_global_out;
+
+/* ---- lifecycle -------------------------------------------------------------------- */
+int wrapd_create(wrapd_ctx** ctx, int device);
+int wrapd_destroy(wrapd_ctx* ctx);
+const char* wrapd_last_error(wrapd_ctx* ctx);
+
+/* TetElements constructor (src/TetElements.cpp:36-143) fed as create_tets_from_mesh
+ * (src/TetElements.hpp:250-288) and binding::add_tetmesh (samples/utils/AddMeshes.hpp:61-132)
+ * would: precomputes Dm^-1, volumes, D_local, w^2 on the device, builds the
+ * vertex->element gather lists and the pattern of A.  mu_e/lambda_e may be NULL. */
+int wrapd_upload_mesh(wrapd_ctx* ctx, int n_verts, const double* rest, int n_elems, const int* tets,
+                      int n_pins, const int* pins, const wrapd_material* mat, const double* mu_e,
+                      const double* lambda_e);
+int wrapd_sizes(wrapd_ctx* ctx, int* n_verts, int* n_elems, int* n_free, int* n_fix);
+
+/* TetElements::update_fix_cache (src/TetElements.cpp:547-562): x_fix (n_fix,3) */
+int wrapd_set_x_fix(wrapd_ctx* ctx, const double* x_fix);
+/* free positions (n_free,3) in and out (ADMM m_curr_x_free) */
+int wrapd_set_x_free(wrapd_ctx* ctx, const double* x_free);
+int wrapd_get_x_free(wrapd_ctx* ctx, double* x_free);
+
+/* |x_free - x_ref| for the x-error log (src/AlgorithmData.cpp:63, 70); x_ref (n_free,3) */
+int wrapd_set_x_ref(wrapd_ctx* ctx, const double* x_ref);
+int wrapd_x_distance(wrapd_ctx* ctx, double* dist);
+
+/* Settings::m_rot_awareness (src/Settings.hpp:107-111): selects what the deformation cache
+ * holds (the symmetric polar factor, or F^T for the rotation-unaware local step). */
+int wrapd_set_rot_aware(wrapd_ctx* ctx, int rot_aware);
+
+/* TetElements::initialize_dual_vars (src/TetElements.cpp:145-183) at the current positions;
+ * returns polar_potential_energy() for rot_aware = 1 (src/TetElements.cpp:638-675). */
+int wrapd_init_duals(wrapd_ctx* ctx, int rot_aware, double* polar_pe);
+
+/* update_defo_cache + potential_energy_x + inversion_count at the current x
+ * (src/TetElements.cpp:564-635, 444-468); also refreshes the cached symmetric factor. */
+int wrapd_defo_energy(wrapd_ctx* ctx, double* energy, int* flips);
+
+/* update_all_candidate_weights (src/TetElements.cpp:391-432) from sigma(F(x))
+ * (midupdate_weights, :360-388); use_hessian = reweighting counter past the delay;
+ * direct = 1 forms F from all four corners at once as midupdate_weights(X) does
+ * (:366-373), direct = 0 as the deformation cache does (:567-587). */
+int wrapd_candidate_weights(wrapd_ctx* ctx, int use_hessian, int direct);
+int wrapd_max_wsq_ratio(wrapd_ctx* ctx, double* ratio);
+/* update_actual_weights (src/TetElements.cpp:435-442) */
+int wrapd_update_actual_weights(wrapd_ctx* ctx);
+
+/* TetElements::polar_update (src/TetElements.cpp:256-356): local step on all elements. */
+int wrapd_local_step(wrapd_ctx* ctx, const wrapd_local_args* args, double pe_prev, wrapd_local_out* out);
+/* TetElements::update (src/TetElements.cpp:193-254): rotation-unaware local step (-ra 0). */
+int wrapd_local_step_unaware(wrapd_ctx* ctx, const wrapd_local_args* args);
+
+/* global_obj_value / global_obj_grad (src/TetElements.cpp:678-787) at the current x;
+ * grad (n_free,3) may be NULL. */
+int wrapd_global_value_grad(wrapd_ctx* ctx, double* value, double* grad);
+
+/* System::get_A / update_A (src/System.cpp:78-100) + LinearSolver::compute/factorize
+ * (deps/mcloptlib/include/MCL/LinearSolver.hpp:288-292): assembles A on the device,
+ * runs the host symbolic analysis once, then the numeric factorization. */
+int wrapd_refactor(wrapd_ctx* ctx);
+/* LinearSolver::solve for the three coordinate columns (SplitLBFGS.hpp:159-171):
+ * b, x (n_free,3) host arrays. */
+int wrapd_solve3(wrapd_ctx* ctx, const double* b, double* x);
+/* A values (upper triangle, CSR over free vertices), for checking. Returns nnz. */
+int wrapd_get_A(wrapd_ctx* ctx, int* row_ptr, int* cols, double* vals);
+/* symbolic statistics: [nsuper, nnz(L), factor flops, max front, leaf tasks, levels, G entries] */
+int wrapd_chol_stats(wrapd_ctx* ctx, double* stats7);
+
+/* SplitLBFGS (deps/mcloptlib/include/MCL/SplitLBFGS.hpp:41-257) */
+int wrapd_lbfgs_configure(wrapd_ctx* ctx, const wrapd_lbfgs_cfg* cfg);
+int wrapd_lbfgs_reset(wrapd_ctx* ctx);              /* update_A0 / clear_history: m_k = 0 */
+/* RotAwareGS::global_step (src/GlobalStepBehavior.hpp:246-258): value, minimize with the
+ * early-exit tolerance |dAL_local|, value; energy and flips at the final x. */
+int wrapd_global_step(wrapd_ctx* ctx, double early_exit_tol, wrapd_global_out* out);
+/* UnawareGS::global_step (src/GlobalStepBehavior.hpp:164-184): x = A^-1 b with
+ * b from add_to_b (src/TetElements.cpp:470-481). */
+int wrapd_global_step_unaware(wrapd_ctx* ctx, wrapd_global_out* out);
+
+/* element state, for checking: "U","Z","wsq","cand_wsq","sigma_last","vol","Di","symF"
+ * with (n_e,9)/(n_e)/(n_e,3)/(n_e,12) row-major shapes. */
+int wrapd_get_elem(wrapd_ctx* ctx, const char* name, double* out);
+int wrapd_set_elem(wrapd_ctx* ctx, const char* name, const double* in);
+
+/* timing of the device work, measured with HIP events on the context's stream */
+int wrapd_profile_enable(wrapd_ctx* ctx, int on);
+int wrapd_profile_get(wrapd_ctx* ctx, const char* kernel, double* total_ms, double* launches);
+int wrapd_synchronize(wrapd_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WRAPD_GPU_H_ */

@@ -164,6 +164,10 @@ WD_HD T backtracking_cubic(const T x[3], const T p[3], const ProxProblem<T>& P, 
         alphap = alpha;
         const T lo = T(0.1) * alpha, hi = T(0.5) * alpha;
         alpha = at < lo ? lo : (at > hi ? hi : at);
+        // A NaN step never satisfies the Armijo test again (every later trial value and
+        // interpolation is NaN), so the loop above would only exhaust max_iters and fail:
+        // return that failure now.
+        if (alpha != alpha) return T(-1);
     }
     if (iter >= max_iters) return T(-1);
     return alpha;
@@ -295,27 +299,34 @@ WD_HD int prox_newton(const ProxProblem<T>& P, T x[3], int max_iters, T rtol, T&
         if (gn <= rtol * scale * (xn > T(1) ? xn : T(1))) return it;
         M3<T> H;
         nh_hess_density_raw(x, P.p, H);
-        // shift: lower bound on lambda_min via Gershgorin, then exact check by Cholesky
-        T shift = T(0);
-        {
-            const T r0 = H(0, 0) - wabs(H(0, 1)) - wabs(H(0, 2));
-            const T r1 = H(1, 1) - wabs(H(1, 0)) - wabs(H(1, 2));
-            const T r2 = H(2, 2) - wabs(H(2, 0)) - wabs(H(2, 1));
-            T lo = r0 < r1 ? r0 : r1;
-            lo = lo < r2 ? lo : r2;
-            if (lo < T(0)) shift = -lo;
-        }
-        H(0, 0) += P.wov + shift;
-        H(1, 1) += P.wov + shift;
-        H(2, 2) += P.wov + shift;
+        M3<T> Hw = H;
+        Hw(0, 0) += P.wov;
+        Hw(1, 1) += P.wov;
+        Hw(2, 2) += P.wov;
         T d[3];
-        if (!spd_solve3(H, g, d)) {
-            const T inv = T(1) / scale;
-            d[0] = g[0] * inv; d[1] = g[1] * inv; d[2] = g[2] * inv;
+        if (!spd_solve3(Hw, g, d)) {
+            // indefinite density Hessian: lift its spectrum to >= 0 (the projection the
+            // reference's makePD performs, src/Math.hpp:243-261, here as a shift by the
+            // exact smallest eigenvalue) before adding w^2/vol
+            M3<T> Hn = H;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Hn.m[i] = -H.m[i];
+            const T lmin = -sym3_lambda_max(Hn);
+            const T shift = lmin < T(0) ? -lmin * (T(1) + T(16) * Eps<T>::v) : T(0);
+            Hw(0, 0) += shift;
+            Hw(1, 1) += shift;
+            Hw(2, 2) += shift;
+            if (!spd_solve3(Hw, g, d)) {
+                const T inv = T(1) / scale;
+                d[0] = g[0] * inv; d[1] = g[1] * inv; d[2] = g[2] * inv;
+            }
         }
-        // Armijo backtracking; infeasible points cost 1e15 through the barrier
+        // Armijo backtracking; infeasible points cost 1e15 through the barrier.  Near the
+        // minimiser the predicted decrease falls below the rounding of f itself, so the
+        // test allows a few ulps of |f| (otherwise the full Newton step is rejected there).
         T a = T(1);
         const T gtd = dot3(g, d);
+        const T fslack = T(8) * Eps<T>::v * wabs(f);
         T xn3[3];
         T fn = f;
         bool ok = false;
@@ -323,7 +334,7 @@ WD_HD int prox_newton(const ProxProblem<T>& P, T x[3], int max_iters, T rtol, T&
 #pragma unroll
             for (int c = 0; c < 3; ++c) xn3[c] = x[c] - a * d[c];
             fn = P.value(xn3);
-            if (fn <= f - T(1e-4) * a * gtd) { ok = true; break; }
+            if (fn <= f - T(1e-4) * a * gtd + fslack) { ok = true; break; }
             a *= T(0.5);
         }
         if (!ok) return it;   // no further decrease possible at this precision

@@ -1,0 +1,860 @@
+// C ABI (include/wrapd_gpu.h): context management, mesh upload, and the host control of
+// the device work.  The control flow of the global step follows SplitLBFGS::minimize
+// (deps/mcloptlib/include/MCL/SplitLBFGS.hpp:114-256) decision for decision; only the
+// vector arithmetic runs on the device.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "wd_ctx.hpp"
+
+using wd::check_hip;
+
+wrapd_ctx::~wrapd_ctx() {
+    auto f = [](void* p) { if (p) (void)hipFree(p); };
+    f(vidx); f(Di); f(vol); f(kst); f(mu_e); f(lam_e); f(wsq); f(cand); f(U); f(P); f(symF); f(sig); f(gstage);
+    f(xall); f(grad); f(grad_old); f(qv); f(rv); f(bv); f(xref);
+    for (int i = 0; i < wd::kMaxHist; ++i) { f(hS[i]); f(hY[i]); }
+    f(v2e_ptr); f(v2e_idx); f(a_cptr); f(a_cidx); f(Ax);
+    f(G); f(ysol); f(xsol); f(ubuf); f(d_perm); f(d_iperm); f(d_col0); f(d_fsize); f(d_rowptr); f(d_rows);
+    f(d_goff); f(d_uoff); f(d_asm_ptr); f(d_asm_idx); f(d_leaf_ptr); f(d_leaf_sn); f(d_fwd_tasks); f(d_bwd_tasks);
+    f(partials); f(scal);
+    if (scal_host) (void)hipHostFree(scal_host);
+    for (auto& p : prof.pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (auto e : prof.pool) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace wd {
+
+int check_hip(wrapd_ctx* c, hipError_t e, const char* what) {
+    if (e == hipSuccess) return 0;
+    if (c) c->last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return 1;
+}
+
+static hipEvent_t ev_get(wrapd_ctx* c) {
+    if (!c->prof.pool.empty()) {
+        hipEvent_t e = c->prof.pool.back();
+        c->prof.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+ProfScope::ProfScope(wrapd_ctx* ctx, const char* name) : c(ctx) {
+    if (!c->prof.on) return;
+    Prof::Pending p;
+    p.name = name;
+    p.a = ev_get(c);
+    p.b = ev_get(c);
+    (void)hipEventRecord(p.a, c->stream);
+    c->prof.pending.push_back(p);
+    idx = static_cast<int>(c->prof.pending.size()) - 1;
+}
+
+ProfScope::~ProfScope() {
+    if (idx >= 0) (void)hipEventRecord(c->prof.pending[idx].b, c->stream);
+}
+
+static void prof_flush(wrapd_ctx* c) {
+    if (c->prof.pending.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& p : c->prof.pending) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, p.a, p.b);
+        auto& a = c->prof.acc[p.name];
+        a.first += ms;
+        a.second += 1.0;
+        c->prof.pool.push_back(p.a);
+        c->prof.pool.push_back(p.b);
+    }
+    c->prof.pending.clear();
+}
+
+template <typename T>
+static int dalloc(wrapd_ctx* c, T*& p, size_t n) {
+    if (p) { (void)hipFree(p); p = nullptr; }
+    return check_hip(c, hipMalloc(&p, std::max<size_t>(1, n) * sizeof(T)), "hipMalloc");
+}
+
+template <typename T>
+static int h2d(wrapd_ctx* c, T* d, const T* h, size_t n) {
+    if (n == 0) return 0;
+    return check_hip(c, hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, c->stream), "H2D");
+}
+
+template <typename T>
+static int d2h(wrapd_ctx* c, T* h, const T* d, size_t n) {
+    if (n == 0) return 0;
+    if (check_hip(c, hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, c->stream), "D2H")) return 1;
+    return check_hip(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+// read n device scalars starting at slot `at`
+static const double* scal_read(wrapd_ctx* c, int at, int n) {
+    if (check_hip(c, hipMemcpyAsync(c->scal_host + at, c->scal + at, n * sizeof(double), hipMemcpyDeviceToHost,
+                                    c->stream), "scal D2H"))
+        return nullptr;
+    if (check_hip(c, hipStreamSynchronize(c->stream), "sync")) return nullptr;
+    return c->scal_host + at;
+}
+
+static int launch_status(wrapd_ctx* c) {
+    return check_hip(c, hipGetLastError(), "kernel launch") ? WRAPD_ERR_HIP : WRAPD_OK;
+}
+
+static int elem_blocks(wrapd_ctx* c) { return blocks_for(c->n_e, kElemThreads); }
+static int local_blocks(wrapd_ctx* c) { return blocks_for(c->n_e, kLocalThreads); }
+
+}  // namespace wd
+
+using namespace wd;
+
+#define CTX_OK(c) do { if (!(c)) return WRAPD_ERR_ARG; } while (0)
+#define MESH_OK(c) do { if (!(c)->mesh_ready) { (c)->last_error = "mesh not uploaded"; return WRAPD_ERR_STATE; } } while (0)
+
+extern "C" {
+
+int wrapd_create(wrapd_ctx** out, int device) {
+    if (!out) return WRAPD_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return WRAPD_ERR_NO_DEVICE;
+    wrapd_ctx* c = new wrapd_ctx();
+    c->device = device;
+    if (check_hip(c, hipSetDevice(device), "hipSetDevice") ||
+        check_hip(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate") ||
+        dalloc(c, c->partials, static_cast<size_t>(kMaxBlocks) * kRedStride) || dalloc(c, c->scal, S_NUM) ||
+        check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&c->scal_host), S_NUM * sizeof(double)), "hipHostMalloc")) {
+        delete c;
+        return WRAPD_ERR_HIP;
+    }
+    launch_fill(c, c->scal, 0.0, S_NUM);
+    *out = c;
+    return WRAPD_OK;
+}
+
+int wrapd_destroy(wrapd_ctx* c) {
+    if (!c) return WRAPD_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    delete c;
+    return WRAPD_OK;
+}
+
+const char* wrapd_last_error(wrapd_ctx* c) { return c ? c->last_error.c_str() : "null context"; }
+
+int wrapd_sizes(wrapd_ctx* c, int* nv, int* ne, int* nf, int* nfix) {
+    CTX_OK(c);
+    if (nv) *nv = c->n_v;
+    if (ne) *ne = c->n_e;
+    if (nf) *nf = c->n_f;
+    if (nfix) *nfix = c->n_fix;
+    return WRAPD_OK;
+}
+
+int wrapd_upload_mesh(wrapd_ctx* c, int n_v, const double* rest, int n_e, const int* tets, int n_pins,
+                      const int* pins, const wrapd_material* mat, const double* mu_e, const double* lambda_e) {
+    CTX_OK(c);
+    if (n_v <= 0 || n_e <= 0 || !rest || !tets || !mat || n_pins < 0 || (n_pins > 0 && !pins)) {
+        c->last_error = "wrapd_upload_mesh: bad arguments";
+        return WRAPD_ERR_ARG;
+    }
+    if (mat->model != 0 && mat->model != 1) { c->last_error = "unknown model"; return WRAPD_ERR_ARG; }
+    for (long long i = 0; i < 4ll * n_e; ++i)
+        if (tets[i] < 0 || tets[i] >= n_v) { c->last_error = "tet index out of range"; return WRAPD_ERR_ARG; }
+    (void)hipSetDevice(c->device);
+    c->n_v = n_v;
+    c->n_e = n_e;
+    c->n_ep = ((n_e + 255) / 256) * 256;
+    c->mat = *mat;
+    c->tets.assign(tets, tets + 4ll * n_e);
+    c->rest.assign(rest, rest + 3ll * n_v);
+    // binding::add_tetmesh (samples/utils/AddMeshes.hpp:89-110)
+    c->pins.assign(pins, pins + n_pins);
+    std::sort(c->pins.begin(), c->pins.end());
+    for (size_t i = 0; i < c->pins.size(); ++i)
+        if (c->pins[i] < 0 || c->pins[i] >= n_v || (i > 0 && c->pins[i] == c->pins[i - 1])) {
+            c->last_error = "pins out of range or repeated";
+            return WRAPD_ERR_ARG;
+        }
+    c->free_idx.assign(n_v, -1);
+    c->fix_idx.assign(n_v, -1);
+    int j = 0, k = 0;
+    for (int i = 0; i < n_v; ++i) {
+        if (j < n_pins && i == c->pins[j]) { c->fix_idx[i] = j; j++; }
+        else { c->free_idx[i] = k; k++; }
+    }
+    c->n_f = k;
+    c->n_fix = j;
+    if (c->n_f == 0) { c->last_error = "no free vertices"; return WRAPD_ERR_ARG; }
+    const int n_ep = c->n_ep, n_f = c->n_f;
+    // element -> combined position index
+    std::vector<int> vidx(4ll * n_ep, 0);
+    for (int e = 0; e < n_e; ++e)
+        for (int q = 0; q < 4; ++q) {
+            const int v = tets[4 * e + q];
+            vidx[4 * e + q] = c->free_idx[v] >= 0 ? c->free_idx[v] : n_f + c->fix_idx[v];
+        }
+    // A pattern (upper triangle over free vertices) and its per-entry contributions
+    std::vector<long long> keys;
+    keys.reserve(10ll * n_e);
+    for (int e = 0; e < n_e; ++e)
+        for (int a = 0; a < 4; ++a) {
+            const int fa = c->free_idx[tets[4 * e + a]];
+            if (fa < 0) continue;
+            for (int b = 0; b < 4; ++b) {
+                const int fb = c->free_idx[tets[4 * e + b]];
+                if (fb >= 0 && fa <= fb) keys.push_back(static_cast<long long>(fa) * n_f + fb);
+            }
+        }
+    std::vector<long long> ukeys = keys;
+    std::sort(ukeys.begin(), ukeys.end());
+    ukeys.erase(std::unique(ukeys.begin(), ukeys.end()), ukeys.end());
+    c->nnzA = static_cast<int>(ukeys.size());
+    c->Ap.assign(n_f + 1, 0);
+    c->Aj.resize(c->nnzA);
+    for (int p = 0; p < c->nnzA; ++p) {
+        c->Ap[ukeys[p] / n_f + 1]++;
+        c->Aj[p] = static_cast<int>(ukeys[p] % n_f);
+    }
+    for (int i = 0; i < n_f; ++i) c->Ap[i + 1] += c->Ap[i];
+    std::vector<int> ccnt(c->nnzA + 1, 0), cidx;
+    std::vector<int> slot_of;   // per (e, a, b) in key order
+    slot_of.reserve(keys.size());
+    {
+        size_t t = 0;
+        for (int e = 0; e < n_e; ++e)
+            for (int a = 0; a < 4; ++a) {
+                const int fa = c->free_idx[tets[4 * e + a]];
+                if (fa < 0) continue;
+                for (int b = 0; b < 4; ++b) {
+                    const int fb = c->free_idx[tets[4 * e + b]];
+                    if (!(fb >= 0 && fa <= fb)) continue;
+                    const int* row = c->Aj.data() + c->Ap[fa];
+                    const int* rend = c->Aj.data() + c->Ap[fa + 1];
+                    const int p = c->Ap[fa] + static_cast<int>(std::lower_bound(row, rend, fb) - row);
+                    slot_of.push_back(p);
+                    ccnt[p + 1]++;
+                    ++t;
+                }
+            }
+    }
+    for (int p = 0; p < c->nnzA; ++p) ccnt[p + 1] += ccnt[p];
+    cidx.assign(ccnt[c->nnzA], 0);
+    {
+        std::vector<int> fill(ccnt.begin(), ccnt.end() - 1);
+        size_t t = 0;
+        for (int e = 0; e < n_e; ++e)
+            for (int a = 0; a < 4; ++a) {
+                const int fa = c->free_idx[tets[4 * e + a]];
+                if (fa < 0) continue;
+                for (int b = 0; b < 4; ++b) {
+                    const int fb = c->free_idx[tets[4 * e + b]];
+                    if (!(fb >= 0 && fa <= fb)) continue;
+                    cidx[fill[slot_of[t++]]++] = e * 16 + a * 4 + b;
+                }
+            }
+    }
+    // vertex -> element-corner gather lists (element order)
+    std::vector<int> vptr(n_f + 1, 0), vlist;
+    for (int e = 0; e < n_e; ++e)
+        for (int q = 0; q < 4; ++q) {
+            const int f = c->free_idx[tets[4 * e + q]];
+            if (f >= 0) vptr[f + 1]++;
+        }
+    for (int i = 0; i < n_f; ++i) vptr[i + 1] += vptr[i];
+    vlist.assign(vptr[n_f], 0);
+    {
+        std::vector<int> fill(vptr.begin(), vptr.end() - 1);
+        for (int e = 0; e < n_e; ++e)
+            for (int q = 0; q < 4; ++q) {
+                const int f = c->free_idx[tets[4 * e + q]];
+                if (f >= 0) vlist[fill[f]++] = 4 * e + q;
+            }
+    }
+    // device allocations
+    int rc = 0;
+    rc |= dalloc(c, c->vidx, 4ull * n_ep);
+    rc |= dalloc(c, c->Di, 12ull * n_ep);
+    rc |= dalloc(c, c->vol, n_ep);
+    rc |= dalloc(c, c->kst, n_ep);
+    rc |= dalloc(c, c->wsq, n_ep);
+    rc |= dalloc(c, c->cand, n_ep);
+    rc |= dalloc(c, c->U, 9ull * n_ep);
+    rc |= dalloc(c, c->P, 9ull * n_ep);
+    rc |= dalloc(c, c->symF, 9ull * n_ep);
+    rc |= dalloc(c, c->sig, 3ull * n_ep);
+    rc |= dalloc(c, c->gstage, 12ull * n_e);
+    rc |= dalloc(c, c->xall, 3ull * (n_f + c->n_fix));
+    rc |= dalloc(c, c->grad, 3ull * n_f);
+    rc |= dalloc(c, c->grad_old, 3ull * n_f);
+    rc |= dalloc(c, c->qv, 3ull * n_f);
+    rc |= dalloc(c, c->rv, 3ull * n_f);
+    rc |= dalloc(c, c->bv, 3ull * n_f);
+    for (int i = 0; i < kMaxHist; ++i) { rc |= dalloc(c, c->hS[i], 3ull * n_f); rc |= dalloc(c, c->hY[i], 3ull * n_f); }
+    rc |= dalloc(c, c->v2e_ptr, vptr.size());
+    rc |= dalloc(c, c->v2e_idx, vlist.size());
+    rc |= dalloc(c, c->a_cptr, ccnt.size());
+    rc |= dalloc(c, c->a_cidx, cidx.size());
+    rc |= dalloc(c, c->Ax, c->nnzA);
+    if (mu_e) { rc |= dalloc(c, c->mu_e, n_ep); } else if (c->mu_e) { (void)hipFree(c->mu_e); c->mu_e = nullptr; }
+    if (lambda_e) { rc |= dalloc(c, c->lam_e, n_ep); } else if (c->lam_e) { (void)hipFree(c->lam_e); c->lam_e = nullptr; }
+    if (rc) return WRAPD_ERR_HIP;
+    rc |= h2d(c, c->vidx, vidx.data(), vidx.size());
+    rc |= h2d(c, c->v2e_ptr, vptr.data(), vptr.size());
+    rc |= h2d(c, c->v2e_idx, vlist.data(), vlist.size());
+    rc |= h2d(c, c->a_cptr, ccnt.data(), ccnt.size());
+    rc |= h2d(c, c->a_cidx, cidx.data(), cidx.size());
+    std::vector<double> me, le;
+    if (mu_e) { me.assign(n_ep, mat->mu); std::copy(mu_e, mu_e + n_e, me.begin()); rc |= h2d(c, c->mu_e, me.data(), n_ep); }
+    if (lambda_e) { le.assign(n_ep, mat->lambda); std::copy(lambda_e, lambda_e + n_e, le.begin()); rc |= h2d(c, c->lam_e, le.data(), n_ep); }
+    c->hetero = mu_e || lambda_e;
+    // setup kernel on the rest positions
+    int* d_tv = nullptr;
+    double* d_rest = nullptr;
+    int* d_bad = nullptr;
+    std::vector<int> tvp(4ll * n_ep, 0);
+    std::copy(tets, tets + 4ll * n_e, tvp.begin());
+    rc |= dalloc(c, d_tv, tvp.size());
+    rc |= dalloc(c, d_rest, 3ull * n_v);
+    rc |= dalloc(c, d_bad, 1);
+    if (rc) return WRAPD_ERR_HIP;
+    rc |= h2d(c, d_tv, tvp.data(), tvp.size());
+    rc |= h2d(c, d_rest, rest, 3ull * n_v);
+    int zero = 0;
+    rc |= h2d(c, d_bad, &zero, 1);
+    launch_setup(c, d_tv, d_rest, d_bad);
+    int bad = 0;
+    rc |= launch_status(c);
+    rc |= d2h(c, &bad, d_bad, 1);
+    (void)hipFree(d_tv);
+    (void)hipFree(d_rest);
+    (void)hipFree(d_bad);
+    if (rc) return WRAPD_ERR_HIP;
+    // zero the dual state so that reads before init_duals are defined
+    launch_fill(c, c->U, 0.0, 9ll * n_ep);
+    launch_fill(c, c->P, 0.0, 9ll * n_ep);
+    launch_fill(c, c->symF, 0.0, 9ll * n_ep);
+    launch_fill(c, c->cand, 1.0, n_ep);
+    launch_fill(c, c->grad_old, 0.0, 3ll * n_f);
+    c->analyzed = c->factored = false;
+    c->m_k = 0;
+    for (int i = 0; i < kMaxHist; ++i) c->pos2slot[i] = i;
+    if (bad) { c->last_error = "Inverted tet in the rest mesh"; return WRAPD_ERR_INVERTED_REST; }
+    c->mesh_ready = true;
+    return launch_status(c);
+}
+
+int wrapd_set_x_fix(wrapd_ctx* c, const double* x_fix) {
+    CTX_OK(c); MESH_OK(c);
+    if (c->n_fix == 0) return WRAPD_OK;
+    if (!x_fix) return WRAPD_ERR_ARG;
+    return h2d(c, c->xall + 3ll * c->n_f, x_fix, 3ull * c->n_fix) ? WRAPD_ERR_HIP : WRAPD_OK;
+}
+
+int wrapd_set_x_free(wrapd_ctx* c, const double* x) {
+    CTX_OK(c); MESH_OK(c);
+    if (!x) return WRAPD_ERR_ARG;
+    return h2d(c, c->xall, x, 3ull * c->n_f) ? WRAPD_ERR_HIP : WRAPD_OK;
+}
+
+int wrapd_get_x_free(wrapd_ctx* c, double* x) {
+    CTX_OK(c); MESH_OK(c);
+    if (!x) return WRAPD_ERR_ARG;
+    return d2h(c, x, c->xall, 3ull * c->n_f) ? WRAPD_ERR_HIP : WRAPD_OK;
+}
+
+int wrapd_set_x_ref(wrapd_ctx* c, const double* x) {
+    CTX_OK(c); MESH_OK(c);
+    if (!x) return WRAPD_ERR_ARG;
+    if (!c->xref && dalloc(c, c->xref, 3ull * c->n_f)) return WRAPD_ERR_HIP;
+    return h2d(c, c->xref, x, 3ull * c->n_f) ? WRAPD_ERR_HIP : WRAPD_OK;
+}
+
+int wrapd_x_distance(wrapd_ctx* c, double* dist) {
+    CTX_OK(c); MESH_OK(c);
+    if (!c->xref) { c->last_error = "no reference positions"; return WRAPD_ERR_STATE; }
+    launch_diffnorm(c, c->xall, c->xref);
+    const double* r = scal_read(c, S_RED, 1);
+    if (!r) return WRAPD_ERR_HIP;
+    if (dist) *dist = std::sqrt(r[0]);
+    return launch_status(c);
+}
+
+int wrapd_set_rot_aware(wrapd_ctx* c, int rot_aware) {
+    CTX_OK(c);
+    c->rot_aware = rot_aware ? 1 : 0;
+    return WRAPD_OK;
+}
+
+int wrapd_init_duals(wrapd_ctx* c, int rot_aware, double* polar_pe) {
+    CTX_OK(c); MESH_OK(c);
+    c->rot_aware = rot_aware ? 1 : 0;
+    launch_init_duals(c, c->rot_aware);
+    launch_finalize(c, elem_blocks(c), 1, 0u, S_RED);
+    const double* r = scal_read(c, S_RED, 1);
+    if (!r) return WRAPD_ERR_HIP;
+    if (polar_pe) *polar_pe = r[0];
+    return launch_status(c);
+}
+
+int wrapd_defo_energy(wrapd_ctx* c, double* energy, int* flips) {
+    CTX_OK(c); MESH_OK(c);
+    launch_defo(c, c->rot_aware ? (DM_WRITE_SYMF | DM_ENERGY) : (DM_FT_CACHE | DM_ENERGY), 0);
+    launch_finalize(c, elem_blocks(c), 3, 0u, S_RED);
+    const double* r = scal_read(c, S_RED, 3);
+    if (!r) return WRAPD_ERR_HIP;
+    c->last_energy = r[1];
+    c->last_flips = static_cast<int>(r[2]);
+    if (energy) *energy = r[1];
+    if (flips) *flips = static_cast<int>(r[2]);
+    return launch_status(c);
+}
+
+int wrapd_candidate_weights(wrapd_ctx* c, int use_hessian, int direct) {
+    CTX_OK(c); MESH_OK(c);
+    if (c->mat.model != 1) { c->last_error = "Adaptive weights are only supported for the NH model."; return WRAPD_ERR_STATE; }
+    launch_defo(c, direct ? (DM_CAND | DM_DIRECT) : DM_CAND, use_hessian);
+    return launch_status(c);
+}
+
+int wrapd_max_wsq_ratio(wrapd_ctx* c, double* ratio) {
+    CTX_OK(c); MESH_OK(c);
+    launch_max_ratio(c);
+    launch_finalize(c, elem_blocks(c), 1, 1u, S_RED);
+    const double* r = scal_read(c, S_RED, 1);
+    if (!r) return WRAPD_ERR_HIP;
+    if (ratio) *ratio = r[0];
+    return launch_status(c);
+}
+
+int wrapd_update_actual_weights(wrapd_ctx* c) {
+    CTX_OK(c); MESH_OK(c);
+    launch_update_weights(c);
+    return launch_status(c);
+}
+
+int wrapd_local_step(wrapd_ctx* c, const wrapd_local_args* a, double pe_prev, wrapd_local_out* out) {
+    CTX_OK(c); MESH_OK(c);
+    if (!a) return WRAPD_ERR_ARG;
+    if (a->update_cand && c->mat.model != 1) {
+        c->last_error = "Adaptive weights are only supported for the NH model.";
+        return WRAPD_ERR_STATE;
+    }
+    launch_local(c, a);
+    launch_finalize(c, local_blocks(c), 10, (1u << 3) | (1u << 9), S_RED);
+    const double* r = scal_read(c, S_RED, 10);
+    if (!r) return WRAPD_ERR_HIP;
+    if (out) {
+        out->bef = r[0];
+        out->aft = r[1];
+        out->pe = r[2];
+        out->max_wsq_ratio = r[3];
+        out->prox_iters_total = r[4];
+        out->fallback_count = r[5];
+        out->neg_sigma_count = r[6];
+        out->learned_accepted = r[7];
+        out->evals_total = r[8];
+        out->evals_max = r[9];
+        out->delta_auglag = r[1] - r[0] + r[2] - pe_prev;
+    }
+    int st = launch_status(c);
+    if (st == WRAPD_OK && r[6] > 0) {
+        c->last_error = "Negative singular value in solution";
+        return WRAPD_ERR_NEG_SIGMA;
+    }
+    return st;
+}
+
+int wrapd_local_step_unaware(wrapd_ctx* c, const wrapd_local_args* a) {
+    CTX_OK(c); MESH_OK(c);
+    if (!a) return WRAPD_ERR_ARG;
+    launch_local_unaware(c, a);
+    return launch_status(c);
+}
+
+int wrapd_global_value_grad(wrapd_ctx* c, double* value, double* grad) {
+    CTX_OK(c); MESH_OK(c);
+    launch_defo(c, DM_GRAD | DM_VALUE, 0);
+    launch_finalize(c, elem_blocks(c), 1, 0u, S_VAL);
+    launch_gather_grad(c, c->grad, 0, 0, nullptr, 0);
+    const double* r = scal_read(c, S_VAL, 1);
+    if (!r) return WRAPD_ERR_HIP;
+    if (value) *value = r[0];
+    if (grad && d2h(c, grad, c->grad, 3ull * c->n_f)) return WRAPD_ERR_HIP;
+    return launch_status(c);
+}
+
+int wrapd_refactor(wrapd_ctx* c) {
+    CTX_OK(c); MESH_OK(c);
+    launch_assemble_A(c);
+    std::vector<double> Ax(c->nnzA);
+    if (d2h(c, Ax.data(), c->Ax, Ax.size())) return WRAPD_ERR_HIP;
+    if (!c->analyzed) {
+        std::vector<double> xyz(3ull * c->n_f);
+        for (int v = 0; v < c->n_v; ++v) {
+            const int f = c->free_idx[v];
+            if (f >= 0) for (int q = 0; q < 3; ++q) xyz[3 * f + q] = c->rest[3 * v + q];
+        }
+        wrapd::SymbolicOptions opt;
+        try {
+            wrapd::chol_analyze(c->n_f, c->Ap, c->Aj, xyz.data(), opt, c->sym);
+        } catch (const std::exception& ex) {
+            c->last_error = ex.what();
+            return WRAPD_ERR_STATE;
+        }
+        int st = chol_prepare_device(c);
+        if (st) return st;
+        c->analyzed = true;
+    }
+    if (!wrapd::chol_numeric_host(c->sym, c->Ap, c->Aj, Ax, c->G_host)) {
+        c->last_error = "factorization: non-positive pivot (A not positive definite)";
+        c->factored = false;
+        return WRAPD_ERR_NOT_SPD;
+    }
+    int st = chol_upload_factor(c);
+    if (st) return st;
+    c->factored = true;
+    return launch_status(c);
+}
+
+int wrapd_solve3(wrapd_ctx* c, const double* b, double* x) {
+    CTX_OK(c); MESH_OK(c);
+    if (!c->factored) { c->last_error = "no factorization"; return WRAPD_ERR_STATE; }
+    if (!b || !x) return WRAPD_ERR_ARG;
+    if (h2d(c, c->bv, b, 3ull * c->n_f)) return WRAPD_ERR_HIP;
+    launch_solve(c, c->bv, c->rv);
+    if (d2h(c, x, c->rv, 3ull * c->n_f)) return WRAPD_ERR_HIP;
+    return launch_status(c);
+}
+
+int wrapd_get_A(wrapd_ctx* c, int* row_ptr, int* cols, double* vals) {
+    CTX_OK(c); MESH_OK(c);
+    if (row_ptr) std::copy(c->Ap.begin(), c->Ap.end(), row_ptr);
+    if (cols) std::copy(c->Aj.begin(), c->Aj.end(), cols);
+    if (vals) {
+        launch_assemble_A(c);
+        if (d2h(c, vals, c->Ax, c->nnzA)) return -WRAPD_ERR_HIP;
+    }
+    return c->nnzA;
+}
+
+int wrapd_chol_stats(wrapd_ctx* c, double* st) {
+    CTX_OK(c);
+    if (!c->analyzed) return WRAPD_ERR_STATE;
+    st[0] = c->sym.nsuper;
+    st[1] = c->sym.nnz_L;
+    st[2] = c->sym.flops_factor;
+    st[3] = c->sym.max_front;
+    st[4] = static_cast<double>(c->sym.leaf_ptr.size()) - 1;
+    st[5] = static_cast<double>(c->sym.level_ptr.size()) - 1;
+    st[6] = c->sym.g_entries;
+    return WRAPD_OK;
+}
+
+int wrapd_lbfgs_configure(wrapd_ctx* c, const wrapd_lbfgs_cfg* cfg) {
+    CTX_OK(c);
+    if (!cfg || cfg->memory < 1 || cfg->memory > kMaxHist || cfg->max_iters < 0) {
+        if (c) c->last_error = "L-BFGS memory must be in [1, 8]";
+        return WRAPD_ERR_ARG;
+    }
+    c->lcfg = *cfg;
+    c->m_k = 0;
+    return WRAPD_OK;
+}
+
+int wrapd_lbfgs_reset(wrapd_ctx* c) {
+    CTX_OK(c);
+    c->m_k = 0;
+    return WRAPD_OK;
+}
+
+// SplitLBFGS::minimize between the two GlobalProblem::value calls of RotAwareGS::global_step.
+int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
+    CTX_OK(c); MESH_OK(c);
+    if (!c->factored) { c->last_error = "A0 is required but wasn't supplied."; return WRAPD_ERR_STATE; }
+    const int M = c->lcfg.memory;
+    const long long n3 = 3ll * c->n_f;
+    const double m_tol = tol;
+    const double kappa = c->lcfg.kappa;
+    wrapd_global_out o{};
+    o.exit_reason = 0;
+    if (c->m_k == 0) launch_fill(c, c->grad_old, 0.0, n3);   // minimize() re-zeros its vectors when m_k == 0
+    // value(x) fused with the first gradient (the gradient from the cache at the same x)
+    launch_defo(c, DM_GRAD | DM_VALUE, 0);
+    launch_finalize(c, elem_blocks(c), 1, 0u, S_VAL);
+    launch_gather_grad(c, c->grad, 1, 0, nullptr, 0);
+    launch_finalize(c, blocks_for(c->n_f, 256), 1, 0u, S_DG);
+    const double* r = scal_read(c, S_DG, 2);   // S_DG, S_VAL
+    if (!r) return WRAPD_ERR_HIP;
+    const double dg = r[0];
+    o.obj_bef = r[1];
+    o.obj_aft = o.obj_bef;
+    o.energy = c->last_energy;
+    o.flips = c->last_flips;
+    if (dg < 1.e-13) {
+        o.iters = 0;
+        o.exit_reason = 3;
+        if (out) *out = o;
+        return launch_status(c);
+    }
+    int global_iter = 1;
+    int max_iters = c->lcfg.max_iters;
+    bool breaking_out = false;
+    bool have_final = false;
+    double alpha_init = 1.0;
+    for (; global_iter <= max_iters; ++global_iter) {
+        // x_old = x (kept implicitly by the x-step), grad_old = grad
+        std::swap(c->grad, c->grad_old);
+        const double* g = c->grad_old;
+        const int it = std::min(M, c->m_k);
+        int slots[kMaxHist];
+        for (int i = 0; i < it; ++i) slots[i] = c->pos2slot[i];
+        double dir;
+        if (it > 0) {
+            double* vs[kMaxHist];
+            for (int i = 0; i < it; ++i) vs[i] = c->hS[slots[i]];
+            launch_multidot(c, g, vs, it);
+            launch_lbfgs_alpha(c, slots, it);
+            launch_axpy_q(c, g, slots, it);
+            launch_solve(c, c->qv, c->rv);
+            for (int i = 0; i < it; ++i) vs[i] = c->hY[slots[i]];
+            launch_multidot(c, c->rv, vs, it);
+            launch_lbfgs_beta(c, slots, it);
+            launch_axpy_r(c, g, slots, it);
+            const double* d = scal_read(c, S_DIR, 1);
+            if (!d) return WRAPD_ERR_HIP;
+            dir = d[0];
+        } else {
+            launch_solve(c, g, c->rv);
+            launch_dot(c, g, c->rv);
+            const double* d = scal_read(c, S_RED, 1);
+            if (!d) return WRAPD_ERR_HIP;
+            dir = d[0];
+        }
+        if (std::isnan(dir)) {
+            c->last_error = "Decrement is NaN.";
+            return WRAPD_ERR_NAN;
+        }
+        const double* stepdir = c->rv;
+        if (dir <= 0) {
+            stepdir = g;   // q = grad
+            max_iters -= c->m_k;
+            c->m_k = 0;
+            if (c->lcfg.linesearch == 1 && m_tol > 1.e-12) {
+                launch_dot(c, g, g);
+                const double* d = scal_read(c, S_RED, 2);
+                if (!d) return WRAPD_ERR_HIP;
+                alpha_init = std::min(1.0, 1.0 / d[1]);
+            }
+        }
+        double rate = 1.0;
+        if (m_tol > 1.e-12 && c->lcfg.linesearch == 1) {
+            // Backtracking::search on n x 3 (deps/mcloptlib/include/MCL/Backtracking.hpp:71-98), p = -q
+            launch_dot(c, stepdir, stepdir);
+            const double* d = scal_read(c, S_RED, 1);
+            if (!d) return WRAPD_ERR_HIP;
+            const double pnorm = std::sqrt(d[0]);
+            if (pnorm <= 2.220446049250313e-16) {
+                rate = 1e-4;
+            } else {
+                launch_dot(c, g, stepdir);
+                d = scal_read(c, S_RED, 1);
+                if (!d) return WRAPD_ERR_HIP;
+                const double gtp = -d[0];
+                launch_defo(c, DM_VALUE, 0);
+                launch_finalize(c, elem_blocks(c), 1, 0u, S_RED);
+                d = scal_read(c, S_RED, 1);
+                if (!d) return WRAPD_ERR_HIP;
+                const double fx0 = d[0];
+                launch_copy(c, c->bv, c->xall, n3);   // save x
+                double alpha = alpha_init;
+                int lsit = 0;
+                for (; lsit < 100000; ++lsit) {
+                    launch_armijo_x(c, -alpha, stepdir, c->bv);   // x = x_save - alpha q
+                    launch_defo(c, DM_VALUE, 0);
+                    launch_finalize(c, elem_blocks(c), 1, 0u, S_RED);
+                    d = scal_read(c, S_RED, 1);
+                    if (!d) return WRAPD_ERR_HIP;
+                    if (d[0] <= fx0 + alpha * 1e-4 * gtp) break;
+                    alpha *= 0.7;
+                }
+                launch_copy(c, c->xall, c->bv, n3);   // restore x
+                rate = lsit >= 100000 ? -1.0 : alpha;
+            }
+            if (rate <= 0) { c->last_error = "Failure in linesearch"; return WRAPD_ERR_LINESEARCH; }
+        }
+        // position of the new (s, y) pair
+        int new_pos, new_slot;
+        if (c->m_k < M) {
+            new_pos = c->m_k;
+            new_slot = c->pos2slot[new_pos];
+        } else {
+            new_slot = c->pos2slot[0];
+            for (int i = 0; i < M - 1; ++i) c->pos2slot[i] = c->pos2slot[i + 1];
+            c->pos2slot[M - 1] = new_slot;
+            new_pos = M - 1;
+        }
+        const int nlist = new_pos + 1;
+        int lslots[kMaxHist];
+        for (int i = 0; i < nlist; ++i) lslots[i] = c->pos2slot[i];
+        launch_xstep(c, rate, stepdir, c->hS[new_slot]);
+        const bool early = dir > 0 && (dir < kappa * m_tol || dir < kappa * 1.e-12);
+        const bool last = early || (global_iter + 1 > max_iters);
+        launch_defo(c, last ? (DM_GRAD | DM_VALUE | DM_WRITE_SYMF | DM_ENERGY) : (DM_GRAD | DM_VALUE), 0);
+        launch_finalize(c, elem_blocks(c), last ? 3 : 1, 0u, S_VAL);
+        launch_gather_grad(c, c->grad, 2, new_pos, lslots, nlist);
+        const double* d = scal_read(c, S_DENOM, 5);   // DENOM, DG, VAL, EN, FLIPS
+        if (!d) return WRAPD_ERR_HIP;
+        const double denom = d[0];
+        o.obj_aft = d[2];
+        if (last) {
+            o.energy = d[3];
+            o.flips = static_cast<int>(d[4]);
+            have_final = true;
+        }
+        if (early) {
+            c->m_k += 1;
+            breaking_out = true;
+            o.exit_reason = 1;
+            break;
+        }
+        if (std::fabs(denom) <= 0) {
+            breaking_out = true;
+            o.exit_reason = 2;
+            have_final = false;
+            break;
+        }
+        alpha_init = 1.0;
+        c->m_k++;
+    }
+    c->m_k = std::min(M, c->m_k);
+    o.iters = breaking_out ? global_iter : global_iter - 1;
+    if (!have_final) {
+        launch_defo(c, DM_VALUE | DM_WRITE_SYMF | DM_ENERGY, 0);
+        launch_finalize(c, elem_blocks(c), 3, 0u, S_RED);
+        const double* d = scal_read(c, S_RED, 3);
+        if (!d) return WRAPD_ERR_HIP;
+        o.obj_aft = d[0];
+        o.energy = d[1];
+        o.flips = static_cast<int>(d[2]);
+    }
+    c->last_energy = o.energy;
+    c->last_flips = o.flips;
+    if (out) *out = o;
+    return launch_status(c);
+}
+
+// UnawareGS::global_step: b = add_to_b, x = A^-1 b, refresh the cache (F and sigma)
+int wrapd_global_step_unaware(wrapd_ctx* c, wrapd_global_out* out) {
+    CTX_OK(c); MESH_OK(c);
+    if (!c->factored) { c->last_error = "no factorization"; return WRAPD_ERR_STATE; }
+    launch_defo(c, DM_BVEC, 0);
+    launch_gather_grad(c, c->bv, 0, 0, nullptr, 0);
+    launch_solve(c, c->bv, c->xall);
+    launch_defo(c, DM_FT_CACHE | DM_ENERGY, 0);
+    launch_finalize(c, elem_blocks(c), 3, 0u, S_RED);
+    const double* d = scal_read(c, S_RED, 3);
+    if (!d) return WRAPD_ERR_HIP;
+    wrapd_global_out o{};
+    o.iters = 1;
+    o.energy = d[1];
+    o.flips = static_cast<int>(d[2]);
+    c->last_energy = o.energy;
+    c->last_flips = o.flips;
+    if (out) *out = o;
+    return launch_status(c);
+}
+
+int wrapd_get_elem(wrapd_ctx* c, const char* name, double* outp) {
+    CTX_OK(c); MESH_OK(c);
+    if (!name || !outp) return WRAPD_ERR_ARG;
+    const std::string n(name);
+    const int ne = c->n_e, nep = c->n_ep;
+    auto soa = [&](const double* d, int ncomp) -> int {
+        std::vector<double> h(static_cast<size_t>(ncomp) * nep);
+        if (d2h(c, h.data(), d, h.size())) return WRAPD_ERR_HIP;
+        for (int e = 0; e < ne; ++e)
+            for (int q = 0; q < ncomp; ++q) outp[static_cast<size_t>(e) * ncomp + q] = h[static_cast<size_t>(q) * nep + e];
+        return WRAPD_OK;
+    };
+    if (n == "U") return soa(c->U, 9);
+    if (n == "P") return soa(c->P, 9);
+    if (n == "symF") return soa(c->symF, 9);
+    if (n == "wsq") return soa(c->wsq, 1);
+    if (n == "cand_wsq") return soa(c->cand, 1);
+    if (n == "sigma_last") return soa(c->sig, 3);
+    if (n == "vol") return soa(c->vol, 1);
+    if (n == "Di") return soa(c->Di, 12);
+    if (n == "Z") {
+        std::vector<double> u(9ull * ne);
+        int st = soa(c->U, 9);
+        if (st) return st;
+        std::copy(outp, outp + 9ull * ne, u.begin());
+        st = soa(c->P, 9);
+        if (st) return st;
+        for (size_t i = 0; i < u.size(); ++i) outp[i] += u[i];
+        return WRAPD_OK;
+    }
+    c->last_error = "unknown element array";
+    return WRAPD_ERR_ARG;
+}
+
+int wrapd_set_elem(wrapd_ctx* c, const char* name, const double* in) {
+    CTX_OK(c); MESH_OK(c);
+    if (!name || !in) return WRAPD_ERR_ARG;
+    const std::string n(name);
+    const int ne = c->n_e, nep = c->n_ep;
+    auto soa = [&](double* d, int ncomp) -> int {
+        std::vector<double> h(static_cast<size_t>(ncomp) * nep, 0.0);
+        if (d2h(c, h.data(), d, h.size())) return WRAPD_ERR_HIP;
+        for (int e = 0; e < ne; ++e)
+            for (int q = 0; q < ncomp; ++q) h[static_cast<size_t>(q) * nep + e] = in[static_cast<size_t>(e) * ncomp + q];
+        if (h2d(c, d, h.data(), h.size())) return WRAPD_ERR_HIP;
+        return check_hip(c, hipStreamSynchronize(c->stream), "sync") ? WRAPD_ERR_HIP : WRAPD_OK;
+    };
+    if (n == "U") return soa(c->U, 9);
+    if (n == "P") return soa(c->P, 9);
+    if (n == "symF") return soa(c->symF, 9);
+    if (n == "wsq") return soa(c->wsq, 1);
+    if (n == "cand_wsq") return soa(c->cand, 1);
+    if (n == "sigma_last") return soa(c->sig, 3);
+    c->last_error = "unknown or read-only element array";
+    return WRAPD_ERR_ARG;
+}
+
+int wrapd_profile_enable(wrapd_ctx* c, int on) {
+    CTX_OK(c);
+    prof_flush(c);
+    c->prof.on = on != 0;
+    if (on) c->prof.acc.clear();
+    return WRAPD_OK;
+}
+
+int wrapd_profile_get(wrapd_ctx* c, const char* kernel, double* total_ms, double* launches) {
+    CTX_OK(c);
+    prof_flush(c);
+    auto it = c->prof.acc.find(kernel ? kernel : "");
+    if (it == c->prof.acc.end()) {
+        if (total_ms) *total_ms = 0.0;
+        if (launches) *launches = 0.0;
+        return WRAPD_OK;
+    }
+    if (total_ms) *total_ms = it->second.first;
+    if (launches) *launches = it->second.second;
+    return WRAPD_OK;
+}
+
+int wrapd_synchronize(wrapd_ctx* c) {
+    CTX_OK(c);
+    return check_hip(c, hipStreamSynchronize(c->stream), "sync") ? WRAPD_ERR_HIP : WRAPD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,183 @@
+// Internal: the device context behind wrapd_ctx and the launchers shared by the .hip files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/wrapd_gpu.h"
+#include "chol_symbolic.hpp"
+
+namespace wd {
+
+constexpr int kElemThreads = 256;     // element kernels
+constexpr int kLocalThreads = 128;    // local step (register heavy)
+constexpr int kRedStride = 24;        // reduction slots per block in `partials`
+constexpr int kMaxHist = 8;           // largest L-BFGS memory supported on the device
+constexpr int kMaxBlocks = 1 << 16;   // capacity of the partials buffer (blocks)
+
+// device scalar slots (ctx->scal)
+enum ScalarSlot {
+    S_RED = 0,          // 0..23: result of the last finalize
+    S_ALPHA = 32,       // 8: two-loop alpha (logical order)
+    S_COEF = 40,        // 8: alpha - beta
+    S_DIR = 48, S_DENOM = 49, S_DG = 50, S_VAL = 51, S_EN = 52, S_FLIPS = 53, S_GTP = 54, S_GMAX = 55,
+    S_GRAM = 64,        // 64: s_i . y_j by slot (8 x 8)
+    S_NUM = 128
+};
+
+// defo-pass mode bits
+enum DefoMode {
+    DM_WRITE_SYMF = 1, DM_VALUE = 2, DM_GRAD = 4, DM_ENERGY = 8, DM_CAND = 16, DM_DIRECT = 32,
+    DM_FT_CACHE = 64, DM_BVEC = 128
+};
+
+struct Prof {
+    bool on = false;
+    struct Pending { std::string name; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, std::pair<double, double>> acc;   // ms, launches
+};
+
+}  // namespace wd
+
+struct wrapd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    wd::Prof prof;
+
+    // sizes
+    int n_v = 0, n_e = 0, n_ep = 0, n_f = 0, n_fix = 0;
+    bool mesh_ready = false;
+    // host copies
+    std::vector<int> tets, pins, free_idx, fix_idx;
+    std::vector<double> rest;
+    wrapd_material mat{};
+    bool hetero = false;
+
+    // element arrays (structure of arrays, stride n_ep)
+    int* vidx = nullptr;        // int4 per element: index into xall (free: [0,n_f), fixed: n_f + j)
+    double* Di = nullptr;       // 12 x n_ep: Di[(r*4+k)*n_ep + e] = D_local(r,k)
+    double* vol = nullptr;
+    double* kst = nullptr;      // material stiffness k_e
+    double* mu_e = nullptr;     // optional
+    double* lam_e = nullptr;    // optional
+    double* wsq = nullptr;
+    double* cand = nullptr;
+    double* U = nullptr;        // 9 x n_ep
+    double* P = nullptr;        // 9 x n_ep, P = Z - U
+    double* symF = nullptr;     // 9 x n_ep (Ft for the rotation-unaware mode)
+    double* sig = nullptr;      // 3 x n_ep last prox solution
+    double* gstage = nullptr;   // n_e x 12 gradient staging (AoS)
+
+    // vertex vectors (n_f x 3 AoS)
+    double* xall = nullptr;     // (n_f + n_fix) x 3
+    double* grad = nullptr;
+    double* grad_old = nullptr;
+    double* qv = nullptr;
+    double* rv = nullptr;
+    double* bv = nullptr;
+    double* xref = nullptr;     // reference positions for the x-error log
+    double* hS[wd::kMaxHist] = {};
+    double* hY[wd::kMaxHist] = {};
+    int* v2e_ptr = nullptr;     // free vertex -> element corners (e*4+k), element order
+    int* v2e_idx = nullptr;
+
+    // A
+    std::vector<int> Ap, Aj;    // upper CSR over free vertices (host)
+    int nnzA = 0;
+    int* a_cptr = nullptr;      // A slot -> contributions (e*16 + a*4 + b)
+    int* a_cidx = nullptr;
+    double* Ax = nullptr;
+
+    // Cholesky
+    wrapd::CholSymbolic sym;
+    bool analyzed = false, factored = false;
+    double* G = nullptr;
+    double* ysol = nullptr;     // n_f x 3 forward result (new numbering)
+    double* xsol = nullptr;     // n_f x 3 backward result (new numbering)
+    double* ubuf = nullptr;     // forward update rows x 3
+    int* d_perm = nullptr;
+    int* d_iperm = nullptr;
+    int* d_col0 = nullptr;
+    int* d_fsize = nullptr;
+    int64_t* d_rowptr = nullptr;
+    int* d_rows = nullptr;
+    int64_t* d_goff = nullptr;
+    int64_t* d_uoff = nullptr;
+    int64_t* d_asm_ptr = nullptr;
+    int64_t* d_asm_idx = nullptr;
+    int* d_leaf_ptr = nullptr;
+    int* d_leaf_sn = nullptr;
+    int* d_fwd_tasks = nullptr;  // per level: (sn, row block)
+    int* d_bwd_tasks = nullptr;  // per level: (sn, col block)
+    std::vector<int> fwd_task_ptr, bwd_task_ptr;
+    std::vector<double> G_host;
+
+    // reductions
+    double* partials = nullptr;
+    double* scal = nullptr;      // device scalars
+    double* scal_host = nullptr; // pinned host mirror
+
+    // SplitLBFGS host-side state (deps/mcloptlib/include/MCL/SplitLBFGS.hpp:48-62)
+    wrapd_lbfgs_cfg lcfg{6, 100, 1.0, 0};
+    int m_k = 0;
+    int pos2slot[wd::kMaxHist] = {0, 1, 2, 3, 4, 5, 6, 7};   // reference position -> device slot
+    bool grad_old_valid = false;
+    int rot_aware = 1;
+    double last_energy = 0.0;
+    int last_flips = 0;
+
+    ~wrapd_ctx();
+};
+
+namespace wd {
+
+// profiling scope around a launch
+struct ProfScope {
+    wrapd_ctx* c;
+    int idx = -1;
+    ProfScope(wrapd_ctx* ctx, const char* name);
+    ~ProfScope();
+};
+
+inline int blocks_for(long long n, int nt) { return static_cast<int>((n + nt - 1) / nt); }
+
+// launchers (wd_elem.hip)
+void launch_setup(wrapd_ctx* c, const int* d_tv, const double* d_rest, int* d_bad);
+void launch_init_duals(wrapd_ctx* c, int rot_aware);
+void launch_defo(wrapd_ctx* c, int mode, int use_hessian);   // partial slots: 0 value, 1 energy, 2 flips
+void launch_local(wrapd_ctx* c, const wrapd_local_args* a);  // slots: 0 bef,1 aft,2 pe,3 maxratio,4 iters,5 fallback,6 neg,7 accepted
+void launch_local_unaware(wrapd_ctx* c, const wrapd_local_args* a);
+void launch_update_weights(wrapd_ctx* c);
+void launch_max_ratio(wrapd_ctx* c);
+void launch_assemble_A(wrapd_ctx* c);
+// (wd_vec.hip)
+void launch_finalize(wrapd_ctx* c, int nblk, int nslots, unsigned maxmask, int dst);
+void launch_gather_grad(wrapd_ctx* c, double* out, int mode, int new_slot, const int* list_slots,
+                        int nlist);   // mode: 0 plain, 1 first (dg), 2 step (y + Gram)
+void launch_multidot(wrapd_ctx* c, const double* base, double* const* vecs, int nvec);
+void launch_lbfgs_alpha(wrapd_ctx* c, const int* slots, int n);
+void launch_lbfgs_beta(wrapd_ctx* c, const int* slots, int n);
+void launch_axpy_q(wrapd_ctx* c, const double* g, const int* slots, int n);   // qv = g - sum alpha y
+void launch_axpy_r(wrapd_ctx* c, const double* g, const int* slots, int n);   // rv += sum coef s, dir = g.r
+void launch_dot(wrapd_ctx* c, const double* a, const double* b); // partial slot 0
+void launch_xstep(wrapd_ctx* c, double rate, const double* dirv, double* s_out);
+void launch_diffnorm(wrapd_ctx* c, const double* a, const double* b);   // |a - b|^2 -> S_RED
+void launch_copy(wrapd_ctx* c, double* dst, const double* src, long long n);
+void launch_fill(wrapd_ctx* c, double* dst, double v, long long n);
+void launch_armijo_x(wrapd_ctx* c, double alpha, const double* p, double* xbuf); // xbuf = x + alpha p
+int finalize_blocks_n(wrapd_ctx* c);
+// (wd_solve.hip)
+int chol_prepare_device(wrapd_ctx* c);
+int chol_upload_factor(wrapd_ctx* c);
+void launch_solve(wrapd_ctx* c, const double* b, double* x);      // n_f x 3 device arrays
+
+int check_hip(wrapd_ctx* c, hipError_t e, const char* what);
+
+}  // namespace wd

@@ -1,0 +1,647 @@
+// Element kernels: one tetrahedron per lane over structure-of-arrays element state.
+//
+//   k_setup       TetElements ctor (src/TetElements.cpp:90-142)
+//   k_init_duals  initialize_dual_vars + polar_potential_energy (src/TetElements.cpp:145-183, 638-675)
+//   k_defo        update_defo_cache fused with global_obj_value/grad, potential_energy_x,
+//                 inversion_count, candidate weights and add_to_b
+//                 (src/TetElements.cpp:564-596, 678-787, 598-635, 444-468, 391-432, 470-481)
+//   k_local       polar_update (src/TetElements.cpp:256-356) with the candidate weights fused
+//   k_local_ua    update (src/TetElements.cpp:193-254), rotation-unaware local step
+//   k_update_w    update_actual_weights (src/TetElements.cpp:435-442)
+//   k_assemble_A  update_A_coeffs (src/TetElements.cpp:498-510) as a per-entry gather
+#include <type_traits>
+
+#include "wd_ctx.hpp"
+#include "device/wd_math.hpp"
+#include "device/wd_nh.hpp"
+#include "device/wd_reduce.hpp"
+
+namespace wd {
+
+__device__ __forceinline__ double ldD(const double* Di, int n_ep, int r, int k, int e) {
+    return Di[(r * 4 + k) * n_ep + e];
+}
+
+struct MatArgs {
+    const double* mu_e;   // nullptr: homogeneous
+    const double* lam_e;
+    double mu, lam;
+    double beta_min, beta_max;
+};
+
+__device__ __forceinline__ NHParams<double> mat_of(const MatArgs& m, int e) {
+    NHParams<double> p;
+    p.mu = m.mu_e ? m.mu_e[e] : m.mu;
+    p.lambda = m.lam_e ? m.lam_e[e] : m.lam;
+    return p;
+}
+
+// ---------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kElemThreads)
+k_setup(int n_e, int n_ep, const int4* __restrict__ tv, const double* __restrict__ rest,
+        double* __restrict__ Di, double* __restrict__ vol, double* __restrict__ kst,
+        double* __restrict__ wsq, double* __restrict__ sig, MatArgs m, int model, double beta_static,
+        int* __restrict__ bad) {
+    const int e = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (e >= n_ep) return;
+    if (e >= n_e) {   // padding lanes: harmless values
+#pragma unroll
+        for (int i = 0; i < 12; ++i) Di[i * n_ep + e] = 0.0;
+        vol[e] = 1.0; kst[e] = 1.0; wsq[e] = 1.0;
+        sig[e] = sig[n_ep + e] = sig[2 * n_ep + e] = 1.0;
+        return;
+    }
+    const int4 t = tv[e];
+    const int id[4] = {t.x, t.y, t.z, t.w};
+    M3<double> E;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) E(r, c) = rest[3 * id[c + 1] + r] - rest[3 * id[0] + r];
+    M3<double> C;
+    C(0, 0) = E(1, 1) * E(2, 2) - E(1, 2) * E(2, 1);
+    C(0, 1) = E(0, 2) * E(2, 1) - E(0, 1) * E(2, 2);
+    C(0, 2) = E(0, 1) * E(1, 2) - E(0, 2) * E(1, 1);
+    C(1, 0) = E(1, 2) * E(2, 0) - E(1, 0) * E(2, 2);
+    C(1, 1) = E(0, 0) * E(2, 2) - E(0, 2) * E(2, 0);
+    C(1, 2) = E(0, 2) * E(1, 0) - E(0, 0) * E(1, 2);
+    C(2, 0) = E(1, 0) * E(2, 1) - E(1, 1) * E(2, 0);
+    C(2, 1) = E(0, 1) * E(2, 0) - E(0, 0) * E(2, 1);
+    C(2, 2) = E(0, 0) * E(1, 1) - E(0, 1) * E(1, 0);
+    const double det = E(0, 0) * C(0, 0) + E(0, 1) * C(1, 0) + E(0, 2) * C(2, 0);
+    const double id_ = 1.0 / det;
+    M3<double> Ei;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Ei.m[i] = C.m[i] * id_;
+    const double v = det3(E) / 6.0;
+    if (v < 0.0) bad[0] = 1;
+    vol[e] = v;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        Di[(r * 4 + 0) * n_ep + e] = -Ei(0, r) - Ei(1, r) - Ei(2, r);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) Di[(r * 4 + i + 1) * n_ep + e] = Ei(i, r);
+    }
+    const NHParams<double> p = mat_of(m, e);
+    const double k = model == 1 ? 2.0066 * p.mu + 1.0122 * p.lambda : 1.0;
+    kst[e] = k;
+    wsq[e] = beta_static * k * v;
+    sig[e] = sig[n_ep + e] = sig[2 * n_ep + e] = 1.0;
+}
+
+// F from positions; `split` keeps free and fixed corner sums apart as the cached
+// deformation of the reference does (free part + D_fix x_fix, src/TetElements.cpp:575-585).
+// Ft(r,c) = sum_k D(r,k) x_k[c]; F = Ft^T.  Fx receives the fixed part of Ft.
+template <bool SPLIT>
+__device__ __forceinline__ void gather_F(const int4 vi, const double* __restrict__ Di, int n_ep, int e, int n_f,
+                                         const double* __restrict__ xall, M3<double>& F, double d[3][4],
+                                         M3<double>* Fx_t) {
+    const int id[4] = {vi.x, vi.y, vi.z, vi.w};
+    double x[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[k][c] = xall[3 * id[k] + c];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) d[r][k] = ldD(Di, n_ep, r, k, e);
+    M3<double> Ff, Fx;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) { Ff.m[i] = 0.0; Fx.m[i] = 0.0; }
+    bool anyfix = false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const bool fr = !SPLIT || id[k] < n_f;
+        anyfix |= !fr;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (fr) Ff(r, c) += d[r][k] * x[k][c];
+                else Fx(r, c) += d[r][k] * x[k][c];
+            }
+    }
+    if (anyfix) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Ff.m[i] += Fx.m[i];
+    }
+    if (Fx_t) *Fx_t = Fx;
+    F = transpose(Ff);
+}
+
+__device__ __forceinline__ void ld9(const double* a, int n_ep, int e, M3<double>& M) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) M.m[i] = a[i * n_ep + e];
+}
+__device__ __forceinline__ void st9(double* a, int n_ep, int e, const M3<double>& M) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) a[i * n_ep + e] = M.m[i];
+}
+
+// ---------------------------------------------------------------------------------
+template <int RA>
+__global__ void __launch_bounds__(kElemThreads)
+k_init_duals(int n_e, int n_ep, int n_f, const int4* __restrict__ vidx, const double* __restrict__ Di,
+             const double* __restrict__ xall, const double* __restrict__ vol, double* __restrict__ U,
+             double* __restrict__ P, MatArgs m, int model, double* __restrict__ partials) {
+    __shared__ double sh[kElemThreads / 64];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int e = lb * blockDim.x + threadIdx.x;
+    double pe = 0.0;
+    if (e < n_e) {
+        M3<double> F;
+        double d[3][4];
+        gather_F<false>(reinterpret_cast<const int4*>(vidx)[e], Di, n_ep, e, n_f, xall, F, d, nullptr);
+        M3<double> Z;
+        if (RA) {
+            double s[3];
+            M3<double> Uu, V;
+            svd3(F, s, Uu, V);
+            Z = transpose(udvt(V, s, V));
+            if (model == 1) {
+                double s2[3];
+                M3<double> U2, V2;
+                svd3(transpose(Z), s2, U2, V2);
+                pe = vol[e] * nh_psi(s2, mat_of(m, e));
+            } else {
+                M3<double> D = Z;
+                D(0, 0) -= 1.0; D(1, 1) -= 1.0; D(2, 2) -= 1.0;
+                pe = 0.5 * 1.0 * vol[e] * sqnorm9(D);
+            }
+        } else {
+            Z = transpose(F);
+        }
+        M3<double> zero;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) zero.m[i] = 0.0;
+        st9(U, n_ep, e, zero);
+        st9(P, n_ep, e, Z);
+    }
+    const double t = block_sum<kElemThreads>(pe, sh);
+    if (threadIdx.x == 0) partials[lb * kRedStride + 0] = t;
+}
+
+// ---------------------------------------------------------------------------------
+template <int MODE, int MODEL>
+__global__ void __launch_bounds__(kElemThreads)
+k_defo(int n_e, int n_ep, int n_f, const int4* __restrict__ vidx, const double* __restrict__ Di,
+       const double* __restrict__ xall, const double* __restrict__ P, const double* __restrict__ wsq,
+       const double* __restrict__ vol, const double* __restrict__ kst, double* __restrict__ symF,
+       double* __restrict__ gstage, double* __restrict__ cand, MatArgs m, int use_hessian,
+       double* __restrict__ partials) {
+    __shared__ double sh[kElemThreads / 64];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int e = lb * blockDim.x + threadIdx.x;
+    double val = 0.0, en = 0.0, flip = 0.0;
+    if (e < n_e) {
+        M3<double> F, Fx;
+        double d[3][4];
+        const int4 vi = vidx[e];
+        if (MODE & DM_DIRECT) gather_F<false>(vi, Di, n_ep, e, n_f, xall, F, d, &Fx);
+        else gather_F<true>(vi, Di, n_ep, e, n_f, xall, F, d, &Fx);
+        if (MODE & DM_FT_CACHE) st9(symF, n_ep, e, transpose(F));
+        double s[3];
+        M3<double> Uu, V;
+        const bool need_svd = (MODE & (DM_WRITE_SYMF | DM_VALUE | DM_GRAD | DM_ENERGY | DM_CAND)) &&
+                              !((MODE & DM_FT_CACHE) && !(MODE & (DM_ENERGY | DM_CAND)));
+        if (need_svd) svd3(F, s, Uu, V);
+        M3<double> S;
+        if (MODE & (DM_WRITE_SYMF | DM_VALUE | DM_GRAD)) S = udvt(V, s, V);
+        if ((MODE & DM_WRITE_SYMF) && !(MODE & DM_FT_CACHE)) st9(symF, n_ep, e, S);
+        const double w2 = (MODE & (DM_VALUE | DM_GRAD | DM_BVEC)) ? wsq[e] : 0.0;
+        M3<double> Pm;
+        if (MODE & (DM_VALUE | DM_GRAD | DM_BVEC)) ld9(P, n_ep, e, Pm);
+        if (MODE & (DM_VALUE | DM_GRAD)) {
+            // P here is (Z - U)^T in the reference's notation: P_ref = (Z - U)^T
+            const M3<double> Pt = transpose(Pm);
+            if (MODE & DM_VALUE) {
+                M3<double> D;
+#pragma unroll
+                for (int i = 0; i < 9; ++i) D.m[i] = S.m[i] - Pt.m[i];
+                val = (0.5 * w2) * sqnorm9(D);
+            }
+            if (MODE & DM_GRAD) {
+                const M3<double> Q = matmul_tn(V, matmul(Pt, V));
+                M3<double> K;
+                K(0, 0) = Q(0, 0) / s[0];
+                K(1, 1) = Q(1, 1) / s[1];
+                K(2, 2) = Q(2, 2) / s[2];
+                K(0, 1) = K(1, 0) = (Q(0, 1) + Q(1, 0)) / (s[0] + s[1]);
+                K(0, 2) = K(2, 0) = (Q(0, 2) + Q(2, 0)) / (s[0] + s[2]);
+                K(1, 2) = K(2, 1) = (Q(1, 2) + Q(2, 1)) / (s[1] + s[2]);
+                M3<double> US = Uu;
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) US(r, c) *= s[c];
+                const M3<double> T = matmul(matmul(US, K), transpose(V));
+                M3<double> Mm;
+#pragma unroll
+                for (int i = 0; i < 9; ++i) Mm.m[i] = F.m[i] - T.m[i];
+                double* g = gstage + 12ll * e;
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        g[3 * k + c] = d[0][k] * (w2 * Mm(c, 0)) + d[1][k] * (w2 * Mm(c, 1)) + d[2][k] * (w2 * Mm(c, 2));
+            }
+        }
+        if (MODE & DM_BVEC) {
+            // add_to_b: w^2 D^T (Z - U - D_fix x_fix), the cache here holds Ft so P = Z - U
+            double* g = gstage + 12ll * e;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) acc += (w2 * d[r][k]) * (Pm(r, c) - Fx(r, c));
+                    g[3 * k + c] = acc;
+                }
+        }
+        if (MODE & DM_ENERGY) {
+            if (MODEL == 1) {
+                en = vol[e] * nh_psi(s, mat_of(m, e));
+            } else {
+                const M3<double> R = matmul(Uu, transpose(V));
+                M3<double> D;
+#pragma unroll
+                for (int i = 0; i < 9; ++i) D.m[i] = F.m[i] - R.m[i];
+                en = 0.5 * 1.0 * vol[e] * sqnorm9(D);
+            }
+            flip = (s[0] < 0.0 || s[1] < 0.0 || s[2] < 0.0) ? 1.0 : 0.0;
+        }
+        if (MODE & DM_CAND) {
+            const NHParams<double> p = mat_of(m, e);
+            const double k = kst[e];
+            double cs = k;
+            if (use_hessian) cs = nh_candidate_stiffness(s, p, k * m.beta_min, k * m.beta_max);
+            cand[e] = vol[e] * cs;
+        }
+    }
+    if (MODE & DM_VALUE) {
+        const double t = block_sum<kElemThreads>(val, sh);
+        if (threadIdx.x == 0) partials[lb * kRedStride + 0] = t;
+    }
+    if (MODE & DM_ENERGY) {
+        const double t1 = block_sum<kElemThreads>(en, sh);
+        const double t2 = block_sum<kElemThreads>(flip, sh);
+        if (threadIdx.x == 0) {
+            partials[lb * kRedStride + 1] = t1;
+            partials[lb * kRedStride + 2] = t2;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Local step.  T is the precision of the SVD and the prox; the ADMM state stays FP64.
+template <typename T>
+__device__ __forceinline__ M3<T> cvt(const M3<double>& a) {
+    M3<T> b;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) b.m[i] = static_cast<T>(a.m[i]);
+    return b;
+}
+
+// Stretch prox of one element in precision T.  Returns the new Z (FP64) and the
+// prox solution; iters gets the solver iterations.  ok = false when the result is unusable.
+template <typename T, int PROX>
+__device__ __forceinline__ void elem_prox(const M3<double>& Sin, const double sl[3], NHParams<double> pd, double wov,
+                                          int prox_iters, double rtol, M3<double>& Znew, double x_out[3],
+                                          double& iters, bool& ok, double& shat_min, double& evals_out) {
+    const M3<T> St = cvt<T>(Sin);
+    T sh[3];
+    M3<T> Ut, Vt;
+    svd3(St, sh, Ut, Vt);
+    shat_min = static_cast<double>(sh[2]);
+    ProxProblem<T> P;
+    P.p.mu = static_cast<T>(pd.mu);
+    P.p.lambda = static_cast<T>(pd.lambda);
+    P.x0[0] = sh[0]; P.x0[1] = sh[1]; P.x0[2] = sh[2];
+    P.wov = static_cast<T>(wov);
+    T x[3] = {static_cast<T>(sl[0]), static_cast<T>(sl[1]), static_cast<T>(sl[2])};
+    int it;
+    if (PROX == WRAPD_PROX_LBFGS) {
+        int evals = 0;
+        it = prox_lbfgs(P, x, prox_iters, evals);
+        evals_out += static_cast<double>(evals);
+    } else {
+        T gn = T(0);
+        it = prox_newton(P, x, prox_iters, static_cast<T>(rtol), gn);
+        evals_out += static_cast<double>(it < 0 ? 0 : it);
+    }
+    iters = it < 0 ? 0.0 : static_cast<double>(it);
+    const M3<T> Zt = udvt(Vt, x, Ut);
+    ok = true;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        Znew.m[i] = static_cast<double>(Zt.m[i]);
+        ok &= isfinite(Znew.m[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) x_out[i] = static_cast<double>(x[i]);
+}
+
+template <int MODEL, int PREC, int PROX>
+__global__ void __launch_bounds__(kLocalThreads)
+k_local(int n_e, int n_ep, const double* __restrict__ symF, double* __restrict__ U, double* __restrict__ P,
+        const double* __restrict__ wsq, const double* __restrict__ vol, const double* __restrict__ kst,
+        double* __restrict__ cand, double* __restrict__ sig, MatArgs m, int prox_iters, double rtol,
+        double fallback_sigma, int update_cand, int use_hessian, double* __restrict__ partials) {
+    __shared__ double sh[kLocalThreads / 64];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int e = lb * blockDim.x + threadIdx.x;
+    double bef = 0.0, aft = 0.0, pe = 0.0, ratio = 0.0, iters = 0.0, fallback = 0.0, neg = 0.0, evals = 0.0;
+    if (e < n_e) {
+        M3<double> S, Uo, Po;
+        ld9(symF, n_ep, e, S);
+        ld9(U, n_ep, e, Uo);
+        ld9(P, n_ep, e, Po);
+        const double w2 = wsq[e], v = vol[e];
+        {   // bef = 1/2 w^2 |symF - (Z - U)|^2 with P = Z - U stored transposed-free
+            M3<double> D;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) D.m[i] = S.m[i] - Po.m[i];
+            bef = 0.5 * w2 * sqnorm9(D);
+        }
+        M3<double> Zn;
+        double x[3] = {1.0, 1.0, 1.0};
+        if (MODEL == 1) {
+            M3<double> Sin;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Sin.m[i] = S.m[i] + Uo.m[i];
+            Sin = transpose(Sin);
+            const NHParams<double> pd = mat_of(m, e);
+            const double sl[3] = {sig[e], sig[n_ep + e], sig[2 * n_ep + e]};
+            bool ok;
+            double smin;
+            elem_prox<typename std::conditional<PREC == WRAPD_FP32, float, double>::type, PROX>(
+                Sin, sl, pd, w2 / v, prox_iters, rtol, Zn, x, iters, ok, smin, evals);
+            if (PREC == WRAPD_FP32) {
+                if (!ok || smin < fallback_sigma || x[0] < 0.0 || x[1] < 0.0 || x[2] < 0.0) {
+                    fallback = 1.0;
+                    elem_prox<double, PROX>(Sin, sl, pd, w2 / v, prox_iters, PROX == WRAPD_PROX_LBFGS ? rtol : 1e-12,
+                                            Zn, x, iters, ok, smin, evals);
+                }
+            }
+            pe = v * nh_psi(x, pd);
+            neg = (x[0] < 0.0 || x[1] < 0.0 || x[2] < 0.0) ? 1.0 : 0.0;
+            sig[e] = x[0];
+            sig[n_ep + e] = x[1];
+            sig[2 * n_ep + e] = x[2];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Zn.m[i] = S.m[i] + Uo.m[i];
+            Zn.m[0] += 1.0; Zn.m[4] += 1.0; Zn.m[8] += 1.0;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Zn.m[i] *= 0.5;
+            M3<double> D = S;
+            D.m[0] -= 1.0; D.m[4] -= 1.0; D.m[8] -= 1.0;
+            pe = 0.5 * 1.0 * v * sqnorm9(D);
+        }
+        M3<double> Un, Pn;
+        {
+            M3<double> D;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) D.m[i] = S.m[i] - (Zn.m[i] - Uo.m[i]);
+            aft = 0.5 * w2 * sqnorm9(D);
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            Un.m[i] = Uo.m[i] + (S.m[i] - Zn.m[i]);
+            Pn.m[i] = Zn.m[i] - Un.m[i];
+        }
+        st9(U, n_ep, e, Un);
+        st9(P, n_ep, e, Pn);
+        if (update_cand) {
+            const double k = kst[e];
+            double cs = k;
+            if (use_hessian) cs = nh_candidate_stiffness(x, mat_of(m, e), k * m.beta_min, k * m.beta_max);
+            const double cw = v * cs;
+            cand[e] = cw;
+            const double a = w2 / cw, b = 1.0 / a;
+            ratio = a > b ? a : b;
+        }
+    }
+    const double r0 = block_sum<kLocalThreads>(bef, sh);
+    const double r1 = block_sum<kLocalThreads>(aft, sh);
+    const double r2 = block_sum<kLocalThreads>(pe, sh);
+    const double r3 = block_max<kLocalThreads>(ratio, sh);
+    const double r4 = block_sum<kLocalThreads>(iters, sh);
+    const double r5 = block_sum<kLocalThreads>(fallback, sh);
+    const double r6 = block_sum<kLocalThreads>(neg, sh);
+    const double r8 = block_sum<kLocalThreads>(evals, sh);
+    const double r9 = block_max<kLocalThreads>(evals, sh);
+    if (threadIdx.x == 0) {
+        double* o = partials + lb * kRedStride;
+        o[0] = r0; o[1] = r1; o[2] = r2; o[3] = r3; o[4] = r4; o[5] = r5; o[6] = r6; o[7] = 0.0;
+        o[8] = r8; o[9] = r9;
+    }
+}
+
+// Rotation-unaware local step (src/TetElements.cpp:193-254); the cache holds Ft = F^T.
+template <int MODEL>
+__global__ void __launch_bounds__(kLocalThreads)
+k_local_ua(int n_e, int n_ep, const double* __restrict__ Ftc, double* __restrict__ U, double* __restrict__ P,
+           const double* __restrict__ wsq, const double* __restrict__ vol, const double* __restrict__ kst,
+           double* __restrict__ cand, double* __restrict__ sig, MatArgs m, int prox_iters, int update_cand,
+           int use_hessian) {
+    const int e = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (e >= n_e) return;
+    M3<double> Dix, Uo;
+    ld9(Ftc, n_ep, e, Dix);
+    ld9(U, n_ep, e, Uo);
+    M3<double> Ft;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Ft.m[i] = Dix.m[i] + Uo.m[i];
+    const M3<double> F = transpose(Ft);
+    double s[3];
+    M3<double> Us, Vs;
+    svd3(F, s, Us, Vs);
+    double x[3] = {1.0, 1.0, 1.0};
+    if (MODEL == 1) {
+        const NHParams<double> pd = mat_of(m, e);
+        ProxProblem<double> Pp;
+        Pp.p = pd;
+        Pp.x0[0] = s[0]; Pp.x0[1] = s[1]; Pp.x0[2] = s[2];
+        Pp.wov = wsq[e] / vol[e];
+        x[0] = sig[e]; x[1] = sig[n_ep + e]; x[2] = sig[2 * n_ep + e];
+        int evals = 0;
+        prox_lbfgs(Pp, x, prox_iters, evals);
+        sig[e] = x[0]; sig[n_ep + e] = x[1]; sig[2 * n_ep + e] = x[2];
+        Ft = udvt(Vs, x, Us);
+    } else {
+        const M3<double> R = matmul(Us, transpose(Vs));
+        M3<double> Fr;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Fr.m[i] = 0.5 * (F.m[i] + R.m[i]);
+        Ft = transpose(Fr);
+    }
+    M3<double> Un, Pn;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        Un.m[i] = Uo.m[i] + (Dix.m[i] - Ft.m[i]);
+        Pn.m[i] = Ft.m[i] - Un.m[i];
+    }
+    st9(U, n_ep, e, Un);
+    st9(P, n_ep, e, Pn);
+    if (update_cand) {
+        const double k = kst[e];
+        double cs = k;
+        if (use_hessian) cs = nh_candidate_stiffness(x, mat_of(m, e), k * m.beta_min, k * m.beta_max);
+        cand[e] = vol[e] * cs;
+    }
+}
+
+__global__ void __launch_bounds__(kElemThreads)
+k_update_w(int n_e, int n_ep, double* __restrict__ U, double* __restrict__ P, double* __restrict__ wsq,
+           const double* __restrict__ cand) {
+    const int e = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (e >= n_e) return;
+    const double r = wsq[e] / cand[e];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const double u = U[i * n_ep + e];
+        const double z = P[i * n_ep + e] + u;
+        const double un = r * u;
+        U[i * n_ep + e] = un;
+        P[i * n_ep + e] = z - un;
+    }
+    wsq[e] = cand[e];
+}
+
+__global__ void __launch_bounds__(kElemThreads)
+k_max_ratio(int n_e, const double* __restrict__ wsq, const double* __restrict__ cand, double* __restrict__ partials) {
+    __shared__ double sh[kElemThreads / 64];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int e = lb * blockDim.x + threadIdx.x;
+    double r = 0.0;
+    if (e < n_e) {
+        const double a = wsq[e] / cand[e], b = 1.0 / a;
+        r = a > b ? a : b;
+    }
+    const double t = block_max<kElemThreads>(r, sh);
+    if (threadIdx.x == 0) partials[lb * kRedStride + 0] = t;
+}
+
+// A(slot) = sum over (element, a, b) contributions of w_e^2 d_a . d_b, in element order
+__global__ void __launch_bounds__(kElemThreads)
+k_assemble_A(int nnz, int n_ep, const int* __restrict__ cptr, const int* __restrict__ cidx,
+             const double* __restrict__ Di, const double* __restrict__ wsq, double* __restrict__ Ax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnz) return;
+    double acc = 0.0;
+    for (int q = cptr[i]; q < cptr[i + 1]; ++q) {
+        const int c = cidx[q];
+        const int e = c >> 4, a = (c >> 2) & 3, b = c & 3;
+        const double dd = ldD(Di, n_ep, 0, a, e) * ldD(Di, n_ep, 0, b, e) +
+                          ldD(Di, n_ep, 1, a, e) * ldD(Di, n_ep, 1, b, e) +
+                          ldD(Di, n_ep, 2, a, e) * ldD(Di, n_ep, 2, b, e);
+        acc += wsq[e] * dd;
+    }
+    Ax[i] = acc;
+}
+
+// ---------------------------------------------------------------------------------
+static MatArgs mat_args(wrapd_ctx* c) {
+    MatArgs m;
+    m.mu_e = c->mu_e;
+    m.lam_e = c->lam_e;
+    m.mu = c->mat.mu;
+    m.lam = c->mat.lambda;
+    m.beta_min = c->mat.beta_min;
+    m.beta_max = c->mat.beta_max;
+    return m;
+}
+
+void launch_setup(wrapd_ctx* c, const int* d_tv, const double* d_rest, int* d_bad) {
+    ProfScope ps(c, "setup");
+    hipLaunchKernelGGL(k_setup, dim3(blocks_for(c->n_ep, kElemThreads)), dim3(kElemThreads), 0, c->stream, c->n_e,
+                       c->n_ep, reinterpret_cast<const int4*>(d_tv), d_rest, c->Di, c->vol, c->kst, c->wsq, c->sig,
+                       mat_args(c), c->mat.model, c->mat.beta_static, d_bad);
+}
+
+void launch_init_duals(wrapd_ctx* c, int ra) {
+    ProfScope ps(c, "init_duals");
+    const int nb = blocks_for(c->n_e, kElemThreads);
+    auto k = ra ? k_init_duals<1> : k_init_duals<0>;
+    hipLaunchKernelGGL(k, dim3(nb), dim3(kElemThreads), 0, c->stream, c->n_e, c->n_ep, c->n_f,
+                       reinterpret_cast<const int4*>(c->vidx), c->Di, c->xall, c->vol, c->U, c->P, mat_args(c),
+                       c->mat.model, c->partials);
+}
+
+template <int MODE>
+static void defo_mode(wrapd_ctx* c, int use_hessian) {
+    const int nb = blocks_for(c->n_e, kElemThreads);
+    auto k = c->mat.model == 1 ? k_defo<MODE, 1> : k_defo<MODE, 0>;
+    hipLaunchKernelGGL(k, dim3(nb), dim3(kElemThreads), 0, c->stream, c->n_e, c->n_ep, c->n_f,
+                       reinterpret_cast<const int4*>(c->vidx), c->Di, c->xall, c->P, c->wsq, c->vol, c->kst,
+                       c->symF, c->gstage, c->cand, mat_args(c), use_hessian, c->partials);
+}
+
+void launch_defo(wrapd_ctx* c, int mode, int use_hessian) {
+    ProfScope ps(c, (mode & DM_GRAD) ? "defo_grad" : ((mode & DM_BVEC) ? "defo_b" : "defo"));
+    switch (mode) {
+        case DM_GRAD | DM_VALUE: defo_mode<DM_GRAD | DM_VALUE>(c, use_hessian); break;
+        case DM_GRAD | DM_VALUE | DM_WRITE_SYMF | DM_ENERGY:
+            defo_mode<DM_GRAD | DM_VALUE | DM_WRITE_SYMF | DM_ENERGY>(c, use_hessian); break;
+        case DM_VALUE | DM_WRITE_SYMF | DM_ENERGY: defo_mode<DM_VALUE | DM_WRITE_SYMF | DM_ENERGY>(c, use_hessian); break;
+        case DM_VALUE: defo_mode<DM_VALUE>(c, use_hessian); break;
+        case DM_WRITE_SYMF | DM_ENERGY: defo_mode<DM_WRITE_SYMF | DM_ENERGY>(c, use_hessian); break;
+        case DM_CAND: defo_mode<DM_CAND>(c, use_hessian); break;
+        case DM_CAND | DM_DIRECT: defo_mode<DM_CAND | DM_DIRECT>(c, use_hessian); break;
+        case DM_FT_CACHE | DM_ENERGY: defo_mode<DM_FT_CACHE | DM_ENERGY>(c, use_hessian); break;
+        case DM_FT_CACHE | DM_CAND: defo_mode<DM_FT_CACHE | DM_CAND>(c, use_hessian); break;
+        case DM_BVEC: defo_mode<DM_BVEC>(c, use_hessian); break;
+        default: break;
+    }
+}
+
+template <int MODEL, int PREC, int PROX>
+static void local_inst(wrapd_ctx* c, const wrapd_local_args* a) {
+    const int nb = blocks_for(c->n_e, kLocalThreads);
+    hipLaunchKernelGGL((k_local<MODEL, PREC, PROX>), dim3(nb), dim3(kLocalThreads), 0, c->stream, c->n_e, c->n_ep,
+                       c->symF, c->U, c->P, c->wsq, c->vol, c->kst, c->cand, c->sig, mat_args(c), a->prox_iters,
+                       a->newton_rtol, a->fallback_sigma, a->update_cand, a->cand_use_hessian, c->partials);
+}
+
+void launch_local(wrapd_ctx* c, const wrapd_local_args* a) {
+    ProfScope ps(c, "local_step");
+    if (c->mat.model != 1) { local_inst<0, WRAPD_FP64, WRAPD_PROX_LBFGS>(c, a); return; }
+    const bool f32 = a->precision == WRAPD_FP32;
+    const bool newton = a->prox_mode != WRAPD_PROX_LBFGS;
+    if (!f32 && !newton) local_inst<1, WRAPD_FP64, WRAPD_PROX_LBFGS>(c, a);
+    else if (!f32 && newton) local_inst<1, WRAPD_FP64, WRAPD_PROX_NEWTON>(c, a);
+    else if (f32 && newton) local_inst<1, WRAPD_FP32, WRAPD_PROX_NEWTON>(c, a);
+    else local_inst<1, WRAPD_FP32, WRAPD_PROX_LBFGS>(c, a);
+}
+
+void launch_local_unaware(wrapd_ctx* c, const wrapd_local_args* a) {
+    ProfScope ps(c, "local_step_ua");
+    const int nb = blocks_for(c->n_e, kLocalThreads);
+    auto k = c->mat.model == 1 ? k_local_ua<1> : k_local_ua<0>;
+    hipLaunchKernelGGL(k, dim3(nb), dim3(kLocalThreads), 0, c->stream, c->n_e, c->n_ep, c->symF, c->U, c->P, c->wsq,
+                       c->vol, c->kst, c->cand, c->sig, mat_args(c), a->prox_iters, a->update_cand,
+                       a->cand_use_hessian);
+}
+
+void launch_update_weights(wrapd_ctx* c) {
+    ProfScope ps(c, "update_weights");
+    hipLaunchKernelGGL(k_update_w, dim3(blocks_for(c->n_e, kElemThreads)), dim3(kElemThreads), 0, c->stream, c->n_e,
+                       c->n_ep, c->U, c->P, c->wsq, c->cand);
+}
+
+void launch_max_ratio(wrapd_ctx* c) {
+    ProfScope ps(c, "max_ratio");
+    hipLaunchKernelGGL(k_max_ratio, dim3(blocks_for(c->n_e, kElemThreads)), dim3(kElemThreads), 0, c->stream, c->n_e,
+                       c->wsq, c->cand, c->partials);
+}
+
+void launch_assemble_A(wrapd_ctx* c) {
+    ProfScope ps(c, "assemble_A");
+    hipLaunchKernelGGL(k_assemble_A, dim3(blocks_for(c->nnzA, kElemThreads)), dim3(kElemThreads), 0, c->stream,
+                       c->nnzA, c->n_ep, c->a_cptr, c->a_cidx, c->Di, c->wsq, c->Ax);
+}
+
+}  // namespace wd

@@ -1,0 +1,341 @@
+// Supernodal triangular solves with three right-hand sides (the x, y, z columns of
+// SplitLBFGS's H0 application, deps/mcloptlib/include/MCL/SplitLBFGS.hpp:159-171).
+//
+// Each supernode s stores G_s = [L11^{-1}; -L21 L11^{-1}] (f_s x ns_s, column-major), so
+//   forward  (L y = b):   [y_J ; u_s] = [0 ; f_R] + G_s c_s,  c_s = b_J + child updates
+//   backward (L^T x = y): x_J = G_s^T [y_J ; x_R]
+// Both are one dense product per supernode: rows (forward) or columns (backward) are
+// independent, so a wide supernode is a single step of the schedule.  Child updates are
+// gathered by the parent from precomputed lists (deterministic, no atomics).
+//
+// Schedule: small supernodes are grouped into leaf subtrees, each walked by one
+// workgroup in postorder; the remaining supernodes run level by level with
+// (supernode, row block) tasks forward and (supernode, column block) tasks backward.
+#include "wd_ctx.hpp"
+#include "device/wd_reduce.hpp"
+
+namespace wd {
+
+constexpr int kSolveThreads = 256;
+constexpr int kBwdCols = 16;          // columns per backward task (4 per wave)
+constexpr int kBwdChunk = 2048;       // rows of [y_J; x_R] staged in LDS at a time
+constexpr int kNsMax = 1024;          // supernode width limit (symbolic ns_max)
+constexpr int kSmallFront = 256;      // leaf-subtree front limit (symbolic small_front)
+
+struct SolveMaps {
+    const int* col0;
+    const int* fsize;
+    const int64_t* rowptr;
+    const int* rows;
+    const int64_t* goff;
+    const int64_t* uoff;
+    const int64_t* asm_ptr;
+    const int64_t* asm_idx;
+    const int* perm;
+};
+
+__device__ __forceinline__ void assemble_row(const SolveMaps& M, int64_t prow, const double* __restrict__ ubuf,
+                                             double a[3]) {
+    for (int64_t q = M.asm_ptr[prow]; q < M.asm_ptr[prow + 1]; ++q) {
+        const double* u = ubuf + 3 * M.asm_idx[q];
+        a[0] += u[0];
+        a[1] += u[1];
+        a[2] += u[2];
+    }
+}
+
+__global__ void __launch_bounds__(kSolveThreads)
+k_fwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, SolveMaps M,
+           const double* __restrict__ b, const double* __restrict__ G, double* __restrict__ ysol,
+           double* __restrict__ ubuf) {
+    __shared__ double cv[kSmallFront * 3];
+    const int t = blockIdx.x;
+    for (int q = leaf_ptr[t]; q < leaf_ptr[t + 1]; ++q) {
+        const int s = leaf_sn[q];
+        const int c0 = M.col0[s];
+        const int ns = M.col0[s + 1] - c0;
+        const int f = M.fsize[s];
+        const int64_t rp = M.rowptr[s];
+        for (int r = threadIdx.x; r < f; r += kSolveThreads) {
+            double a[3] = {0.0, 0.0, 0.0};
+            if (r < ns) {
+                const int old = M.perm[c0 + r];
+                a[0] = b[3 * old + 0];
+                a[1] = b[3 * old + 1];
+                a[2] = b[3 * old + 2];
+            }
+            assemble_row(M, rp + r, ubuf, a);
+            cv[3 * r + 0] = a[0];
+            cv[3 * r + 1] = a[1];
+            cv[3 * r + 2] = a[2];
+        }
+        __syncthreads();
+        const double* Gs = G + M.goff[s];
+        for (int r = threadIdx.x; r < f; r += kSolveThreads) {
+            double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+            const int jend = r < ns ? r + 1 : ns;   // L11^{-1} is lower triangular
+            for (int j = 0; j < jend; ++j) {
+                const double g = Gs[static_cast<int64_t>(j) * f + r];
+                t0 += g * cv[3 * j + 0];
+                t1 += g * cv[3 * j + 1];
+                t2 += g * cv[3 * j + 2];
+            }
+            if (r < ns) {
+                double* y = ysol + 3ll * (c0 + r);
+                y[0] = t0; y[1] = t1; y[2] = t2;
+            } else {
+                double* u = ubuf + 3 * (M.uoff[s] + (r - ns));
+                u[0] = cv[3 * r + 0] + t0;
+                u[1] = cv[3 * r + 1] + t1;
+                u[2] = cv[3 * r + 2] + t2;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kSolveThreads)
+k_fwd_level(const int2* __restrict__ tasks, SolveMaps M, const double* __restrict__ b, const double* __restrict__ G,
+            double* __restrict__ ysol, double* __restrict__ ubuf) {
+    __shared__ double cv[kNsMax * 3];
+    const int2 task = tasks[blockIdx.x];
+    const int s = task.x, rb = task.y;
+    const int c0 = M.col0[s];
+    const int ns = M.col0[s + 1] - c0;
+    const int f = M.fsize[s];
+    const int64_t rp = M.rowptr[s];
+    for (int j = threadIdx.x; j < ns; j += kSolveThreads) {
+        const int old = M.perm[c0 + j];
+        double a[3] = {b[3 * old + 0], b[3 * old + 1], b[3 * old + 2]};
+        assemble_row(M, rp + j, ubuf, a);
+        cv[3 * j + 0] = a[0];
+        cv[3 * j + 1] = a[1];
+        cv[3 * j + 2] = a[2];
+    }
+    __syncthreads();
+    const int r = rb * kSolveThreads + threadIdx.x;
+    if (r >= f) return;
+    const double* Gs = G + M.goff[s];
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    const int jend = r < ns ? r + 1 : ns;
+    for (int j = 0; j < jend; ++j) {
+        const double g = Gs[static_cast<int64_t>(j) * f + r];
+        t0 += g * cv[3 * j + 0];
+        t1 += g * cv[3 * j + 1];
+        t2 += g * cv[3 * j + 2];
+    }
+    if (r < ns) {
+        double* y = ysol + 3ll * (c0 + r);
+        y[0] = t0; y[1] = t1; y[2] = t2;
+    } else {
+        double a[3] = {0.0, 0.0, 0.0};
+        assemble_row(M, rp + r, ubuf, a);
+        double* u = ubuf + 3 * (M.uoff[s] + (r - ns));
+        u[0] = a[0] + t0;
+        u[1] = a[1] + t1;
+        u[2] = a[2] + t2;
+    }
+}
+
+// backward, one supernode column block per workgroup: x_J = G^T [y_J; x_R]
+__global__ void __launch_bounds__(kSolveThreads)
+k_bwd_level(const int2* __restrict__ tasks, SolveMaps M, const double* __restrict__ G,
+            const double* __restrict__ ysol, double* __restrict__ xsol) {
+    __shared__ double vv[kBwdChunk * 3];
+    const int2 task = tasks[blockIdx.x];
+    const int s = task.x, cb = task.y;
+    const int c0 = M.col0[s];
+    const int ns = M.col0[s + 1] - c0;
+    const int f = M.fsize[s];
+    const int64_t rp = M.rowptr[s];
+    const double* Gs = G + M.goff[s];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double acc[4][3];
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) acc[jj][0] = acc[jj][1] = acc[jj][2] = 0.0;
+    for (int r0 = 0; r0 < f; r0 += kBwdChunk) {
+        const int rc = min(kBwdChunk, f - r0);
+        for (int r = threadIdx.x; r < rc; r += kSolveThreads) {
+            const int row = r0 + r;
+            const double* src = row < ns ? ysol + 3ll * (c0 + row) : xsol + 3ll * M.rows[rp + row];
+            vv[3 * r + 0] = src[0];
+            vv[3 * r + 1] = src[1];
+            vv[3 * r + 2] = src[2];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            const int j = cb * kBwdCols + w * 4 + jj;
+            if (j < ns) {
+                const double* gc = Gs + static_cast<int64_t>(j) * f + r0;
+                const int rstart = j > r0 ? j - r0 : 0;   // rows above the diagonal are zero
+                for (int r = rstart + lane; r < rc; r += 64) {
+                    const double g = gc[r];
+                    acc[jj][0] += g * vv[3 * r + 0];
+                    acc[jj][1] += g * vv[3 * r + 1];
+                    acc[jj][2] += g * vv[3 * r + 2];
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+        const int j = cb * kBwdCols + w * 4 + jj;
+        const double a0 = wave_sum(acc[jj][0]);
+        const double a1 = wave_sum(acc[jj][1]);
+        const double a2 = wave_sum(acc[jj][2]);
+        if (lane == 0 && j < ns) {
+            double* x = xsol + 3ll * (c0 + j);
+            x[0] = a0; x[1] = a1; x[2] = a2;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kSolveThreads)
+k_bwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, SolveMaps M,
+           const double* __restrict__ G, const double* __restrict__ ysol, double* __restrict__ xsol) {
+    __shared__ double vv[kSmallFront * 3];
+    const int t = blockIdx.x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int q = leaf_ptr[t + 1] - 1; q >= leaf_ptr[t]; --q) {
+        const int s = leaf_sn[q];
+        const int c0 = M.col0[s];
+        const int ns = M.col0[s + 1] - c0;
+        const int f = M.fsize[s];
+        const int64_t rp = M.rowptr[s];
+        for (int r = threadIdx.x; r < f; r += kSolveThreads) {
+            const double* src = r < ns ? ysol + 3ll * (c0 + r) : xsol + 3ll * M.rows[rp + r];
+            vv[3 * r + 0] = src[0];
+            vv[3 * r + 1] = src[1];
+            vv[3 * r + 2] = src[2];
+        }
+        __syncthreads();
+        const double* Gs = G + M.goff[s];
+        for (int j = w; j < ns; j += kSolveThreads / 64) {
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+            const double* gc = Gs + static_cast<int64_t>(j) * f;
+            for (int r = j + lane; r < f; r += 64) {
+                const double g = gc[r];
+                a0 += g * vv[3 * r + 0];
+                a1 += g * vv[3 * r + 1];
+                a2 += g * vv[3 * r + 2];
+            }
+            a0 = wave_sum(a0);
+            a1 = wave_sum(a1);
+            a2 = wave_sum(a2);
+            if (lane == 0) {
+                double* x = xsol + 3ll * (c0 + j);
+                x[0] = a0; x[1] = a1; x[2] = a2;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kSolveThreads)
+k_unpermute(int n, const int* __restrict__ iperm, const double* __restrict__ xsol, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* s = xsol + 3ll * iperm[i];
+    out[3 * i + 0] = s[0];
+    out[3 * i + 1] = s[1];
+    out[3 * i + 2] = s[2];
+}
+
+// ---------------------------------------------------------------------------------
+template <typename T>
+static int up(wrapd_ctx* c, T*& dptr, const std::vector<T>& h) {
+    if (dptr) { (void)hipFree(dptr); dptr = nullptr; }
+    size_t bytes = std::max<size_t>(sizeof(T), h.size() * sizeof(T));
+    if (check_hip(c, hipMalloc(&dptr, bytes), "hipMalloc(chol)")) return 1;
+    if (!h.empty() && check_hip(c, hipMemcpy(dptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy(chol)"))
+        return 1;
+    return 0;
+}
+
+int chol_prepare_device(wrapd_ctx* c) {
+    const wrapd::CholSymbolic& S = c->sym;
+    if (S.max_ns > kNsMax) { c->last_error = "supernode wider than the solve kernel supports"; return WRAPD_ERR_ARG; }
+    int rc = 0;
+    rc |= up(c, c->d_perm, S.perm);
+    rc |= up(c, c->d_iperm, S.iperm);
+    rc |= up(c, c->d_col0, S.col0);
+    rc |= up(c, c->d_fsize, S.fsize);
+    rc |= up(c, c->d_rowptr, S.rowptr);
+    rc |= up(c, c->d_rows, S.rows);
+    rc |= up(c, c->d_goff, S.goff);
+    rc |= up(c, c->d_uoff, S.uoff);
+    rc |= up(c, c->d_asm_ptr, S.asm_ptr);
+    rc |= up(c, c->d_asm_idx, S.asm_idx);
+    rc |= up(c, c->d_leaf_ptr, S.leaf_ptr);
+    rc |= up(c, c->d_leaf_sn, S.leaf_sn);
+    // level task lists
+    std::vector<int> ft, bt;
+    c->fwd_task_ptr.assign(1, 0);
+    c->bwd_task_ptr.assign(1, 0);
+    const int nlev = static_cast<int>(S.level_ptr.size()) - 1;
+    for (int l = 0; l < nlev; ++l) {
+        for (int q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+            const int s = S.level_sn[q];
+            const int ns = S.col0[s + 1] - S.col0[s];
+            for (int rb = 0; rb * kSolveThreads < S.fsize[s]; ++rb) { ft.push_back(s); ft.push_back(rb); }
+            for (int cb = 0; cb * kBwdCols < ns; ++cb) { bt.push_back(s); bt.push_back(cb); }
+        }
+        c->fwd_task_ptr.push_back(static_cast<int>(ft.size() / 2));
+        c->bwd_task_ptr.push_back(static_cast<int>(bt.size() / 2));
+    }
+    rc |= up(c, c->d_fwd_tasks, ft);
+    rc |= up(c, c->d_bwd_tasks, bt);
+    std::vector<double> z3(3ull * std::max(1, c->n_f), 0.0);
+    rc |= up(c, c->ysol, z3);
+    rc |= up(c, c->xsol, z3);
+    std::vector<double> zu(3ull * std::max<int64_t>(1, S.uoff[S.nsuper]), 0.0);
+    rc |= up(c, c->ubuf, zu);
+    if (c->G) { (void)hipFree(c->G); c->G = nullptr; }
+    rc |= check_hip(c, hipMalloc(&c->G, std::max<size_t>(8, S.goff[S.nsuper] * sizeof(double))), "hipMalloc(G)");
+    return rc ? WRAPD_ERR_HIP : WRAPD_OK;
+}
+
+int chol_upload_factor(wrapd_ctx* c) {
+    return check_hip(c, hipMemcpyAsync(c->G, c->G_host.data(), c->G_host.size() * sizeof(double),
+                                       hipMemcpyHostToDevice, c->stream), "upload G") ? WRAPD_ERR_HIP : WRAPD_OK;
+}
+
+void launch_solve(wrapd_ctx* c, const double* b, double* x) {
+    const wrapd::CholSymbolic& S = c->sym;
+    SolveMaps M{c->d_col0, c->d_fsize, c->d_rowptr, c->d_rows, c->d_goff, c->d_uoff, c->d_asm_ptr, c->d_asm_idx,
+                c->d_perm};
+    const int nleaf = static_cast<int>(S.leaf_ptr.size()) - 1;
+    const int nlev = static_cast<int>(c->fwd_task_ptr.size()) - 1;
+    const int2* ft = reinterpret_cast<const int2*>(c->d_fwd_tasks);
+    const int2* bt = reinterpret_cast<const int2*>(c->d_bwd_tasks);
+    {
+        ProfScope ps(c, "solve_fwd");
+        if (nleaf > 0)
+            hipLaunchKernelGGL(k_fwd_leaf, dim3(nleaf), dim3(kSolveThreads), 0, c->stream, c->d_leaf_ptr, c->d_leaf_sn,
+                               M, b, c->G, c->ysol, c->ubuf);
+        for (int l = 0; l < nlev; ++l) {
+            const int n = c->fwd_task_ptr[l + 1] - c->fwd_task_ptr[l];
+            if (n > 0)
+                hipLaunchKernelGGL(k_fwd_level, dim3(n), dim3(kSolveThreads), 0, c->stream, ft + c->fwd_task_ptr[l], M,
+                                   b, c->G, c->ysol, c->ubuf);
+        }
+    }
+    {
+        ProfScope ps(c, "solve_bwd");
+        for (int l = nlev - 1; l >= 0; --l) {
+            const int n = c->bwd_task_ptr[l + 1] - c->bwd_task_ptr[l];
+            if (n > 0)
+                hipLaunchKernelGGL(k_bwd_level, dim3(n), dim3(kSolveThreads), 0, c->stream, bt + c->bwd_task_ptr[l], M,
+                                   c->G, c->ysol, c->xsol);
+        }
+        if (nleaf > 0)
+            hipLaunchKernelGGL(k_bwd_leaf, dim3(nleaf), dim3(kSolveThreads), 0, c->stream, c->d_leaf_ptr, c->d_leaf_sn,
+                               M, c->G, c->ysol, c->xsol);
+        hipLaunchKernelGGL(k_unpermute, dim3(blocks_for(c->n_f, kSolveThreads)), dim3(kSolveThreads), 0, c->stream,
+                           c->n_f, c->d_iperm, c->xsol, x);
+    }
+}
+
+}  // namespace wd
