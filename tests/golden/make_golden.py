@@ -1,0 +1,162 @@
+"""Writes the golden vectors under tests/golden/ from the reference build (oracle/_ref).
+
+Run in a container where the reference tree is present, after `make -C oracle ref`:
+    OMP_NUM_THREADS=1 python tests/golden/make_golden.py
+The reference runs single-threaded (its OpenMP reductions are order dependent).  Only
+inputs and outputs are stored; the mesh inputs of the sample case come from the
+reference's own data files (samples/data/untwist).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from wrapd_amd import meshgen  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+REF_TREE = "/root/reference"
+ENV = dict(os.environ, OMP_NUM_THREADS="1")
+
+
+def ref_lines(mode, rows):
+    inp = "\n".join(" ".join(f"{v:.17g}" for v in r) for r in rows) + "\n"
+    out = subprocess.run([REF, mode], input=inp, capture_output=True, text=True, env=ENV, check=True).stdout
+    return np.array([[float(t) for t in line.split()] for line in out.strip().split("\n")])
+
+
+def lame(E, nu):
+    return E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+
+
+def kat_svd(rng):
+    F = []
+    for i in range(300):
+        kind = i % 6
+        if kind == 0:
+            A = rng.standard_normal((3, 3))
+        elif kind == 1:   # near identity (typical deformation gradients)
+            A = np.eye(3) + 0.2 * rng.standard_normal((3, 3))
+        elif kind == 2:   # inverted
+            A = np.eye(3) + 0.2 * rng.standard_normal((3, 3))
+            A[:, 2] *= -1
+        elif kind == 3:   # repeated singular values (rotation times scaled identity)
+            Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+            A = Q * rng.uniform(0.5, 2.0)
+        elif kind == 4:   # rank deficient
+            A = rng.standard_normal((3, 2)) @ rng.standard_normal((2, 3))
+        else:             # strongly anisotropic
+            U, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+            V, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+            A = U @ np.diag([3.0, 0.7, 0.05]) @ V.T
+        F.append(A.reshape(-1))
+    F = np.array(F)
+    out = ref_lines("svd", F)
+    return dict(F=F, sigma=out[:, :3], U=out[:, 3:12], V=out[:, 12:21])
+
+
+def kat_prox(rng):
+    rows = []
+    for i in range(600):
+        E = [1e5, 1e6, 1e7][i % 3]
+        nu = [0.3, 0.45, 0.49][(i // 3) % 3]
+        mu, lam = lame(E, nu)
+        k = 2.0066 * mu + 1.0122 * lam
+        s = np.sort(rng.uniform(0.1, 2.5, 3))[::-1]
+        if i % 10 == 0:
+            s[2] = -rng.uniform(0.0, 0.3)
+        wov = k * np.exp(rng.uniform(np.log(0.1), np.log(10.0)))
+        last = np.ones(3) if i % 4 == 0 else np.abs(s) * rng.uniform(0.85, 1.15, 3) + 0.01
+        rows.append([s[0], s[1], s[2], wov, mu, lam, last[0], last[1], last[2]])
+    rows = np.array(rows)
+    out = ref_lines("prox", rows)
+    return dict(inputs=rows, x=out[:, :3], iters=out[:, 3].astype(int))
+
+
+def kat_hess(rng):
+    rows = []
+    for i in range(300):
+        E = [1e5, 1e6, 1e7][i % 3]
+        mu, lam = lame(E, 0.45)
+        s = rng.uniform(0.05, 2.0, 3)
+        if i % 5 == 0:   # the J < J0 branch
+            s = rng.uniform(0.05, 0.4, 3)
+        rows.append([s[0], s[1], s[2], mu, lam])
+    rows = np.array(rows)
+    out = ref_lines("hess", rows)
+    return dict(inputs=rows, kmax=out[:, 0])
+
+
+def run_trace(prob, flags, youngs=None, threads=1):
+    with tempfile.TemporaryDirectory() as d:
+        p = meshgen.write_problem(prob, d)
+        cmd = [REF, "solve", "-rest", p["rest"], "-init", p["init"], "-handles", p["handles"], "-p", "0", "-s", "0",
+               "-out", d, "-tol", "0"] + flags
+        if youngs is not None:
+            cmd += ["-E", repr(youngs)]
+        subprocess.run(cmd, env=dict(ENV, OMP_NUM_THREADS=str(threads)), check=True, capture_output=True)
+        rd = lambda n: np.atleast_1d(np.loadtxt(os.path.join(d, n)))  # noqa: E731
+        return dict(objectives=rd("objectives.txt"), reweighted=rd("reweighted.txt").astype(int),
+                    inner_iters=rd("inner_iters.txt").astype(int), ls_dal=rd("ls_dal.txt"),
+                    gs_dal=rd("gs_dal.txt"), x=np.loadtxt(os.path.join(d, "final_x.txt")))
+
+
+def save_case(name, prob, flags, youngs=None, store_mesh=False, x_every=1):
+    t = run_trace(prob, flags, youngs)
+    # the reference's own run-to-run spread: the same run on 8 OpenMP threads
+    t8 = run_trace(prob, flags, youngs, threads=8)
+    spread_e = float(np.max(np.abs(t["objectives"] - t8["objectives"])) / abs(t["objectives"][0]))
+    spread_x = float(np.linalg.norm(t["x"] - t8["x"]) / np.linalg.norm(t["x"]))
+    arrs = dict(flags=np.array(" ".join(flags)), x_rows=np.arange(0, prob.n_verts, x_every),
+                spread_e=spread_e, spread_x=spread_x,
+                spread_inner_equal=bool(np.array_equal(t["inner_iters"], t8["inner_iters"])))
+    arrs.update({k: v for k, v in t.items() if k != "x"})
+    arrs["x"] = t["x"][::x_every]
+    if store_mesh:
+        arrs.update(rest=prob.rest, init=prob.init, tets=prob.tets.astype(np.int32), pins=prob.pins.astype(np.int32))
+    np.savez_compressed(os.path.join(HERE, f"trace_{name}.npz"), **arrs)
+    print(name, "iters", len(t["objectives"]) - 1, "E0", t["objectives"][0], "spread_e %.2e spread_x %.2e" % (spread_e, spread_x))
+
+
+def main():
+    if not os.path.exists(REF):
+        sys.exit("build the reference first: make -C oracle ref")
+    rng = np.random.default_rng(20211)
+    np.savez_compressed(os.path.join(HERE, "kat_svd.npz"), **kat_svd(rng))
+    np.savez_compressed(os.path.join(HERE, "kat_prox.npz"), **kat_prox(rng))
+    np.savez_compressed(os.path.join(HERE, "kat_hess.npz"), **kat_hess(rng))
+    c1 = meshgen.config_problem(1)
+    save_case("c1_arap", c1, ["-model", "arap", "-rw", "0", "-it", "100"])
+    nh = meshgen.small_beam(20, 4, 4, seed=1, pin_far_end=False)
+    save_case("c1geom_nh", nh, ["-it", "100"])
+    save_case("c1geom_nh_E1e6", nh, ["-it", "80"], youngs=1e6)
+    save_case("c1geom_nh_ls", nh, ["-it", "60", "-global_ls", "1"])
+    save_case("c1geom_nh_kappa", nh, ["-it", "60", "-kappa", "0.1", "-gamma", "1.2", "-rw_delay", "10"])
+    sb = meshgen.small_beam(12, 3, 3, seed=2)
+    save_case("nh_ra0", sb, ["-ra", "0", "-it", "40"])
+    save_case("arap_ra0", c1, ["-model", "arap", "-rw", "0", "-ra", "0", "-it", "40"])
+    push = meshgen.small_beam(16, 3, 3, seed=4, twist_deg=120.0, axial=-0.3)
+    save_case("push_twist_nh", push, ["-it", "80"], youngs=1e6)
+    # the reference's own sample: untwist (samples/data/untwist), 30 iterations
+    ut = os.path.join(REF_TREE, "samples", "data", "untwist")
+    if os.path.isdir(ut):
+        rest, tets = meshgen.load_elenode(os.path.join(ut, "untwist_rest"))
+        init, _ = meshgen.load_elenode(os.path.join(ut, "untwist_init"))
+        soln, _ = meshgen.load_elenode(os.path.join(ut, "untwist_nh_soln"))
+        pins = np.sort(meshgen.load_handles(os.path.join(ut, "untwist_handles.txt")))
+        prob = meshgen.Problem("untwist", rest, init, tets, pins)
+        save_case("untwist", prob, ["-it", "30"], store_mesh=True)
+        np.savez_compressed(os.path.join(HERE, "untwist_soln.npz"), soln=soln)
+    # the benchmark beam at the reference's defaults except E (-E 1e6), 20 iterations
+    c2 = meshgen.config_problem(2)
+    save_case("c2_it20", c2, ["-it", "20"], youngs=1e6, x_every=37)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,82 @@
+"""The GPU solver against the reference's own outputs (tests/golden/), through the C ABI,
+and the `quasi` CLI on the reference's sample data.  Same bars as the oracle's golden tests:
+equal inner-iteration counts and reweighting iterations, energies within
+max(5e-9, 4 x reference thread spread) E_0, positions within max(1e-8, 4 x spread).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from test_oracle_golden import TRACES, load, trace_bars, untwist_problem
+from wrapd_amd import _lib, meshgen
+from wrapd_amd.solver import Solver
+
+pytestmark = pytest.mark.gpu
+
+CASES = dict(TRACES)
+CASES["untwist"] = lambda: (untwist_problem(), {})
+CASES["c2_it20"] = lambda: (meshgen.config_problem(2), dict(lbfgs_m=6, max_admm_iters=5000))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_gpu_trace_matches_reference(name):
+    prob, over = CASES[name]()
+    z = load(f"trace_{name}.npz")
+    iters = len(z["objectives"]) - 1
+    s = Solver(prob, tol=0.0, **over)
+    s.run(iters)
+    e, e_ref = s.log("objectives"), z["objectives"]
+    bar_e, bar_x = trace_bars(z)
+    print(f"{name}: max|dE|/E0 {np.max(np.abs(e - e_ref)) / abs(e_ref[0]):.2e} (bar {bar_e:.1e})")
+    assert len(e) == len(e_ref)
+    assert np.array_equal(s.log("inner_iters").astype(int), z["inner_iters"])
+    assert np.array_equal(s.log("reweighted").astype(int), z["reweighted"])
+    assert np.max(np.abs(e - e_ref)) <= bar_e * abs(e_ref[0])
+    x = s.x()[z["x_rows"]]
+    assert np.linalg.norm(x - z["x"]) <= bar_x * np.linalg.norm(z["x"])
+
+
+def test_quasi_cli_untwist(tmp_path):
+    """quasi on the reference's untwist sample writes the reference's logs and meshes."""
+    prob = untwist_problem()
+    p = meshgen.write_problem(prob, str(tmp_path))
+    out = tmp_path / "out"
+    r = subprocess.run([_lib.QUASI_PATH, "-rest", p["rest"] + ".node", "-init", p["init"] + ".node", "-handles",
+                        p["handles"], "-it", "30", "-tol", "0", "-p", "0", "-s", "1", "-out", str(out)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    z = load("trace_untwist.npz")
+    obj = np.loadtxt(out / "objectives.txt")
+    assert obj.shape == (31, 2) and np.array_equal(obj[:, 0], np.arange(31))
+    bar_e, bar_x = trace_bars(z)
+    assert np.max(np.abs(obj[:, 1] - z["objectives"])) <= bar_e * z["objectives"][0]
+    rw = np.loadtxt(out / "reweighted.txt")
+    assert np.array_equal(rw[:, 1].astype(int), z["reweighted"])
+    X, T = meshgen.load_elenode(str(out / "final_mesh"))
+    assert np.array_equal(T, prob.tets)
+    assert np.linalg.norm(X - z["x"]) <= bar_x * np.linalg.norm(z["x"])
+    assert (out / "final_mesh.obj").exists()
+
+
+def test_fp32_local_step_final_positions_within_1e6():
+    """FP32 local step (FP64 fallback) with a Newton prox reaches the FP64 fixed point."""
+    prob = meshgen.small_beam(20, 4, 4, seed=1, pin_far_end=False, youngs=1e6)
+    ref = Solver(prob, tol=0.0)
+    ref.run(300)
+    fast = Solver(prob, tol=0.0, precision=1, prox_mode=1)
+    fast.run(300)
+    xr, xf = ref.x(), fast.x()
+    assert np.linalg.norm(xf - xr) <= 1e-6 * np.linalg.norm(xr)
+    er, ef = ref.log("objectives"), fast.log("objectives")
+    assert abs(ef[-1] - er[-1]) <= 1e-6 * abs(er[0])
+
+
+def test_newton_fp64_prox_same_fixed_point():
+    prob = meshgen.small_beam(16, 3, 3, seed=4, twist_deg=120.0, axial=-0.3, youngs=1e6)
+    ref = Solver(prob, tol=0.0)
+    ref.run(200)
+    nt = Solver(prob, tol=0.0, prox_mode=1)
+    nt.run(200)
+    assert np.linalg.norm(nt.x() - ref.x()) <= 1e-7 * np.linalg.norm(ref.x())
