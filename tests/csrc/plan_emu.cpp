@@ -1,0 +1,164 @@
+// Test helper (not part of the library): runs the GPU factorization schedule built by
+// chol_plan sequentially on the CPU, task by task, with the tile geometry the kernels in
+// wrapd_amd/csrc/wd_factor.hip use, and records every read and write so that two tasks
+// of one launch touching the same entry (one of them writing) are reported.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../wrapd_amd/csrc/chol_plan.hpp"
+#include "../../wrapd_amd/csrc/chol_symbolic.hpp"
+
+using namespace wrapd;
+
+namespace {
+CholSymbolic S;
+FactorPlan PL;
+std::vector<int> Ap, Aj;
+std::vector<double> pool, G;
+std::vector<int64_t> wF, rF, wG, rG;
+int64_t stamp = 0, cur_launch = 0, conflicts = 0;
+
+inline int64_t launch_of(int64_t st) { return st >> 32; }
+inline void on_read(std::vector<int64_t>& w, std::vector<int64_t>& r, int64_t e) {
+    if (w[e] && launch_of(w[e]) == cur_launch && w[e] != stamp) conflicts++;
+    r[e] = stamp;
+}
+inline void on_write(std::vector<int64_t>& w, std::vector<int64_t>& r, int64_t e) {
+    if (w[e] && launch_of(w[e]) == cur_launch && w[e] != stamp) conflicts++;
+    if (r[e] && launch_of(r[e]) == cur_launch && r[e] != stamp) conflicts++;
+    w[e] = stamp;
+}
+inline double rdF(int64_t e) { on_read(wF, rF, e); return pool[e]; }
+inline void wrF(int64_t e, double v) { on_write(wF, rF, e); pool[e] = v; }
+inline double rdG(int64_t e) { on_read(wG, rG, e); return G[e]; }
+inline void wrG(int64_t e, double v) { on_write(wG, rG, e); G[e] = v; }
+
+void tile(int t, int ns, int f, int& st, int& sz) {
+    const int P = tile_count_l11(ns);
+    if (t < P) { st = kFactorTile * t; sz = std::min(kFactorTile, ns - st); }
+    else { st = ns + kFactorTile * (t - P); sz = std::min(kFactorTile, f - st); }
+}
+}  // namespace
+
+extern "C" int pe_analyze(int n, const int* ap, const int* aj, const double* xyz, int leaf_size, double* st) {
+    Ap.assign(ap, ap + n + 1);
+    Aj.assign(aj, aj + ap[n]);
+    SymbolicOptions o;
+    o.leaf_size = leaf_size;
+    chol_analyze(n, Ap, Aj, xyz, o, S);
+    chol_plan(S, PL);
+    st[0] = S.nsuper; st[1] = PL.n_levels; st[2] = static_cast<double>(PL.launches.size());
+    st[3] = S.max_ns; st[4] = static_cast<double>(S.goff[S.nsuper]); st[5] = static_cast<double>(PL.pool_size);
+    st[6] = static_cast<double>(PL.pool_no_reuse);
+    return 0;
+}
+
+// perm (n), col0 (nsuper+1), fsize (nsuper), goff (nsuper+1), rowptr (nsuper+1), rows
+extern "C" void pe_layout(int* perm, int* col0, int* fsize, int64_t* goff, int64_t* rowptr, int* rows) {
+    std::copy(S.perm.begin(), S.perm.end(), perm);
+    std::copy(S.col0.begin(), S.col0.end(), col0);
+    std::copy(S.fsize.begin(), S.fsize.end(), fsize);
+    std::copy(S.goff.begin(), S.goff.end(), goff);
+    std::copy(S.rowptr.begin(), S.rowptr.end(), rowptr);
+    if (rows) std::copy(S.rows.begin(), S.rows.end(), rows);
+}
+
+// returns 0 on success, 1 on a non-positive pivot; G_out gets G; conflicts_out the count
+extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
+    pool.assign(PL.pool_size, std::nan(""));
+    G.assign(S.goff[S.nsuper], std::nan(""));
+    wF.assign(pool.size(), 0); rF.assign(pool.size(), 0);
+    wG.assign(G.size(), 0); rG.assign(G.size(), 0);
+    conflicts = 0;
+    int fail = 0;
+    for (size_t li = 0; li < PL.launches.size(); ++li) {
+        const FactorLaunch& L = PL.launches[li];
+        cur_launch = static_cast<int64_t>(li) + 1;
+        for (int q = 0; q < L.count; ++q) {
+            stamp = (cur_launch << 32) | (q + 1);
+            const int s = PL.tasks[2 * (L.offset + q)], y = PL.tasks[2 * (L.offset + q) + 1];
+            const int ns = S.col0[s + 1] - S.col0[s], f = S.fsize[s];
+            const int64_t fo = PL.foff[s], go = S.goff[s];
+            const int k = L.panel, c0 = kFactorTile * k, nb = std::min(kFactorTile, ns - c0);
+            auto Fi = [&](int r, int c) { return fo + static_cast<int64_t>(c) * f + r; };
+            auto Gi = [&](int r, int c) { return go + static_cast<int64_t>(c) * f + r; };
+            if (L.kind == FK_ASSEMBLE) {
+                const int j = y;
+                const int64_t gc = PL.cbase[s] + j;
+                for (int r = j; r < f; ++r) wrF(Fi(r, j), 0.0);
+                if (j < ns) for (int r = 0; r < f; ++r) wrG(Gi(r, j), 0.0);
+                for (int64_t a = PL.acol_ptr[gc]; a < PL.acol_ptr[gc + 1]; ++a)
+                    wrF(Fi(PL.acol[2 * a], j), ax[PL.acol[2 * a + 1]]);
+                for (int64_t a = PL.ccol_ptr[gc]; a < PL.ccol_ptr[gc + 1]; ++a) {
+                    const int ch = PL.ccol[2 * a], jj = PL.ccol[2 * a + 1];
+                    const int fc = S.fsize[ch], nsc = S.col0[ch + 1] - S.col0[ch], mc = fc - nsc;
+                    const int* rl = &S.rel[S.relptr[ch]];
+                    for (int ii = jj; ii < mc; ++ii) {
+                        const double v = rdF(PL.foff[ch] + static_cast<int64_t>(nsc + jj) * fc + nsc + ii);
+                        wrF(Fi(rl[ii], j), rdF(Fi(rl[ii], j)) + v);
+                    }
+                }
+            } else if (L.kind == FK_POTRF) {
+                for (int p = 0; p < nb; ++p) {
+                    double d = rdF(Fi(c0 + p, c0 + p));
+                    if (!(d > 0)) { fail = 1; d = 1.0; }
+                    d = std::sqrt(d);
+                    wrF(Fi(c0 + p, c0 + p), d);
+                    for (int r = p + 1; r < nb; ++r) wrF(Fi(c0 + r, c0 + p), rdF(Fi(c0 + r, c0 + p)) / d);
+                    for (int c = p + 1; c < nb; ++c)
+                        for (int r = c; r < nb; ++r)
+                            wrF(Fi(c0 + r, c0 + c), rdF(Fi(c0 + r, c0 + c)) - rdF(Fi(c0 + r, c0 + p)) * rdF(Fi(c0 + c, c0 + p)));
+                }
+            } else if (L.kind == FK_TRSM) {
+                const int kind = y >> kTaskKindShift, t = y & ((1 << kTaskKindShift) - 1);
+                int t0, nt;
+                tile(t, ns, f, t0, nt);
+                if (kind == 0) {
+                    for (int p = 0; p < nb; ++p) {
+                        const double d = rdF(Fi(c0 + p, c0 + p));
+                        for (int r = 0; r < nt; ++r) wrF(Fi(t0 + r, c0 + p), rdF(Fi(t0 + r, c0 + p)) / d);
+                        for (int c = p + 1; c < nb; ++c) {
+                            const double l = rdF(Fi(c0 + c, c0 + p));
+                            for (int r = 0; r < nt; ++r)
+                                wrF(Fi(t0 + r, c0 + c), rdF(Fi(t0 + r, c0 + c)) - rdF(Fi(t0 + r, c0 + p)) * l);
+                        }
+                    }
+                } else {
+                    std::vector<double> X(kFactorTile * kFactorTile, 0.0);
+                    for (int j = 0; j < nt; ++j)
+                        for (int r = 0; r < nb; ++r)
+                            X[j * 64 + r] = (t == k) ? (r == j ? 1.0 : 0.0) : rdG(Gi(c0 + r, t0 + j));
+                    for (int p = 0; p < nb; ++p) {
+                        const double d = rdF(Fi(c0 + p, c0 + p));
+                        for (int j = 0; j < nt; ++j) X[j * 64 + p] /= d;
+                        for (int r = p + 1; r < nb; ++r) {
+                            const double l = rdF(Fi(c0 + r, c0 + p));
+                            for (int j = 0; j < nt; ++j) X[j * 64 + r] -= l * X[j * 64 + p];
+                        }
+                    }
+                    for (int j = 0; j < nt; ++j)
+                        for (int r = 0; r < nb; ++r) wrG(Gi(c0 + r, t0 + j), X[j * 64 + r]);
+                }
+            } else {
+                const int kind = y >> kTaskKindShift;
+                const int a = (y >> kTaskTileShift) & kTaskTileMask, b = y & kTaskTileMask;
+                int a0, na, b0, nbt;
+                tile(a, ns, f, a0, na);
+                tile(b, ns, f, b0, nbt);
+                for (int j = 0; j < nbt; ++j)
+                    for (int i = (kind == 0 && a == b) ? j : 0; i < na; ++i) {
+                        double acc = 0;
+                        for (int kk = 0; kk < nb; ++kk)
+                            acc += rdF(Fi(a0 + i, c0 + kk)) * (kind == 0 ? rdF(Fi(b0 + j, c0 + kk)) : rdG(Gi(c0 + kk, b0 + j)));
+                        if (kind == 0) wrF(Fi(a0 + i, b0 + j), rdF(Fi(a0 + i, b0 + j)) - acc);
+                        else wrG(Gi(a0 + i, b0 + j), rdG(Gi(a0 + i, b0 + j)) - acc);
+                    }
+            }
+        }
+    }
+    std::memcpy(G_out, G.data(), G.size() * sizeof(double));
+    *conflicts_out = conflicts;
+    return fail;
+}

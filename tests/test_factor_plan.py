@@ -1,0 +1,116 @@
+"""The GPU factorization schedule (wrapd_amd/csrc/chol_plan.cpp) on the CPU.
+
+tests/csrc/plan_emu.cpp executes the plan's launches task by task with the tiles the HIP
+kernels use.  Checked: G_s = [L11^{-1}; -L21 L11^{-1}] for every supernode against a
+dense LAPACK Cholesky of the permuted matrix, no two tasks of one launch touching the
+same entry with one of them writing, front-pool reuse never handing out live memory
+(the pool starts as NaN), and an indefinite matrix flagged.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from wrapd_amd import meshgen
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def emu(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("planemu") / "libplanemu.so")
+    src = [os.path.join(ROOT, "tests", "csrc", "plan_emu.cpp"),
+           os.path.join(ROOT, "wrapd_amd", "csrc", "chol_symbolic.cpp"),
+           os.path.join(ROOT, "wrapd_amd", "csrc", "chol_plan.cpp")]
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", out] + src, check=True)
+    lib = C.CDLL(out)
+    lib.pe_run.restype = C.c_int
+    return lib
+
+
+def _upper_csr(prob):
+    o = ol.Oracle(prob)
+    o.initialize()
+    r, c, v = o.A_coo()
+    n = o.n_f
+    m = r <= c
+    r, c, v = r[m], c[m], v[m]
+    order = np.lexsort((c, r))
+    r, c, v = r[order], c[order], v[order]
+    Ap = np.zeros(n + 1, np.int32)
+    np.add.at(Ap, r + 1, 1)
+    return n, np.cumsum(Ap).astype(np.int32), np.ascontiguousarray(c.astype(np.int32)), np.ascontiguousarray(v)
+
+
+def _run(emu, prob, leaf_size, negate=False):
+    n, Ap, Aj, Ax = _upper_csr(prob)
+    free = np.setdiff1d(np.arange(prob.n_verts), prob.pins)
+    st = np.zeros(7)
+    emu.pe_analyze(n, ol.ip(Ap), ol.ip(Aj), ol.dp(np.ascontiguousarray(prob.rest[free])), leaf_size, ol.dp(st))
+    nsup = int(st[0])
+    perm = np.zeros(n, np.int32)
+    col0 = np.zeros(nsup + 1, np.int32)
+    fsize = np.zeros(nsup, np.int32)
+    goff = np.zeros(nsup + 1, np.int64)
+    rowptr = np.zeros(nsup + 1, np.int64)
+    emu.pe_layout(ol.ip(perm), ol.ip(col0), ol.ip(fsize), goff.ctypes.data_as(C.c_void_p),
+                  rowptr.ctypes.data_as(C.c_void_p), None)
+    rows = np.zeros(int(rowptr[-1]), np.int32)
+    emu.pe_layout(ol.ip(perm), ol.ip(col0), ol.ip(fsize), goff.ctypes.data_as(C.c_void_p),
+                  rowptr.ctypes.data_as(C.c_void_p), ol.ip(rows))
+    G = np.zeros(int(goff[-1]))
+    conflicts = C.c_int64(0)
+    fail = emu.pe_run(ol.dp(-Ax if negate else Ax), ol.dp(G), C.byref(conflicts))
+    A = np.zeros((n, n))
+    rr = np.repeat(np.arange(n), np.diff(Ap))
+    A[rr, Aj] = Ax
+    A = A + np.triu(A, 1).T
+    return dict(st=st, perm=perm, col0=col0, fsize=fsize, goff=goff, rowptr=rowptr, rows=rows, G=G, fail=fail,
+                conflicts=conflicts.value, A=A)
+
+
+CASES = {
+    "tiny": (lambda: meshgen.small_beam(6, 3, 3, seed=2), 48),
+    "deep_tree": (lambda: meshgen.small_beam(24, 5, 4, seed=3), 8),
+    "two_panel_separators": (lambda: meshgen.small_beam(16, 10, 10, seed=4), 48),
+    "ragged": (lambda: meshgen.small_beam(9, 7, 5, seed=5, twist_deg=30.0), 16),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_plan_matches_dense_cholesky(emu, name):
+    make, leaf = CASES[name]
+    R = _run(emu, make(), leaf)
+    assert R["fail"] == 0
+    assert R["conflicts"] == 0
+    if name == "two_panel_separators":
+        assert R["st"][3] > 64   # a supernode spans more than one 64-column panel
+    p = R["perm"]
+    L = np.linalg.cholesky(R["A"][np.ix_(p, p)])
+    worst = 0.0
+    for s in range(len(R["fsize"])):
+        c0, c1, f = R["col0"][s], R["col0"][s + 1], R["fsize"][s]
+        ns = c1 - c0
+        rows = R["rows"][R["rowptr"][s]:R["rowptr"][s + 1]]
+        assert np.array_equal(rows[:ns], np.arange(c0, c1))
+        Linv = np.linalg.inv(L[c0:c1, c0:c1])
+        expect = np.vstack([Linv, -L[np.ix_(rows[ns:], np.arange(c0, c1))] @ Linv])
+        got = R["G"][R["goff"][s]:R["goff"][s + 1]].reshape(ns, f).T
+        worst = max(worst, float(np.max(np.abs(got - expect)) / np.max(np.abs(expect))))
+    assert worst < 1e-12, worst
+    # every factor entry outside the supernodes' structure is zero
+    Ls = np.zeros_like(L)
+    for s in range(len(R["fsize"])):
+        c0, c1 = R["col0"][s], R["col0"][s + 1]
+        rows = R["rows"][R["rowptr"][s]:R["rowptr"][s + 1]]
+        Ls[np.ix_(rows, np.arange(c0, c1))] = L[np.ix_(rows, np.arange(c0, c1))]
+    assert np.allclose(np.tril(L), np.tril(Ls), atol=1e-12 * np.abs(L).max())
+
+
+def test_plan_pool_reuse_and_indefinite(emu):
+    R = _run(emu, meshgen.small_beam(24, 5, 4, seed=3), 8, negate=True)
+    assert R["fail"] == 1
+    assert R["st"][5] < R["st"][6]   # reuse makes the pool smaller than the sum of fronts

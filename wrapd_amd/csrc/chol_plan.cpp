@@ -1,0 +1,208 @@
+// Builds the GPU factorization schedule: levels, front pool offsets with reuse,
+// per-column assembly lists and the batched tile task lists.
+#include "chol_plan.hpp"
+
+#include <algorithm>
+#include <iterator>
+#include <map>
+#include <stdexcept>
+
+namespace wrapd {
+
+namespace {
+
+// first-fit allocator over a growing pool, in doubles, with coalescing of free blocks
+struct PoolAlloc {
+    std::map<int64_t, int64_t> free_blocks;   // offset -> size
+    int64_t end = 0;
+    static int64_t round(int64_t n) { return (n + 31) / 32 * 32; }
+    int64_t alloc(int64_t n) {
+        n = round(std::max<int64_t>(n, 1));
+        for (auto it = free_blocks.begin(); it != free_blocks.end(); ++it) {
+            if (it->second >= n) {
+                const int64_t off = it->first;
+                const int64_t rest = it->second - n;
+                free_blocks.erase(it);
+                if (rest > 0) free_blocks[off + n] = rest;
+                return off;
+            }
+        }
+        // extend the pool, merging a free tail block if there is one
+        if (!free_blocks.empty()) {
+            auto last = std::prev(free_blocks.end());
+            if (last->first + last->second == end) {
+                const int64_t off = last->first;
+                free_blocks.erase(last);
+                end = off + n;
+                return off;
+            }
+        }
+        const int64_t off = end;
+        end += n;
+        return off;
+    }
+    void release(int64_t off, int64_t n) {
+        n = round(std::max<int64_t>(n, 1));
+        auto it = free_blocks.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_blocks.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_blocks.erase(nx);
+        }
+        if (it != free_blocks.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_blocks.erase(it);
+            }
+        }
+    }
+};
+
+}  // namespace
+
+void chol_plan(const CholSymbolic& S, FactorPlan& P) {
+    const int nsup = S.nsuper;
+    P = FactorPlan();
+    std::vector<int> ns(nsup), f(nsup);
+    for (int s = 0; s < nsup; ++s) {
+        ns[s] = S.col0[s + 1] - S.col0[s];
+        f[s] = S.fsize[s];
+        if (ns[s] <= 0 || f[s] < ns[s]) throw std::runtime_error("chol_plan: bad supernode");
+        if (tile_count(ns[s], f[s]) > kTaskTileMask) throw std::runtime_error("chol_plan: front too large");
+    }
+    // levels: leaves 0, parent above every child (parent index > child index)
+    std::vector<int> lev(nsup, 0);
+    for (int s = 0; s < nsup; ++s) {
+        const int p = S.parent[s];
+        if (p >= 0) {
+            if (p <= s) throw std::runtime_error("chol_plan: supernodes not postordered");
+            lev[p] = std::max(lev[p], lev[s] + 1);
+        }
+    }
+    int nlev = 0;
+    for (int s = 0; s < nsup; ++s) nlev = std::max(nlev, lev[s] + 1);
+    P.n_levels = nlev;
+    std::vector<std::vector<int>> by_level(nlev);
+    for (int s = 0; s < nsup; ++s) by_level[lev[s]].push_back(s);
+    std::vector<std::vector<int>> children(nsup);
+    for (int s = 0; s < nsup; ++s)
+        if (S.parent[s] >= 0) children[S.parent[s]].push_back(s);
+
+    // pool offsets: a level's fronts are allocated before its assembly; the children it
+    // consumes are released after it, for later levels to reuse
+    PoolAlloc pool;
+    P.foff.assign(nsup, 0);
+    for (int l = 0; l < nlev; ++l) {
+        for (int s : by_level[l]) {
+            P.foff[s] = pool.alloc(static_cast<int64_t>(f[s]) * f[s]);
+            P.pool_no_reuse += static_cast<int64_t>(f[s]) * f[s];
+        }
+        for (int s : by_level[l])
+            for (int c : children[s]) pool.release(P.foff[c], static_cast<int64_t>(f[c]) * f[c]);
+    }
+    P.pool_size = pool.end;
+
+    // assembly lists per global front column
+    P.cbase.assign(nsup, 0);
+    int64_t ncols = 0;
+    for (int s = 0; s < nsup; ++s) {
+        P.cbase[s] = ncols;
+        ncols += f[s];
+    }
+    std::vector<int64_t> acnt(ncols + 1, 0), ccnt(ncols + 1, 0);
+    const int64_t nnzA = static_cast<int64_t>(S.a_sn.size());
+    for (int64_t q = 0; q < nnzA; ++q) {
+        const int s = S.a_sn[q];
+        const int64_t col = S.a_pos[q] / f[s];
+        acnt[P.cbase[s] + col + 1]++;
+    }
+    for (int s = 0; s < nsup; ++s)
+        for (int c : children[s]) {
+            const int mc = f[c] - ns[c];
+            for (int jj = 0; jj < mc; ++jj) ccnt[P.cbase[s] + S.rel[S.relptr[c] + jj] + 1]++;
+        }
+    for (int64_t i = 0; i < ncols; ++i) {
+        acnt[i + 1] += acnt[i];
+        ccnt[i + 1] += ccnt[i];
+    }
+    P.acol_ptr = acnt;
+    P.ccol_ptr = ccnt;
+    P.acol.assign(2 * acnt[ncols], 0);
+    P.ccol.assign(2 * ccnt[ncols], 0);
+    {
+        std::vector<int64_t> pos(acnt.begin(), acnt.end() - 1);
+        for (int64_t q = 0; q < nnzA; ++q) {
+            const int s = S.a_sn[q];
+            const int64_t col = S.a_pos[q] / f[s];
+            const int64_t row = S.a_pos[q] % f[s];
+            if (row < col) throw std::runtime_error("chol_plan: A entry above the diagonal");
+            const int64_t k = pos[P.cbase[s] + col]++;
+            P.acol[2 * k] = static_cast<int>(row);
+            P.acol[2 * k + 1] = static_cast<int>(q);
+        }
+        std::vector<int64_t> cp(ccnt.begin(), ccnt.end() - 1);
+        for (int s = 0; s < nsup; ++s)
+            for (int c : children[s]) {   // ascending child index: the summation order
+                const int mc = f[c] - ns[c];
+                for (int jj = 0; jj < mc; ++jj) {
+                    const int64_t k = cp[P.cbase[s] + S.rel[S.relptr[c] + jj]]++;
+                    P.ccol[2 * k] = c;
+                    P.ccol[2 * k + 1] = jj;
+                }
+            }
+    }
+
+    // launches
+    auto begin = [&](int kind, int panel) {
+        P.launches.push_back(FactorLaunch{kind, panel, static_cast<int64_t>(P.tasks.size() / 2), 0});
+    };
+    auto push = [&](int x, int y) {
+        P.tasks.push_back(x);
+        P.tasks.push_back(y);
+        P.launches.back().count++;
+    };
+    const double T = kFactorTile;
+    for (int l = 0; l < nlev; ++l) {
+        const std::vector<int>& L = by_level[l];
+        begin(FK_ASSEMBLE, 0);
+        for (int s : L)
+            for (int j = 0; j < f[s]; ++j) push(s, j);
+        int pmax = 0;
+        for (int s : L) pmax = std::max(pmax, tile_count_l11(ns[s]));
+        for (int k = 0; k < pmax; ++k) {
+            begin(FK_POTRF, k);
+            for (int s : L)
+                if (k < tile_count_l11(ns[s])) push(s, 0);
+            begin(FK_TRSM, k);
+            for (int s : L) {
+                if (k >= tile_count_l11(ns[s])) continue;
+                const int nt = tile_count(ns[s], f[s]);
+                for (int t = k + 1; t < nt; ++t) push(s, t);
+                for (int c = 0; c <= k; ++c) push(s, (1 << kTaskKindShift) | c);
+            }
+            begin(FK_UPDATE, k);
+            for (int s : L) {
+                if (k >= tile_count_l11(ns[s])) continue;
+                const int nt = tile_count(ns[s], f[s]);
+                for (int a = k + 1; a < nt; ++a)
+                    for (int b = k + 1; b <= a; ++b) {
+                        push(s, (a << kTaskTileShift) | b);
+                        P.flops += 2 * T * T * T;
+                    }
+                for (int i = k + 1; i < nt; ++i)
+                    for (int c = 0; c <= k; ++c) {
+                        push(s, (1 << kTaskKindShift) | (i << kTaskTileShift) | c);
+                        P.flops += 2 * T * T * T;
+                    }
+            }
+        }
+    }
+    // drop empty launches (a level whose panel list is empty for a kind)
+    std::vector<FactorLaunch> kept;
+    for (const FactorLaunch& L : P.launches)
+        if (L.count > 0) kept.push_back(L);
+    P.launches.swap(kept);
+}
+
+}  // namespace wrapd

@@ -1,0 +1,72 @@
+// Execution plan for the numeric supernodal factorization on the GPU (the numeric
+// half of the reference's LinearSolver::factorize, deps/mcloptlib/include/MCL/
+// LinearSolver.hpp:279-298, redone whenever reweighting changes W, src/System.cpp:50-63).
+//
+// The elimination tree is processed level by level (a level holds supernodes whose
+// children all sit at lower levels).  Per level:
+//   1. assemble: one wave per front column zeroes the column, stores the A entries and
+//      adds the children's Schur-complement columns (children in index order, so every
+//      entry is summed in a fixed order, no atomics);
+//   2. per 64-column panel k of the supernode columns, three batched launches:
+//        POTRF         factor the panel's diagonal tile L_kk
+//        TRSM + GDIAG  L[t, k] = F[t, k] L_kk^{-T} for row tiles t > k, and
+//                      X[k, c] = L_kk^{-1} X[k, c] for column tiles c <= k
+//        SYRK + GUPD   F[a, b] -= L[a, k] L[b, k]^T for lower tiles a >= b > k
+//                      (remaining supernode columns and the Schur complement), and
+//                      X[i, c] -= L[i, k] X[k, c] for row tiles i > k, c <= k
+//      Every task of a launch writes its own tile and reads only tiles that no task of
+//      the same launch writes.  X solves [L11 0; L21 I] X = [I; 0], i.e. X is the solve layout
+//      G_s = [L11^{-1}; -L21 L11^{-1}] (f_s x ns_s, column-major).
+// Front tiles: the ns columns are cut into P = ceil(ns/64) tiles from 0, the m = f - ns
+// update rows into Q = ceil(m/64) tiles from ns; tile t < P starts at 64 t, tile t >= P
+// at ns + 64 (t - P).  Fronts are f x f column-major doubles in one pool; a front's
+// memory is handed to later levels once its parent has assembled it.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "chol_symbolic.hpp"
+
+namespace wrapd {
+
+enum FactorKind { FK_ASSEMBLE = 0, FK_POTRF = 1, FK_TRSM = 2, FK_UPDATE = 3 };
+
+// task pair (s, y): FK_ASSEMBLE y = front column j; FK_POTRF y = 0;
+// FK_TRSM y = (kind << 30) | t with kind 0 TRSM (row tile t), 1 GDIAG (column tile t);
+// FK_UPDATE y = (kind << 30) | (a << 15) | b with kind 0 SYRK (tiles a, b), 1 GUPD (row
+// tile a, column tile b).
+constexpr int kTaskKindShift = 30;
+constexpr int kTaskTileShift = 15;
+constexpr int kTaskTileMask = (1 << 15) - 1;
+
+struct FactorLaunch {
+    int kind;
+    int panel;          // k (unused for FK_ASSEMBLE)
+    int64_t offset;     // first task (int pairs) in tasks
+    int count;          // tasks
+};
+
+struct FactorPlan {
+    int n_levels = 0;
+    std::vector<int64_t> foff;      // [nsuper] front offset in the pool (doubles)
+    int64_t pool_size = 0;          // doubles, peak of the reuse schedule
+    int64_t pool_no_reuse = 0;      // sum f_s^2, for the record
+    std::vector<int64_t> cbase;     // [nsuper] first global front column of s
+    std::vector<int64_t> acol_ptr;  // [total columns + 1]
+    std::vector<int> acol;          // pairs (row in front, A slot)
+    std::vector<int64_t> ccol_ptr;  // [total columns + 1]
+    std::vector<int> ccol;          // pairs (child supernode, child update column jj)
+    std::vector<int> tasks;         // pairs, all launches concatenated
+    std::vector<FactorLaunch> launches;   // in stream order
+    double flops = 0;               // dense flops issued by the tile tasks (incl. G)
+};
+
+constexpr int kFactorTile = 64;
+
+inline int tile_count_l11(int ns) { return (ns + kFactorTile - 1) / kFactorTile; }
+inline int tile_count(int ns, int f) { return tile_count_l11(ns) + (f - ns + kFactorTile - 1) / kFactorTile; }
+
+void chol_plan(const CholSymbolic& S, FactorPlan& P);
+
+}  // namespace wrapd

@@ -439,126 +439,4 @@ void chol_analyze(int n, const std::vector<int>& Ap, const std::vector<int>& Aj,
     S.iperm = iperm;
 }
 
-bool chol_numeric_host(const CholSymbolic& S, const std::vector<int>& Ap, const std::vector<int>& Aj,
-                       const std::vector<double>& Ax, std::vector<double>& G) {
-    (void)Ap;
-    (void)Aj;
-    const int ns_total = S.nsuper;
-    G.assign(S.goff[ns_total], 0.0);
-    std::vector<std::vector<int>> slots(ns_total);
-    for (size_t p = 0; p < Ax.size(); ++p) slots[S.a_sn[p]].push_back(static_cast<int>(p));
-    std::vector<std::vector<int>> children(ns_total);
-    for (int s = 0; s < ns_total; ++s) if (S.parent[s] >= 0) children[S.parent[s]].push_back(s);
-    std::vector<std::vector<double>> upd(ns_total);
-    std::vector<double> F;
-    for (int s = 0; s < ns_total; ++s) {
-        const int ns = S.col0[s + 1] - S.col0[s];
-        const int f = S.fsize[s];
-        const int m = f - ns;
-        F.assign(static_cast<size_t>(f) * f, 0.0);
-        for (int p : slots[s]) F[S.a_pos[p]] += Ax[p];
-        for (int c : children[s]) {
-            const int mc = S.fsize[c] - (S.col0[c + 1] - S.col0[c]);
-            const int* rl = &S.rel[S.relptr[c]];
-            const std::vector<double>& Uc = upd[c];
-            for (int jj = 0; jj < mc; ++jj)
-                for (int ii = jj; ii < mc; ++ii)
-                    F[static_cast<size_t>(rl[jj]) * f + rl[ii]] += Uc[static_cast<size_t>(jj) * mc + ii];
-            upd[c].clear();
-            upd[c].shrink_to_fit();
-        }
-        for (int k = 0; k < ns; ++k) {
-            double* Fk = &F[static_cast<size_t>(k) * f];
-            double d = Fk[k];
-            if (!(d > 0.0)) return false;
-            d = std::sqrt(d);
-            Fk[k] = d;
-            for (int i = k + 1; i < f; ++i) Fk[i] /= d;
-            for (int j = k + 1; j < f; ++j) {
-                double lj = Fk[j];
-                if (lj == 0.0) continue;
-                double* Fj = &F[static_cast<size_t>(j) * f];
-                for (int i = j; i < f; ++i) Fj[i] -= Fk[i] * lj;
-            }
-        }
-        if (m > 0) {
-            upd[s].assign(static_cast<size_t>(m) * m, 0.0);
-            for (int jj = 0; jj < m; ++jj)
-                for (int ii = jj; ii < m; ++ii)
-                    upd[s][static_cast<size_t>(jj) * m + ii] = F[static_cast<size_t>(ns + jj) * f + ns + ii];
-        }
-        // Linv (ns x ns lower) and -W = -L21 Linv into G (f x ns column-major)
-        double* Gs = &G[S.goff[s]];
-        for (int c = 0; c < ns; ++c) {
-            double* gc = Gs + static_cast<size_t>(c) * f;
-            gc[c] = 1.0 / F[static_cast<size_t>(c) * f + c];
-            for (int i = c + 1; i < ns; ++i) {
-                double acc = 0.0;
-                for (int k = c; k < i; ++k) acc += F[static_cast<size_t>(k) * f + i] * gc[k];
-                gc[i] = -acc / F[static_cast<size_t>(i) * f + i];
-            }
-            for (int r = ns; r < f; ++r) {
-                double acc = 0.0;
-                for (int k = c; k < ns; ++k) acc += F[static_cast<size_t>(k) * f + r] * gc[k];
-                gc[r] = -acc;
-            }
-        }
-    }
-    return true;
-}
-
-void chol_solve_host(const CholSymbolic& S, const std::vector<double>& G, const double* b, double* x) {
-    const int n = S.n;
-    std::vector<double> sol(3 * static_cast<size_t>(n)), u(3 * static_cast<size_t>(S.uoff[S.nsuper]));
-    std::vector<double> c, v;
-    for (int s = 0; s < S.nsuper; ++s) {
-        const int ns = S.col0[s + 1] - S.col0[s], f = S.fsize[s];
-        const int64_t rp = S.rowptr[s];
-        c.assign(3 * static_cast<size_t>(f), 0.0);
-        for (int r = 0; r < f; ++r) {
-            double a[3] = {0, 0, 0};
-            if (r < ns) {
-                int old = S.perm[S.col0[s] + r];
-                for (int k = 0; k < 3; ++k) a[k] = b[3 * old + k];
-            }
-            for (int64_t q = S.asm_ptr[rp + r]; q < S.asm_ptr[rp + r + 1]; ++q)
-                for (int k = 0; k < 3; ++k) a[k] += u[3 * S.asm_idx[q] + k];
-            for (int k = 0; k < 3; ++k) c[3 * r + k] = a[k];
-        }
-        const double* Gs = &G[S.goff[s]];
-        for (int r = 0; r < f; ++r) {
-            double t[3] = {0, 0, 0};
-            for (int j = 0; j < ns; ++j) {
-                double gv = Gs[static_cast<size_t>(j) * f + r];
-                for (int k = 0; k < 3; ++k) t[k] += gv * c[3 * j + k];
-            }
-            if (r < ns) {
-                for (int k = 0; k < 3; ++k) sol[3 * (S.col0[s] + r) + k] = t[k];
-            } else {
-                for (int k = 0; k < 3; ++k) u[3 * (S.uoff[s] + r - ns) + k] = c[3 * r + k] + t[k];
-            }
-        }
-    }
-    for (int s = S.nsuper - 1; s >= 0; --s) {
-        const int ns = S.col0[s + 1] - S.col0[s], f = S.fsize[s];
-        const int64_t rp = S.rowptr[s];
-        v.assign(3 * static_cast<size_t>(f), 0.0);
-        for (int r = 0; r < f; ++r) {
-            int row = S.rows[rp + r];
-            for (int k = 0; k < 3; ++k) v[3 * r + k] = sol[3 * row + k];
-        }
-        const double* Gs = &G[S.goff[s]];
-        for (int j = 0; j < ns; ++j) {
-            double t[3] = {0, 0, 0};
-            for (int r = 0; r < f; ++r) {
-                double gv = Gs[static_cast<size_t>(j) * f + r];
-                for (int k = 0; k < 3; ++k) t[k] += gv * v[3 * r + k];
-            }
-            for (int k = 0; k < 3; ++k) sol[3 * (S.col0[s] + j) + k] = t[k];
-        }
-    }
-    for (int i = 0; i < n; ++i)
-        for (int k = 0; k < 3; ++k) x[3 * i + k] = sol[3 * S.iperm[i] + k];
-}
-
 }  // namespace wrapd

@@ -8,6 +8,7 @@
 //   * supernodes s with columns [col0[s], col0[s+1]) and a front of f_s rows: its own
 //     ns_s columns followed by the m_s rows of its off-diagonal structure (sorted);
 //   * per supernode a dense column-major block G_s of f_s x ns_s doubles holding
+//     (filled by the numeric factorization, wd_factor.hip, scheduled by chol_plan)
 //       G_s = [ L11^{-1} ; -L21 L11^{-1} ]
 //     so the forward solve is   [y_J ; u] = G_s c_s   and the backward solve is
 //     x_J = G_s^T [y_J ; x_R]  (both single dense products, no in-block recursion);
@@ -66,14 +67,5 @@ struct SymbolicOptions {
 // xyz: (n,3) coordinates used for the dissection.
 void chol_analyze(int n, const std::vector<int>& Ap, const std::vector<int>& Aj, const double* xyz,
                   const SymbolicOptions& opt, CholSymbolic& out);
-
-// Host reference numeric factorization into the G layout (multifrontal, dense
-// kernels in plain loops).  Ax: values aligned with Aj.  Returns false if a pivot is
-// not positive.
-bool chol_numeric_host(const CholSymbolic& S, const std::vector<int>& Ap, const std::vector<int>& Aj,
-                       const std::vector<double>& Ax, std::vector<double>& G);
-
-// Host reference solve with the G layout (b and x are (n,3) row-major, old numbering).
-void chol_solve_host(const CholSymbolic& S, const std::vector<double>& G, const double* b, double* x);
 
 }  // namespace wrapd

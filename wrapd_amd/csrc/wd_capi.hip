@@ -21,6 +21,8 @@ wrapd_ctx::~wrapd_ctx() {
     f(v2e_ptr); f(v2e_idx); f(a_cptr); f(a_cidx); f(Ax);
     f(G); f(ysol); f(xsol); f(ubuf); f(d_perm); f(d_iperm); f(d_col0); f(d_fsize); f(d_rowptr); f(d_rows);
     f(d_goff); f(d_uoff); f(d_asm_ptr); f(d_asm_idx); f(d_leaf_ptr); f(d_leaf_sn); f(d_fwd_tasks); f(d_bwd_tasks);
+    f(pool); f(d_foff); f(d_cbase); f(d_acol_ptr); f(d_acol); f(d_ccol_ptr); f(d_ccol); f(d_relptr); f(d_rel);
+    f(d_ftasks); f(d_ffail);
     f(partials); f(scal);
     if (scal_host) (void)hipHostFree(scal_host);
     for (auto& p : prof.pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -495,8 +497,6 @@ int wrapd_global_value_grad(wrapd_ctx* c, double* value, double* grad) {
 int wrapd_refactor(wrapd_ctx* c) {
     CTX_OK(c); MESH_OK(c);
     launch_assemble_A(c);
-    std::vector<double> Ax(c->nnzA);
-    if (d2h(c, Ax.data(), c->Ax, Ax.size())) return WRAPD_ERR_HIP;
     if (!c->analyzed) {
         std::vector<double> xyz(3ull * c->n_f);
         for (int v = 0; v < c->n_v; ++v) {
@@ -511,16 +511,23 @@ int wrapd_refactor(wrapd_ctx* c) {
             return WRAPD_ERR_STATE;
         }
         int st = chol_prepare_device(c);
+        if (!st) st = factor_prepare_device(c);
         if (st) return st;
         c->analyzed = true;
     }
-    if (!wrapd::chol_numeric_host(c->sym, c->Ap, c->Aj, Ax, c->G_host)) {
+    {
+        ProfScope ps(c, "refactor");
+        launch_factor(c);
+    }
+    int fail = 0;
+    if (check_hip(c, hipMemcpyAsync(&fail, c->d_ffail, sizeof(int), hipMemcpyDeviceToHost, c->stream), "factor status") ||
+        check_hip(c, hipStreamSynchronize(c->stream), "factor"))
+        return WRAPD_ERR_HIP;
+    if (fail) {
         c->last_error = "factorization: non-positive pivot (A not positive definite)";
         c->factored = false;
         return WRAPD_ERR_NOT_SPD;
     }
-    int st = chol_upload_factor(c);
-    if (st) return st;
     c->factored = true;
     return launch_status(c);
 }
@@ -556,6 +563,10 @@ int wrapd_chol_stats(wrapd_ctx* c, double* st) {
     st[4] = static_cast<double>(c->sym.leaf_ptr.size()) - 1;
     st[5] = static_cast<double>(c->sym.level_ptr.size()) - 1;
     st[6] = c->sym.g_entries;
+    st[7] = c->fplan.n_levels;
+    st[8] = static_cast<double>(c->fplan.launches.size());
+    st[9] = c->fplan.flops;
+    st[10] = static_cast<double>(c->fplan.pool_size);
     return WRAPD_OK;
 }
 

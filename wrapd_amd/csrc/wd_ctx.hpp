@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/wrapd_gpu.h"
+#include "chol_plan.hpp"
 #include "chol_symbolic.hpp"
 
 namespace wd {
@@ -117,7 +118,19 @@ struct wrapd_ctx {
     int* d_fwd_tasks = nullptr;  // per level: (sn, row block)
     int* d_bwd_tasks = nullptr;  // per level: (sn, col block)
     std::vector<int> fwd_task_ptr, bwd_task_ptr;
-    std::vector<double> G_host;
+    // numeric factorization (plan from chol_plan, fronts in one pool)
+    wrapd::FactorPlan fplan;
+    double* pool = nullptr;
+    int64_t* d_foff = nullptr;
+    int64_t* d_cbase = nullptr;
+    int64_t* d_acol_ptr = nullptr;
+    int* d_acol = nullptr;
+    int64_t* d_ccol_ptr = nullptr;
+    int* d_ccol = nullptr;
+    int64_t* d_relptr = nullptr;
+    int* d_rel = nullptr;
+    int* d_ftasks = nullptr;
+    int* d_ffail = nullptr;
 
     // reductions
     double* partials = nullptr;
@@ -175,8 +188,10 @@ void launch_armijo_x(wrapd_ctx* c, double alpha, const double* p, double* xbuf);
 int finalize_blocks_n(wrapd_ctx* c);
 // (wd_solve.hip)
 int chol_prepare_device(wrapd_ctx* c);
-int chol_upload_factor(wrapd_ctx* c);
 void launch_solve(wrapd_ctx* c, const double* b, double* x);      // n_f x 3 device arrays
+// (wd_factor.hip)
+int factor_prepare_device(wrapd_ctx* c);
+void launch_factor(wrapd_ctx* c);   // status flag in c->d_ffail
 
 int check_hip(wrapd_ctx* c, hipError_t e, const char* what);
 

@@ -297,11 +297,6 @@ int chol_prepare_device(wrapd_ctx* c) {
     return rc ? WRAPD_ERR_HIP : WRAPD_OK;
 }
 
-int chol_upload_factor(wrapd_ctx* c) {
-    return check_hip(c, hipMemcpyAsync(c->G, c->G_host.data(), c->G_host.size() * sizeof(double),
-                                       hipMemcpyHostToDevice, c->stream), "upload G") ? WRAPD_ERR_HIP : WRAPD_OK;
-}
-
 void launch_solve(wrapd_ctx* c, const double* b, double* x) {
     const wrapd::CholSymbolic& S = c->sym;
     SolveMaps M{c->d_col0, c->d_fsize, c->d_rowptr, c->d_rows, c->d_goff, c->d_uoff, c->d_asm_ptr, c->d_asm_idx,
