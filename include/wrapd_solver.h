@@ -32,6 +32,7 @@ typedef struct {
     int prox_mode;            /* WRAPD_PROX_* */
     double newton_rtol;
     double fallback_sigma;
+    double fp64_switch;       /* WRAPD_FP32: FP64 local steps once |dE|/(|E|+1) < this (sticky) */
     int soln_provided;
     int verbose;
     int device;
@@ -56,8 +57,9 @@ int wrapd_solver_num_iters(wrapd_solver* s);
 int wrapd_solver_get_log(wrapd_solver* s, const char* name, double* out, int max_len);
 int wrapd_solver_get_x(wrapd_solver* s, double* X);           /* (n_verts,3) after finish     */
 int wrapd_solver_get_x_free(wrapd_solver* s, double* x);      /* current free positions       */
-/* [setup_ms, local_ms, global_ms, refactor_ms, inner_iters, finish_ms, prox_iters, fallback] */
-int wrapd_solver_get_runtime(wrapd_solver* s, double* out8);
+/* [setup_ms, local_ms, global_ms, refactor_ms, inner_iters, finish_ms, prox_iters, fallback,
+ *  learned_accepted, fp64_switch_iter (-1: none)] */
+int wrapd_solver_get_runtime(wrapd_solver* s, double* out10);
 wrapd_ctx* wrapd_solver_ctx(wrapd_solver* s);
 
 #ifdef __cplusplus

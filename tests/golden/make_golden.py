@@ -93,6 +93,14 @@ def kat_hess(rng):
     return dict(inputs=rows, kmax=out[:, 0])
 
 
+def perturbed(prob, seed=7, rel=1e-15):
+    """The same problem with the initial positions moved by ~1e-15 relative (round-off size)."""
+    rng = np.random.default_rng(seed)
+    init = prob.init * (1.0 + rel * rng.uniform(-1.0, 1.0, prob.init.shape))
+    return meshgen.Problem(prob.name, prob.rest, init, prob.tets, prob.pins, prob.model, prob.youngs, prob.poisson,
+                           prob.youngs_per_elem, dict(prob.settings))
+
+
 def run_trace(prob, flags, youngs=None, threads=1):
     with tempfile.TemporaryDirectory() as d:
         p = meshgen.write_problem(prob, d)
@@ -113,15 +121,21 @@ def save_case(name, prob, flags, youngs=None, store_mesh=False, x_every=1):
     t8 = run_trace(prob, flags, youngs, threads=8)
     spread_e = float(np.max(np.abs(t["objectives"] - t8["objectives"])) / abs(t["objectives"][0]))
     spread_x = float(np.linalg.norm(t["x"] - t8["x"]) / np.linalg.norm(t["x"]))
+    # and its sensitivity to round-off: the run from initial positions perturbed by 1e-15
+    tp = run_trace(perturbed(prob), flags, youngs)
+    pert_e = float(np.max(np.abs(t["objectives"] - tp["objectives"])) / abs(t["objectives"][0]))
+    pert_x = float(np.linalg.norm(t["x"] - tp["x"]) / np.linalg.norm(t["x"]))
     arrs = dict(flags=np.array(" ".join(flags)), x_rows=np.arange(0, prob.n_verts, x_every),
-                spread_e=spread_e, spread_x=spread_x,
-                spread_inner_equal=bool(np.array_equal(t["inner_iters"], t8["inner_iters"])))
+                spread_e=spread_e, spread_x=spread_x, pert_e=pert_e, pert_x=pert_x,
+                spread_inner_equal=bool(np.array_equal(t["inner_iters"], t8["inner_iters"])),
+                pert_inner_equal=bool(np.array_equal(t["inner_iters"], tp["inner_iters"])))
     arrs.update({k: v for k, v in t.items() if k != "x"})
     arrs["x"] = t["x"][::x_every]
     if store_mesh:
         arrs.update(rest=prob.rest, init=prob.init, tets=prob.tets.astype(np.int32), pins=prob.pins.astype(np.int32))
     np.savez_compressed(os.path.join(HERE, f"trace_{name}.npz"), **arrs)
-    print(name, "iters", len(t["objectives"]) - 1, "E0", t["objectives"][0], "spread_e %.2e spread_x %.2e" % (spread_e, spread_x))
+    print(name, "iters", len(t["objectives"]) - 1, "E0", t["objectives"][0],
+          "spread_e %.2e spread_x %.2e pert_e %.2e pert_x %.2e" % (spread_e, spread_x, pert_e, pert_x), flush=True)
 
 
 def main():

@@ -1,7 +1,8 @@
 """The GPU solver against the reference's own outputs (tests/golden/), through the C ABI,
 and the `quasi` CLI on the reference's sample data.  Same bars as the oracle's golden tests:
 equal inner-iteration counts and reweighting iterations, energies within
-max(5e-9, 4 x reference thread spread) E_0, positions within max(1e-8, 4 x spread).
+max(5e-9, 4 s_E) E_0, positions within max(1e-8, 8 s_x), s being the reference's measured
+round-off spread of the same run (tests/test_oracle_golden.py).
 """
 import os
 import subprocess
@@ -61,16 +62,24 @@ def test_quasi_cli_untwist(tmp_path):
 
 
 def test_fp32_local_step_final_positions_within_1e6():
-    """FP32 local step (FP64 fallback) with a Newton prox reaches the FP64 fixed point."""
-    prob = meshgen.small_beam(20, 4, 4, seed=1, pin_far_end=False, youngs=1e6)
-    ref = Solver(prob, tol=0.0)
-    ref.run(300)
-    fast = Solver(prob, tol=0.0, precision=1, prox_mode=1)
-    fast.run(300)
+    """Mixed precision (FP32 local steps with a Newton prox, FP64 from residual 1e-8 on)
+    reaches the FP64 fixed point: both runs stop on the reference's own criterion
+    (|dE|/(|E|+1) < 1e-12, no flips) on a pushed and twisted beam (the oracle converges it
+    in 272 iterations); final positions and energy within 1e-6 of the FP64 run."""
+    prob = meshgen.small_beam(10, 3, 3, seed=9, twist_deg=90.0, axial=-0.2, youngs=1e6)
+    ref = Solver(prob, tol=1e-12)
+    n_ref = ref.run(3000)
+    fast = Solver(prob, tol=1e-12, precision=1, prox_mode=1, fp64_switch=1e-8)
+    n_fast = fast.run(3000)
+    assert n_ref < 3000 and n_fast < 3000
+    rt = fast.runtime()
+    assert 0 < rt["fp64_switch_iter"] < n_fast
     xr, xf = ref.x(), fast.x()
+    print(f"FP64 {n_ref} iterations, mixed {n_fast} (FP64 from {rt['fp64_switch_iter']:.0f}): "
+          f"x rel err {np.linalg.norm(xf - xr) / np.linalg.norm(xr):.2e}")
     assert np.linalg.norm(xf - xr) <= 1e-6 * np.linalg.norm(xr)
     er, ef = ref.log("objectives"), fast.log("objectives")
-    assert abs(ef[-1] - er[-1]) <= 1e-6 * abs(er[0])
+    assert abs(ef[-1] - er[-1]) <= 1e-6 * abs(er[-1])
 
 
 def test_newton_fp64_prox_same_fixed_point():

@@ -3,8 +3,11 @@ from the reference compiled in oracle/_ref, single-threaded).
 
 Bars: singular values and prox solutions to the precision their algorithms define;
 ADMM traces at equal iteration counts: identical inner-iteration counts and reweighting
-iterations, energies within 5e-9 E_0 (the reference's own 1- vs 8-thread spread on these
-beams is 1.25e-9 E_0), final positions within 1e-8 relative.
+iterations; energies within max(5e-9, 4 s_E) E_0 and final positions within
+max(1e-8, 8 s_x) relative, where s is the larger of two measured round-off spreads of
+the reference's own trajectory: 1 vs 8 OpenMP threads, and initial positions perturbed
+by 1e-15 relative.  Most traces have s_x below 1e-9; the kappa = 0.1 case amplifies
+round-off to s_x = 5e-6 (any independent implementation lands a few s_x away).
 """
 import os
 
@@ -70,9 +73,10 @@ def untwist_problem():
 
 
 def trace_bars(z):
-    """Energy (relative to E_0) and position bars: 5e-9 / 1e-8, or 4x the reference's own
-    1- vs 8-thread spread on the same run where that is larger."""
-    return max(5e-9, 4.0 * float(z["spread_e"])), max(1e-8, 4.0 * float(z["spread_x"]))
+    """Energy (relative to E_0) and position bars from the reference's round-off spreads."""
+    s_e = max(float(z["spread_e"]), float(z["pert_e"]))
+    s_x = max(float(z["spread_x"]), float(z["pert_x"]))
+    return max(5e-9, 4.0 * s_e), max(1e-8, 8.0 * s_x)
 
 
 def compare_trace(name, prob, over):

@@ -53,7 +53,7 @@ class SolverSettings(C.Structure):
                 ("global_max_iters", C.c_int), ("global_ls", C.c_int), ("kappa", C.c_double),
                 ("lbfgs_m", C.c_int), ("prox_iters", C.c_int), ("precision", C.c_int),
                 ("prox_mode", C.c_int), ("newton_rtol", C.c_double), ("fallback_sigma", C.c_double),
-                ("soln_provided", C.c_int), ("verbose", C.c_int), ("device", C.c_int)]
+                ("fp64_switch", C.c_double), ("soln_provided", C.c_int), ("verbose", C.c_int), ("device", C.c_int)]
 
 
 _PROTOS = {

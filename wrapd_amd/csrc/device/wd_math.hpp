@@ -27,6 +27,7 @@ struct M3 {
 template <typename T> WD_HD T wsqrt(T x) { return sqrt(x); }
 template <typename T> WD_HD T wlog(T x) { return log(x); }
 template <typename T> WD_HD T wabs(T x) { return x < T(0) ? -x : x; }
+template <typename T> WD_HD T wmax(T a, T b) { return a > b ? a : b; }
 
 template <typename T>
 WD_HD M3<T> eye3() {

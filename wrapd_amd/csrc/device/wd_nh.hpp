@@ -13,8 +13,10 @@
 //                 iterations (deps/mcloptlib/include/MCL/LBFGS.hpp:52-152,
 //                 Backtracking.hpp:111-175, src/TetElements.hpp:40-42).
 //   prox_newton - Newton on the same objective with the PSD-projected density Hessian
-//                 (src/TetElements.hpp:109-149) plus w^2/vol I and an Armijo backtrack;
-//                 stops on a gradient-norm tolerance relative to the objective's scale.
+//                 (src/TetElements.hpp:109-149) plus w^2/vol I; full steps once they are
+//                 small (where an Armijo test only sees rounding noise), Armijo
+//                 backtracking before that; stops on a gradient-norm tolerance relative
+//                 to the objective's scale or on a step below the resolution of x.
 #pragma once
 
 #include "wd_math.hpp"
@@ -321,9 +323,25 @@ WD_HD int prox_newton(const ProxProblem<T>& P, T x[3], int max_iters, T rtol, T&
                 d[0] = g[0] * inv; d[1] = g[1] * inv; d[2] = g[2] * inv;
             }
         }
-        // Armijo backtracking; infeasible points cost 1e15 through the barrier.  Near the
-        // minimiser the predicted decrease falls below the rounding of f itself, so the
-        // test allows a few ulps of |f| (otherwise the full Newton step is rejected there).
+        // Near the minimiser the predicted decrease g.d falls below the rounding error of f
+        // (f sums terms far larger than itself), so an Armijo test there only sees noise:
+        // steps of at most 1e-4 |x| (FP64) or 1e-2 |x| (FP32) are taken whole (Newton's
+        // quadratic region), and a step below the resolution of x ends the iteration.
+        const T xmax = wmax(T(1), wmax(wabs(x[0]), wmax(wabs(x[1]), wabs(x[2]))));
+        const T dmax = wmax(wabs(d[0]), wmax(wabs(d[1]), wabs(d[2])));
+        const T full_step = sizeof(T) == 4 ? T(1e-2) : T(1e-4);
+        if (dmax <= full_step * xmax && x[0] - d[0] > T(0) && x[1] - d[1] > T(0) && x[2] - d[2] > T(0)) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) x[c] -= d[c];
+            f = P.gradient(x, g);
+            if (dmax <= T(4) * Eps<T>::v * xmax) {
+                gnorm_out = wsqrt(dot3(g, g));
+                return it + 1;
+            }
+            continue;
+        }
+        // Armijo backtracking; infeasible points cost 1e15 through the barrier.  The test
+        // allows a few ulps of |f| for the same rounding reason.
         T a = T(1);
         const T gtd = dot3(g, d);
         const T fslack = T(8) * Eps<T>::v * wabs(f);

@@ -57,6 +57,7 @@ void wrapd_solver_default_settings(wrapd_solver_settings* s) {
     s->prox_mode = WRAPD_PROX_LBFGS;
     s->newton_rtol = 0.0;   // precision default
     s->fallback_sigma = 0.2;
+    s->fp64_switch = 1e-8;
     s->soln_provided = 0;
     s->verbose = 0;
     s->device = 0;
@@ -92,6 +93,7 @@ int wrapd_solver_create(wrapd_solver** out, const wrapd_solver_settings* cs, int
         st.m_prox_mode = cs->prox_mode;
         st.m_newton_rtol = cs->newton_rtol;
         st.m_fallback_sigma = cs->fallback_sigma;
+        st.m_fp64_switch = cs->fp64_switch;
         st.m_io.verbose(cs->verbose);
         if (cs->soln_provided) st.m_io.soln_mesh("given");
         st.m_device = cs->device;
@@ -220,6 +222,8 @@ int wrapd_solver_get_runtime(wrapd_solver* s, double* o) {
     o[5] = r.finish_solve_ms;
     o[6] = r.prox_iters;
     o[7] = r.fallback_elems;
+    o[8] = r.learned_accepted;
+    o[9] = r.fp64_switch_iter;
     return WRAPD_OK;
 }
 

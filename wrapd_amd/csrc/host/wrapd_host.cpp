@@ -89,7 +89,8 @@ bool Settings::parse_args(int argc, char** argv) {
             else if (s == "newton") m_prox_mode = WRAPD_PROX_NEWTON;
             else if (s == "learned") m_prox_mode = WRAPD_PROX_LEARNED;
             else throw std::runtime_error("Error: -prox must be lbfgs, newton or learned");
-        } else if (arg == "-device") { val >> m_device; }
+        } else if (arg == "-fp64_switch") { val >> m_fp64_switch; }
+        else if (arg == "-device") { val >> m_device; }
         else throw std::runtime_error("Error: Unrecognized command-line argument: " + arg);
     }
     return false;
@@ -126,6 +127,7 @@ std::string Settings::help_text() {
            " -prox_iters: prox iteration cap                            [Default: 20] \n"
            " -precision: local step arithmetic                          [Default: fp64]    (fp64, fp32 with fp64 fallback) \n"
            " -prox: stretch prox solver                                 [Default: lbfgs]   (lbfgs=reference, newton, learned) \n"
+           " -fp64_switch: fp32 runs switch to fp64 below this residual [Default: 1e-8] \n"
            " -device: HIP device index                                  [Default: 0] \n"
            " -out: output directory                                     [Default: output] \n"
            "==========================================\n";
@@ -193,7 +195,7 @@ void TetElements::initialize_dual_vars() {
 double TetElements::polar_update(bool update_cand) {
     // src/TetElements.cpp:256-356; the candidate-weight counter of :404-428
     wrapd_local_args a;
-    a.precision = m_settings.m_precision;
+    a.precision = m_fp64_tail ? WRAPD_FP64 : m_settings.m_precision;
     a.prox_mode = m_settings.m_prox_mode;
     a.prox_iters = m_settings.m_model_settings.elastic_prox_iters();
     a.update_cand = update_cand ? 1 : 0;
@@ -477,6 +479,13 @@ void ADMM::setup_solve(math::MatX3& X) {
 void ADMM::iterate() {
     AlgorithmData& ad = *m_algorithm_data;
     const bool upd = m_settings.m_reweighting == Settings::Reweighting::ENABLED;
+    // mixed precision: once the objective residual is below fp64_switch, the remaining
+    // local steps run in FP64, so the run ends at the FP64 fixed point
+    if (m_settings.m_precision == WRAPD_FP32 && ad.m_runtime.fp64_switch_iter < 0 && ad.m_iter > 0 &&
+        ad.current_normalized_objective_residual() < m_settings.m_fp64_switch) {
+        m_system->tet_elements()->set_fp64_tail(true);
+        ad.m_runtime.fp64_switch_iter = ad.m_iter;
+    }
     m_micro_timer.reset();
     if (m_settings.m_rot_awareness == Settings::RotAwareness::DISABLED) {
         m_system->local_update(upd);

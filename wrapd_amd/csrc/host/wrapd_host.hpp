@@ -144,6 +144,7 @@ struct Settings {
     int m_prox_mode = WRAPD_PROX_LBFGS;
     double m_newton_rtol = 0.0;      // 0: 1e-12 (FP64) / 1e-6 (FP32)
     double m_fallback_sigma = 0.2;
+    double m_fp64_switch = 1e-8;     // mixed precision: FP64 local steps below this residual
     std::vector<double> m_mu_e, m_lambda_e;   // optional per-element material
 };
 
@@ -154,6 +155,7 @@ struct RuntimeData {
     double finish_solve_ms = 0;
     // device-path extras
     double prox_iters = 0, fallback_elems = 0, learned_accepted = 0;
+    double fp64_switch_iter = -1;   // first iteration of the FP64 tail of a mixed run
 };
 
 class MicroTimer {
@@ -198,6 +200,7 @@ class TetElements {
     double x_distance();
     void set_x_ref(const math::MatX3& x);
     const wrapd_local_out& last_local() const { return m_last_local; }
+    void set_fp64_tail(bool on) { m_fp64_tail = on; }
 
     double m_curr_pe = 0.0;
     int m_cand_count = 0;       // src/TetElements.cpp:404 (function-static counter in the reference)
@@ -210,6 +213,7 @@ class TetElements {
     double m_energy = 0.0;
     int m_flips = 0;
     wrapd_local_out m_last_local{};
+    bool m_fp64_tail = false;   // mixed precision: FP64 local steps from here on
 };
 
 template <typename IN_SCALAR, typename TYPE>

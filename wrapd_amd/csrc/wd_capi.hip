@@ -231,7 +231,6 @@ int wrapd_upload_mesh(wrapd_ctx* c, int n_v, const double* rest, int n_e, const 
     std::vector<int> slot_of;   // per (e, a, b) in key order
     slot_of.reserve(keys.size());
     {
-        size_t t = 0;
         for (int e = 0; e < n_e; ++e)
             for (int a = 0; a < 4; ++a) {
                 const int fa = c->free_idx[tets[4 * e + a]];
@@ -244,7 +243,6 @@ int wrapd_upload_mesh(wrapd_ctx* c, int n_v, const double* rest, int n_e, const 
                     const int p = c->Ap[fa] + static_cast<int>(std::lower_bound(row, rend, fb) - row);
                     slot_of.push_back(p);
                     ccnt[p + 1]++;
-                    ++t;
                 }
             }
     }

@@ -115,8 +115,9 @@ struct wrapd_ctx {
     int64_t* d_asm_idx = nullptr;
     int* d_leaf_ptr = nullptr;
     int* d_leaf_sn = nullptr;
-    int* d_fwd_tasks = nullptr;  // per level: (sn, row block)
-    int* d_bwd_tasks = nullptr;  // per level: (sn, col block)
+    int* d_fwd_tasks = nullptr;  // per level: (sn, block of fwd_rows[l] rows)
+    int* d_bwd_tasks = nullptr;  // per level: (sn, block of 4 bwd_cpw[l] columns)
+    std::vector<int> fwd_rows, bwd_cpw;
     std::vector<int> fwd_task_ptr, bwd_task_ptr;
     // numeric factorization (plan from chol_plan, fronts in one pool)
     wrapd::FactorPlan fplan;

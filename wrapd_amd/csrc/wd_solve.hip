@@ -9,16 +9,21 @@
 // gathered by the parent from precomputed lists (deterministic, no atomics).
 //
 // Schedule: small supernodes are grouped into leaf subtrees, each walked by one
-// workgroup in postorder; the remaining supernodes run level by level with
-// (supernode, row block) tasks forward and (supernode, column block) tasks backward.
+// workgroup in postorder; the remaining supernodes run level by level.  A level with
+// a few wide supernodes must still fill 256 CUs, so each level picks its task size:
+//   forward  tasks (supernode, block of R rows), R in {256, 128, 64, 32}: the
+//            256 threads are 256/R column groups, reduced in LDS in a fixed order;
+//   backward tasks (supernode, block of 4 CPW columns), CPW columns per wave in {4, 2, 1}.
+// No task depends on another task of its launch (no cross-workgroup reduction: on this
+// 8-XCD part a device-scope fence costs an L2 writeback).
 #include "wd_ctx.hpp"
 #include "device/wd_reduce.hpp"
 
 namespace wd {
 
 constexpr int kSolveThreads = 256;
-constexpr int kBwdCols = 16;          // columns per backward task (4 per wave)
 constexpr int kBwdChunk = 2048;       // rows of [y_J; x_R] staged in LDS at a time
+constexpr int kLevelTasks = 512;      // task count a level aims for
 constexpr int kNsMax = 1024;          // supernode width limit (symbolic ns_max)
 constexpr int kSmallFront = 256;      // leaf-subtree front limit (symbolic small_front)
 
@@ -94,10 +99,13 @@ k_fwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, So
     }
 }
 
+// forward: task (s, row block rb) with R rows; thread (g, r) = (tid / R, tid % R) sums
+// columns j = g, g + 256/R, ... of row r
 __global__ void __launch_bounds__(kSolveThreads)
-k_fwd_level(const int2* __restrict__ tasks, SolveMaps M, const double* __restrict__ b, const double* __restrict__ G,
-            double* __restrict__ ysol, double* __restrict__ ubuf) {
+k_fwd_level(const int2* __restrict__ tasks, int R, SolveMaps M, const double* __restrict__ b,
+            const double* __restrict__ G, double* __restrict__ ysol, double* __restrict__ ubuf) {
     __shared__ double cv[kNsMax * 3];
+    __shared__ double red[kSolveThreads * 3];
     const int2 task = tasks[blockIdx.x];
     const int s = task.x, rb = task.y;
     const int c0 = M.col0[s];
@@ -113,17 +121,44 @@ k_fwd_level(const int2* __restrict__ tasks, SolveMaps M, const double* __restric
         cv[3 * j + 2] = a[2];
     }
     __syncthreads();
-    const int r = rb * kSolveThreads + threadIdx.x;
-    if (r >= f) return;
-    const double* Gs = G + M.goff[s];
+    const int ng = kSolveThreads / R;
+    const int g = threadIdx.x / R, rl = threadIdx.x - g * R;
+    const int r = rb * R + rl;
     double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-    const int jend = r < ns ? r + 1 : ns;
-    for (int j = 0; j < jend; ++j) {
-        const double g = Gs[static_cast<int64_t>(j) * f + r];
-        t0 += g * cv[3 * j + 0];
-        t1 += g * cv[3 * j + 1];
-        t2 += g * cv[3 * j + 2];
+    if (r < f) {
+        const double* Gs = G + M.goff[s] + r;
+        const int jend = r < ns ? r + 1 : ns;   // L11^{-1} is lower triangular
+        int j = g;
+        for (; j + 3 * ng < jend; j += 4 * ng) {
+            const double g0 = Gs[static_cast<int64_t>(j) * f];
+            const double g1 = Gs[static_cast<int64_t>(j + ng) * f];
+            const double g2 = Gs[static_cast<int64_t>(j + 2 * ng) * f];
+            const double g3 = Gs[static_cast<int64_t>(j + 3 * ng) * f];
+            t0 += g0 * cv[3 * j + 0] + g1 * cv[3 * (j + ng) + 0] + g2 * cv[3 * (j + 2 * ng) + 0] + g3 * cv[3 * (j + 3 * ng) + 0];
+            t1 += g0 * cv[3 * j + 1] + g1 * cv[3 * (j + ng) + 1] + g2 * cv[3 * (j + 2 * ng) + 1] + g3 * cv[3 * (j + 3 * ng) + 1];
+            t2 += g0 * cv[3 * j + 2] + g1 * cv[3 * (j + ng) + 2] + g2 * cv[3 * (j + 2 * ng) + 2] + g3 * cv[3 * (j + 3 * ng) + 2];
+        }
+        for (; j < jend; j += ng) {
+            const double gv = Gs[static_cast<int64_t>(j) * f];
+            t0 += gv * cv[3 * j + 0];
+            t1 += gv * cv[3 * j + 1];
+            t2 += gv * cv[3 * j + 2];
+        }
     }
+    if (ng > 1) {
+        red[3 * threadIdx.x + 0] = t0;
+        red[3 * threadIdx.x + 1] = t1;
+        red[3 * threadIdx.x + 2] = t2;
+        __syncthreads();
+        if (g != 0) return;
+        for (int k = 1; k < ng; ++k) {
+            const double* o = red + 3 * (k * R + rl);
+            t0 += o[0];
+            t1 += o[1];
+            t2 += o[2];
+        }
+    }
+    if (r >= f) return;
     if (r < ns) {
         double* y = ysol + 3ll * (c0 + r);
         y[0] = t0; y[1] = t1; y[2] = t2;
@@ -137,7 +172,8 @@ k_fwd_level(const int2* __restrict__ tasks, SolveMaps M, const double* __restric
     }
 }
 
-// backward, one supernode column block per workgroup: x_J = G^T [y_J; x_R]
+// backward: task (s, column block cb) of 4 * CPW columns, CPW per wave: x_J = G^T [y_J; x_R]
+template <int CPW>
 __global__ void __launch_bounds__(kSolveThreads)
 k_bwd_level(const int2* __restrict__ tasks, SolveMaps M, const double* __restrict__ G,
             const double* __restrict__ ysol, double* __restrict__ xsol) {
@@ -150,10 +186,11 @@ k_bwd_level(const int2* __restrict__ tasks, SolveMaps M, const double* __restric
     const int64_t rp = M.rowptr[s];
     const double* Gs = G + M.goff[s];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double acc[4][3];
+    const int jfirst = cb * 4 * CPW;
+    double acc[CPW][3];
 #pragma unroll
-    for (int jj = 0; jj < 4; ++jj) acc[jj][0] = acc[jj][1] = acc[jj][2] = 0.0;
-    for (int r0 = 0; r0 < f; r0 += kBwdChunk) {
+    for (int jj = 0; jj < CPW; ++jj) acc[jj][0] = acc[jj][1] = acc[jj][2] = 0.0;
+    for (int r0 = jfirst - jfirst % 64; r0 < f; r0 += kBwdChunk) {   // rows above jfirst are zero
         const int rc = min(kBwdChunk, f - r0);
         for (int r = threadIdx.x; r < rc; r += kSolveThreads) {
             const int row = r0 + r;
@@ -164,24 +201,24 @@ k_bwd_level(const int2* __restrict__ tasks, SolveMaps M, const double* __restric
         }
         __syncthreads();
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-            const int j = cb * kBwdCols + w * 4 + jj;
+        for (int jj = 0; jj < CPW; ++jj) {
+            const int j = jfirst + w * CPW + jj;
             if (j < ns) {
                 const double* gc = Gs + static_cast<int64_t>(j) * f + r0;
                 const int rstart = j > r0 ? j - r0 : 0;   // rows above the diagonal are zero
                 for (int r = rstart + lane; r < rc; r += 64) {
-                    const double g = gc[r];
-                    acc[jj][0] += g * vv[3 * r + 0];
-                    acc[jj][1] += g * vv[3 * r + 1];
-                    acc[jj][2] += g * vv[3 * r + 2];
+                    const double gv = gc[r];
+                    acc[jj][0] += gv * vv[3 * r + 0];
+                    acc[jj][1] += gv * vv[3 * r + 1];
+                    acc[jj][2] += gv * vv[3 * r + 2];
                 }
             }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int jj = 0; jj < 4; ++jj) {
-        const int j = cb * kBwdCols + w * 4 + jj;
+    for (int jj = 0; jj < CPW; ++jj) {
+        const int j = jfirst + w * CPW + jj;
         const double a0 = wave_sum(acc[jj][0]);
         const double a1 = wave_sum(acc[jj][1]);
         const double a2 = wave_sum(acc[jj][2]);
@@ -270,18 +307,37 @@ int chol_prepare_device(wrapd_ctx* c) {
     rc |= up(c, c->d_asm_idx, S.asm_idx);
     rc |= up(c, c->d_leaf_ptr, S.leaf_ptr);
     rc |= up(c, c->d_leaf_sn, S.leaf_sn);
-    // level task lists
+    // level task lists: forward (s, row block) with R rows per task, backward
+    // (s, column block) with 4 CPW columns per task, both chosen per level
     std::vector<int> ft, bt;
     c->fwd_task_ptr.assign(1, 0);
     c->bwd_task_ptr.assign(1, 0);
+    c->fwd_rows.clear();
+    c->bwd_cpw.clear();
     const int nlev = static_cast<int>(S.level_ptr.size()) - 1;
     for (int l = 0; l < nlev; ++l) {
-        for (int q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
-            const int s = S.level_sn[q];
-            const int ns = S.col0[s + 1] - S.col0[s];
-            for (int rb = 0; rb * kSolveThreads < S.fsize[s]; ++rb) { ft.push_back(s); ft.push_back(rb); }
-            for (int cb = 0; cb * kBwdCols < ns; ++cb) { bt.push_back(s); bt.push_back(cb); }
+        int R = 256, cpw = 4;
+        for (; R > 32; R /= 2) {
+            int64_t n = 0;
+            for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) n += (S.fsize[S.level_sn[i]] + R - 1) / R;
+            if (n >= kLevelTasks) break;
         }
+        for (; cpw > 1; cpw /= 2) {
+            int64_t n = 0;
+            for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) {
+                const int sn = S.level_sn[i];
+                n += (S.col0[sn + 1] - S.col0[sn] + 4 * cpw - 1) / (4 * cpw);
+            }
+            if (n >= kLevelTasks) break;
+        }
+        for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) {
+            const int sn = S.level_sn[i];
+            const int ns = S.col0[sn + 1] - S.col0[sn];
+            for (int rb = 0; rb * R < S.fsize[sn]; ++rb) { ft.push_back(sn); ft.push_back(rb); }
+            for (int cb = 0; cb * 4 * cpw < ns; ++cb) { bt.push_back(sn); bt.push_back(cb); }
+        }
+        c->fwd_rows.push_back(R);
+        c->bwd_cpw.push_back(cpw);
         c->fwd_task_ptr.push_back(static_cast<int>(ft.size() / 2));
         c->bwd_task_ptr.push_back(static_cast<int>(bt.size() / 2));
     }
@@ -313,17 +369,22 @@ void launch_solve(wrapd_ctx* c, const double* b, double* x) {
         for (int l = 0; l < nlev; ++l) {
             const int n = c->fwd_task_ptr[l + 1] - c->fwd_task_ptr[l];
             if (n > 0)
-                hipLaunchKernelGGL(k_fwd_level, dim3(n), dim3(kSolveThreads), 0, c->stream, ft + c->fwd_task_ptr[l], M,
-                                   b, c->G, c->ysol, c->ubuf);
+                hipLaunchKernelGGL(k_fwd_level, dim3(n), dim3(kSolveThreads), 0, c->stream, ft + c->fwd_task_ptr[l],
+                                   c->fwd_rows[l], M, b, c->G, c->ysol, c->ubuf);
         }
     }
     {
         ProfScope ps(c, "solve_bwd");
         for (int l = nlev - 1; l >= 0; --l) {
             const int n = c->bwd_task_ptr[l + 1] - c->bwd_task_ptr[l];
-            if (n > 0)
-                hipLaunchKernelGGL(k_bwd_level, dim3(n), dim3(kSolveThreads), 0, c->stream, bt + c->bwd_task_ptr[l], M,
-                                   c->G, c->ysol, c->xsol);
+            if (n == 0) continue;
+            const int2* t = bt + c->bwd_task_ptr[l];
+            if (c->bwd_cpw[l] == 4)
+                hipLaunchKernelGGL(k_bwd_level<4>, dim3(n), dim3(kSolveThreads), 0, c->stream, t, M, c->G, c->ysol, c->xsol);
+            else if (c->bwd_cpw[l] == 2)
+                hipLaunchKernelGGL(k_bwd_level<2>, dim3(n), dim3(kSolveThreads), 0, c->stream, t, M, c->G, c->ysol, c->xsol);
+            else
+                hipLaunchKernelGGL(k_bwd_level<1>, dim3(n), dim3(kSolveThreads), 0, c->stream, t, M, c->G, c->ysol, c->xsol);
         }
         if (nleaf > 0)
             hipLaunchKernelGGL(k_bwd_leaf, dim3(nleaf), dim3(kSolveThreads), 0, c->stream, c->d_leaf_ptr, c->d_leaf_sn,

@@ -143,10 +143,10 @@ class Solver:
         return x
 
     def runtime(self):
-        o = np.zeros(8)
+        o = np.zeros(10)
         lib().wrapd_solver_get_runtime(self.h, dp(o))
         keys = ["setup_ms", "local_ms", "global_ms", "refactor_ms", "inner_iters", "finish_ms", "prox_iters",
-                "fallback_elems"]
+                "fallback_elems", "learned_accepted", "fp64_switch_iter"]
         return dict(zip(keys, o.tolist()))
 
     def device(self):
