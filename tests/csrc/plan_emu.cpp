@@ -141,6 +141,21 @@ extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
                     for (int j = 0; j < nt; ++j)
                         for (int r = 0; r < nb; ++r) wrG(Gi(c0 + r, t0 + j), X[j * 64 + r]);
                 }
+            } else if (L.kind == FK_FINAL) {
+                const int kind = y >> kTaskKindShift;
+                const int a = (y >> kTaskTileShift) & kTaskTileMask, b = y & kTaskTileMask;
+                const int P = tile_count_l11(ns);
+                int a0, na, b0, nbt;
+                tile(a, ns, f, a0, na);
+                tile(b, ns, f, b0, nbt);
+                for (int j = 0; j < nbt; ++j)
+                    for (int i = (kind == 0 && a == b) ? j : 0; i < na; ++i) {
+                        double acc = 0;
+                        for (int q = kind == 0 ? 0 : b * kFactorTile; q < P * kFactorTile && q < ns; ++q)
+                            acc += rdF(Fi(a0 + i, q)) * (kind == 0 ? rdF(Fi(b0 + j, q)) : rdG(Gi(q, b0 + j)));
+                        if (kind == 0) wrF(Fi(a0 + i, b0 + j), rdF(Fi(a0 + i, b0 + j)) - acc);
+                        else wrG(Gi(a0 + i, b0 + j), -acc);
+                    }
             } else {
                 const int kind = y >> kTaskKindShift;
                 const int a = (y >> kTaskTileShift) & kTaskTileMask, b = y & kTaskTileMask;

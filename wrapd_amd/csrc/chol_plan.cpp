@@ -183,19 +183,35 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P) {
             }
             begin(FK_UPDATE, k);
             for (int s : L) {
-                if (k >= tile_count_l11(ns[s])) continue;
+                const int p11 = tile_count_l11(ns[s]);
+                if (k >= p11) continue;
                 const int nt = tile_count(ns[s], f[s]);
-                for (int a = k + 1; a < nt; ++a)
-                    for (int b = k + 1; b <= a; ++b) {
+                for (int b = k + 1; b < p11; ++b)
+                    for (int a = b; a < nt; ++a) {
                         push(s, (a << kTaskTileShift) | b);
                         P.flops += 2 * T * T * T;
                     }
-                for (int i = k + 1; i < nt; ++i)
+                for (int i = k + 1; i < p11; ++i)
                     for (int c = 0; c <= k; ++c) {
                         push(s, (1 << kTaskKindShift) | (i << kTaskTileShift) | c);
                         P.flops += 2 * T * T * T;
                     }
             }
+        }
+        begin(FK_FINAL, 0);
+        for (int s : L) {
+            const int p11 = tile_count_l11(ns[s]);
+            const int nt = tile_count(ns[s], f[s]);
+            for (int a = p11; a < nt; ++a)
+                for (int b = p11; b <= a; ++b) {
+                    push(s, (a << kTaskTileShift) | b);
+                    P.flops += 2 * T * T * T * p11;
+                }
+            for (int a = p11; a < nt; ++a)
+                for (int c = 0; c < p11; ++c) {
+                    push(s, (1 << kTaskKindShift) | (a << kTaskTileShift) | c);
+                    P.flops += 2 * T * T * T * (p11 - c);
+                }
         }
     }
     // drop empty launches (a level whose panel list is empty for a kind)

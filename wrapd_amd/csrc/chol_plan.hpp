@@ -8,15 +8,19 @@
 //      adds the children's Schur-complement columns (children in index order, so every
 //      entry is summed in a fixed order, no atomics);
 //   2. per 64-column panel k of the supernode columns, three batched launches:
-//        POTRF         factor the panel's diagonal tile L_kk
+//        POTRF         factor the panel's diagonal tile L_kk (and invert it)
 //        TRSM + GDIAG  L[t, k] = F[t, k] L_kk^{-T} for row tiles t > k, and
 //                      X[k, c] = L_kk^{-1} X[k, c] for column tiles c <= k
-//        SYRK + GUPD   F[a, b] -= L[a, k] L[b, k]^T for lower tiles a >= b > k
-//                      (remaining supernode columns and the Schur complement), and
-//                      X[i, c] -= L[i, k] X[k, c] for row tiles i > k, c <= k
+//        SYRK + GUPD   F[a, b] -= L[a, k] L[b, k]^T for tiles a >= b, k < b < P (the
+//                      supernode's remaining columns only), and X[i, c] -= L[i, k] X[k, c]
+//                      for L11 row tiles k < i < P, c <= k
+//      after which X = L11^{-1} (the top of G_s);
+//   3. one launch for the rest, each tile a single pass over all ns columns:
+//        SCHUR         F[a, b] -= sum_q L[a, q] L[b, q]^T for update tiles a >= b >= P
+//        GBOT          G[a, c] = -sum_q L[a, q] X[q, c] for update rows a >= P, c < P
+//      so the Schur complement is read and written once (not once per panel).
 //      Every task of a launch writes its own tile and reads only tiles that no task of
-//      the same launch writes.  X solves [L11 0; L21 I] X = [I; 0], i.e. X is the solve layout
-//      G_s = [L11^{-1}; -L21 L11^{-1}] (f_s x ns_s, column-major).
+//      the same launch writes.  G_s = [L11^{-1}; -L21 L11^{-1}] (f_s x ns_s, column-major).
 // Front tiles: the ns columns are cut into P = ceil(ns/64) tiles from 0, the m = f - ns
 // update rows into Q = ceil(m/64) tiles from ns; tile t < P starts at 64 t, tile t >= P
 // at ns + 64 (t - P).  Fronts are f x f column-major doubles in one pool; a front's
@@ -30,12 +34,12 @@
 
 namespace wrapd {
 
-enum FactorKind { FK_ASSEMBLE = 0, FK_POTRF = 1, FK_TRSM = 2, FK_UPDATE = 3 };
+enum FactorKind { FK_ASSEMBLE = 0, FK_POTRF = 1, FK_TRSM = 2, FK_UPDATE = 3, FK_FINAL = 4 };
 
 // task pair (s, y): FK_ASSEMBLE y = front column j; FK_POTRF y = 0;
 // FK_TRSM y = (kind << 30) | t with kind 0 TRSM (row tile t), 1 GDIAG (column tile t);
-// FK_UPDATE y = (kind << 30) | (a << 15) | b with kind 0 SYRK (tiles a, b), 1 GUPD (row
-// tile a, column tile b).
+// FK_UPDATE / FK_FINAL y = (kind << 30) | (a << 15) | b with kind 0 SYRK / SCHUR
+// (tiles a, b), 1 GUPD / GBOT (row tile a, column tile b).
 constexpr int kTaskKindShift = 30;
 constexpr int kTaskTileShift = 15;
 constexpr int kTaskTileMask = (1 << 15) - 1;

@@ -7,10 +7,19 @@
 // the host.  Four kernels, one launch per (level, step):
 //   k_fassemble  one wave per front column: zero, store A entries, add children's
 //                Schur-complement columns in child order (deterministic, no atomics)
-//   k_fpotrf     one workgroup per supernode: Cholesky of the panel's diagonal tile in LDS
-//   k_ftrsm      TRSM tiles below the diagonal tile, or GDIAG tiles of X = L^{-1}
-//   k_fupdate    SYRK/GEMM trailing updates of F, or GEMM updates of X
-// All arithmetic is FP64.  The two LDS tiles of k_ftrsm/k_fupdate take 64 KB.
+//   k_fpotrf     one workgroup per supernode: Cholesky L_kk of the panel's diagonal tile
+//                and its inverse in LDS; L_kk goes to the tile's lower half, the strict
+//                lower part of L_kk^{-1} (transposed) to its otherwise unused upper half
+//   k_ftrsm      TRSM tiles F[t, k] <- F[t, k] L_kk^{-T}, or GDIAG tiles
+//                X[k, c] <- L_kk^{-1} X[k, c], both as products with the stored inverse
+//   k_fupdate    SYRK/GEMM updates of the supernode's own later columns, or GEMM
+//                updates of X = L11^{-1}
+//   k_ffinal     the Schur complement F[a, b] -= L21 L21^T and G's bottom -L21 L11^{-1},
+//                each output tile accumulated over all ns columns in registers
+// Every product runs on the matrix cores in 64x64 output tiles (v_mfma_f64_16x16x4:
+// each of the 4 waves owns a 32x32 quarter as 2x2 16x16 accumulators) over 64-deep
+// operand chunks staged in two 32 KB LDS tiles; the accumulator layout puts
+// consecutive lanes on consecutive rows, so tile stores are coalesced.  All FP64.
 #include "wd_ctx.hpp"
 #include "chol_plan.hpp"
 
@@ -82,8 +91,76 @@ k_fassemble(const int2* __restrict__ tasks, FactArgs A, const double* __restrict
 }
 
 // ---------------------------------------------------------------------------------
-// Cholesky of a diagonal tile held column-major in LDS (D[c * 64 + r], lower part)
-__device__ __forceinline__ void lds_potrf(double* D, int nb, int* fail) {
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// acc += sum_kk As[kk][i] Bs[kk][j], kk < nk (a multiple of 4, operands zero-padded).
+// v_mfma_f64_16x16x4 lane layout: A[row l%16][k l/16], B[k l/16][col l%16], D register q
+// of lane l = D[4q + l/16][l%16].  The product is formed as D = Bs^T-side x As-side, so
+// that D's column index (consecutive lanes) is the output row i.
+__device__ __forceinline__ void tile_zero(dbl4 acc[2][2]) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+}
+
+__device__ __forceinline__ void tile_mma(const double* __restrict__ As, const double* __restrict__ Bs, int nk,
+                                         dbl4 acc[2][2]) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int j0 = 32 * (w >> 1), i0 = 32 * (w & 1);
+    const int li = l & 15, lk = l >> 4;
+    for (int kk = 0; kk < nk; kk += 4) {
+        const int k = (kk + lk) * kT;
+        const double b0 = Bs[k + j0 + li], b1 = Bs[k + j0 + 16 + li];
+        const double a0 = As[k + i0 + li], a1 = As[k + i0 + 16 + li];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
+    }
+}
+
+// visits every accumulator entry: fn(i, j, value) with (i, j) in the 64x64 tile; within a
+// wave, lanes l and l+1 hold rows i and i+1 of the same column j
+template <typename Fn>
+__device__ __forceinline__ void tile_for_each(const dbl4 acc[2][2], Fn fn) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int j0 = 32 * (w >> 1), i0 = 32 * (w & 1);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) fn(i0 + 16 * b + (l & 15), j0 + 16 * a + 4 * q + (l >> 4), acc[a][b][q]);
+}
+
+// Linv[i][q] of panel (c0, nb) from the diagonal tile: 1/L_ii on the diagonal, the stored
+// transpose above it, zero below the diagonal of the inverse's upper part
+__device__ __forceinline__ double linv_at(const double* __restrict__ F, int f, int c0, int i, int q) {
+    if (q > i) return 0.0;
+    if (q == i) return 1.0 / F[static_cast<int64_t>(c0 + i) * f + c0 + i];
+    return F[static_cast<int64_t>(c0 + i) * f + c0 + q];
+}
+
+// ---------------------------------------------------------------------------------
+// Cholesky of the diagonal tile and the inverse of its factor, both in LDS
+// (column-major: D[c * 64 + r]); the inverse by substitution on the identity
+__global__ void __launch_bounds__(kFactThreads)
+k_fpotrf(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ fail) {
+    __shared__ double D[kT * kT];
+    __shared__ double X[kT * kT];
+    const int s = tasks[blockIdx.x].x;
+    const int f = A.fsize[s];
+    const int ns = A.col0[s + 1] - A.col0[s];
+    const int c0 = kT * k;
+    const int nb = min(kT, ns - c0);
+    double* F = A.pool + A.foff[s];
+    for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
+        const int c = idx >> 6, r = idx & 63;
+        D[idx] = (c < nb && r < nb && r >= c) ? F[static_cast<int64_t>(c0 + c) * f + c0 + r] : 0.0;
+        X[idx] = (c == r && c < nb) ? 1.0 : 0.0;
+    }
+    __syncthreads();
     const int r = threadIdx.x & 63, cg = threadIdx.x >> 6;
     for (int p = 0; p < nb; ++p) {
         if (threadIdx.x == 0) {
@@ -103,34 +180,31 @@ __device__ __forceinline__ void lds_potrf(double* D, int nb, int* fail) {
         }
         __syncthreads();
     }
-}
-
-__global__ void __launch_bounds__(kFactThreads)
-k_fpotrf(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ fail) {
-    __shared__ double D[kT * kT];
-    const int s = tasks[blockIdx.x].x;
-    const int f = A.fsize[s];
-    const int ns = A.col0[s + 1] - A.col0[s];
-    const int c0 = kT * k;
-    const int nb = min(kT, ns - c0);
-    double* F = A.pool + A.foff[s];
-    for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
-        const int c = idx >> 6, r = idx & 63;
-        D[idx] = (c < nb && r < nb && r >= c) ? F[static_cast<int64_t>(c0 + c) * f + c0 + r] : 0.0;
+    // X = L^{-1}: row p of X is final after dividing by L_pp; then eliminate below
+    for (int p = 0; p < nb; ++p) {
+        if (threadIdx.x <= p) X[threadIdx.x * 64 + p] /= D[p * 65];
+        __syncthreads();
+        if (r > p && r < nb) {
+            const double l = D[p * 64 + r];
+            for (int j = cg; j <= p; j += 4) X[j * 64 + r] -= l * X[j * 64 + p];
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    lds_potrf(D, nb, fail);
     for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
-        const int c = idx >> 6, r = idx & 63;
-        if (c < nb && r < nb && r >= c) F[static_cast<int64_t>(c0 + c) * f + c0 + r] = D[idx];
+        const int c = idx >> 6, rr = idx & 63;
+        if (c >= nb || rr >= nb) continue;
+        double* dst = F + static_cast<int64_t>(c0 + c) * f + c0 + rr;
+        if (rr >= c) *dst = D[idx];                 // L_kk (lower, with the diagonal)
+        else *dst = X[rr * 64 + c];                 // (L_kk^{-1})[c][rr] above the diagonal
     }
 }
 
 // ---------------------------------------------------------------------------------
+// TRSM (kind 0): F[t, k] <- F[t, k] L_kk^{-T};  GDIAG (kind 1): X[k, c] <- L_kk^{-1} X[k, c]
 __global__ void __launch_bounds__(kFactThreads)
 k_ftrsm(const int2* __restrict__ tasks, FactArgs A, int k) {
-    __shared__ double D[kT * kT];
-    __shared__ double B[kT * kT];
+    __shared__ double As[kT * kT];   // As[q * 64 + i]
+    __shared__ double Bs[kT * kT];   // Bs[q * 64 + j]
     const int2 task = tasks[blockIdx.x];
     const int s = task.x;
     const int kind = task.y >> wrapd::kTaskKindShift;
@@ -139,63 +213,52 @@ k_ftrsm(const int2* __restrict__ tasks, FactArgs A, int k) {
     const int ns = A.col0[s + 1] - A.col0[s];
     const int c0 = kT * k;
     const int nb = min(kT, ns - c0);
-    const double* F = A.pool + A.foff[s];
-    for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
-        const int c = idx >> 6, r = idx & 63;
-        D[idx] = (c < nb && r < nb && r >= c) ? F[static_cast<int64_t>(c0 + c) * f + c0 + r] : 0.0;
-    }
-    const int r = threadIdx.x & 63, cg = threadIdx.x >> 6;
+    double* F = A.pool + A.foff[s];
+    double* Gs = A.G + A.goff[s];
     int t0, nt;
     front_tile(t, ns, f, t0, nt);
+    if (kind == 1 && t == k) {   // X[k, k] = L_kk^{-1}
+        for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
+            const int j = idx >> 6, i = idx & 63;
+            if (j < nb && i < nb) Gs[static_cast<int64_t>(c0 + j) * f + c0 + i] = linv_at(F, f, c0, i, j);
+        }
+        return;
+    }
     if (kind == 0) {
-        // rows [t0, t0+nt) of panel k: X L_kk^T = B
-        double* Fw = A.pool + A.foff[s];
+        // C[i][j] = sum_q B[i][q] Linv[j][q]
         for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
-            const int c = idx >> 6, rr = idx & 63;
-            B[idx] = (c < nb && rr < nt) ? Fw[static_cast<int64_t>(c0 + c) * f + t0 + rr] : 0.0;
-        }
-        __syncthreads();
-        for (int p = 0; p < nb; ++p) {
-            if (threadIdx.x < nt) B[p * 64 + threadIdx.x] /= D[p * 65];
-            __syncthreads();
-            if (r < nt) {
-                const double xp = B[p * 64 + r];
-                for (int c = p + 1 + cg; c < nb; c += 4) B[c * 64 + r] -= xp * D[p * 64 + c];
-            }
-            __syncthreads();
-        }
-        for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
-            const int c = idx >> 6, rr = idx & 63;
-            if (c < nb && rr < nt) Fw[static_cast<int64_t>(c0 + c) * f + t0 + rr] = B[idx];
+            const int q = idx >> 6, i = idx & 63;
+            As[idx] = (q < nb && i < nt) ? F[static_cast<int64_t>(c0 + q) * f + t0 + i] : 0.0;
+            Bs[idx] = (q < nb && i < nb) ? linv_at(F, f, c0, i, q) : 0.0;   // here i plays j
         }
     } else {
-        // X[k, t] = L_kk^{-1} X[k, t]: rows [c0, c0+nb), columns [t0, t0+nt) of G_s
-        double* Gs = A.G + A.goff[s];
+        // C[i][j] = sum_q Linv[i][q] X[q][j], X = G rows [c0, c0+nb), columns [t0, t0+nt)
         for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
-            const int j = idx >> 6, rr = idx & 63;
-            double v = 0.0;
-            if (j < nt && rr < nb) v = (t == k) ? (rr == j ? 1.0 : 0.0) : Gs[static_cast<int64_t>(t0 + j) * f + c0 + rr];
-            B[idx] = v;
-        }
-        __syncthreads();
-        for (int p = 0; p < nb; ++p) {
-            if (threadIdx.x < nt) B[threadIdx.x * 64 + p] /= D[p * 65];
-            __syncthreads();
-            if (r > p && r < nb) {
-                const double l = D[p * 64 + r];
-                for (int j = cg; j < nt; j += 4) B[j * 64 + r] -= l * B[j * 64 + p];
-            }
-            __syncthreads();
+            const int q = idx >> 6, i = idx & 63;
+            As[idx] = (q < nb && i < nb) ? linv_at(F, f, c0, i, q) : 0.0;
         }
         for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
-            const int j = idx >> 6, rr = idx & 63;
-            if (j < nt && rr < nb) Gs[static_cast<int64_t>(t0 + j) * f + c0 + rr] = B[idx];
+            const int j = idx >> 6, q = idx & 63;
+            Bs[q * 64 + j] = (q < nb && j < nt) ? Gs[static_cast<int64_t>(t0 + j) * f + c0 + q] : 0.0;
         }
+    }
+    __syncthreads();
+    dbl4 acc[2][2];
+    tile_zero(acc);
+    tile_mma(As, Bs, (nb + 3) & ~3, acc);
+    if (kind == 0) {
+        tile_for_each(acc, [&](int i, int j, double v) {
+            if (i < nt && j < nb) F[static_cast<int64_t>(c0 + j) * f + t0 + i] = v;
+        });
+    } else {
+        tile_for_each(acc, [&](int i, int j, double v) {
+            if (i < nb && j < nt) Gs[static_cast<int64_t>(t0 + j) * f + c0 + i] = v;
+        });
     }
 }
 
 // ---------------------------------------------------------------------------------
-// C[a, b] -= P_a P_b^T over the panel's nb columns; 16x16 threads, 4x4 outputs each
+// SYRK/GEMM (kind 0): F[a, b] -= L[a, k] L[b, k]^T;  GUPD (kind 1): X[i, c] -= L[i, k] X[k, c]
 __global__ void __launch_bounds__(kFactThreads)
 k_fupdate(const int2* __restrict__ tasks, FactArgs A, int k) {
     __shared__ double As[kT * kT];   // As[kk * 64 + i]
@@ -230,35 +293,84 @@ k_fupdate(const int2* __restrict__ tasks, FactArgs A, int k) {
         }
     }
     __syncthreads();
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    double acc[4][4];
-#pragma unroll
-    for (int ii = 0; ii < 4; ++ii)
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) acc[ii][jj] = 0.0;
-    for (int kk = 0; kk < nb; ++kk) {
-        double av[4], bv[4];
-#pragma unroll
-        for (int ii = 0; ii < 4; ++ii) av[ii] = As[kk * 64 + tx + 16 * ii];
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) bv[jj] = Bs[kk * 64 + ty + 16 * jj];
-#pragma unroll
-        for (int ii = 0; ii < 4; ++ii)
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj) acc[ii][jj] = fma(av[ii], bv[jj], acc[ii][jj]);
-    }
+    dbl4 acc[2][2];
+    tile_zero(acc);
+    tile_mma(As, Bs, (nb + 3) & ~3, acc);
     double* C = kind == 0 ? F : Gs;
+    const bool diag = kind == 0 && a == b;
+    tile_for_each(acc, [&](int i, int j, double v) {
+        if (i < na && j < nbt && !(diag && i < j)) C[static_cast<int64_t>(b0 + j) * f + a0 + i] -= v;
+    });
+}
+
+
+// ---------------------------------------------------------------------------------
+// SCHUR (kind 0): F[a, b] -= sum_q L[a, q] L[b, q]^T over all panels q < P, tiles a >= b >= P;
+// GBOT (kind 1): G[a, c] = -sum_{q >= c} L[a, q] X[q, c], X = L11^{-1}, rows a >= P
+__global__ void __launch_bounds__(kFactThreads)
+k_ffinal(const int2* __restrict__ tasks, FactArgs A) {
+    __shared__ double As[kT * kT];   // As[kk * 64 + i]
+    __shared__ double Bs[kT * kT];   // Bs[kk * 64 + j]
+    const int2 task = tasks[blockIdx.x];
+    const int s = task.x;
+    const int kind = task.y >> wrapd::kTaskKindShift;
+    const int a = (task.y >> wrapd::kTaskTileShift) & wrapd::kTaskTileMask;
+    const int b = task.y & wrapd::kTaskTileMask;
+    const int f = A.fsize[s];
+    const int ns = A.col0[s + 1] - A.col0[s];
+    const int P = (ns + kT - 1) / kT;
+    int a0, na, b0, nbt;
+    front_tile(a, ns, f, a0, na);
+    front_tile(b, ns, f, b0, nbt);
+    const double* F = A.pool + A.foff[s];
+    double* Fw = A.pool + A.foff[s];
+    double* Gs = A.G + A.goff[s];
+    // chunk q+1 is fetched into registers while chunk q runs on the matrix cores
+    constexpr int kPer = kT * kT / kFactThreads;   // 16 entries per thread and operand
+    double ra[kPer], rb[kPer];
+    auto fetch = [&](int q) {
+        const int q0 = kT * q;
+        const int nq = min(kT, ns - q0);
 #pragma unroll
-    for (int jj = 0; jj < 4; ++jj) {
-        const int j = ty + 16 * jj;
-        if (j >= nbt) continue;
-#pragma unroll
-        for (int ii = 0; ii < 4; ++ii) {
-            const int i = tx + 16 * ii;
-            if (i >= na) continue;
-            if (kind == 0 && a == b && i < j) continue;   // diagonal tiles: lower part only
-            C[static_cast<int64_t>(b0 + j) * f + a0 + i] -= acc[ii][jj];
+        for (int r = 0; r < kPer; ++r) {
+            const int idx = threadIdx.x + kFactThreads * r;
+            const int kk = idx >> 6, i = idx & 63;
+            ra[r] = (kk < nq && i < na) ? F[static_cast<int64_t>(q0 + kk) * f + a0 + i] : 0.0;
+            if (kind == 0) {
+                rb[r] = (kk < nq && i < nbt) ? F[static_cast<int64_t>(q0 + kk) * f + b0 + i] : 0.0;
+            } else {   // transposed: consecutive threads read consecutive rows kk of G
+                const int j = idx >> 6, k2 = idx & 63;
+                rb[r] = (k2 < nq && j < nbt) ? Gs[static_cast<int64_t>(b0 + j) * f + q0 + k2] : 0.0;
+            }
         }
+    };
+    dbl4 acc[2][2];
+    tile_zero(acc);
+    const int qfirst = kind == 0 ? 0 : b;
+    fetch(qfirst);
+    for (int q = qfirst; q < P; ++q) {
+        const int nq = min(kT, ns - kT * q);
+        __syncthreads();   // the previous chunk has been consumed
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const int idx = threadIdx.x + kFactThreads * r;
+            As[idx] = ra[r];
+            if (kind == 0) Bs[idx] = rb[r];
+            else Bs[(idx & 63) * 64 + (idx >> 6)] = rb[r];
+        }
+        __syncthreads();
+        if (q + 1 < P) fetch(q + 1);
+        tile_mma(As, Bs, (nq + 3) & ~3, acc);
+    }
+    if (kind == 0) {
+        const bool diag = a == b;
+        tile_for_each(acc, [&](int i, int j, double v) {
+            if (i < na && j < nbt && !(diag && i < j)) Fw[static_cast<int64_t>(b0 + j) * f + a0 + i] -= v;
+        });
+    } else {
+        tile_for_each(acc, [&](int i, int j, double v) {
+            if (i < na && j < nbt) Gs[static_cast<int64_t>(b0 + j) * f + a0 + i] = -v;
+        });
     }
 }
 
@@ -320,8 +432,11 @@ void launch_factor(wrapd_ctx* c) {
         case wrapd::FK_TRSM:
             hipLaunchKernelGGL(k_ftrsm, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A, L.panel);
             break;
-        default:
+        case wrapd::FK_UPDATE:
             hipLaunchKernelGGL(k_fupdate, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A, L.panel);
+            break;
+        default:
+            hipLaunchKernelGGL(k_ffinal, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A);
             break;
         }
     }
