@@ -59,6 +59,7 @@ typedef struct {
     double learned_accepted;    /* learned warm starts accepted without a Newton iteration   */
     double evals_total;         /* prox objective evaluations (line-search trials / Newton steps) */
     double evals_max;           /* largest per-element count                                 */
+    double learned_used;        /* elements whose Newton prox started from the learned guess */
 } wrapd_local_out;
 
 typedef struct {
@@ -163,6 +164,37 @@ int wrapd_set_elem(wrapd_ctx* ctx, const char* name, const double* in);
 int wrapd_profile_enable(wrapd_ctx* ctx, int on);
 int wrapd_profile_get(wrapd_ctx* ctx, const char* kernel, double* total_ms, double* launches);
 int wrapd_synchronize(wrapd_ctx* ctx);
+
+/* ---- learned prox warm start ----------------------------------------------------------
+ * An extension of the stretch prox (src/TetElements.cpp:302-318, solved there by
+ * LBFGS<double,3,8>): no counterpart in the reference.  An MLP 6 -> 64 -> 64 -> 3 (GELU,
+ * BF16 matrix-core products, FP32 accumulation) maps (sorted s_hat, w^2/vol, mu, lambda)
+ * to a guess of the prox solution; with WRAPD_PROX_LEARNED the local step starts its
+ * Newton prox from that guess when its optimality residual beats the previous solution's.
+ * Parameters: 4803 floats W1[64][6] b1[64] W2[64][64] b2[64] W3[3][64] b3[3];
+ * normalisation: 18 floats in_mean[6] in_std[6] out_mean[3] out_std[3] (features s0 s1 s2
+ * log(w^2/vol/mu) log(lambda/mu) log(mu); outputs x - s_hat).
+ * Tuples: 9 floats s0 s1 s2 w^2/vol mu lambda | x0 x1 x2. */
+int wrapd_mlp_init(wrapd_ctx* ctx, unsigned long long seed);            /* Glorot weights, identity norm */
+int wrapd_mlp_set_params(wrapd_ctx* ctx, const float* params4803, const float* norm18);
+int wrapd_mlp_get_params(wrapd_ctx* ctx, float* params4803, float* norm18);
+/* in: n x 6 (s0 s1 s2 w^2/vol mu lambda), out: n x 3 predicted prox solutions (host arrays) */
+int wrapd_mlp_infer(wrapd_ctx* ctx, const float* in, int n, float* out);
+/* mean squared error (standardised targets) and its gradient on the n given tuples */
+int wrapd_mlp_grad(wrapd_ctx* ctx, const float* tuples, int n, float* grad4803, float* loss);
+/* tuple log: while started, every NH local step appends its n_e (input, solution) tuples */
+int wrapd_mlp_log_start(wrapd_ctx* ctx, long long capacity);
+int wrapd_mlp_log_stop(wrapd_ctx* ctx);
+long long wrapd_mlp_log_count(wrapd_ctx* ctx);
+int wrapd_mlp_log_get(wrapd_ctx* ctx, long long offset, long long n, float* out);
+int wrapd_mlp_log_set(wrapd_ctx* ctx, const float* tuples, long long n);
+/* normalisation from the logged tuples (a strided sample of at most 1e6) */
+int wrapd_mlp_fit_norm(wrapd_ctx* ctx);
+/* w^2/vol redrawn log-uniformly over [w_lo, w_hi] (<= 0: the logged range) and the
+ * targets re-solved by the FP64 Newton prox */
+int wrapd_mlp_augment(wrapd_ctx* ctx, double w_lo, double w_hi, unsigned long long seed);
+/* Adam steps on minibatches drawn from the log; loss_hist (steps floats) may be NULL */
+int wrapd_mlp_train(wrapd_ctx* ctx, int steps, int batch, float lr, unsigned long long seed, float* loss_hist);
 
 #ifdef __cplusplus
 }

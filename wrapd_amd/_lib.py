@@ -15,6 +15,7 @@ QUASI_PATH = os.path.join(LIB_DIR, "quasi")
 
 D = C.POINTER(C.c_double)
 I = C.POINTER(C.c_int)
+F = C.POINTER(C.c_float)
 P = C.c_void_p
 
 
@@ -33,7 +34,8 @@ class LocalOut(C.Structure):
     _fields_ = [("delta_auglag", C.c_double), ("bef", C.c_double), ("aft", C.c_double), ("pe", C.c_double),
                 ("max_wsq_ratio", C.c_double), ("prox_iters_total", C.c_double),
                 ("fallback_count", C.c_double), ("neg_sigma_count", C.c_double),
-                ("learned_accepted", C.c_double), ("evals_total", C.c_double), ("evals_max", C.c_double)]
+                ("learned_accepted", C.c_double), ("evals_total", C.c_double), ("evals_max", C.c_double),
+                ("learned_used", C.c_double)]
 
 
 class LbfgsCfg(C.Structure):
@@ -90,6 +92,19 @@ _PROTOS = {
     "wrapd_profile_enable": (C.c_int, [P, C.c_int]),
     "wrapd_profile_get": (C.c_int, [P, C.c_char_p, D, D]),
     "wrapd_synchronize": (C.c_int, [P]),
+    "wrapd_mlp_init": (C.c_int, [P, C.c_ulonglong]),
+    "wrapd_mlp_set_params": (C.c_int, [P, F, F]),
+    "wrapd_mlp_get_params": (C.c_int, [P, F, F]),
+    "wrapd_mlp_infer": (C.c_int, [P, F, C.c_int, F]),
+    "wrapd_mlp_grad": (C.c_int, [P, F, C.c_int, F, F]),
+    "wrapd_mlp_log_start": (C.c_int, [P, C.c_longlong]),
+    "wrapd_mlp_log_stop": (C.c_int, [P]),
+    "wrapd_mlp_log_count": (C.c_longlong, [P]),
+    "wrapd_mlp_log_get": (C.c_int, [P, C.c_longlong, C.c_longlong, F]),
+    "wrapd_mlp_log_set": (C.c_int, [P, F, C.c_longlong]),
+    "wrapd_mlp_fit_norm": (C.c_int, [P]),
+    "wrapd_mlp_augment": (C.c_int, [P, C.c_double, C.c_double, C.c_ulonglong]),
+    "wrapd_mlp_train": (C.c_int, [P, C.c_int, C.c_int, C.c_float, C.c_ulonglong, F]),
     # wrapd_solver.h
     "wrapd_solver_default_settings": (None, [C.POINTER(SolverSettings)]),
     "wrapd_solver_create": (C.c_int, [C.POINTER(P), C.POINTER(SolverSettings), C.c_int, D, D, D, C.c_int, I,
@@ -137,3 +152,7 @@ def dp(a):
 
 def ip(a):
     return a.ctypes.data_as(I)
+
+
+def fp(a):
+    return None if a is None else a.ctypes.data_as(F)

@@ -23,6 +23,8 @@ wrapd_ctx::~wrapd_ctx() {
     f(d_goff); f(d_uoff); f(d_asm_ptr); f(d_asm_idx); f(d_leaf_ptr); f(d_leaf_sn); f(d_fwd_tasks); f(d_bwd_tasks);
     f(pool); f(d_foff); f(d_cbase); f(d_acol_ptr); f(d_acol); f(d_ccol_ptr); f(d_ccol); f(d_relptr); f(d_rel);
     f(d_ftasks); f(d_ffail);
+    f(mlp_master); f(mlp_m1); f(mlp_m2); f(mlp_gsum); f(mlp_norm); f(mlp_bf); f(mlp_loss); f(mlp_partial);
+    f(mlp_lpart); f(mlp_log);
     f(partials); f(scal);
     if (scal_host) (void)hipHostFree(scal_host);
     for (auto& p : prof.pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -448,9 +450,13 @@ int wrapd_local_step(wrapd_ctx* c, const wrapd_local_args* a, double pe_prev, wr
         c->last_error = "Adaptive weights are only supported for the NH model.";
         return WRAPD_ERR_STATE;
     }
+    if (a->prox_mode == WRAPD_PROX_LEARNED && !c->mlp_ready) {
+        c->last_error = "learned prox: no network (wrapd_mlp_init / wrapd_mlp_set_params)";
+        return WRAPD_ERR_STATE;
+    }
     launch_local(c, a);
-    launch_finalize(c, local_blocks(c), 10, (1u << 3) | (1u << 9), S_RED);
-    const double* r = scal_read(c, S_RED, 10);
+    launch_finalize(c, local_blocks(c), 11, (1u << 3) | (1u << 9), S_RED);
+    const double* r = scal_read(c, S_RED, 11);
     if (!r) return WRAPD_ERR_HIP;
     if (out) {
         out->bef = r[0];
@@ -463,6 +469,7 @@ int wrapd_local_step(wrapd_ctx* c, const wrapd_local_args* a, double pe_prev, wr
         out->learned_accepted = r[7];
         out->evals_total = r[8];
         out->evals_max = r[9];
+        out->learned_used = r[10];
         out->delta_auglag = r[1] - r[0] + r[2] - pe_prev;
     }
     int st = launch_status(c);
@@ -864,6 +871,237 @@ int wrapd_profile_get(wrapd_ctx* c, const char* kernel, double* total_ms, double
 int wrapd_synchronize(wrapd_ctx* c) {
     CTX_OK(c);
     return check_hip(c, hipStreamSynchronize(c->stream), "sync") ? WRAPD_ERR_HIP : WRAPD_OK;
+}
+
+}  // extern "C"
+
+// ---- learned prox warm start -------------------------------------------------------
+namespace {
+std::vector<float> mlp_download_sample(wrapd_ctx* c, long long max_n) {
+    const long long n = c->mlp_log_count;
+    const long long stride = std::max<long long>(1, n / std::max<long long>(1, max_n));
+    const long long m = (n + stride - 1) / stride;
+    std::vector<float> out(static_cast<size_t>(m) * 9);
+    if (m == 0) return out;
+    // strided rows: 2-D copy, 9 floats per row
+    if (hipMemcpy2D(out.data(), 9 * sizeof(float), c->mlp_log, stride * 9 * sizeof(float), 9 * sizeof(float),
+                    static_cast<size_t>(m), hipMemcpyDeviceToHost) != hipSuccess)
+        out.clear();
+    return out;
+}
+}  // namespace
+
+extern "C" {
+
+int wrapd_mlp_set_params(wrapd_ctx* c, const float* params, const float* norm) {
+    CTX_OK(c);
+    if (!params || !norm) return WRAPD_ERR_ARG;
+    for (int i = 6; i < 12; ++i)
+        if (!(norm[i] > 0.0f)) { c->last_error = "mlp: input standard deviations must be positive"; return WRAPD_ERR_ARG; }
+    for (int i = 15; i < 18; ++i)
+        if (!(norm[i] > 0.0f)) { c->last_error = "mlp: output standard deviations must be positive"; return WRAPD_ERR_ARG; }
+    int st = mlp_upload(c, params, norm);
+    if (st) return st;
+    if (check_hip(c, hipMemsetAsync(c->mlp_m1, 0, kMlpParams * sizeof(float), c->stream), "mlp") ||
+        check_hip(c, hipMemsetAsync(c->mlp_m2, 0, kMlpParams * sizeof(float), c->stream), "mlp"))
+        return WRAPD_ERR_HIP;
+    c->mlp_t = 0;
+    return launch_status(c);
+}
+
+int wrapd_mlp_get_params(wrapd_ctx* c, float* params, float* norm) {
+    CTX_OK(c);
+    if (!c->mlp_ready) { c->last_error = "mlp: no network"; return WRAPD_ERR_STATE; }
+    if ((params && check_hip(c, hipMemcpyAsync(params, c->mlp_master, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, c->stream), "mlp")) ||
+        (norm && check_hip(c, hipMemcpyAsync(norm, c->mlp_norm, kMlpNorm * sizeof(float), hipMemcpyDeviceToHost, c->stream), "mlp")))
+        return WRAPD_ERR_HIP;
+    return check_hip(c, hipStreamSynchronize(c->stream), "mlp") ? WRAPD_ERR_HIP : WRAPD_OK;
+}
+
+int wrapd_mlp_init(wrapd_ctx* c, unsigned long long seed) {
+    CTX_OK(c);
+    // Glorot-uniform weights, zero biases, identity normalisation
+    std::vector<float> p(kMlpParams, 0.0f), norm(kMlpNorm, 0.0f);
+    uint64_t z = seed;
+    auto next = [&z]() {
+        z += 0x9E3779B97F4A7C15ull;
+        uint64_t r = z;
+        r = (r ^ (r >> 30)) * 0xBF58476D1CE4E5B9ull;
+        r = (r ^ (r >> 27)) * 0x94D049BB133111EBull;
+        r ^= r >> 31;
+        return static_cast<double>(r >> 11) * (1.0 / 9007199254740992.0);
+    };
+    auto fill = [&](int off, int rows, int cols) {
+        const double a = std::sqrt(6.0 / (rows + cols));
+        for (int i = 0; i < rows * cols; ++i) p[off + i] = static_cast<float>((2.0 * next() - 1.0) * a);
+    };
+    fill(kMlpOffW1, kMlpHid, kMlpIn);
+    fill(kMlpOffW2, kMlpHid, kMlpHid);
+    fill(kMlpOffW3, kMlpOut, kMlpHid);
+    for (int i = 6; i < 12; ++i) norm[i] = 1.0f;
+    for (int i = 15; i < 18; ++i) norm[i] = 1.0f;
+    return wrapd_mlp_set_params(c, p.data(), norm.data());
+}
+
+int wrapd_mlp_infer(wrapd_ctx* c, const float* in, int n, float* out) {
+    CTX_OK(c);
+    if (!c->mlp_ready) { c->last_error = "mlp: no network"; return WRAPD_ERR_STATE; }
+    if (n < 0 || (n > 0 && (!in || !out))) return WRAPD_ERR_ARG;
+    if (n == 0) return WRAPD_OK;
+    float *din = nullptr, *dout = nullptr;
+    int st = WRAPD_OK;
+    if (check_hip(c, hipMalloc(&din, 6ull * n * sizeof(float)), "mlp infer") ||
+        check_hip(c, hipMalloc(&dout, 3ull * n * sizeof(float)), "mlp infer") ||
+        check_hip(c, hipMemcpyAsync(din, in, 6ull * n * sizeof(float), hipMemcpyHostToDevice, c->stream), "mlp infer")) {
+        st = WRAPD_ERR_HIP;
+    } else {
+        mlp_infer(c, din, n, dout);
+        if (check_hip(c, hipMemcpyAsync(out, dout, 3ull * n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "mlp infer") ||
+            check_hip(c, hipStreamSynchronize(c->stream), "mlp infer"))
+            st = WRAPD_ERR_HIP;
+    }
+    if (din) (void)hipFree(din);
+    if (dout) (void)hipFree(dout);
+    return st ? st : launch_status(c);
+}
+
+int wrapd_mlp_grad(wrapd_ctx* c, const float* tuples, int n, float* grad, float* loss) {
+    CTX_OK(c);
+    if (!c->mlp_ready) { c->last_error = "mlp: no network"; return WRAPD_ERR_STATE; }
+    if (n <= 0 || !tuples) return WRAPD_ERR_ARG;
+    float* dt = nullptr;
+    int* didx = nullptr;
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    int st = WRAPD_OK;
+    if (check_hip(c, hipMalloc(&dt, 9ull * n * sizeof(float)), "mlp grad") ||
+        check_hip(c, hipMalloc(&didx, static_cast<size_t>(n) * sizeof(int)), "mlp grad") ||
+        check_hip(c, hipMemcpyAsync(dt, tuples, 9ull * n * sizeof(float), hipMemcpyHostToDevice, c->stream), "mlp grad") ||
+        check_hip(c, hipMemcpyAsync(didx, idx.data(), static_cast<size_t>(n) * sizeof(int), hipMemcpyHostToDevice, c->stream), "mlp grad")) {
+        st = WRAPD_ERR_HIP;
+    } else {
+        st = mlp_step(c, dt, n, didx, n, 0, 0.0f, 1);
+        if (!st && ((grad && check_hip(c, hipMemcpyAsync(grad, c->mlp_gsum, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, c->stream), "mlp grad")) ||
+                    (loss && check_hip(c, hipMemcpyAsync(loss, c->mlp_loss, sizeof(float), hipMemcpyDeviceToHost, c->stream), "mlp grad")) ||
+                    check_hip(c, hipStreamSynchronize(c->stream), "mlp grad")))
+            st = WRAPD_ERR_HIP;
+    }
+    if (dt) (void)hipFree(dt);
+    if (didx) (void)hipFree(didx);
+    return st ? st : launch_status(c);
+}
+
+int wrapd_mlp_log_start(wrapd_ctx* c, long long capacity) {
+    CTX_OK(c);
+    if (capacity < 0) return WRAPD_ERR_ARG;
+    if (capacity > c->mlp_log_cap) {
+        if (c->mlp_log) (void)hipFree(c->mlp_log);
+        c->mlp_log = nullptr;
+        c->mlp_log_cap = 0;
+        if (check_hip(c, hipMalloc(&c->mlp_log, std::max<long long>(1, capacity) * 9 * sizeof(float)), "mlp log"))
+            return WRAPD_ERR_HIP;
+        c->mlp_log_cap = capacity;
+    }
+    c->mlp_log_count = 0;
+    c->mlp_logging = true;
+    return WRAPD_OK;
+}
+
+int wrapd_mlp_log_stop(wrapd_ctx* c) {
+    CTX_OK(c);
+    c->mlp_logging = false;
+    return WRAPD_OK;
+}
+
+long long wrapd_mlp_log_count(wrapd_ctx* c) { return c ? c->mlp_log_count : -1; }
+
+int wrapd_mlp_log_get(wrapd_ctx* c, long long offset, long long n, float* out) {
+    CTX_OK(c);
+    if (offset < 0 || n < 0 || offset + n > c->mlp_log_count || (n > 0 && !out)) return WRAPD_ERR_ARG;
+    if (n == 0) return WRAPD_OK;
+    if (check_hip(c, hipMemcpyAsync(out, c->mlp_log + 9 * offset, 9 * n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "mlp log") ||
+        check_hip(c, hipStreamSynchronize(c->stream), "mlp log"))
+        return WRAPD_ERR_HIP;
+    return WRAPD_OK;
+}
+
+int wrapd_mlp_log_set(wrapd_ctx* c, const float* tuples, long long n) {
+    CTX_OK(c);
+    if (n < 0 || (n > 0 && !tuples)) return WRAPD_ERR_ARG;
+    int st = wrapd_mlp_log_start(c, n);
+    c->mlp_logging = false;   // a set log is data, not a recording
+    if (st) return st;
+    if (n > 0 && check_hip(c, hipMemcpyAsync(c->mlp_log, tuples, 9 * n * sizeof(float), hipMemcpyHostToDevice, c->stream), "mlp log"))
+        return WRAPD_ERR_HIP;
+    c->mlp_log_count = n;
+    return launch_status(c);
+}
+
+int wrapd_mlp_fit_norm(wrapd_ctx* c) {
+    CTX_OK(c);
+    if (!c->mlp_ready) { c->last_error = "mlp: no network"; return WRAPD_ERR_STATE; }
+    if (c->mlp_log_count < 2) { c->last_error = "mlp: no logged tuples"; return WRAPD_ERR_STATE; }
+    const std::vector<float> t = mlp_download_sample(c, 1000000);
+    if (t.empty()) { c->last_error = "mlp: log download failed"; return WRAPD_ERR_HIP; }
+    const size_t m = t.size() / 9;
+    double s[9] = {0}, q[9] = {0};
+    for (size_t i = 0; i < m; ++i) {
+        const float* r = &t[9 * i];
+        const double v[9] = {r[0], r[1], r[2], std::log(r[3] / r[4]), std::log(r[5] / r[4]), std::log(r[4]),
+                             r[6] - r[0], r[7] - r[1], r[8] - r[2]};
+        for (int k = 0; k < 9; ++k) { s[k] += v[k]; q[k] += v[k] * v[k]; }
+    }
+    std::vector<float> norm(kMlpNorm);
+    for (int k = 0; k < 9; ++k) {
+        const double mean = s[k] / m;
+        const double sd = std::sqrt(std::max(0.0, q[k] / m - mean * mean));
+        const double sdc = sd > 1e-6 * (1.0 + std::fabs(mean)) ? sd : 1.0;   // constant features: unit scale
+        if (k < 6) { norm[k] = static_cast<float>(mean); norm[6 + k] = static_cast<float>(sdc); }
+        else { norm[12 + k - 6] = static_cast<float>(mean); norm[15 + k - 6] = static_cast<float>(sdc); }
+    }
+    if (check_hip(c, hipMemcpyAsync(c->mlp_norm, norm.data(), kMlpNorm * sizeof(float), hipMemcpyHostToDevice, c->stream), "mlp norm") ||
+        check_hip(c, hipStreamSynchronize(c->stream), "mlp norm"))
+        return WRAPD_ERR_HIP;
+    return WRAPD_OK;
+}
+
+int wrapd_mlp_augment(wrapd_ctx* c, double w_lo, double w_hi, unsigned long long seed) {
+    CTX_OK(c);
+    if (c->mlp_log_count < 1) { c->last_error = "mlp: no logged tuples"; return WRAPD_ERR_STATE; }
+    if (!(w_lo > 0.0) || !(w_hi > 0.0)) {   // the logged range of w^2/vol
+        const std::vector<float> t = mlp_download_sample(c, 1000000);
+        if (t.empty()) return WRAPD_ERR_HIP;
+        double lo = 1e300, hi = 0.0;
+        for (size_t i = 0; i < t.size() / 9; ++i) { lo = std::min(lo, double(t[9 * i + 3])); hi = std::max(hi, double(t[9 * i + 3])); }
+        w_lo = lo;
+        w_hi = hi;
+    }
+    if (!(w_hi >= w_lo)) return WRAPD_ERR_ARG;
+    launch_mlp_augment(c, c->mlp_log, c->mlp_log_count, w_lo, w_hi, seed);
+    return launch_status(c);
+}
+
+int wrapd_mlp_train(wrapd_ctx* c, int steps, int batch, float lr, unsigned long long seed, float* loss_hist) {
+    CTX_OK(c);
+    if (!c->mlp_ready) { c->last_error = "mlp: no network"; return WRAPD_ERR_STATE; }
+    if (steps < 0 || batch <= 0 || !(lr > 0.0f)) return WRAPD_ERR_ARG;
+    if (c->mlp_log_count < 1) { c->last_error = "mlp: no logged tuples"; return WRAPD_ERR_STATE; }
+    float* dl = nullptr;
+    if (loss_hist && check_hip(c, hipMalloc(&dl, std::max(1, steps) * sizeof(float)), "mlp train")) return WRAPD_ERR_HIP;
+    int st = WRAPD_OK;
+    {
+        ProfScope ps(c, "mlp_train");
+        for (int k = 0; k < steps && !st; ++k) {
+            st = mlp_step(c, c->mlp_log, c->mlp_log_count, nullptr, batch, seed * 0x9E3779B97F4A7C15ull + k, lr, ++c->mlp_t);
+            if (!st && dl) st = check_hip(c, hipMemcpyAsync(dl + k, c->mlp_loss, sizeof(float), hipMemcpyDeviceToDevice, c->stream), "mlp train") ? WRAPD_ERR_HIP : WRAPD_OK;
+        }
+    }
+    if (!st && dl && steps > 0 &&
+        (check_hip(c, hipMemcpyAsync(loss_hist, dl, steps * sizeof(float), hipMemcpyDeviceToHost, c->stream), "mlp train") ||
+         check_hip(c, hipStreamSynchronize(c->stream), "mlp train")))
+        st = WRAPD_ERR_HIP;
+    if (dl) (void)hipFree(dl);
+    return st ? st : launch_status(c);
 }
 
 }  // extern "C"

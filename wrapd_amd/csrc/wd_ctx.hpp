@@ -10,6 +10,7 @@
 
 #include "../../include/wrapd_gpu.h"
 #include "chol_plan.hpp"
+#include "device/wd_mlp.hpp"
 #include "chol_symbolic.hpp"
 
 namespace wd {
@@ -133,6 +134,23 @@ struct wrapd_ctx {
     int* d_ftasks = nullptr;
     int* d_ffail = nullptr;
 
+    // learned prox warm start (wd_mlp.hip)
+    float* mlp_master = nullptr;   // FP32 parameters
+    float* mlp_m1 = nullptr;       // Adam moments
+    float* mlp_m2 = nullptr;
+    float* mlp_gsum = nullptr;     // last summed gradient
+    float* mlp_norm = nullptr;
+    uint16_t* mlp_bf = nullptr;    // BF16 matrix-core tiles
+    float* mlp_loss = nullptr;
+    float* mlp_partial = nullptr;  // per-wave partial gradients
+    float* mlp_lpart = nullptr;
+    int mlp_partial_waves = 0;
+    bool mlp_ready = false;
+    int mlp_t = 0;                 // Adam step
+    float* mlp_log = nullptr;      // training tuples, 9 floats each
+    long long mlp_log_cap = 0, mlp_log_count = 0;
+    bool mlp_logging = false;
+
     // reductions
     double* partials = nullptr;
     double* scal = nullptr;      // device scalars
@@ -190,6 +208,13 @@ int finalize_blocks_n(wrapd_ctx* c);
 // (wd_solve.hip)
 int chol_prepare_device(wrapd_ctx* c);
 void launch_solve(wrapd_ctx* c, const double* b, double* x);      // n_f x 3 device arrays
+// (wd_mlp.hip)
+MlpDevice mlp_device(wrapd_ctx* c);
+int mlp_upload(wrapd_ctx* c, const float* params, const float* norm);
+int mlp_infer(wrapd_ctx* c, const float* d_in, int n, float* d_out);
+int mlp_step(wrapd_ctx* c, const float* d_tuples, long long n_data, const int* d_idx, long long n, uint64_t seed,
+             float lr, int t);   // loss to c->mlp_loss[0], summed gradient to c->mlp_gsum
+void launch_mlp_augment(wrapd_ctx* c, float* d_tuples, long long n, double lo, double hi, uint64_t seed);
 // (wd_factor.hip)
 int factor_prepare_device(wrapd_ctx* c);
 void launch_factor(wrapd_ctx* c);   // status flag in c->d_ffail

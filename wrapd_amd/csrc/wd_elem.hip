@@ -14,6 +14,7 @@
 #include "wd_ctx.hpp"
 #include "device/wd_math.hpp"
 #include "device/wd_nh.hpp"
+#include "device/wd_mlp.hpp"
 #include "device/wd_reduce.hpp"
 
 namespace wd {
@@ -304,17 +305,36 @@ __device__ __forceinline__ M3<T> cvt(const M3<double>& a) {
     return b;
 }
 
+// one training tuple (s_hat, w^2/vol, mu, lambda | prox solution) for the learned warm start
+__device__ __forceinline__ void log_tuple(float* __restrict__ tlog, int e, const double shat[3], double wov,
+                                          const NHParams<double>& pd, const double x[3]) {
+    float* t = tlog + 9ll * e;
+    t[0] = static_cast<float>(shat[0]);
+    t[1] = static_cast<float>(shat[1]);
+    t[2] = static_cast<float>(shat[2]);
+    t[3] = static_cast<float>(wov);
+    t[4] = static_cast<float>(pd.mu);
+    t[5] = static_cast<float>(pd.lambda);
+    t[6] = static_cast<float>(x[0]);
+    t[7] = static_cast<float>(x[1]);
+    t[8] = static_cast<float>(x[2]);
+}
+
 // Stretch prox of one element in precision T.  Returns the new Z (FP64) and the
 // prox solution; iters gets the solver iterations.  ok = false when the result is unusable.
 template <typename T, int PROX>
 __device__ __forceinline__ void elem_prox(const M3<double>& Sin, const double sl[3], NHParams<double> pd, double wov,
                                           int prox_iters, double rtol, M3<double>& Znew, double x_out[3],
-                                          double& iters, bool& ok, double& shat_min, double& evals_out) {
+                                          double& iters, bool& ok, double& shat_min, double& evals_out,
+                                          double sh_out[3]) {
     const M3<T> St = cvt<T>(Sin);
     T sh[3];
     M3<T> Ut, Vt;
     svd3(St, sh, Ut, Vt);
     shat_min = static_cast<double>(sh[2]);
+    sh_out[0] = static_cast<double>(sh[0]);
+    sh_out[1] = static_cast<double>(sh[1]);
+    sh_out[2] = static_cast<double>(sh[2]);
     ProxProblem<T> P;
     P.p.mu = static_cast<T>(pd.mu);
     P.p.lambda = static_cast<T>(pd.lambda);
@@ -348,7 +368,8 @@ __global__ void __launch_bounds__(kLocalThreads)
 k_local(int n_e, int n_ep, const double* __restrict__ symF, double* __restrict__ U, double* __restrict__ P,
         const double* __restrict__ wsq, const double* __restrict__ vol, const double* __restrict__ kst,
         double* __restrict__ cand, double* __restrict__ sig, MatArgs m, int prox_iters, double rtol,
-        double fallback_sigma, int update_cand, int use_hessian, double* __restrict__ partials) {
+        double fallback_sigma, int update_cand, int use_hessian, double* __restrict__ partials,
+        float* __restrict__ tlog) {
     __shared__ double sh[kLocalThreads / 64];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int e = lb * blockDim.x + threadIdx.x;
@@ -375,16 +396,17 @@ k_local(int n_e, int n_ep, const double* __restrict__ symF, double* __restrict__
             const NHParams<double> pd = mat_of(m, e);
             const double sl[3] = {sig[e], sig[n_ep + e], sig[2 * n_ep + e]};
             bool ok;
-            double smin;
+            double smin, shat[3];
             elem_prox<typename std::conditional<PREC == WRAPD_FP32, float, double>::type, PROX>(
-                Sin, sl, pd, w2 / v, prox_iters, rtol, Zn, x, iters, ok, smin, evals);
+                Sin, sl, pd, w2 / v, prox_iters, rtol, Zn, x, iters, ok, smin, evals, shat);
             if (PREC == WRAPD_FP32) {
                 if (!ok || smin < fallback_sigma || x[0] < 0.0 || x[1] < 0.0 || x[2] < 0.0) {
                     fallback = 1.0;
                     elem_prox<double, PROX>(Sin, sl, pd, w2 / v, prox_iters, PROX == WRAPD_PROX_LBFGS ? rtol : 1e-12,
-                                            Zn, x, iters, ok, smin, evals);
+                                            Zn, x, iters, ok, smin, evals, shat);
                 }
             }
+            if (tlog) log_tuple(tlog, e, shat, w2 / v, pd, x);
             pe = v * nh_psi(x, pd);
             neg = (x[0] < 0.0 || x[1] < 0.0 || x[2] < 0.0) ? 1.0 : 0.0;
             sig[e] = x[0];
@@ -436,7 +458,150 @@ k_local(int n_e, int n_ep, const double* __restrict__ symF, double* __restrict__
     if (threadIdx.x == 0) {
         double* o = partials + lb * kRedStride;
         o[0] = r0; o[1] = r1; o[2] = r2; o[3] = r3; o[4] = r4; o[5] = r5; o[6] = r6; o[7] = 0.0;
-        o[8] = r8; o[9] = r9;
+        o[8] = r8; o[9] = r9; o[10] = 0.0;
+    }
+}
+
+// Learned warm start (WRAPD_PROX_LEARNED): after the SVD of S_in, the whole wave
+// evaluates the MLP (device/wd_mlp.hpp) on its 64 elements; each element's Newton prox
+// then starts from whichever of the prediction and the previous solution has the smaller
+// optimality residual |grad|, and a prediction already within the Newton tolerance is
+// taken as is.  Everything else (reductions, FP64 fallback, candidate weights) is k_local's.
+template <typename T>
+__device__ __forceinline__ T prox_residual(const ProxProblem<T>& P, const T x[3]) {
+    if (!(x[0] > T(0) && x[1] > T(0) && x[2] > T(0))) return T(INFINITY);
+    T g[3];
+    P.gradient(x, g);
+    const T r = wsqrt(dot3(g, g));
+    return r == r ? r : T(INFINITY);
+}
+
+template <int PREC>
+__global__ void __launch_bounds__(kLocalThreads)
+k_local_learned(int n_e, int n_ep, const double* __restrict__ symF, double* __restrict__ U, double* __restrict__ P,
+                const double* __restrict__ wsq, const double* __restrict__ vol, const double* __restrict__ kst,
+                double* __restrict__ cand, double* __restrict__ sig, MatArgs m, int prox_iters, double rtol,
+                double fallback_sigma, int update_cand, int use_hessian, double* __restrict__ partials,
+                float* __restrict__ tlog, MlpDevice mlp) {
+    using T = typename std::conditional<PREC == WRAPD_FP32, float, double>::type;
+    __shared__ double shr[kLocalThreads / 64];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int e = lb * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool valid = e < n_e;
+    double bef = 0.0, aft = 0.0, pe = 0.0, ratio = 0.0, iters = 0.0, fallback = 0.0, neg = 0.0, evals = 0.0;
+    double accepted = 0.0, used = 0.0;
+    M3<double> S, Uo, Po;
+    double w2 = 1.0, v = 1.0;
+    NHParams<double> pd{1.0, 1.0};
+    double sl[3] = {1.0, 1.0, 1.0};
+#pragma unroll
+    for (int i = 0; i < 9; ++i) S.m[i] = Uo.m[i] = Po.m[i] = 0.0;
+    S.m[0] = S.m[4] = S.m[8] = 1.0;
+    if (valid) {
+        ld9(symF, n_ep, e, S);
+        ld9(U, n_ep, e, Uo);
+        ld9(P, n_ep, e, Po);
+        w2 = wsq[e];
+        v = vol[e];
+        pd = mat_of(m, e);
+        sl[0] = sig[e]; sl[1] = sig[n_ep + e]; sl[2] = sig[2 * n_ep + e];
+        M3<double> D;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) D.m[i] = S.m[i] - Po.m[i];
+        bef = 0.5 * w2 * sqnorm9(D);
+    }
+    M3<double> Sin;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Sin.m[i] = S.m[i] + Uo.m[i];
+    Sin = transpose(Sin);
+    const M3<T> St = cvt<T>(Sin);
+    T shat[3];
+    M3<T> Ut, Vt;
+    svd3(St, shat, Ut, Vt);
+    float feat[kMlpIn];
+    mlp_features(mlp.norm, static_cast<float>(shat[0]), static_cast<float>(shat[1]), static_cast<float>(shat[2]),
+                 static_cast<float>(w2 / v), static_cast<float>(pd.mu), static_cast<float>(pd.lambda), feat);
+    float dpred[kMlpOut];
+    mlp_wave_predict(mlp, feat, lane, dpred);   // every lane of the wave takes part
+    if (valid) {
+        ProxProblem<T> PP;
+        PP.p.mu = static_cast<T>(pd.mu);
+        PP.p.lambda = static_cast<T>(pd.lambda);
+        PP.x0[0] = shat[0]; PP.x0[1] = shat[1]; PP.x0[2] = shat[2];
+        PP.wov = static_cast<T>(w2 / v);
+        const T xp[3] = {shat[0] + static_cast<T>(dpred[0]), shat[1] + static_cast<T>(dpred[1]),
+                         shat[2] + static_cast<T>(dpred[2])};
+        const T xl[3] = {static_cast<T>(sl[0]), static_cast<T>(sl[1]), static_cast<T>(sl[2])};
+        const T rp = prox_residual(PP, xp), rl = prox_residual(PP, xl);
+        const bool use_pred = rp < rl;
+        T x[3] = {use_pred ? xp[0] : xl[0], use_pred ? xp[1] : xl[1], use_pred ? xp[2] : xl[2]};
+        const T xn = wsqrt(dot3(x, x));
+        used = use_pred ? 1.0 : 0.0;
+        accepted = (use_pred && rp <= static_cast<T>(rtol) * (PP.p.mu + PP.wov) * (xn > T(1) ? xn : T(1))) ? 1.0 : 0.0;
+        T gn = T(0);
+        const int it = prox_newton(PP, x, prox_iters, static_cast<T>(rtol), gn);
+        iters = it < 0 ? 0.0 : static_cast<double>(it);
+        evals = iters;
+        const M3<T> Zt = udvt(Vt, x, Ut);
+        M3<double> Zn;
+        bool ok = true;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            Zn.m[i] = static_cast<double>(Zt.m[i]);
+            ok &= isfinite(Zn.m[i]);
+        }
+        double xd[3] = {static_cast<double>(x[0]), static_cast<double>(x[1]), static_cast<double>(x[2])};
+        double shd[3] = {static_cast<double>(shat[0]), static_cast<double>(shat[1]), static_cast<double>(shat[2])};
+        if (PREC == WRAPD_FP32 &&
+            (!ok || static_cast<double>(shat[2]) < fallback_sigma || xd[0] < 0.0 || xd[1] < 0.0 || xd[2] < 0.0)) {
+            fallback = 1.0;
+            double smin;
+            elem_prox<double, WRAPD_PROX_NEWTON>(Sin, sl, pd, w2 / v, prox_iters, 1e-12, Zn, xd, iters, ok, smin, evals,
+                                                 shd);
+        }
+        pe = v * nh_psi(xd, pd);
+        neg = (xd[0] < 0.0 || xd[1] < 0.0 || xd[2] < 0.0) ? 1.0 : 0.0;
+        sig[e] = xd[0];
+        sig[n_ep + e] = xd[1];
+        sig[2 * n_ep + e] = xd[2];
+        if (tlog) log_tuple(tlog, e, shd, w2 / v, pd, xd);
+        M3<double> D, Un, Pn;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) D.m[i] = S.m[i] - (Zn.m[i] - Uo.m[i]);
+        aft = 0.5 * w2 * sqnorm9(D);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            Un.m[i] = Uo.m[i] + (S.m[i] - Zn.m[i]);
+            Pn.m[i] = Zn.m[i] - Un.m[i];
+        }
+        st9(U, n_ep, e, Un);
+        st9(P, n_ep, e, Pn);
+        if (update_cand) {
+            const double k = kst[e];
+            double cs = k;
+            if (use_hessian) cs = nh_candidate_stiffness(xd, pd, k * m.beta_min, k * m.beta_max);
+            const double cw = v * cs;
+            cand[e] = cw;
+            const double a = w2 / cw, b = 1.0 / a;
+            ratio = a > b ? a : b;
+        }
+    }
+    const double r0 = block_sum<kLocalThreads>(bef, shr);
+    const double r1 = block_sum<kLocalThreads>(aft, shr);
+    const double r2 = block_sum<kLocalThreads>(pe, shr);
+    const double r3 = block_max<kLocalThreads>(ratio, shr);
+    const double r4 = block_sum<kLocalThreads>(iters, shr);
+    const double r5 = block_sum<kLocalThreads>(fallback, shr);
+    const double r6 = block_sum<kLocalThreads>(neg, shr);
+    const double r7 = block_sum<kLocalThreads>(accepted, shr);
+    const double r8 = block_sum<kLocalThreads>(evals, shr);
+    const double r9 = block_max<kLocalThreads>(evals, shr);
+    const double r10 = block_sum<kLocalThreads>(used, shr);
+    if (threadIdx.x == 0) {
+        double* o = partials + lb * kRedStride;
+        o[0] = r0; o[1] = r1; o[2] = r2; o[3] = r3; o[4] = r4; o[5] = r5; o[6] = r6; o[7] = r7;
+        o[8] = r8; o[9] = r9; o[10] = r10;
     }
 }
 
@@ -599,22 +764,42 @@ void launch_defo(wrapd_ctx* c, int mode, int use_hessian) {
 }
 
 template <int MODEL, int PREC, int PROX>
-static void local_inst(wrapd_ctx* c, const wrapd_local_args* a) {
+static void local_inst(wrapd_ctx* c, const wrapd_local_args* a, float* tlog) {
     const int nb = blocks_for(c->n_e, kLocalThreads);
     hipLaunchKernelGGL((k_local<MODEL, PREC, PROX>), dim3(nb), dim3(kLocalThreads), 0, c->stream, c->n_e, c->n_ep,
                        c->symF, c->U, c->P, c->wsq, c->vol, c->kst, c->cand, c->sig, mat_args(c), a->prox_iters,
-                       a->newton_rtol, a->fallback_sigma, a->update_cand, a->cand_use_hessian, c->partials);
+                       a->newton_rtol, a->fallback_sigma, a->update_cand, a->cand_use_hessian, c->partials, tlog);
+}
+
+template <int PREC>
+static void local_learned_inst(wrapd_ctx* c, const wrapd_local_args* a, float* tlog) {
+    const int nb = blocks_for(c->n_e, kLocalThreads);
+    hipLaunchKernelGGL((k_local_learned<PREC>), dim3(nb), dim3(kLocalThreads), 0, c->stream, c->n_e, c->n_ep,
+                       c->symF, c->U, c->P, c->wsq, c->vol, c->kst, c->cand, c->sig, mat_args(c), a->prox_iters,
+                       a->newton_rtol, a->fallback_sigma, a->update_cand, a->cand_use_hessian, c->partials, tlog,
+                       mlp_device(c));
 }
 
 void launch_local(wrapd_ctx* c, const wrapd_local_args* a) {
     ProfScope ps(c, "local_step");
-    if (c->mat.model != 1) { local_inst<0, WRAPD_FP64, WRAPD_PROX_LBFGS>(c, a); return; }
+    // tuple log: the whole step is appended when it fits
+    float* tlog = nullptr;
+    if (c->mlp_logging && c->mat.model == 1 && c->mlp_log_count + c->n_e <= c->mlp_log_cap) {
+        tlog = c->mlp_log + 9ll * c->mlp_log_count;
+        c->mlp_log_count += c->n_e;
+    }
+    if (c->mat.model != 1) { local_inst<0, WRAPD_FP64, WRAPD_PROX_LBFGS>(c, a, nullptr); return; }
     const bool f32 = a->precision == WRAPD_FP32;
+    if (a->prox_mode == WRAPD_PROX_LEARNED) {
+        if (f32) local_learned_inst<WRAPD_FP32>(c, a, tlog);
+        else local_learned_inst<WRAPD_FP64>(c, a, tlog);
+        return;
+    }
     const bool newton = a->prox_mode != WRAPD_PROX_LBFGS;
-    if (!f32 && !newton) local_inst<1, WRAPD_FP64, WRAPD_PROX_LBFGS>(c, a);
-    else if (!f32 && newton) local_inst<1, WRAPD_FP64, WRAPD_PROX_NEWTON>(c, a);
-    else if (f32 && newton) local_inst<1, WRAPD_FP32, WRAPD_PROX_NEWTON>(c, a);
-    else local_inst<1, WRAPD_FP32, WRAPD_PROX_LBFGS>(c, a);
+    if (!f32 && !newton) local_inst<1, WRAPD_FP64, WRAPD_PROX_LBFGS>(c, a, tlog);
+    else if (!f32 && newton) local_inst<1, WRAPD_FP64, WRAPD_PROX_NEWTON>(c, a, tlog);
+    else if (f32 && newton) local_inst<1, WRAPD_FP32, WRAPD_PROX_NEWTON>(c, a, tlog);
+    else local_inst<1, WRAPD_FP32, WRAPD_PROX_LBFGS>(c, a, tlog);
 }
 
 void launch_local_unaware(wrapd_ctx* c, const wrapd_local_args* a) {

@@ -12,7 +12,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import dp, ip, lib
+from ._lib import dp, fp, ip, lib
 
 MODEL = {"arap": 0, "nh": 1}
 PRECISION = {"fp64": 0, "fp32": 1}
@@ -319,3 +319,65 @@ class Device:
 
     def synchronize(self):
         self._check(lib().wrapd_synchronize(self.ctx), "synchronize")
+
+    # ---- learned prox warm start (wd_mlp.hip) ----
+    N_PARAMS, N_NORM = 4803, 18
+
+    def mlp_init(self, seed=0):
+        self._check(lib().wrapd_mlp_init(self.ctx, seed), "mlp_init")
+
+    def mlp_set_params(self, params, norm):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        n = np.ascontiguousarray(norm, dtype=np.float32)
+        assert p.size == self.N_PARAMS and n.size == self.N_NORM
+        self._check(lib().wrapd_mlp_set_params(self.ctx, fp(p), fp(n)), "mlp_set_params")
+
+    def mlp_get_params(self):
+        p = np.zeros(self.N_PARAMS, np.float32)
+        n = np.zeros(self.N_NORM, np.float32)
+        self._check(lib().wrapd_mlp_get_params(self.ctx, fp(p), fp(n)), "mlp_get_params")
+        return p, n
+
+    def mlp_infer(self, inputs):
+        x = np.ascontiguousarray(inputs, dtype=np.float32).reshape(-1, 6)
+        out = np.zeros((len(x), 3), np.float32)
+        self._check(lib().wrapd_mlp_infer(self.ctx, fp(x), len(x), fp(out)), "mlp_infer")
+        return out
+
+    def mlp_grad(self, tuples):
+        t = np.ascontiguousarray(tuples, dtype=np.float32).reshape(-1, 9)
+        g = np.zeros(self.N_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(lib().wrapd_mlp_grad(self.ctx, fp(t), len(t), fp(g), fp(loss)), "mlp_grad")
+        return float(loss[0]), g
+
+    def mlp_log_start(self, capacity):
+        self._check(lib().wrapd_mlp_log_start(self.ctx, int(capacity)), "mlp_log_start")
+
+    def mlp_log_stop(self):
+        self._check(lib().wrapd_mlp_log_stop(self.ctx), "mlp_log_stop")
+
+    def mlp_log_count(self):
+        return int(lib().wrapd_mlp_log_count(self.ctx))
+
+    def mlp_log_get(self, offset=0, n=None):
+        if n is None:
+            n = self.mlp_log_count() - offset
+        out = np.zeros((n, 9), np.float32)
+        self._check(lib().wrapd_mlp_log_get(self.ctx, int(offset), int(n), fp(out)), "mlp_log_get")
+        return out
+
+    def mlp_log_set(self, tuples):
+        t = np.ascontiguousarray(tuples, dtype=np.float32).reshape(-1, 9)
+        self._check(lib().wrapd_mlp_log_set(self.ctx, fp(t), len(t)), "mlp_log_set")
+
+    def mlp_fit_norm(self):
+        self._check(lib().wrapd_mlp_fit_norm(self.ctx), "mlp_fit_norm")
+
+    def mlp_augment(self, w_lo=0.0, w_hi=0.0, seed=0):
+        self._check(lib().wrapd_mlp_augment(self.ctx, float(w_lo), float(w_hi), seed), "mlp_augment")
+
+    def mlp_train(self, steps, batch=8192, lr=1e-3, seed=0):
+        hist = np.zeros(max(steps, 1), np.float32)
+        self._check(lib().wrapd_mlp_train(self.ctx, int(steps), int(batch), float(lr), seed, fp(hist)), "mlp_train")
+        return hist[:steps]
