@@ -58,8 +58,8 @@ int wrapd_solver_get_log(wrapd_solver* s, const char* name, double* out, int max
 int wrapd_solver_get_x(wrapd_solver* s, double* X);           /* (n_verts,3) after finish     */
 int wrapd_solver_get_x_free(wrapd_solver* s, double* x);      /* current free positions       */
 /* [setup_ms, local_ms, global_ms, refactor_ms, inner_iters, finish_ms, prox_iters, fallback,
- *  learned_accepted, fp64_switch_iter (-1: none)] */
-int wrapd_solver_get_runtime(wrapd_solver* s, double* out10);
+ *  learned_accepted, fp64_switch_iter (-1: none), learned_used] */
+int wrapd_solver_get_runtime(wrapd_solver* s, double* out11);
 wrapd_ctx* wrapd_solver_ctx(wrapd_solver* s);
 
 #ifdef __cplusplus

@@ -224,6 +224,7 @@ int wrapd_solver_get_runtime(wrapd_solver* s, double* o) {
     o[7] = r.fallback_elems;
     o[8] = r.learned_accepted;
     o[9] = r.fp64_switch_iter;
+    o[10] = r.learned_used;
     return WRAPD_OK;
 }
 

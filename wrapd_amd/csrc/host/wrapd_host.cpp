@@ -497,6 +497,7 @@ void ADMM::iterate() {
     ad.m_runtime.prox_iters += lo.prox_iters_total;
     ad.m_runtime.fallback_elems += lo.fallback_count;
     ad.m_runtime.learned_accepted += lo.learned_accepted;
+    ad.m_runtime.learned_used += lo.learned_used;
     m_global_step_behavior->global_step();
     const double gs_delta = m_global_step_behavior->last_step_delta_auglag();
     m_micro_timer.reset();

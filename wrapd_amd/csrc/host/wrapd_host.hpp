@@ -154,7 +154,7 @@ struct RuntimeData {
     int inner_iters = 0;
     double finish_solve_ms = 0;
     // device-path extras
-    double prox_iters = 0, fallback_elems = 0, learned_accepted = 0;
+    double prox_iters = 0, fallback_elems = 0, learned_accepted = 0, learned_used = 0;
     double fp64_switch_iter = -1;   // first iteration of the FP64 tail of a mixed run
 };
 

@@ -462,11 +462,14 @@ k_local(int n_e, int n_ep, const double* __restrict__ symF, double* __restrict__
     }
 }
 
-// Learned warm start (WRAPD_PROX_LEARNED): after the SVD of S_in, the whole wave
-// evaluates the MLP (device/wd_mlp.hpp) on its 64 elements; each element's Newton prox
-// then starts from whichever of the prediction and the previous solution has the smaller
-// optimality residual |grad|, and a prediction already within the Newton tolerance is
-// taken as is.  Everything else (reductions, FP64 fallback, candidate weights) is k_local's.
+// Learned warm start (WRAPD_PROX_LEARNED): after the SVD of S_in, a wave in which some
+// element's previous solution is a poor start (relative optimality residual above
+// kLearnedGate, where one Newton step no longer suffices) evaluates the MLP
+// (device/wd_mlp.hpp) on its 64 elements; each element's Newton prox then starts from
+// whichever of the prediction and the previous solution has the smaller residual |grad|,
+// and a prediction already within the Newton tolerance is taken as is.  Everything else
+// (reductions, FP64 fallback, candidate weights) is k_local's.
+constexpr double kLearnedGate = 1e-3;
 template <typename T>
 __device__ __forceinline__ T prox_residual(const ProxProblem<T>& P, const T x[3]) {
     if (!(x[0] > T(0) && x[1] > T(0) && x[2] > T(0))) return T(INFINITY);
@@ -519,21 +522,27 @@ k_local_learned(int n_e, int n_ep, const double* __restrict__ symF, double* __re
     T shat[3];
     M3<T> Ut, Vt;
     svd3(St, shat, Ut, Vt);
-    float feat[kMlpIn];
-    mlp_features(mlp.norm, static_cast<float>(shat[0]), static_cast<float>(shat[1]), static_cast<float>(shat[2]),
-                 static_cast<float>(w2 / v), static_cast<float>(pd.mu), static_cast<float>(pd.lambda), feat);
-    float dpred[kMlpOut];
-    mlp_wave_predict(mlp, feat, lane, dpred);   // every lane of the wave takes part
+    ProxProblem<T> PP;
+    PP.p.mu = static_cast<T>(pd.mu);
+    PP.p.lambda = static_cast<T>(pd.lambda);
+    PP.x0[0] = shat[0]; PP.x0[1] = shat[1]; PP.x0[2] = shat[2];
+    PP.wov = static_cast<T>(w2 / v);
+    const T xl[3] = {static_cast<T>(sl[0]), static_cast<T>(sl[1]), static_cast<T>(sl[2])};
+    const T rl = prox_residual(PP, xl);
+    const T xln = wsqrt(dot3(xl, xl));
+    const bool poor = valid && !(rl <= static_cast<T>(kLearnedGate) * (PP.p.mu + PP.wov) * (xln > T(1) ? xln : T(1)));
+    float dpred[kMlpOut] = {0.0f, 0.0f, 0.0f};
+    const bool consult = __any(poor);   // wave-uniform: the MLP is a whole-wave product
+    if (consult) {
+        float feat[kMlpIn];
+        mlp_features(mlp.norm, static_cast<float>(shat[0]), static_cast<float>(shat[1]), static_cast<float>(shat[2]),
+                     static_cast<float>(w2 / v), static_cast<float>(pd.mu), static_cast<float>(pd.lambda), feat);
+        mlp_wave_predict(mlp, feat, lane, dpred);   // every lane of the wave takes part
+    }
     if (valid) {
-        ProxProblem<T> PP;
-        PP.p.mu = static_cast<T>(pd.mu);
-        PP.p.lambda = static_cast<T>(pd.lambda);
-        PP.x0[0] = shat[0]; PP.x0[1] = shat[1]; PP.x0[2] = shat[2];
-        PP.wov = static_cast<T>(w2 / v);
         const T xp[3] = {shat[0] + static_cast<T>(dpred[0]), shat[1] + static_cast<T>(dpred[1]),
                          shat[2] + static_cast<T>(dpred[2])};
-        const T xl[3] = {static_cast<T>(sl[0]), static_cast<T>(sl[1]), static_cast<T>(sl[2])};
-        const T rp = prox_residual(PP, xp), rl = prox_residual(PP, xl);
+        const T rp = poor ? prox_residual(PP, xp) : T(INFINITY);
         const bool use_pred = rp < rl;
         T x[3] = {use_pred ? xp[0] : xl[0], use_pred ? xp[1] : xl[1], use_pred ? xp[2] : xl[2]};
         const T xn = wsqrt(dot3(x, x));

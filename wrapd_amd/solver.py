@@ -143,10 +143,10 @@ class Solver:
         return x
 
     def runtime(self):
-        o = np.zeros(10)
+        o = np.zeros(11)
         lib().wrapd_solver_get_runtime(self.h, dp(o))
         keys = ["setup_ms", "local_ms", "global_ms", "refactor_ms", "inner_iters", "finish_ms", "prox_iters",
-                "fallback_elems", "learned_accepted", "fp64_switch_iter"]
+                "fallback_elems", "learned_accepted", "fp64_switch_iter", "learned_used"]
         return dict(zip(keys, o.tolist()))
 
     def device(self):
