@@ -89,3 +89,19 @@ def test_newton_fp64_prox_same_fixed_point():
     nt = Solver(prob, tol=0.0, prox_mode=1)
     nt.run(200)
     assert np.linalg.norm(nt.x() - ref.x()) <= 1e-7 * np.linalg.norm(ref.x())
+
+
+def test_gpu_uniform_per_element_material_is_the_homogeneous_run():
+    """Per-element material arrays (the heterogeneous extension) at a uniform E follow the
+    reference's homogeneous trace (E = 1e6) on the GPU, to the trace bars."""
+    prob = meshgen.small_beam(20, 4, 4, seed=1, pin_far_end=False, youngs=1e6)
+    prob.youngs_per_elem = np.full(prob.n_elems, 1e6)
+    z = load("trace_c1geom_nh_E1e6.npz")
+    s = Solver(prob, tol=0.0)
+    s.run(len(z["objectives"]) - 1)
+    bar_e, bar_x = trace_bars(z)
+    e = s.log("objectives")
+    assert np.array_equal(s.log("inner_iters").astype(int), z["inner_iters"])
+    assert np.array_equal(s.log("reweighted").astype(int), z["reweighted"])
+    assert np.max(np.abs(e - z["objectives"])) <= bar_e * abs(z["objectives"][0])
+    assert np.linalg.norm(s.x()[z["x_rows"]] - z["x"]) <= bar_x * np.linalg.norm(z["x"])

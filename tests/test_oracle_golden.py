@@ -103,3 +103,12 @@ def test_trace_matches_reference(name):
 @pytest.mark.slow
 def test_untwist_matches_reference():
     compare_trace("untwist", untwist_problem(), {})
+
+
+def test_uniform_per_element_material_is_the_homogeneous_run():
+    """The heterogeneous-material extension (per-element mu, lambda; no reference
+    counterpart) with every element at the same E reproduces the reference's homogeneous
+    trace (its -E path, E = 1e6)."""
+    prob = meshgen.small_beam(20, 4, 4, seed=1, pin_far_end=False, youngs=1e6)
+    prob.youngs_per_elem = np.full(prob.n_elems, 1e6)
+    compare_trace("c1geom_nh_E1e6", prob, {})
