@@ -883,10 +883,7 @@ std::vector<float> mlp_download_sample(wrapd_ctx* c, long long max_n) {
     const long long m = (n + stride - 1) / stride;
     std::vector<float> out(static_cast<size_t>(m) * 9);
     if (m == 0) return out;
-    // strided rows: 2-D copy, 9 floats per row
-    if (hipMemcpy2D(out.data(), 9 * sizeof(float), c->mlp_log, stride * 9 * sizeof(float), 9 * sizeof(float),
-                    static_cast<size_t>(m), hipMemcpyDeviceToHost) != hipSuccess)
-        out.clear();
+    if (mlp_sample(c, stride, m, out.data()) != WRAPD_OK) out.clear();
     return out;
 }
 }  // namespace

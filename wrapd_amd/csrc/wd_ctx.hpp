@@ -119,6 +119,7 @@ struct wrapd_ctx {
     int* d_fwd_tasks = nullptr;  // per level: (sn, block of fwd_rows[l] rows)
     int* d_bwd_tasks = nullptr;  // per level: (sn, block of 4 bwd_cpw[l] columns)
     std::vector<int> fwd_rows, bwd_cpw;
+    size_t leaf_lds_fwd = 0, leaf_lds_bwd = 0;   // dynamic LDS of the leaf-subtree kernels
     std::vector<int> fwd_task_ptr, bwd_task_ptr;
     // numeric factorization (plan from chol_plan, fronts in one pool)
     wrapd::FactorPlan fplan;
@@ -215,6 +216,7 @@ int mlp_infer(wrapd_ctx* c, const float* d_in, int n, float* d_out);
 int mlp_step(wrapd_ctx* c, const float* d_tuples, long long n_data, const int* d_idx, long long n, uint64_t seed,
              float lr, int t);   // loss to c->mlp_loss[0], summed gradient to c->mlp_gsum
 void launch_mlp_augment(wrapd_ctx* c, float* d_tuples, long long n, double lo, double hi, uint64_t seed);
+int mlp_sample(wrapd_ctx* c, long long stride, long long m, float* host);   // every stride-th logged tuple
 // (wd_factor.hip)
 int factor_prepare_device(wrapd_ctx* c);
 void launch_factor(wrapd_ctx* c);   // status flag in c->d_ffail

@@ -354,6 +354,27 @@ int mlp_step(wrapd_ctx* c, const float* d_tuples, long long n_data, const int* d
     return WRAPD_OK;
 }
 
+__global__ void __launch_bounds__(256)
+k_mlp_gather(long long m, long long stride, const float* __restrict__ src, float* __restrict__ dst) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= m * 9) return;
+    const long long row = i / 9, k = i - row * 9;
+    dst[i] = src[row * stride * 9 + k];
+}
+
+// every stride-th tuple of the log, contiguous on the host (m rows)
+int mlp_sample(wrapd_ctx* c, long long stride, long long m, float* host) {
+    float* d = nullptr;
+    if (check_hip(c, hipMalloc(&d, std::max<long long>(1, m) * 9 * sizeof(float)), "mlp sample")) return WRAPD_ERR_HIP;
+    hipLaunchKernelGGL(k_mlp_gather, dim3(blocks_for(m * 9, 256)), dim3(256), 0, c->stream, m, stride, c->mlp_log, d);
+    int st = WRAPD_OK;
+    if (check_hip(c, hipMemcpyAsync(host, d, m * 9 * sizeof(float), hipMemcpyDeviceToHost, c->stream), "mlp sample") ||
+        check_hip(c, hipStreamSynchronize(c->stream), "mlp sample"))
+        st = WRAPD_ERR_HIP;
+    (void)hipFree(d);
+    return st;
+}
+
 void launch_mlp_augment(wrapd_ctx* c, float* d_tuples, long long n, double lo, double hi, uint64_t seed) {
     hipLaunchKernelGGL(k_mlp_augment, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, d_tuples, lo, hi, seed);
 }

@@ -49,13 +49,22 @@ __device__ __forceinline__ void assemble_row(const SolveMaps& M, int64_t prow, c
     }
 }
 
+// Leaf subtrees: one workgroup walks a whole subtree in postorder.  A subtree is a
+// contiguous range of supernodes (postorder), and every update row it produces except its
+// root's is consumed inside it, so those rows stay in LDS (dynamic, sized by the largest
+// subtree); only the root's rows go to the global buffer for the level kernels.
 __global__ void __launch_bounds__(kSolveThreads)
 k_fwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, SolveMaps M,
            const double* __restrict__ b, const double* __restrict__ G, double* __restrict__ ysol,
            double* __restrict__ ubuf) {
-    __shared__ double cv[kSmallFront * 3];
+    extern __shared__ double dyn[];
+    double* cv = dyn;                     // kSmallFront x 3
+    double* ul = dyn + kSmallFront * 3;   // the subtree's update rows x 3
     const int t = blockIdx.x;
-    for (int q = leaf_ptr[t]; q < leaf_ptr[t + 1]; ++q) {
+    const int q0 = leaf_ptr[t], q1 = leaf_ptr[t + 1];
+    const int root = leaf_sn[q1 - 1];
+    const int64_t ubase = M.uoff[leaf_sn[q0]];
+    for (int q = q0; q < q1; ++q) {
         const int s = leaf_sn[q];
         const int c0 = M.col0[s];
         const int ns = M.col0[s + 1] - c0;
@@ -69,7 +78,12 @@ k_fwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, So
                 a[1] = b[3 * old + 1];
                 a[2] = b[3 * old + 2];
             }
-            assemble_row(M, rp + r, ubuf, a);
+            for (int64_t k = M.asm_ptr[rp + r]; k < M.asm_ptr[rp + r + 1]; ++k) {
+                const double* u = ul + 3 * (M.asm_idx[k] - ubase);
+                a[0] += u[0];
+                a[1] += u[1];
+                a[2] += u[2];
+            }
             cv[3 * r + 0] = a[0];
             cv[3 * r + 1] = a[1];
             cv[3 * r + 2] = a[2];
@@ -89,10 +103,14 @@ k_fwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, So
                 double* y = ysol + 3ll * (c0 + r);
                 y[0] = t0; y[1] = t1; y[2] = t2;
             } else {
-                double* u = ubuf + 3 * (M.uoff[s] + (r - ns));
-                u[0] = cv[3 * r + 0] + t0;
-                u[1] = cv[3 * r + 1] + t1;
-                u[2] = cv[3 * r + 2] + t2;
+                const int64_t ui = M.uoff[s] + (r - ns);
+                const double u0 = cv[3 * r + 0] + t0, u1 = cv[3 * r + 1] + t1, u2 = cv[3 * r + 2] + t2;
+                double* u = ul + 3 * (ui - ubase);
+                u[0] = u0; u[1] = u1; u[2] = u2;
+                if (s == root) {
+                    double* ug = ubuf + 3 * ui;
+                    ug[0] = u0; ug[1] = u1; ug[2] = u2;
+                }
             }
         }
         __syncthreads();
@@ -176,7 +194,7 @@ k_fwd_level(const int2* __restrict__ tasks, int R, SolveMaps M, const double* __
 template <int CPW>
 __global__ void __launch_bounds__(kSolveThreads)
 k_bwd_level(const int2* __restrict__ tasks, SolveMaps M, const double* __restrict__ G,
-            const double* __restrict__ ysol, double* __restrict__ xsol) {
+            const double* __restrict__ ysol, double* __restrict__ xsol, double* __restrict__ out) {
     __shared__ double vv[kBwdChunk * 3];
     const int2 task = tasks[blockIdx.x];
     const int s = task.x, cb = task.y;
@@ -225,24 +243,40 @@ k_bwd_level(const int2* __restrict__ tasks, SolveMaps M, const double* __restric
         if (lane == 0 && j < ns) {
             double* x = xsol + 3ll * (c0 + j);
             x[0] = a0; x[1] = a1; x[2] = a2;
+            double* xo = out + 3ll * M.perm[c0 + j];
+            xo[0] = a0; xo[1] = a1; xo[2] = a2;
         }
     }
 }
 
+// The subtree's own solution rows stay in LDS too (its supernodes' columns are the
+// contiguous range [col0[first], col0[root+1])); rows above the subtree come from xsol.
+// Results go to xsol and, in the caller's numbering, to out.
 __global__ void __launch_bounds__(kSolveThreads)
 k_bwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, SolveMaps M,
-           const double* __restrict__ G, const double* __restrict__ ysol, double* __restrict__ xsol) {
-    __shared__ double vv[kSmallFront * 3];
+           const double* __restrict__ G, const double* __restrict__ ysol, double* __restrict__ xsol,
+           double* __restrict__ out) {
+    extern __shared__ double dyn[];
+    double* vv = dyn;                     // kSmallFront x 3
+    double* xl = dyn + kSmallFront * 3;   // the subtree's columns x 3
     const int t = blockIdx.x;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int q = leaf_ptr[t + 1] - 1; q >= leaf_ptr[t]; --q) {
+    const int q0 = leaf_ptr[t], q1 = leaf_ptr[t + 1];
+    const int cfirst = M.col0[leaf_sn[q0]], cend = M.col0[leaf_sn[q1 - 1] + 1];
+    for (int q = q1 - 1; q >= q0; --q) {
         const int s = leaf_sn[q];
         const int c0 = M.col0[s];
         const int ns = M.col0[s + 1] - c0;
         const int f = M.fsize[s];
         const int64_t rp = M.rowptr[s];
         for (int r = threadIdx.x; r < f; r += kSolveThreads) {
-            const double* src = r < ns ? ysol + 3ll * (c0 + r) : xsol + 3ll * M.rows[rp + r];
+            const double* src;
+            if (r < ns) {
+                src = ysol + 3ll * (c0 + r);
+            } else {
+                const int row = M.rows[rp + r];
+                src = (row >= cfirst && row < cend) ? xl + 3 * (row - cfirst) : xsol + 3ll * row;
+            }
             vv[3 * r + 0] = src[0];
             vv[3 * r + 1] = src[1];
             vv[3 * r + 2] = src[2];
@@ -262,22 +296,14 @@ k_bwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, So
             a1 = wave_sum(a1);
             a2 = wave_sum(a2);
             if (lane == 0) {
-                double* x = xsol + 3ll * (c0 + j);
+                double* x = xl + 3 * (c0 + j - cfirst);
                 x[0] = a0; x[1] = a1; x[2] = a2;
+                double* xo = out + 3ll * M.perm[c0 + j];
+                xo[0] = a0; xo[1] = a1; xo[2] = a2;
             }
         }
         __syncthreads();
     }
-}
-
-__global__ void __launch_bounds__(kSolveThreads)
-k_unpermute(int n, const int* __restrict__ iperm, const double* __restrict__ xsol, double* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double* s = xsol + 3ll * iperm[i];
-    out[3 * i + 0] = s[0];
-    out[3 * i + 1] = s[1];
-    out[3 * i + 2] = s[2];
 }
 
 // ---------------------------------------------------------------------------------
@@ -306,6 +332,25 @@ int chol_prepare_device(wrapd_ctx* c) {
     rc |= up(c, c->d_asm_ptr, S.asm_ptr);
     rc |= up(c, c->d_asm_idx, S.asm_idx);
     rc |= up(c, c->d_leaf_ptr, S.leaf_ptr);
+    // LDS of the leaf kernels: the largest subtree's update rows (forward) and columns
+    // (backward); a subtree must be a contiguous supernode range
+    int64_t urows = 0, cols = 0;
+    for (size_t t = 0; t + 1 < S.leaf_ptr.size(); ++t) {
+        const int first = S.leaf_sn[S.leaf_ptr[t]], root = S.leaf_sn[S.leaf_ptr[t + 1] - 1];
+        for (int q = S.leaf_ptr[t]; q < S.leaf_ptr[t + 1]; ++q)
+            if (S.leaf_sn[q] != first + (q - S.leaf_ptr[t])) {
+                c->last_error = "leaf subtree is not a contiguous supernode range";
+                return WRAPD_ERR_STATE;
+            }
+        urows = std::max<int64_t>(urows, S.uoff[root + 1] - S.uoff[first]);
+        cols = std::max<int64_t>(cols, S.col0[root + 1] - S.col0[first]);
+    }
+    c->leaf_lds_fwd = static_cast<size_t>(kSmallFront + urows) * 3 * sizeof(double);
+    c->leaf_lds_bwd = static_cast<size_t>(kSmallFront + cols) * 3 * sizeof(double);
+    if (c->leaf_lds_fwd > 65536 || c->leaf_lds_bwd > 65536) {
+        c->last_error = "leaf subtree too large for the solve kernels' LDS";
+        return WRAPD_ERR_ARG;
+    }
     rc |= up(c, c->d_leaf_sn, S.leaf_sn);
     // level task lists: forward (s, row block) with R rows per task, backward
     // (s, column block) with 4 CPW columns per task, both chosen per level
@@ -364,8 +409,8 @@ void launch_solve(wrapd_ctx* c, const double* b, double* x) {
     {
         ProfScope ps(c, "solve_fwd");
         if (nleaf > 0)
-            hipLaunchKernelGGL(k_fwd_leaf, dim3(nleaf), dim3(kSolveThreads), 0, c->stream, c->d_leaf_ptr, c->d_leaf_sn,
-                               M, b, c->G, c->ysol, c->ubuf);
+            hipLaunchKernelGGL(k_fwd_leaf, dim3(nleaf), dim3(kSolveThreads), c->leaf_lds_fwd, c->stream, c->d_leaf_ptr,
+                               c->d_leaf_sn, M, b, c->G, c->ysol, c->ubuf);
         for (int l = 0; l < nlev; ++l) {
             const int n = c->fwd_task_ptr[l + 1] - c->fwd_task_ptr[l];
             if (n > 0)
@@ -380,17 +425,15 @@ void launch_solve(wrapd_ctx* c, const double* b, double* x) {
             if (n == 0) continue;
             const int2* t = bt + c->bwd_task_ptr[l];
             if (c->bwd_cpw[l] == 4)
-                hipLaunchKernelGGL(k_bwd_level<4>, dim3(n), dim3(kSolveThreads), 0, c->stream, t, M, c->G, c->ysol, c->xsol);
+                hipLaunchKernelGGL(k_bwd_level<4>, dim3(n), dim3(kSolveThreads), 0, c->stream, t, M, c->G, c->ysol, c->xsol, x);
             else if (c->bwd_cpw[l] == 2)
-                hipLaunchKernelGGL(k_bwd_level<2>, dim3(n), dim3(kSolveThreads), 0, c->stream, t, M, c->G, c->ysol, c->xsol);
+                hipLaunchKernelGGL(k_bwd_level<2>, dim3(n), dim3(kSolveThreads), 0, c->stream, t, M, c->G, c->ysol, c->xsol, x);
             else
-                hipLaunchKernelGGL(k_bwd_level<1>, dim3(n), dim3(kSolveThreads), 0, c->stream, t, M, c->G, c->ysol, c->xsol);
+                hipLaunchKernelGGL(k_bwd_level<1>, dim3(n), dim3(kSolveThreads), 0, c->stream, t, M, c->G, c->ysol, c->xsol, x);
         }
         if (nleaf > 0)
-            hipLaunchKernelGGL(k_bwd_leaf, dim3(nleaf), dim3(kSolveThreads), 0, c->stream, c->d_leaf_ptr, c->d_leaf_sn,
-                               M, c->G, c->ysol, c->xsol);
-        hipLaunchKernelGGL(k_unpermute, dim3(blocks_for(c->n_f, kSolveThreads)), dim3(kSolveThreads), 0, c->stream,
-                           c->n_f, c->d_iperm, c->xsol, x);
+            hipLaunchKernelGGL(k_bwd_leaf, dim3(nleaf), dim3(kSolveThreads), c->leaf_lds_bwd, c->stream, c->d_leaf_ptr,
+                               c->d_leaf_sn, M, c->G, c->ysol, c->xsol, x);
     }
 }
 
