@@ -88,7 +88,7 @@ extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
                 const int j = y;
                 const int64_t gc = PL.cbase[s] + j;
                 for (int r = j; r < f; ++r) wrF(Fi(r, j), 0.0);
-                if (j < ns) for (int r = 0; r < f; ++r) wrG(Gi(r, j), 0.0);
+                if (j < ns) for (int r = 0; r < ns; ++r) wrG(Gi(r, j), 0.0);   // L11^{-1} rows only
                 for (int64_t a = PL.acol_ptr[gc]; a < PL.acol_ptr[gc + 1]; ++a)
                     wrF(Fi(PL.acol[2 * a], j), ax[PL.acol[2 * a + 1]]);
                 for (int64_t a = PL.ccol_ptr[gc]; a < PL.ccol_ptr[gc + 1]; ++a) {

@@ -22,6 +22,7 @@
 // consecutive lanes on consecutive rows, so tile stores are coalesced.  All FP64.
 #include "wd_ctx.hpp"
 #include "chol_plan.hpp"
+#include "device/wd_reduce.hpp"
 
 namespace wd {
 
@@ -66,9 +67,9 @@ k_fassemble(const int2* __restrict__ tasks, FactArgs A, const double* __restrict
     const int ns = A.col0[s + 1] - A.col0[s];
     double* F = A.pool + A.foff[s] + static_cast<int64_t>(j) * f;
     for (int r = j + lane; r < f; r += 64) F[r] = 0.0;
-    if (j < ns) {
+    if (j < ns) {   // L11^{-1} rows only: G's bottom rows are assigned whole later
         double* Gc = A.G + A.goff[s] + static_cast<int64_t>(j) * f;
-        for (int r = lane; r < f; r += 64) Gc[r] = 0.0;
+        for (int r = lane; r < ns; r += 64) Gc[r] = 0.0;
     }
     __syncthreads();
     const int64_t gc = cbase[s] + j;
