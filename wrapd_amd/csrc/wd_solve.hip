@@ -39,6 +39,35 @@ struct SolveMaps {
     const int* perm;
 };
 
+// t += sum_k g[k * gstride] * v[k * vstride + 0..2], k < n.  Loads are issued in batches
+// of 8 independent reads before they are used, so a thread has 8 misses in flight
+// instead of one: the solves' dense products are latency-bound otherwise.
+__device__ __forceinline__ void strided_dot3(const double* __restrict__ g, int64_t gstride,
+                                             const double* __restrict__ v, int vstride, int n, double& t0,
+                                             double& t1, double& t2) {
+    constexpr int B = 8;
+    int k = 0;
+    for (; k + B <= n; k += B) {
+        double gv[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) gv[u] = g[static_cast<int64_t>(k + u) * gstride];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const double* vp = v + (k + u) * vstride;
+            t0 += gv[u] * vp[0];
+            t1 += gv[u] * vp[1];
+            t2 += gv[u] * vp[2];
+        }
+    }
+    for (; k < n; ++k) {
+        const double gk = g[static_cast<int64_t>(k) * gstride];
+        const double* vp = v + k * vstride;
+        t0 += gk * vp[0];
+        t1 += gk * vp[1];
+        t2 += gk * vp[2];
+    }
+}
+
 __device__ __forceinline__ void assemble_row(const SolveMaps& M, int64_t prow, const double* __restrict__ ubuf,
                                              double a[3]) {
     for (int64_t q = M.asm_ptr[prow]; q < M.asm_ptr[prow + 1]; ++q) {
@@ -93,12 +122,7 @@ k_fwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, So
         for (int r = threadIdx.x; r < f; r += kSolveThreads) {
             double t0 = 0.0, t1 = 0.0, t2 = 0.0;
             const int jend = r < ns ? r + 1 : ns;   // L11^{-1} is lower triangular
-            for (int j = 0; j < jend; ++j) {
-                const double g = Gs[static_cast<int64_t>(j) * f + r];
-                t0 += g * cv[3 * j + 0];
-                t1 += g * cv[3 * j + 1];
-                t2 += g * cv[3 * j + 2];
-            }
+            strided_dot3(Gs + r, f, cv, 3, jend, t0, t1, t2);
             if (r < ns) {
                 double* y = ysol + 3ll * (c0 + r);
                 y[0] = t0; y[1] = t1; y[2] = t2;
@@ -146,22 +170,9 @@ k_fwd_level(const int2* __restrict__ tasks, int R, SolveMaps M, const double* __
     if (r < f) {
         const double* Gs = G + M.goff[s] + r;
         const int jend = r < ns ? r + 1 : ns;   // L11^{-1} is lower triangular
-        int j = g;
-        for (; j + 3 * ng < jend; j += 4 * ng) {
-            const double g0 = Gs[static_cast<int64_t>(j) * f];
-            const double g1 = Gs[static_cast<int64_t>(j + ng) * f];
-            const double g2 = Gs[static_cast<int64_t>(j + 2 * ng) * f];
-            const double g3 = Gs[static_cast<int64_t>(j + 3 * ng) * f];
-            t0 += g0 * cv[3 * j + 0] + g1 * cv[3 * (j + ng) + 0] + g2 * cv[3 * (j + 2 * ng) + 0] + g3 * cv[3 * (j + 3 * ng) + 0];
-            t1 += g0 * cv[3 * j + 1] + g1 * cv[3 * (j + ng) + 1] + g2 * cv[3 * (j + 2 * ng) + 1] + g3 * cv[3 * (j + 3 * ng) + 1];
-            t2 += g0 * cv[3 * j + 2] + g1 * cv[3 * (j + ng) + 2] + g2 * cv[3 * (j + 2 * ng) + 2] + g3 * cv[3 * (j + 3 * ng) + 2];
-        }
-        for (; j < jend; j += ng) {
-            const double gv = Gs[static_cast<int64_t>(j) * f];
-            t0 += gv * cv[3 * j + 0];
-            t1 += gv * cv[3 * j + 1];
-            t2 += gv * cv[3 * j + 2];
-        }
+        if (g < jend)   // columns g, g + ng, ... below jend
+            strided_dot3(Gs + static_cast<int64_t>(g) * f, static_cast<int64_t>(ng) * f, cv + 3 * g, 3 * ng,
+                         (jend - g + ng - 1) / ng, t0, t1, t2);
     }
     if (ng > 1) {
         red[3 * threadIdx.x + 0] = t0;
@@ -224,12 +235,10 @@ k_bwd_level(const int2* __restrict__ tasks, SolveMaps M, const double* __restric
             if (j < ns) {
                 const double* gc = Gs + static_cast<int64_t>(j) * f + r0;
                 const int rstart = j > r0 ? j - r0 : 0;   // rows above the diagonal are zero
-                for (int r = rstart + lane; r < rc; r += 64) {
-                    const double gv = gc[r];
-                    acc[jj][0] += gv * vv[3 * r + 0];
-                    acc[jj][1] += gv * vv[3 * r + 1];
-                    acc[jj][2] += gv * vv[3 * r + 2];
-                }
+                const int r1 = rstart + lane;
+                if (r1 < rc)
+                    strided_dot3(gc + r1, 64, vv + 3 * r1, 3 * 64, (rc - r1 + 63) / 64, acc[jj][0], acc[jj][1],
+                                 acc[jj][2]);
             }
         }
         __syncthreads();
@@ -286,12 +295,8 @@ k_bwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, So
         for (int j = w; j < ns; j += kSolveThreads / 64) {
             double a0 = 0.0, a1 = 0.0, a2 = 0.0;
             const double* gc = Gs + static_cast<int64_t>(j) * f;
-            for (int r = j + lane; r < f; r += 64) {
-                const double g = gc[r];
-                a0 += g * vv[3 * r + 0];
-                a1 += g * vv[3 * r + 1];
-                a2 += g * vv[3 * r + 2];
-            }
+            const int r1 = j + lane;
+            if (r1 < f) strided_dot3(gc + r1, 64, vv + 3 * r1, 3 * 64, (f - r1 + 63) / 64, a0, a1, a2);
             a0 = wave_sum(a0);
             a1 = wave_sum(a1);
             a2 = wave_sum(a2);
