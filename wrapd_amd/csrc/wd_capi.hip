@@ -20,7 +20,8 @@ wrapd_ctx::~wrapd_ctx() {
     for (int i = 0; i < wd::kMaxHist; ++i) { f(hS[i]); f(hY[i]); }
     f(v2e_ptr); f(v2e_idx); f(a_cptr); f(a_cidx); f(Ax);
     f(G); f(ysol); f(xsol); f(ubuf); f(d_perm); f(d_iperm); f(d_col0); f(d_fsize); f(d_rowptr); f(d_rows);
-    f(d_goff); f(d_uoff); f(d_asm_ptr); f(d_asm_idx); f(d_leaf_ptr); f(d_leaf_sn); f(d_fwd_tasks); f(d_bwd_tasks);
+    f(d_goff); f(d_uoff); f(d_asm_ptr); f(d_asm_idx); f(d_udest); f(d_leaf_ptr); f(d_leaf_sn); f(d_flow_f); f(d_flow_b);
+    f(d_flow_deps); f(d_flow_flags); f(d_flow_abort);
     f(pool); f(d_foff); f(d_cbase); f(d_acol_ptr); f(d_acol); f(d_ccol_ptr); f(d_ccol); f(d_relptr); f(d_rel);
     f(d_ftasks); f(d_ffail);
     f(mlp_master); f(mlp_m1); f(mlp_m2); f(mlp_gsum); f(mlp_norm); f(mlp_bf); f(mlp_loss); f(mlp_partial);
@@ -544,6 +545,7 @@ int wrapd_solve3(wrapd_ctx* c, const double* b, double* x) {
     if (h2d(c, c->bv, b, 3ull * c->n_f)) return WRAPD_ERR_HIP;
     launch_solve(c, c->bv, c->rv);
     if (d2h(c, x, c->rv, 3ull * c->n_f)) return WRAPD_ERR_HIP;
+    if (solve_aborted(c)) { c->last_error = "triangular solve: a dependency wait timed out"; return WRAPD_ERR_HIP; }
     return launch_status(c);
 }
 
@@ -656,6 +658,10 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
             dir = d[0];
         }
         if (std::isnan(dir)) {
+            if (solve_aborted(c)) {
+                c->last_error = "triangular solve: a dependency wait timed out";
+                return WRAPD_ERR_HIP;
+            }
             c->last_error = "Decrement is NaN.";
             return WRAPD_ERR_NAN;
         }
@@ -779,6 +785,10 @@ int wrapd_global_step_unaware(wrapd_ctx* c, wrapd_global_out* out) {
     launch_finalize(c, elem_blocks(c), 3, 0u, S_RED);
     const double* d = scal_read(c, S_RED, 3);
     if (!d) return WRAPD_ERR_HIP;
+    if (std::isnan(d[1]) && solve_aborted(c)) {
+        c->last_error = "triangular solve: a dependency wait timed out";
+        return WRAPD_ERR_HIP;
+    }
     wrapd_global_out o{};
     o.iters = 1;
     o.energy = d[1];

@@ -103,7 +103,7 @@ struct wrapd_ctx {
     double* G = nullptr;
     double* ysol = nullptr;     // n_f x 3 forward result (new numbering)
     double* xsol = nullptr;     // n_f x 3 backward result (new numbering)
-    double* ubuf = nullptr;     // forward update rows x 3
+    double* ubuf = nullptr;     // forward update rows x 3, each at its slot (d_udest)
     int* d_perm = nullptr;
     int* d_iperm = nullptr;
     int* d_col0 = nullptr;
@@ -114,13 +114,20 @@ struct wrapd_ctx {
     int64_t* d_uoff = nullptr;
     int64_t* d_asm_ptr = nullptr;
     int64_t* d_asm_idx = nullptr;
+    int64_t* d_udest = nullptr;
     int* d_leaf_ptr = nullptr;
     int* d_leaf_sn = nullptr;
-    int* d_fwd_tasks = nullptr;  // per level: (sn, block of fwd_rows[l] rows)
-    int* d_bwd_tasks = nullptr;  // per level: (sn, block of 4 bwd_cpw[l] columns)
-    std::vector<int> fwd_rows, bwd_cpw;
     size_t leaf_lds_fwd = 0, leaf_lds_bwd = 0;   // dynamic LDS of the leaf-subtree kernels
-    std::vector<int> fwd_task_ptr, bwd_task_ptr;
+    // dataflow schedule of the supernodes above the leaf subtrees: one launch per
+    // direction, tasks ordered so that each waits only on earlier tasks
+    void* d_flow_f = nullptr;                    // forward tasks
+    void* d_flow_b = nullptr;                    // backward tasks
+    int* d_flow_deps = nullptr;                  // forward: the tasks each task waits for
+    unsigned* d_flow_flags = nullptr;            // per task (forward, then backward): epoch when done
+    int* d_flow_abort = nullptr;                 // sticky: a bounded wait ran out
+    int n_flow_f = 0, n_flow_b = 0;
+    std::vector<std::pair<int, int>> bwd_segs;   // backward launches: (first task, columns per wave)
+    unsigned flow_epoch = 0;
     // numeric factorization (plan from chol_plan, fronts in one pool)
     wrapd::FactorPlan fplan;
     double* pool = nullptr;
@@ -209,6 +216,7 @@ int finalize_blocks_n(wrapd_ctx* c);
 // (wd_solve.hip)
 int chol_prepare_device(wrapd_ctx* c);
 void launch_solve(wrapd_ctx* c, const double* b, double* x);      // n_f x 3 device arrays
+int solve_aborted(wrapd_ctx* c);   // nonzero once a solve's dependency wait has timed out (syncs)
 // (wd_mlp.hip)
 MlpDevice mlp_device(wrapd_ctx* c);
 int mlp_upload(wrapd_ctx* c, const float* params, const float* norm);

@@ -9,13 +9,15 @@
 // gathered by the parent from precomputed lists (deterministic, no atomics).
 //
 // Schedule: small supernodes are grouped into leaf subtrees, each walked by one
-// workgroup in postorder; the remaining supernodes run level by level.  A level with
-// a few wide supernodes must still fill 256 CUs, so each level picks its task size:
-//   forward  tasks (supernode, block of R rows), R in {256, 128, 64, 32}: the
-//            256 threads are 256/R column groups, reduced in LDS in a fixed order;
-//   backward tasks (supernode, block of 4 CPW columns), CPW columns per wave in {4, 2, 1}.
-// No task depends on another task of its launch (no cross-workgroup reduction: on this
-// 8-XCD part a device-scope fence costs an L2 writeback).
+// workgroup in postorder.  The remaining supernodes are split into tasks whose size is
+// chosen per level of the supernodal tree, so that a level with a few wide supernodes
+// still fills 256 CUs:
+//   forward  tasks (supernode, block of R rows), R in {256, ..., 16}: the 256 threads
+//            are 256/R column groups, reduced in LDS in a fixed order;
+//   backward tasks (supernode, block of 4 cpw columns), cpw columns per wave in {4, 2, 1}.
+// These tasks run as one dataflow launch per direction (see k_fwd_flow): a task waits for
+// the supernodes it reads, not for the end of its level.  Sums are taken in a fixed
+// order, so results do not depend on timing.
 #include "wd_ctx.hpp"
 #include "device/wd_reduce.hpp"
 
@@ -26,6 +28,8 @@ constexpr int kBwdChunk = 2048;       // rows of [y_J; x_R] staged in LDS at a t
 constexpr int kLevelTasks = 512;      // task count a level aims for
 constexpr int kNsMax = 1024;          // supernode width limit (symbolic ns_max)
 constexpr int kSmallFront = 256;      // leaf-subtree front limit (symbolic small_front)
+constexpr int kFwdPrefetch = 16;      // G terms per thread loaded before a forward task's wait
+constexpr int kBwdPrefetch = 16;      // G terms per thread loaded before a backward task's wait (all its columns)
 
 struct SolveMaps {
     const int* col0;
@@ -37,11 +41,13 @@ struct SolveMaps {
     const int64_t* asm_ptr;
     const int64_t* asm_idx;
     const int* perm;
+    const int64_t* udest;   // update row -> its slot (its position in asm_idx)
 };
 
-// t += sum_k g[k * gstride] * v[k * vstride + 0..2], k < n.  Loads are issued in batches
-// of 8 independent reads before they are used, so a thread has 8 misses in flight
-// instead of one: the solves' dense products are latency-bound otherwise.
+// t += sum_k g[k * gstride] * v[k * vstride + 0..2], k < n, in order of k.  Loads are
+// issued in batches of 8 independent reads before they are used (the last batch
+// predicated), so a thread has up to 8 misses in flight instead of one: the solves'
+// dense products are latency-bound otherwise.
 __device__ __forceinline__ void strided_dot3(const double* __restrict__ g, int64_t gstride,
                                              const double* __restrict__ v, int vstride, int n, double& t0,
                                              double& t1, double& t2) {
@@ -59,33 +65,29 @@ __device__ __forceinline__ void strided_dot3(const double* __restrict__ g, int64
             t2 += gv[u] * vp[2];
         }
     }
-    for (; k < n; ++k) {
-        const double gk = g[static_cast<int64_t>(k) * gstride];
-        const double* vp = v + k * vstride;
-        t0 += gk * vp[0];
-        t1 += gk * vp[1];
-        t2 += gk * vp[2];
-    }
-}
-
-__device__ __forceinline__ void assemble_row(const SolveMaps& M, int64_t prow, const double* __restrict__ ubuf,
-                                             double a[3]) {
-    for (int64_t q = M.asm_ptr[prow]; q < M.asm_ptr[prow + 1]; ++q) {
-        const double* u = ubuf + 3 * M.asm_idx[q];
-        a[0] += u[0];
-        a[1] += u[1];
-        a[2] += u[2];
+    if (k < n) {
+        double gv[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) gv[u] = k + u < n ? g[static_cast<int64_t>(k + u) * gstride] : 0.0;
+#pragma unroll
+        for (int u = 0; u < B; ++u)
+            if (k + u < n) {
+                const double* vp = v + (k + u) * vstride;
+                t0 += gv[u] * vp[0];
+                t1 += gv[u] * vp[1];
+                t2 += gv[u] * vp[2];
+            }
     }
 }
 
 // Leaf subtrees: one workgroup walks a whole subtree in postorder.  A subtree is a
 // contiguous range of supernodes (postorder), and every update row it produces except its
 // root's is consumed inside it, so those rows stay in LDS (dynamic, sized by the largest
-// subtree); only the root's rows go to the global buffer for the level kernels.
+// subtree); only the root's rows go to global memory, to their slots.
 __global__ void __launch_bounds__(kSolveThreads)
 k_fwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, SolveMaps M,
            const double* __restrict__ b, const double* __restrict__ G, double* __restrict__ ysol,
-           double* __restrict__ ubuf) {
+           double* __restrict__ slots) {
     extern __shared__ double dyn[];
     double* cv = dyn;                     // kSmallFront x 3
     double* ul = dyn + kSmallFront * 3;   // the subtree's update rows x 3
@@ -132,7 +134,7 @@ k_fwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, So
                 double* u = ul + 3 * (ui - ubase);
                 u[0] = u0; u[1] = u1; u[2] = u2;
                 if (s == root) {
-                    double* ug = ubuf + 3 * ui;
+                    double* ug = slots + 3 * M.udest[ui];
                     ug[0] = u0; ug[1] = u1; ug[2] = u2;
                 }
             }
@@ -141,119 +143,379 @@ k_fwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, So
     }
 }
 
-// forward: task (s, row block rb) with R rows; thread (g, r) = (tid / R, tid % R) sums
-// columns j = g, g + 256/R, ... of row r
-__global__ void __launch_bounds__(kSolveThreads)
-k_fwd_level(const int2* __restrict__ tasks, int R, SolveMaps M, const double* __restrict__ b,
-            const double* __restrict__ G, double* __restrict__ ysol, double* __restrict__ ubuf) {
-    __shared__ double cv[kNsMax * 3];
-    __shared__ double red[kSolveThreads * 3];
-    const int2 task = tasks[blockIdx.x];
-    const int s = task.x, rb = task.y;
-    const int c0 = M.col0[s];
-    const int ns = M.col0[s + 1] - c0;
-    const int f = M.fsize[s];
-    const int64_t rp = M.rowptr[s];
-    for (int j = threadIdx.x; j < ns; j += kSolveThreads) {
-        const int old = M.perm[c0 + j];
-        double a[3] = {b[3 * old + 0], b[3 * old + 1], b[3 * old + 2]};
-        assemble_row(M, rp + j, ubuf, a);
-        cv[3 * j + 0] = a[0];
-        cv[3 * j + 1] = a[1];
-        cv[3 * j + 2] = a[2];
+// ---------------------------------------------------------------------------------
+// Dataflow schedule above the leaf subtrees: all forward tasks form one launch, and the
+// backward tasks one launch per task width.  A task waits for the tasks it reads from,
+// not for the end of a level: a level boundary costs a launch (~2.3 us) plus the
+// dependent loads of its first tasks (~1.2 us each on data the previous launch wrote).
+//  - Task i is workgroup i.  Forward tasks are ordered bottom-up and wait for the tasks
+//    of their non-leaf children; backward tasks are ordered top-down and wait for the
+//    tasks of their parent, whose completion implies its ancestors'.  So a task only
+//    waits on lower-numbered workgroups, which are dispatched before it.
+//  - A finished task stores the solve's epoch in its own flag.  There is no shared
+//    counter or ticket: atomics on one address serialize at ~25 ns each, which on
+//    thousands of tasks costs more than the levels themselves.
+//  - Rows handed from one task to another in the same launch are stored and loaded at
+//    agent scope (written through to memory, read past this XCD's L2).  A task waits for
+//    its stores to complete before it sets its flag.
+//  - Every wait is bounded.  When one expires it sets a sticky abort flag, and from then
+//    on tasks write NaN instead of waiting; the host reports the flag.
+// Operands that no task of the launch writes (G, b, the maps) are loaded before the
+// wait, so their latency overlaps the children's work.
+struct SolveTask {
+    int s, first, width, ns;   // forward: rows [first, first + width); backward: columns, width = 4 x cpw
+    int f, c0, dep0, ndep;     // forward: tasks deps[dep0, dep0 + ndep); backward: tasks [dep0, dep0 + ndep)
+    long long goff, rowptr;
+    long long uoff;
+    int pad0, pad1;
+};
+static_assert(sizeof(SolveTask) == 64, "SolveTask layout");
+static_assert(kBwdChunk % kSolveThreads == 0 && kNsMax % kSolveThreads == 0, "staging layout");
+
+constexpr int kSpinLimit = 1 << 20;   // polls (about a second) before a wait gives up
+
+__device__ __forceinline__ double ld_agent(const double* p) {
+    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits until *flag == epoch.  False when this wait or another one has given up.
+__device__ bool wait_flag(unsigned* flag, unsigned epoch, int* abortf) {
+    for (int polls = 1; __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch; ++polls) {
+        if ((polls & 63) == 0) {
+            if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+            if (polls >= kSpinLimit) {
+                __hip_atomic_store(abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return false;
+            }
+        }
+        __builtin_amdgcn_s_sleep(2);
     }
+    return true;
+}
+
+// Marks the task done once every thread's stores have completed.
+__device__ __forceinline__ void finish_task(unsigned* flag, unsigned epoch) {
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    const int ng = kSolveThreads / R;
+    if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Forward task (s, rows [first, first + R)): thread (g, r) = (tid / R, tid % R) sums
+// columns j = g, g + 256/R, ... of row r; the 256/R partial sums are added in LDS in a
+// fixed order.  Update rows travel in `slots`: row i of u_s is written once, to
+// slots[udest[i]], inside its parent front row's range [asm_ptr[p], asm_ptr[p + 1]).  A
+// task's inputs are therefore two contiguous slot ranges (those of the supernode's pivot
+// rows and those of the task's update rows), loaded in parallel rounds of kStageSlots
+// into LDS and then summed per row in slot order.  B: G terms per thread loaded before
+// the wait.
+template <int B>
+__global__ void __launch_bounds__(kSolveThreads)
+k_fwd_flow(const SolveTask* __restrict__ tasks, const int* __restrict__ deps, unsigned* flags, unsigned epoch,
+           int* abortf, SolveMaps M, const double* __restrict__ b, const double* __restrict__ G,
+           double* __restrict__ ysol, double* __restrict__ slots) {
+    constexpr int kRowsPerThread = kNsMax / kSolveThreads;
+    constexpr int kStageSlots = 4 * kSolveThreads;
+    __shared__ double cv[kNsMax * 3];
+    __shared__ double sred[kStageSlots * 3];   // staged slots, then the column-group partial sums
+    __shared__ int sh_ok;
+    if (threadIdx.x == 0) sh_ok = __hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 0 : 1;
+    const SolveTask T = tasks[blockIdx.x];
+    const int ns = T.ns, f = T.f, c0 = T.c0;
+    const int R = T.width, ng = kSolveThreads / R;
     const int g = threadIdx.x / R, rl = threadIdx.x - g * R;
-    const int r = rb * R + rl;
-    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-    if (r < f) {
-        const double* Gs = G + M.goff[s] + r;
-        const int jend = r < ns ? r + 1 : ns;   // L11^{-1} is lower triangular
-        if (g < jend)   // columns g, g + ng, ... below jend
-            strided_dot3(Gs + static_cast<int64_t>(g) * f, static_cast<int64_t>(ng) * f, cv + 3 * g, 3 * ng,
-                         (jend - g + ng - 1) / ng, t0, t1, t2);
+    const int r = T.first + rl;
+    // Before the wait: b of the pivot rows into LDS, the slot ranges, this thread's
+    // first G terms and, for an update row, where its result goes.
+    int64_t qlo[kRowsPerThread], qhi[kRowsPerThread];
+    int old[kRowsPerThread];
+#pragma unroll
+    for (int k = 0; k < kRowsPerThread; ++k) {   // independent loads first, then their uses
+        const int j = threadIdx.x + k * kSolveThreads;
+        const bool in = j < ns;
+        old[k] = in ? M.perm[c0 + j] : 0;
+        qlo[k] = in ? M.asm_ptr[T.rowptr + j] : 0;
+        qhi[k] = in ? M.asm_ptr[T.rowptr + j + 1] : 0;
     }
-    if (ng > 1) {
-        red[3 * threadIdx.x + 0] = t0;
-        red[3 * threadIdx.x + 1] = t1;
-        red[3 * threadIdx.x + 2] = t2;
-        __syncthreads();
-        if (g != 0) return;
-        for (int k = 1; k < ng; ++k) {
-            const double* o = red + 3 * (k * R + rl);
-            t0 += o[0];
-            t1 += o[1];
-            t2 += o[2];
+#pragma unroll
+    for (int k = 0; k < kRowsPerThread; ++k) {
+        const int j = threadIdx.x + k * kSolveThreads;
+        double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+        if (j < ns) {
+            b0 = b[3 * old[k] + 0];
+            b1 = b[3 * old[k] + 1];
+            b2 = b[3 * old[k] + 2];
+        }
+        if (j < ns) {
+            cv[3 * j + 0] = b0;
+            cv[3 * j + 1] = b1;
+            cv[3 * j + 2] = b2;
         }
     }
-    if (r >= f) return;
-    if (r < ns) {
+    const int64_t qa = M.asm_ptr[T.rowptr], np = M.asm_ptr[T.rowptr + ns] - qa;
+    const int ur0 = max(T.first, ns), ur1 = min(T.first + R, f);
+    const int64_t uq0 = ur0 < ur1 ? M.asm_ptr[T.rowptr + ur0] : 0;
+    const int64_t nu = ur0 < ur1 ? M.asm_ptr[T.rowptr + ur1] - uq0 : 0;
+    const int jend = r < ns ? r + 1 : ns;   // L11^{-1} is lower triangular
+    const int nterm = (r < f && g < jend) ? (jend - g + ng - 1) / ng : 0;
+    const double* gp = G + T.goff + r + static_cast<int64_t>(g) * f;
+    const int64_t gstride = static_cast<int64_t>(ng) * f;
+    double gv[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) gv[u] = u < nterm ? gp[u * gstride] : 0.0;
+    const bool urow = g == 0 && r >= ns && r < f;
+    int64_t ulo = 0, uhi = 0, udst = 0;
+    if (urow) {   // as indices into the staged sequence: pivot slots first, then update-row slots
+        ulo = np + M.asm_ptr[T.rowptr + r] - uq0;
+        uhi = np + M.asm_ptr[T.rowptr + r + 1] - uq0;
+        udst = M.udest[T.uoff + (r - ns)];
+    }
+    __syncthreads();   // sh_ok is set
+    if (sh_ok)
+        for (int k = threadIdx.x; k < T.ndep; k += kSolveThreads)
+            if (!wait_flag(flags + deps[T.dep0 + k], epoch, abortf)) sh_ok = 0;
+    __syncthreads();
+    // After the wait: the children's updates, staged and summed in slot order.
+    double ua0 = 0.0, ua1 = 0.0, ua2 = 0.0;
+    const int64_t ntot = np + nu;
+    for (int64_t base = 0; base < ntot; base += kStageSlots) {
+        double v[kStageSlots / kSolveThreads][3];
+#pragma unroll
+        for (int k = 0; k < kStageSlots / kSolveThreads; ++k) {
+            const int64_t i = base + threadIdx.x + k * kSolveThreads;
+            v[k][0] = v[k][1] = v[k][2] = 0.0;
+            if (i < ntot) {
+                const double* sp = slots + 3 * (i < np ? qa + i : uq0 + (i - np));
+                v[k][0] = ld_agent(sp + 0);
+                v[k][1] = ld_agent(sp + 1);
+                v[k][2] = ld_agent(sp + 2);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kStageSlots / kSolveThreads; ++k) {
+            const int i = threadIdx.x + k * kSolveThreads;
+            sred[3 * i + 0] = v[k][0];
+            sred[3 * i + 1] = v[k][1];
+            sred[3 * i + 2] = v[k][2];
+        }
+        __syncthreads();
+        const int64_t wend = base + kStageSlots;
+#pragma unroll
+        for (int k = 0; k < kRowsPerThread; ++k) {
+            const int j = threadIdx.x + k * kSolveThreads;
+            const int64_t i0 = max(qlo[k] - qa, base), i1 = min(qhi[k] - qa, wend);
+            if (j < ns && i0 < i1) {
+                double a0 = cv[3 * j + 0], a1 = cv[3 * j + 1], a2 = cv[3 * j + 2];
+                for (int64_t i = i0; i < i1; ++i) {
+                    const double* sv = sred + 3 * (i - base);
+                    a0 += sv[0];
+                    a1 += sv[1];
+                    a2 += sv[2];
+                }
+                cv[3 * j + 0] = a0;
+                cv[3 * j + 1] = a1;
+                cv[3 * j + 2] = a2;
+            }
+        }
+        for (int64_t i = max(ulo, base); i < min(uhi, wend); ++i) {
+            const double* sv = sred + 3 * (i - base);
+            ua0 += sv[0];
+            ua1 += sv[1];
+            ua2 += sv[2];
+        }
+        __syncthreads();
+    }
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+        if (u < nterm) {
+            const double* vp = cv + 3 * (g + u * ng);
+            t0 += gv[u] * vp[0];
+            t1 += gv[u] * vp[1];
+            t2 += gv[u] * vp[2];
+        }
+    if (nterm > B) strided_dot3(gp + B * gstride, gstride, cv + 3 * (g + B * ng), 3 * ng, nterm - B, t0, t1, t2);
+    if (ng > 1) {
+        sred[3 * threadIdx.x + 0] = t0;
+        sred[3 * threadIdx.x + 1] = t1;
+        sred[3 * threadIdx.x + 2] = t2;
+        __syncthreads();
+        if (g == 0)
+            for (int k = 1; k < ng; ++k) {
+                const double* o = sred + 3 * (k * R + rl);
+                t0 += o[0];
+                t1 += o[1];
+                t2 += o[2];
+            }
+    }
+    const double nan = __builtin_nan("");
+    const bool ok = sh_ok != 0;
+    if (urow) {   // read by the parent's tasks in this launch: complete before the flag
+        double* u = slots + 3 * udst;
+        st_agent(u + 0, ok ? ua0 + t0 : nan);
+        st_agent(u + 1, ok ? ua1 + t1 : nan);
+        st_agent(u + 2, ok ? ua2 + t2 : nan);
+    }
+    finish_task(flags + blockIdx.x, epoch);
+    if (g == 0 && r < ns) {   // read by the next launch only
         double* y = ysol + 3ll * (c0 + r);
-        y[0] = t0; y[1] = t1; y[2] = t2;
-    } else {
-        double a[3] = {0.0, 0.0, 0.0};
-        assemble_row(M, rp + r, ubuf, a);
-        double* u = ubuf + 3 * (M.uoff[s] + (r - ns));
-        u[0] = a[0] + t0;
-        u[1] = a[1] + t1;
-        u[2] = a[2] + t2;
+        y[0] = ok ? t0 : nan;
+        y[1] = ok ? t1 : nan;
+        y[2] = ok ? t2 : nan;
     }
 }
 
-// backward: task (s, column block cb) of 4 * CPW columns, CPW per wave: x_J = G^T [y_J; x_R]
-template <int CPW>
+// Backward task (s, columns [first, first + 4 CPW)): x_J = G^T [y_J; x_R], CPW columns
+// per wave, rows staged through LDS in chunks.  BT / CPW G terms per column (the first
+// chunk's) are loaded before the wait.
+template <int CPW, int BT>
 __global__ void __launch_bounds__(kSolveThreads)
-k_bwd_level(const int2* __restrict__ tasks, SolveMaps M, const double* __restrict__ G,
-            const double* __restrict__ ysol, double* __restrict__ xsol, double* __restrict__ out) {
+k_bwd_flow(const SolveTask* __restrict__ tasks, int task0, unsigned* flags, unsigned epoch, int* abortf, SolveMaps M,
+           const double* __restrict__ G, const double* __restrict__ ysol, double* __restrict__ xsol,
+           double* __restrict__ out) {
     __shared__ double vv[kBwdChunk * 3];
-    const int2 task = tasks[blockIdx.x];
-    const int s = task.x, cb = task.y;
-    const int c0 = M.col0[s];
-    const int ns = M.col0[s + 1] - c0;
-    const int f = M.fsize[s];
-    const int64_t rp = M.rowptr[s];
-    const double* Gs = G + M.goff[s];
+    __shared__ int sh_ok;
+    const int ti = task0 + static_cast<int>(blockIdx.x);
+    if (threadIdx.x == 0) sh_ok = __hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 0 : 1;
+    const SolveTask T = tasks[ti];
+    const int ns = T.ns, f = T.f, c0 = T.c0;
+    constexpr int B = BT / CPW;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int jfirst = cb * 4 * CPW;
+    const int jfirst = T.first;
+    const int rbase = jfirst - jfirst % 64;   // rows above jfirst are zero
+    const double* Gs = G + T.goff;
+    double gv[CPW][B];
+    {
+        const int rc = min(kBwdChunk, f - rbase);
+#pragma unroll
+        for (int jj = 0; jj < CPW; ++jj) {
+            const int j = jfirst + w * CPW + jj;
+            int n = 0;
+            const double* gc = Gs;
+            if (j < ns) {
+                const int r1 = (j > rbase ? j - rbase : 0) + lane;
+                if (r1 < rc) {
+                    n = (rc - r1 + 63) / 64;
+                    gc = Gs + static_cast<int64_t>(j) * f + rbase + r1;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < B; ++u) gv[jj][u] = u < n ? gc[64 * u] : 0.0;
+        }
+    }
+    // Before the wait, the first chunk's rows of this supernode (its forward result,
+    // written by the previous launch) go to LDS, and the ancestor rows' indices to registers.
+    constexpr int kStage = kBwdChunk / kSolveThreads;
+    int srow[kStage];
+    {
+        const int rc = min(kBwdChunk, f - rbase);
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+            const int r = threadIdx.x + k * kSolveThreads, row = rbase + r;
+            srow[k] = -1;
+            if (r < rc) {
+                if (row < ns) {
+                    const double* src = ysol + 3ll * (c0 + row);
+                    vv[3 * r + 0] = src[0];
+                    vv[3 * r + 1] = src[1];
+                    vv[3 * r + 2] = src[2];
+                } else {
+                    srow[k] = M.rows[T.rowptr + row];
+                }
+            }
+        }
+    }
+    __syncthreads();   // sh_ok is set
+    if (sh_ok)
+        for (int k = threadIdx.x; k < T.ndep; k += kSolveThreads)
+            if (!wait_flag(flags + T.dep0 + k, epoch, abortf)) sh_ok = 0;
+    __syncthreads();
     double acc[CPW][3];
 #pragma unroll
     for (int jj = 0; jj < CPW; ++jj) acc[jj][0] = acc[jj][1] = acc[jj][2] = 0.0;
-    for (int r0 = jfirst - jfirst % 64; r0 < f; r0 += kBwdChunk) {   // rows above jfirst are zero
+    for (int r0 = rbase; r0 < f; r0 += kBwdChunk) {
         const int rc = min(kBwdChunk, f - r0);
-        for (int r = threadIdx.x; r < rc; r += kSolveThreads) {
-            const int row = r0 + r;
-            const double* src = row < ns ? ysol + 3ll * (c0 + row) : xsol + 3ll * M.rows[rp + row];
-            vv[3 * r + 0] = src[0];
-            vv[3 * r + 1] = src[1];
-            vv[3 * r + 2] = src[2];
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+            const int r = threadIdx.x + k * kSolveThreads, row = r0 + r;
+            if (r < rc) {
+                int sr = srow[k];
+                if (r0 != rbase) {   // later chunks are staged here, both kinds of rows
+                    if (row < ns) {
+                        const double* src = ysol + 3ll * (c0 + row);
+                        vv[3 * r + 0] = src[0];
+                        vv[3 * r + 1] = src[1];
+                        vv[3 * r + 2] = src[2];
+                        sr = -1;
+                    } else {
+                        sr = M.rows[T.rowptr + row];
+                    }
+                }
+                if (sr >= 0) {   // an ancestor's solution, written in this launch
+                    const double* src = xsol + 3ll * sr;
+                    vv[3 * r + 0] = ld_agent(src + 0);
+                    vv[3 * r + 1] = ld_agent(src + 1);
+                    vv[3 * r + 2] = ld_agent(src + 2);
+                }
+            }
         }
         __syncthreads();
 #pragma unroll
         for (int jj = 0; jj < CPW; ++jj) {
             const int j = jfirst + w * CPW + jj;
             if (j < ns) {
-                const double* gc = Gs + static_cast<int64_t>(j) * f + r0;
-                const int rstart = j > r0 ? j - r0 : 0;   // rows above the diagonal are zero
-                const int r1 = rstart + lane;
-                if (r1 < rc)
-                    strided_dot3(gc + r1, 64, vv + 3 * r1, 3 * 64, (rc - r1 + 63) / 64, acc[jj][0], acc[jj][1],
-                                 acc[jj][2]);
+                const int r1 = (j > r0 ? j - r0 : 0) + lane;   // rows above the diagonal are zero
+                if (r1 < rc) {
+                    const int n = (rc - r1 + 63) / 64;
+                    int k0 = 0;
+                    if (r0 == rbase) {
+#pragma unroll
+                        for (int u = 0; u < B; ++u)
+                            if (u < n) {
+                                const double* vp = vv + 3 * (r1 + 64 * u);
+                                acc[jj][0] += gv[jj][u] * vp[0];
+                                acc[jj][1] += gv[jj][u] * vp[1];
+                                acc[jj][2] += gv[jj][u] * vp[2];
+                            }
+                        k0 = B;
+                    }
+                    if (n > k0)
+                        strided_dot3(Gs + static_cast<int64_t>(j) * f + r0 + r1 + 64 * k0, 64, vv + 3 * (r1 + 64 * k0),
+                                     3 * 64, n - k0, acc[jj][0], acc[jj][1], acc[jj][2]);
+                }
             }
         }
         __syncthreads();
     }
+    const double nan = __builtin_nan("");
+    const bool ok = sh_ok != 0;
 #pragma unroll
     for (int jj = 0; jj < CPW; ++jj) {
         const int j = jfirst + w * CPW + jj;
         const double a0 = wave_sum(acc[jj][0]);
         const double a1 = wave_sum(acc[jj][1]);
         const double a2 = wave_sum(acc[jj][2]);
-        if (lane == 0 && j < ns) {
+        acc[jj][0] = a0;
+        acc[jj][1] = a1;
+        acc[jj][2] = a2;
+        if (lane == 0 && j < ns) {   // read by the descendants' tasks in this launch
             double* x = xsol + 3ll * (c0 + j);
-            x[0] = a0; x[1] = a1; x[2] = a2;
+            st_agent(x + 0, ok ? a0 : nan);
+            st_agent(x + 1, ok ? a1 : nan);
+            st_agent(x + 2, ok ? a2 : nan);
+        }
+    }
+    finish_task(flags + ti, epoch);
+#pragma unroll
+    for (int jj = 0; jj < CPW; ++jj) {   // the caller's copy, read after this launch only
+        const int j = jfirst + w * CPW + jj;
+        if (lane == 0 && j < ns) {
             double* xo = out + 3ll * M.perm[c0 + j];
-            xo[0] = a0; xo[1] = a1; xo[2] = a2;
+            xo[0] = ok ? acc[jj][0] : nan;
+            xo[1] = ok ? acc[jj][1] : nan;
+            xo[2] = ok ? acc[jj][2] : nan;
         }
     }
 }
@@ -336,6 +598,22 @@ int chol_prepare_device(wrapd_ctx* c) {
     rc |= up(c, c->d_uoff, S.uoff);
     rc |= up(c, c->d_asm_ptr, S.asm_ptr);
     rc |= up(c, c->d_asm_idx, S.asm_idx);
+    // each update row adds into exactly one parent front row: its slot is its position in asm_idx
+    {
+        const int64_t nu = S.uoff[S.nsuper];
+        std::vector<int64_t> udest(std::max<int64_t>(1, nu), -1);
+        bool bad = static_cast<int64_t>(S.asm_idx.size()) != nu;
+        for (size_t q = 0; q < S.asm_idx.size() && !bad; ++q) {
+            const int64_t i = S.asm_idx[q];
+            if (i < 0 || i >= nu || udest[i] >= 0) bad = true;
+            else udest[i] = static_cast<int64_t>(q);
+        }
+        if (bad) {
+            c->last_error = "supernode update rows: each must add into exactly one parent row";
+            return WRAPD_ERR_STATE;
+        }
+        rc |= up(c, c->d_udest, udest);
+    }
     rc |= up(c, c->d_leaf_ptr, S.leaf_ptr);
     // LDS of the leaf kernels: the largest subtree's update rows (forward) and columns
     // (backward); a subtree must be a contiguous supernode range
@@ -357,21 +635,25 @@ int chol_prepare_device(wrapd_ctx* c) {
         return WRAPD_ERR_ARG;
     }
     rc |= up(c, c->d_leaf_sn, S.leaf_sn);
-    // level task lists: forward (s, row block) with R rows per task, backward
-    // (s, column block) with 4 CPW columns per task, both chosen per level
-    std::vector<int> ft, bt;
-    c->fwd_task_ptr.assign(1, 0);
-    c->bwd_task_ptr.assign(1, 0);
-    c->fwd_rows.clear();
-    c->bwd_cpw.clear();
+    // Task widths are the widest that still give a level about kLevelTasks tasks:
+    // forward (s, block of R rows), R in {256, ..., 16}; backward (s, block of 4 cpw
+    // columns), cpw in {4, 2, 1}, made non-decreasing from the root down so that each
+    // width is one launch.  Wide tasks keep G's reads long and the per-task costs (the
+    // right-hand side is gathered by every task of a supernode) few; narrow ones split the
+    // few supernodes near the root.  Forward tasks go bottom-up, backward tasks top-down.
     const int nlev = static_cast<int>(S.level_ptr.size()) - 1;
+    std::vector<char> in_level(S.nsuper, 0);
+    for (int i = S.level_ptr[0]; i < S.level_ptr[nlev]; ++i) in_level[S.level_sn[i]] = 1;
+    std::vector<int> lev_cpw(nlev, 1), lev_rows(nlev, 256), nft(S.nsuper, 0), nbt(S.nsuper, 0);
     for (int l = 0; l < nlev; ++l) {
-        int R = 256, cpw = 4;
-        for (; R > 32; R /= 2) {
+        int R = 256;
+        for (; R > 16; R /= 2) {
             int64_t n = 0;
             for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) n += (S.fsize[S.level_sn[i]] + R - 1) / R;
             if (n >= kLevelTasks) break;
         }
+        lev_rows[l] = R;
+        int cpw = 4;
         for (; cpw > 1; cpw /= 2) {
             int64_t n = 0;
             for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) {
@@ -380,19 +662,73 @@ int chol_prepare_device(wrapd_ctx* c) {
             }
             if (n >= kLevelTasks) break;
         }
+        lev_cpw[l] = cpw;
+    }
+    for (int l = nlev - 2; l >= 0; --l) lev_cpw[l] = std::max(lev_cpw[l], lev_cpw[l + 1]);
+    for (int l = 0; l < nlev; ++l)
         for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) {
             const int sn = S.level_sn[i];
             const int ns = S.col0[sn + 1] - S.col0[sn];
-            for (int rb = 0; rb * R < S.fsize[sn]; ++rb) { ft.push_back(sn); ft.push_back(rb); }
-            for (int cb = 0; cb * 4 * cpw < ns; ++cb) { bt.push_back(sn); bt.push_back(cb); }
+            nft[sn] = (S.fsize[sn] + lev_rows[l] - 1) / lev_rows[l];
+            nbt[sn] = (ns + 4 * lev_cpw[l] - 1) / (4 * lev_cpw[l]);
         }
-        c->fwd_rows.push_back(R);
-        c->bwd_cpw.push_back(cpw);
-        c->fwd_task_ptr.push_back(static_cast<int>(ft.size() / 2));
-        c->bwd_task_ptr.push_back(static_cast<int>(bt.size() / 2));
+    std::vector<std::vector<int>> kids(S.nsuper);
+    for (int i = S.level_ptr[0]; i < S.level_ptr[nlev]; ++i) {
+        const int sn = S.level_sn[i], p = S.parent[sn];
+        if (p >= 0 && !in_level[p]) {
+            c->last_error = "supernode schedule: a level supernode's parent is in a leaf subtree";
+            return WRAPD_ERR_STATE;
+        }
+        if (p >= 0) kids[p].push_back(sn);
     }
-    rc |= up(c, c->d_fwd_tasks, ft);
-    rc |= up(c, c->d_bwd_tasks, bt);
+    auto task_of = [&](int sn, int first, int width, int dep0, int ndep) {
+        SolveTask t{};
+        t.s = sn;
+        t.first = first;
+        t.width = width;
+        t.ns = S.col0[sn + 1] - S.col0[sn];
+        t.f = S.fsize[sn];
+        t.c0 = S.col0[sn];
+        t.dep0 = dep0;
+        t.ndep = ndep;
+        t.goff = S.goff[sn];
+        t.rowptr = S.rowptr[sn];
+        t.uoff = S.uoff[sn];
+        return t;
+    };
+    // a forward task waits for every task of its non-leaf children (their indices listed
+    // in deps); a backward task for every task of its parent (a contiguous range)
+    std::vector<SolveTask> ft, bt;
+    std::vector<int> deps, first_task(S.nsuper, -1);
+    for (int l = 0; l < nlev; ++l)
+        for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) {
+            const int sn = S.level_sn[i];
+            const int dep0 = static_cast<int>(deps.size());
+            for (int k : kids[sn])
+                for (int q = 0; q < nft[k]; ++q) deps.push_back(first_task[k] + q);
+            const int ndep = static_cast<int>(deps.size()) - dep0;
+            first_task[sn] = static_cast<int>(ft.size());
+            for (int rb = 0; rb < nft[sn]; ++rb) ft.push_back(task_of(sn, rb * lev_rows[l], lev_rows[l], dep0, ndep));
+        }
+    c->bwd_segs.clear();
+    for (int l = nlev - 1; l >= 0; --l) {
+        if (c->bwd_segs.empty() || c->bwd_segs.back().second != lev_cpw[l])
+            c->bwd_segs.push_back({static_cast<int>(bt.size()), lev_cpw[l]});
+        for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) {
+            const int sn = S.level_sn[i], p = S.parent[sn];
+            const int dep0 = p >= 0 ? first_task[p] : 0, ndep = p >= 0 ? nbt[p] : 0;
+            first_task[sn] = static_cast<int>(bt.size());
+            for (int cb = 0; cb < nbt[sn]; ++cb) bt.push_back(task_of(sn, cb * 4 * lev_cpw[l], 4 * lev_cpw[l], dep0, ndep));
+        }
+    }
+    c->n_flow_f = static_cast<int>(ft.size());
+    c->n_flow_b = static_cast<int>(bt.size());
+    rc |= up(c, reinterpret_cast<SolveTask*&>(c->d_flow_f), ft);
+    rc |= up(c, reinterpret_cast<SolveTask*&>(c->d_flow_b), bt);
+    rc |= up(c, c->d_flow_deps, deps);
+    rc |= up(c, c->d_flow_flags, std::vector<unsigned>(static_cast<size_t>(c->n_flow_f) + c->n_flow_b, 0u));
+    rc |= up(c, c->d_flow_abort, std::vector<int>(1, 0));
+    c->flow_epoch = 0;
     std::vector<double> z3(3ull * std::max(1, c->n_f), 0.0);
     rc |= up(c, c->ysol, z3);
     rc |= up(c, c->xsol, z3);
@@ -406,40 +742,46 @@ int chol_prepare_device(wrapd_ctx* c) {
 void launch_solve(wrapd_ctx* c, const double* b, double* x) {
     const wrapd::CholSymbolic& S = c->sym;
     SolveMaps M{c->d_col0, c->d_fsize, c->d_rowptr, c->d_rows, c->d_goff, c->d_uoff, c->d_asm_ptr, c->d_asm_idx,
-                c->d_perm};
+                c->d_perm, c->d_udest};
     const int nleaf = static_cast<int>(S.leaf_ptr.size()) - 1;
-    const int nlev = static_cast<int>(c->fwd_task_ptr.size()) - 1;
-    const int2* ft = reinterpret_cast<const int2*>(c->d_fwd_tasks);
-    const int2* bt = reinterpret_cast<const int2*>(c->d_bwd_tasks);
+    if (++c->flow_epoch == 0) c->flow_epoch = 1;   // flags start at 0
+    const unsigned epoch = c->flow_epoch;
+    const SolveTask* ft = static_cast<const SolveTask*>(c->d_flow_f);
+    const SolveTask* bt = static_cast<const SolveTask*>(c->d_flow_b);
     {
         ProfScope ps(c, "solve_fwd");
         if (nleaf > 0)
             hipLaunchKernelGGL(k_fwd_leaf, dim3(nleaf), dim3(kSolveThreads), c->leaf_lds_fwd, c->stream, c->d_leaf_ptr,
                                c->d_leaf_sn, M, b, c->G, c->ysol, c->ubuf);
-        for (int l = 0; l < nlev; ++l) {
-            const int n = c->fwd_task_ptr[l + 1] - c->fwd_task_ptr[l];
-            if (n > 0)
-                hipLaunchKernelGGL(k_fwd_level, dim3(n), dim3(kSolveThreads), 0, c->stream, ft + c->fwd_task_ptr[l],
-                                   c->fwd_rows[l], M, b, c->G, c->ysol, c->ubuf);
+        if (c->n_flow_f > 0) {
+            hipLaunchKernelGGL(k_fwd_flow<kFwdPrefetch>, dim3(c->n_flow_f), dim3(kSolveThreads), 0, c->stream, ft,
+                               c->d_flow_deps, c->d_flow_flags, epoch, c->d_flow_abort, M, b, c->G, c->ysol, c->ubuf);
         }
     }
     {
         ProfScope ps(c, "solve_bwd");
-        for (int l = nlev - 1; l >= 0; --l) {
-            const int n = c->bwd_task_ptr[l + 1] - c->bwd_task_ptr[l];
-            if (n == 0) continue;
-            const int2* t = bt + c->bwd_task_ptr[l];
-            if (c->bwd_cpw[l] == 4)
-                hipLaunchKernelGGL(k_bwd_level<4>, dim3(n), dim3(kSolveThreads), 0, c->stream, t, M, c->G, c->ysol, c->xsol, x);
-            else if (c->bwd_cpw[l] == 2)
-                hipLaunchKernelGGL(k_bwd_level<2>, dim3(n), dim3(kSolveThreads), 0, c->stream, t, M, c->G, c->ysol, c->xsol, x);
-            else
-                hipLaunchKernelGGL(k_bwd_level<1>, dim3(n), dim3(kSolveThreads), 0, c->stream, t, M, c->G, c->ysol, c->xsol, x);
+        for (size_t k = 0; k < c->bwd_segs.size(); ++k) {
+            const int t0 = c->bwd_segs[k].first, cpw = c->bwd_segs[k].second;
+            const int n = (k + 1 < c->bwd_segs.size() ? c->bwd_segs[k + 1].first : c->n_flow_b) - t0;
+            if (n <= 0) continue;
+            auto kern = cpw == 4 ? k_bwd_flow<4, kBwdPrefetch> : cpw == 2 ? k_bwd_flow<2, kBwdPrefetch>
+                                                                         : k_bwd_flow<1, kBwdPrefetch>;
+            hipLaunchKernelGGL(kern, dim3(n), dim3(kSolveThreads), 0, c->stream, bt, t0, c->d_flow_flags + c->n_flow_f,
+                               epoch, c->d_flow_abort, M, c->G, c->ysol, c->xsol, x);
         }
         if (nleaf > 0)
             hipLaunchKernelGGL(k_bwd_leaf, dim3(nleaf), dim3(kSolveThreads), c->leaf_lds_bwd, c->stream, c->d_leaf_ptr,
                                c->d_leaf_sn, M, c->G, c->ysol, c->xsol, x);
     }
+}
+
+int solve_aborted(wrapd_ctx* c) {
+    int v = 0;
+    if (!c->d_flow_abort) return 0;
+    if (hipMemcpyAsync(&v, c->d_flow_abort, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return 1;
+    return v;
 }
 
 }  // namespace wd
