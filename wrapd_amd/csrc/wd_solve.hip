@@ -81,59 +81,109 @@ __device__ __forceinline__ void strided_dot3(const double* __restrict__ g, int64
 }
 
 // Leaf subtrees: one workgroup walks a whole subtree in postorder.  A subtree is a
-// contiguous range of supernodes (postorder), and every update row it produces except its
-// root's is consumed inside it, so those rows stay in LDS (dynamic, sized by the largest
-// subtree); only the root's rows go to global memory, to their slots.
+// contiguous range of supernodes [first, root] (postorder, checked at set-up), and every
+// update row it produces except its root's is consumed inside it, so those rows stay in
+// LDS (dynamic, sized by the largest subtree); only the root's rows go to global memory,
+// to their slots.  Fronts have at most kSmallFront = kSolveThreads rows, one per thread.
+// Per supernode the loads that do not depend on the previous step (G, perm, the maps,
+// the next supernode's sizes) are issued before those that do.
+constexpr int kLeafPrefetch = 16;   // G terms per row loaded ahead in the leaf kernels
+constexpr int kLeafMaxSn = 64;      // supernodes per leaf subtree (checked at set-up)
+static_assert(kSmallFront == kSolveThreads, "leaf kernels: one front row per thread");
+
+// A leaf subtree's supernode sizes, loaded into LDS once (vector loads: scalar loads
+// would share a wait counter with the LDS traffic of every step).
+struct LeafMeta {
+    int col0[kLeafMaxSn + 1];
+    int fsize[kLeafMaxSn];
+    long long rowptr[kLeafMaxSn], goff[kLeafMaxSn], uoff[kLeafMaxSn];
+};
+__device__ __forceinline__ void load_leaf_meta(const SolveMaps& M, int first, int n, LeafMeta& L) {
+    for (int i = threadIdx.x; i <= n; i += kSolveThreads) {
+        L.col0[i] = M.col0[first + i];
+        if (i < n) {
+            L.fsize[i] = M.fsize[first + i];
+            L.rowptr[i] = M.rowptr[first + i];
+            L.goff[i] = M.goff[first + i];
+            L.uoff[i] = M.uoff[first + i];
+        }
+    }
+    __syncthreads();
+}
 __global__ void __launch_bounds__(kSolveThreads)
 k_fwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, SolveMaps M,
            const double* __restrict__ b, const double* __restrict__ G, double* __restrict__ ysol,
            double* __restrict__ slots) {
+    constexpr int B = kLeafPrefetch;
+    constexpr int KA = 4;   // update rows per front row read ahead (more are read in a loop)
     extern __shared__ double dyn[];
     double* cv = dyn;                     // kSmallFront x 3
     double* ul = dyn + kSmallFront * 3;   // the subtree's update rows x 3
+    __shared__ LeafMeta L;
     const int t = blockIdx.x;
     const int q0 = leaf_ptr[t], q1 = leaf_ptr[t + 1];
-    const int root = leaf_sn[q1 - 1];
-    const int64_t ubase = M.uoff[leaf_sn[q0]];
-    for (int q = q0; q < q1; ++q) {
-        const int s = leaf_sn[q];
-        const int c0 = M.col0[s];
-        const int ns = M.col0[s + 1] - c0;
-        const int f = M.fsize[s];
-        const int64_t rp = M.rowptr[s];
-        for (int r = threadIdx.x; r < f; r += kSolveThreads) {
-            double a[3] = {0.0, 0.0, 0.0};
+    const int first = leaf_sn[q0], nsn = q1 - q0;
+    load_leaf_meta(M, first, nsn, L);
+    const int64_t ubase = L.uoff[0];
+    const int r = threadIdx.x;
+    for (int i = 0; i < nsn; ++i) {
+        const int c0 = L.col0[i], ns = L.col0[i + 1] - c0, f = L.fsize[i];
+        const int64_t rp = L.rowptr[i], go = L.goff[i], uo = L.uoff[i];
+        const int jend = r < ns ? r + 1 : (r < f ? ns : 0);   // L11^{-1} is lower triangular
+        const double* gp = G + go + r;
+        double gv[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) gv[u] = u < jend ? gp[static_cast<int64_t>(u) * f] : 0.0;
+        const int old = r < ns ? M.perm[c0 + r] : 0;
+        const int64_t k0 = r < f ? M.asm_ptr[rp + r] : 0, k1 = r < f ? M.asm_ptr[rp + r + 1] : 0;
+        int64_t ai[KA];
+#pragma unroll
+        for (int i = 0; i < KA; ++i) ai[i] = k0 + i < k1 ? M.asm_idx[k0 + i] : 0;
+        if (r < f) {
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
             if (r < ns) {
-                const int old = M.perm[c0 + r];
-                a[0] = b[3 * old + 0];
-                a[1] = b[3 * old + 1];
-                a[2] = b[3 * old + 2];
+                a0 = b[3 * old + 0];
+                a1 = b[3 * old + 1];
+                a2 = b[3 * old + 2];
             }
-            for (int64_t k = M.asm_ptr[rp + r]; k < M.asm_ptr[rp + r + 1]; ++k) {
+#pragma unroll
+            for (int i = 0; i < KA; ++i)
+                if (k0 + i < k1) {
+                    const double* u = ul + 3 * (ai[i] - ubase);
+                    a0 += u[0];
+                    a1 += u[1];
+                    a2 += u[2];
+                }
+            for (int64_t k = k0 + KA; k < k1; ++k) {
                 const double* u = ul + 3 * (M.asm_idx[k] - ubase);
-                a[0] += u[0];
-                a[1] += u[1];
-                a[2] += u[2];
+                a0 += u[0];
+                a1 += u[1];
+                a2 += u[2];
             }
-            cv[3 * r + 0] = a[0];
-            cv[3 * r + 1] = a[1];
-            cv[3 * r + 2] = a[2];
+            cv[3 * r + 0] = a0;
+            cv[3 * r + 1] = a1;
+            cv[3 * r + 2] = a2;
         }
         __syncthreads();
-        const double* Gs = G + M.goff[s];
-        for (int r = threadIdx.x; r < f; r += kSolveThreads) {
+        if (r < f) {
             double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-            const int jend = r < ns ? r + 1 : ns;   // L11^{-1} is lower triangular
-            strided_dot3(Gs + r, f, cv, 3, jend, t0, t1, t2);
+#pragma unroll
+            for (int u = 0; u < B; ++u)
+                if (u < jend) {
+                    t0 += gv[u] * cv[3 * u + 0];
+                    t1 += gv[u] * cv[3 * u + 1];
+                    t2 += gv[u] * cv[3 * u + 2];
+                }
+            if (jend > B) strided_dot3(gp + static_cast<int64_t>(B) * f, f, cv + 3 * B, 3, jend - B, t0, t1, t2);
             if (r < ns) {
                 double* y = ysol + 3ll * (c0 + r);
                 y[0] = t0; y[1] = t1; y[2] = t2;
             } else {
-                const int64_t ui = M.uoff[s] + (r - ns);
+                const int64_t ui = uo + (r - ns);
                 const double u0 = cv[3 * r + 0] + t0, u1 = cv[3 * r + 1] + t1, u2 = cv[3 * r + 2] + t2;
                 double* u = ul + 3 * (ui - ubase);
                 u[0] = u0; u[1] = u1; u[2] = u2;
-                if (s == root) {
+                if (i == nsn - 1) {   // the subtree's root
                     double* ug = slots + 3 * M.udest[ui];
                     ug[0] = u0; ug[1] = u1; ug[2] = u2;
                 }
@@ -462,30 +512,48 @@ k_bwd_flow(const SolveTask* __restrict__ tasks, int task0, unsigned* flags, unsi
             }
         }
         __syncthreads();
+        // lane l takes rows r1 + 64 u of each column; the columns' loads go out together
+        int r1[CPW], n[CPW], nmax = 0;
 #pragma unroll
         for (int jj = 0; jj < CPW; ++jj) {
             const int j = jfirst + w * CPW + jj;
-            if (j < ns) {
-                const int r1 = (j > r0 ? j - r0 : 0) + lane;   // rows above the diagonal are zero
-                if (r1 < rc) {
-                    const int n = (rc - r1 + 63) / 64;
-                    int k0 = 0;
-                    if (r0 == rbase) {
+            r1[jj] = (j > r0 ? j - r0 : 0) + lane;   // rows above the diagonal are zero
+            n[jj] = (j < ns && r1[jj] < rc) ? (rc - r1[jj] + 63) / 64 : 0;
+            nmax = max(nmax, n[jj]);
+        }
+        int k0 = 0;
+        if (r0 == rbase) {
 #pragma unroll
-                        for (int u = 0; u < B; ++u)
-                            if (u < n) {
-                                const double* vp = vv + 3 * (r1 + 64 * u);
-                                acc[jj][0] += gv[jj][u] * vp[0];
-                                acc[jj][1] += gv[jj][u] * vp[1];
-                                acc[jj][2] += gv[jj][u] * vp[2];
-                            }
-                        k0 = B;
+            for (int jj = 0; jj < CPW; ++jj)
+#pragma unroll
+                for (int u = 0; u < B; ++u)
+                    if (u < n[jj]) {
+                        const double* vp = vv + 3 * (r1[jj] + 64 * u);
+                        acc[jj][0] += gv[jj][u] * vp[0];
+                        acc[jj][1] += gv[jj][u] * vp[1];
+                        acc[jj][2] += gv[jj][u] * vp[2];
                     }
-                    if (n > k0)
-                        strided_dot3(Gs + static_cast<int64_t>(j) * f + r0 + r1 + 64 * k0, 64, vv + 3 * (r1 + 64 * k0),
-                                     3 * 64, n - k0, acc[jj][0], acc[jj][1], acc[jj][2]);
-                }
+            k0 = B;
+        }
+        constexpr int BB = 16 / CPW;   // row terms per column per batch
+        for (int u0 = k0; u0 < nmax; u0 += BB) {
+            double gb[CPW][BB];
+#pragma unroll
+            for (int jj = 0; jj < CPW; ++jj) {
+                const double* gc = Gs + static_cast<int64_t>(jfirst + w * CPW + jj) * f + r0 + r1[jj];
+#pragma unroll
+                for (int u = 0; u < BB; ++u) gb[jj][u] = u0 + u < n[jj] ? gc[64 * (u0 + u)] : 0.0;
             }
+#pragma unroll
+            for (int jj = 0; jj < CPW; ++jj)
+#pragma unroll
+                for (int u = 0; u < BB; ++u)
+                    if (u0 + u < n[jj]) {
+                        const double* vp = vv + 3 * (r1[jj] + 64 * (u0 + u));
+                        acc[jj][0] += gb[jj][u] * vp[0];
+                        acc[jj][1] += gb[jj][u] * vp[1];
+                        acc[jj][2] += gb[jj][u] * vp[2];
+                    }
         }
         __syncthreads();
     }
@@ -521,55 +589,100 @@ k_bwd_flow(const SolveTask* __restrict__ tasks, int task0, unsigned* flags, unsi
 }
 
 // The subtree's own solution rows stay in LDS too (its supernodes' columns are the
-// contiguous range [col0[first], col0[root+1])); rows above the subtree come from xsol.
-// Results go to xsol and, in the caller's numbering, to out.
+// contiguous range [col0[first], col0[root + 1])); rows above the subtree come from xsol.
+// They are copied to out, in the caller's numbering, at the end.  Wave w takes
+// columns w, w + 4, ...; lane l rows j + l + 64 m.  The G terms of KC columns of a wave
+// are loaded together, the first KC before the step's rows are staged.
 __global__ void __launch_bounds__(kSolveThreads)
 k_bwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, SolveMaps M,
            const double* __restrict__ G, const double* __restrict__ ysol, double* __restrict__ xsol,
            double* __restrict__ out) {
+    constexpr int KC = 4;
+    constexpr int MR = kSmallFront / 64;
     extern __shared__ double dyn[];
     double* vv = dyn;                     // kSmallFront x 3
     double* xl = dyn + kSmallFront * 3;   // the subtree's columns x 3
+    __shared__ LeafMeta L;
     const int t = blockIdx.x;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = threadIdx.x;
     const int q0 = leaf_ptr[t], q1 = leaf_ptr[t + 1];
-    const int cfirst = M.col0[leaf_sn[q0]], cend = M.col0[leaf_sn[q1 - 1] + 1];
-    for (int q = q1 - 1; q >= q0; --q) {
-        const int s = leaf_sn[q];
-        const int c0 = M.col0[s];
-        const int ns = M.col0[s + 1] - c0;
-        const int f = M.fsize[s];
-        const int64_t rp = M.rowptr[s];
-        for (int r = threadIdx.x; r < f; r += kSolveThreads) {
-            const double* src;
-            if (r < ns) {
-                src = ysol + 3ll * (c0 + r);
-            } else {
-                const int row = M.rows[rp + r];
-                src = (row >= cfirst && row < cend) ? xl + 3 * (row - cfirst) : xsol + 3ll * row;
+    const int first = leaf_sn[q0], nsn = q1 - q0;
+    load_leaf_meta(M, first, nsn, L);
+    const int cfirst = L.col0[0], cend = L.col0[nsn];
+    int row = -1;   // this thread's staged row above the supernode's own columns
+    {
+        const int i = nsn - 1;
+        if (r < L.fsize[i] && r >= L.col0[i + 1] - L.col0[i]) row = M.rows[L.rowptr[i] + r];
+    }
+    for (int i = nsn - 1; i >= 0; --i) {
+        const int c0 = L.col0[i], ns = L.col0[i + 1] - c0, f = L.fsize[i];
+        const int64_t go = L.goff[i];
+        const double* Gs = G + go;
+        double gv[KC][MR];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            const int j = w + 4 * k;
+#pragma unroll
+            for (int m = 0; m < MR; ++m) {
+                const int rr = j + lane + 64 * m;
+                gv[k][m] = (j < ns && rr < f) ? Gs[static_cast<int64_t>(j) * f + rr] : 0.0;
             }
+        }
+        int prow = -1;   // the next (lower) supernode's row source, loaded ahead
+        if (r < f) {
+            const double* src;
+            if (r < ns) src = ysol + 3ll * (c0 + r);
+            else src = (row >= cfirst && row < cend) ? xl + 3 * (row - cfirst) : xsol + 3ll * row;
             vv[3 * r + 0] = src[0];
             vv[3 * r + 1] = src[1];
             vv[3 * r + 2] = src[2];
         }
+        if (i > 0 && r < L.fsize[i - 1] && r >= c0 - L.col0[i - 1]) prow = M.rows[L.rowptr[i - 1] + r];
         __syncthreads();
-        const double* Gs = G + M.goff[s];
-        for (int j = w; j < ns; j += kSolveThreads / 64) {
-            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-            const double* gc = Gs + static_cast<int64_t>(j) * f;
-            const int r1 = j + lane;
-            if (r1 < f) strided_dot3(gc + r1, 64, vv + 3 * r1, 3 * 64, (f - r1 + 63) / 64, a0, a1, a2);
-            a0 = wave_sum(a0);
-            a1 = wave_sum(a1);
-            a2 = wave_sum(a2);
-            if (lane == 0) {
-                double* x = xl + 3 * (c0 + j - cfirst);
-                x[0] = a0; x[1] = a1; x[2] = a2;
-                double* xo = out + 3ll * M.perm[c0 + j];
-                xo[0] = a0; xo[1] = a1; xo[2] = a2;
+        for (int jc = w; jc < ns; jc += 4 * KC) {
+            if (jc != w) {
+#pragma unroll
+                for (int k = 0; k < KC; ++k) {
+                    const int j = jc + 4 * k;
+#pragma unroll
+                    for (int m = 0; m < MR; ++m) {
+                        const int rr = j + lane + 64 * m;
+                        gv[k][m] = (j < ns && rr < f) ? Gs[static_cast<int64_t>(j) * f + rr] : 0.0;
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < KC; ++k) {
+                const int j = jc + 4 * k;
+                if (j >= ns) break;   // uniform across the wave
+                double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll
+                for (int m = 0; m < MR; ++m) {
+                    const int rr = j + lane + 64 * m;
+                    if (rr < f) {
+                        const double* vp = vv + 3 * rr;
+                        a0 += gv[k][m] * vp[0];
+                        a1 += gv[k][m] * vp[1];
+                        a2 += gv[k][m] * vp[2];
+                    }
+                }
+                a0 = wave_sum(a0);
+                a1 = wave_sum(a1);
+                a2 = wave_sum(a2);
+                if (lane == 0) {
+                    double* x = xl + 3 * (c0 + j - cfirst);
+                    x[0] = a0; x[1] = a1; x[2] = a2;
+                }
             }
         }
         __syncthreads();
+        row = prow;
+    }
+    for (int c = cfirst + threadIdx.x; c < cend; c += kSolveThreads) {
+        const double* x = xl + 3 * (c - cfirst);
+        double* xo = out + 3ll * M.perm[c];
+        xo[0] = x[0]; xo[1] = x[1]; xo[2] = x[2];
     }
 }
 
@@ -624,6 +737,15 @@ int chol_prepare_device(wrapd_ctx* c) {
             if (S.leaf_sn[q] != first + (q - S.leaf_ptr[t])) {
                 c->last_error = "leaf subtree is not a contiguous supernode range";
                 return WRAPD_ERR_STATE;
+            }
+        if (S.leaf_ptr[t + 1] - S.leaf_ptr[t] > kLeafMaxSn) {
+            c->last_error = "leaf subtree has more supernodes than the solve kernels support";
+            return WRAPD_ERR_ARG;
+        }
+        for (int q = S.leaf_ptr[t]; q < S.leaf_ptr[t + 1]; ++q)
+            if (S.fsize[S.leaf_sn[q]] > kSmallFront) {
+                c->last_error = "leaf subtree front larger than the solve kernels support";
+                return WRAPD_ERR_ARG;
             }
         urows = std::max<int64_t>(urows, S.uoff[root + 1] - S.uoff[first]);
         cols = std::max<int64_t>(cols, S.col0[root + 1] - S.col0[first]);
