@@ -46,7 +46,7 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0
 SCOPES = ("local_step", "defo_grad", "gather_grad", "solve_fwd", "solve_bwd", "multidot", "axpy_q", "axpy_r", "xstep",
-          "finalize", "dot", "refactor", "assemble_A", "update_weights", "mlp_train")
+          "dot", "refactor", "assemble_A", "update_weights", "mlp_train")
 
 
 def log(msg):

@@ -58,4 +58,121 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
     return r;
 }
 
+// ---------------------------------------------------------------------------------
+// Reductions finished in the producing launch.  Every block stores its partials
+// (slot k of block lb at partials[k * kTailBlocks + lb]: slot-major, so the reading block's
+// loads are contiguous) at agent scope and then its flag;
+// the block dispatched last (blockIdx.x == gridDim.x - 1: every other block is already
+// running, and none waits for anything) waits for all flags and sums the partials in
+// the order of a 256-thread finalize: lane v takes blocks v, v + 256, ...; lanes are
+// summed in waves of 64 and the 4 waves in order.  So the result is the one a separate
+// one-block finalize launch would produce, without the launch.
+constexpr int kTailSlots = 24;        // reduction slots per block
+constexpr int kTailBlocks = 1 << 16;  // blocks per reducing launch (the partials' row length)
+
+struct RedTail {
+    unsigned* flags = nullptr;   // per block; nullptr: the caller finalizes in a launch of its own
+    int* abortf = nullptr;       // sticky: a wait ran out (the results are then NaN)
+    double* out = nullptr;       // nslots results
+    unsigned epoch = 0;
+    int nslots = 0;
+    unsigned maxmask = 0;        // slots reduced with max instead of sum
+};
+
+__device__ __forceinline__ void put_partial(double* partials, int lb, int k, double v) {
+    __hip_atomic_store(partials + static_cast<long long>(k) * kTailBlocks + lb, v, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Returns true in the block that finished the reduction (the results are in R.out).
+// KS: the most slots this call site reduces (R.nslots <= KS).
+template <int NT, int KS>
+__device__ __forceinline__ bool reduce_tail(const RedTail R, const double* partials) {
+    static_assert(NT == 128 || NT == 256, "reduce_tail: 128 or 256 threads");
+    static_assert(KS >= 1 && KS <= kTailSlots, "reduce_tail: slot count");
+    if (!R.flags) return false;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int nblk = gridDim.x;
+    if (blockIdx.x != nblk - 1) {
+        if (threadIdx.x == 0) __hip_atomic_store(R.flags + blockIdx.x, R.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return false;
+    }
+    bool bad = false;
+    for (int polls = 1;; ++polls) {
+        bool mine = true;
+        for (int b0 = threadIdx.x; b0 < nblk - 1; b0 += 8 * NT) {
+            unsigned v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int b = b0 + u * NT;
+                v[u] = b < nblk - 1 ? __hip_atomic_load(R.flags + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : R.epoch;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) mine = mine && v[u] == R.epoch;
+        }
+        if (__syncthreads_and(mine)) break;
+        const bool stop = polls >= (1 << 20) ||
+                          __hip_atomic_load(R.abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+        if (__syncthreads_or(stop)) {
+            if (threadIdx.x == 0) __hip_atomic_store(R.abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            bad = true;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    // The partials, at agent scope (past this XCD's L2), BB blocks' slots per batch.
+    constexpr int BB = KS > 8 ? 2 : 4;
+    constexpr int V = 256 / NT;   // lanes of the 256-lane order per thread
+    __shared__ double wsum[KS][4];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int ns = R.nslots;
+#pragma unroll
+    for (int h = 0; h < V; ++h) {
+        double acc[KS];
+#pragma unroll
+        for (int k = 0; k < KS; ++k) acc[k] = ((R.maxmask >> k) & 1u) ? -1e308 : 0.0;
+        for (int b0 = threadIdx.x + h * NT; b0 < nblk; b0 += 256 * BB) {
+            double v[BB][KS];
+#pragma unroll
+            for (int u = 0; u < BB; ++u) {
+                const int b = b0 + u * 256;
+#pragma unroll
+                for (int k = 0; k < KS; ++k)
+                    v[u][k] = (b < nblk && k < ns)
+                                  ? __hip_atomic_load(const_cast<double*>(partials) + static_cast<long long>(k) * kTailBlocks + b,
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                  : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < BB; ++u)
+#pragma unroll
+                for (int k = 0; k < KS; ++k)
+                    if (b0 + u * 256 < nblk && k < ns)
+                        acc[k] = ((R.maxmask >> k) & 1u) ? fmax(acc[k], v[u][k]) : acc[k] + v[u][k];
+        }
+#pragma unroll
+        for (int k = 0; k < KS; ++k)
+            if (k < ns) {   // uniform across the block
+                const double t = ((R.maxmask >> k) & 1u) ? wave_max(acc[k]) : wave_sum(acc[k]);
+                if (l == 0) wsum[k][w + h * (NT / 64)] = t;
+            }
+    }
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < ns) {
+        const int k = threadIdx.x;
+        double r;
+        if ((R.maxmask >> k) & 1u) {
+            r = wsum[k][0];
+            for (int i = 1; i < 4; ++i) r = fmax(r, wsum[k][i]);
+        } else {
+            r = 0.0;
+            for (int i = 0; i < 4; ++i) r += wsum[k][i];
+        }
+        R.out[k] = bad ? __builtin_nan("") : r;
+    }
+    __syncthreads();
+    return true;
+}
+
 }  // namespace wd

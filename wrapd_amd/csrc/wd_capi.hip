@@ -26,7 +26,7 @@ wrapd_ctx::~wrapd_ctx() {
     f(d_ftasks); f(d_ffail);
     f(mlp_master); f(mlp_m1); f(mlp_m2); f(mlp_gsum); f(mlp_norm); f(mlp_bf); f(mlp_loss); f(mlp_partial);
     f(mlp_lpart); f(mlp_log);
-    f(partials); f(scal);
+    f(partials); f(scal); f(d_red_flags); f(d_red_abort);
     if (scal_host) (void)hipHostFree(scal_host);
     for (auto& p : prof.pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (auto e : prof.pool) (void)hipEventDestroy(e);
@@ -114,8 +114,6 @@ static int launch_status(wrapd_ctx* c) {
     return check_hip(c, hipGetLastError(), "kernel launch") ? WRAPD_ERR_HIP : WRAPD_OK;
 }
 
-static int elem_blocks(wrapd_ctx* c) { return blocks_for(c->n_e, kElemThreads); }
-static int local_blocks(wrapd_ctx* c) { return blocks_for(c->n_e, kLocalThreads); }
 
 }  // namespace wd
 
@@ -136,7 +134,13 @@ int wrapd_create(wrapd_ctx** out, int device) {
     if (check_hip(c, hipSetDevice(device), "hipSetDevice") ||
         check_hip(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate") ||
         dalloc(c, c->partials, static_cast<size_t>(kMaxBlocks) * kRedStride) || dalloc(c, c->scal, S_NUM) ||
+        dalloc(c, c->d_red_flags, kMaxBlocks) || dalloc(c, c->d_red_abort, 1) ||
         check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&c->scal_host), S_NUM * sizeof(double)), "hipHostMalloc")) {
+        delete c;
+        return WRAPD_ERR_HIP;
+    }
+    if (check_hip(c, hipMemset(c->d_red_flags, 0, kMaxBlocks * sizeof(unsigned)), "hipMemset") ||
+        check_hip(c, hipMemset(c->d_red_abort, 0, sizeof(int)), "hipMemset")) {
         delete c;
         return WRAPD_ERR_HIP;
     }
@@ -174,6 +178,10 @@ int wrapd_upload_mesh(wrapd_ctx* c, int n_v, const double* rest, int n_e, const 
     if (mat->model != 0 && mat->model != 1) { c->last_error = "unknown model"; return WRAPD_ERR_ARG; }
     for (long long i = 0; i < 4ll * n_e; ++i)
         if (tets[i] < 0 || tets[i] >= n_v) { c->last_error = "tet index out of range"; return WRAPD_ERR_ARG; }
+    if (blocks_for(n_e, kLocalThreads) > kMaxBlocks || blocks_for(3ll * n_v, kElemThreads) > kMaxBlocks) {
+        c->last_error = "wrapd_upload_mesh: mesh larger than the reduction buffers (65536 blocks)";
+        return WRAPD_ERR_ARG;
+    }
     (void)hipSetDevice(c->device);
     c->n_v = n_v;
     c->n_e = n_e;
@@ -400,8 +408,8 @@ int wrapd_set_rot_aware(wrapd_ctx* c, int rot_aware) {
 int wrapd_init_duals(wrapd_ctx* c, int rot_aware, double* polar_pe) {
     CTX_OK(c); MESH_OK(c);
     c->rot_aware = rot_aware ? 1 : 0;
+    arm_tail(c, 1, 0u, S_RED);
     launch_init_duals(c, c->rot_aware);
-    launch_finalize(c, elem_blocks(c), 1, 0u, S_RED);
     const double* r = scal_read(c, S_RED, 1);
     if (!r) return WRAPD_ERR_HIP;
     if (polar_pe) *polar_pe = r[0];
@@ -410,8 +418,8 @@ int wrapd_init_duals(wrapd_ctx* c, int rot_aware, double* polar_pe) {
 
 int wrapd_defo_energy(wrapd_ctx* c, double* energy, int* flips) {
     CTX_OK(c); MESH_OK(c);
+    arm_tail(c, 3, 0u, S_RED);
     launch_defo(c, c->rot_aware ? (DM_WRITE_SYMF | DM_ENERGY) : (DM_FT_CACHE | DM_ENERGY), 0);
-    launch_finalize(c, elem_blocks(c), 3, 0u, S_RED);
     const double* r = scal_read(c, S_RED, 3);
     if (!r) return WRAPD_ERR_HIP;
     c->last_energy = r[1];
@@ -430,8 +438,8 @@ int wrapd_candidate_weights(wrapd_ctx* c, int use_hessian, int direct) {
 
 int wrapd_max_wsq_ratio(wrapd_ctx* c, double* ratio) {
     CTX_OK(c); MESH_OK(c);
+    arm_tail(c, 1, 1u, S_RED);
     launch_max_ratio(c);
-    launch_finalize(c, elem_blocks(c), 1, 1u, S_RED);
     const double* r = scal_read(c, S_RED, 1);
     if (!r) return WRAPD_ERR_HIP;
     if (ratio) *ratio = r[0];
@@ -455,8 +463,8 @@ int wrapd_local_step(wrapd_ctx* c, const wrapd_local_args* a, double pe_prev, wr
         c->last_error = "learned prox: no network (wrapd_mlp_init / wrapd_mlp_set_params)";
         return WRAPD_ERR_STATE;
     }
+    arm_tail(c, 11, (1u << 3) | (1u << 9), S_RED);
     launch_local(c, a);
-    launch_finalize(c, local_blocks(c), 11, (1u << 3) | (1u << 9), S_RED);
     const double* r = scal_read(c, S_RED, 11);
     if (!r) return WRAPD_ERR_HIP;
     if (out) {
@@ -490,8 +498,8 @@ int wrapd_local_step_unaware(wrapd_ctx* c, const wrapd_local_args* a) {
 
 int wrapd_global_value_grad(wrapd_ctx* c, double* value, double* grad) {
     CTX_OK(c); MESH_OK(c);
+    arm_tail(c, 1, 0u, S_VAL);
     launch_defo(c, DM_GRAD | DM_VALUE, 0);
-    launch_finalize(c, elem_blocks(c), 1, 0u, S_VAL);
     launch_gather_grad(c, c->grad, 0, 0, nullptr, 0);
     const double* r = scal_read(c, S_VAL, 1);
     if (!r) return WRAPD_ERR_HIP;
@@ -606,10 +614,10 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
     o.exit_reason = 0;
     if (c->m_k == 0) launch_fill(c, c->grad_old, 0.0, n3);   // minimize() re-zeros its vectors when m_k == 0
     // value(x) fused with the first gradient (the gradient from the cache at the same x)
+    arm_tail(c, 1, 0u, S_VAL);
     launch_defo(c, DM_GRAD | DM_VALUE, 0);
-    launch_finalize(c, elem_blocks(c), 1, 0u, S_VAL);
+    arm_tail(c, 1, 0u, S_DG);
     launch_gather_grad(c, c->grad, 1, 0, nullptr, 0);
-    launch_finalize(c, blocks_for(c->n_f, 256), 1, 0u, S_DG);
     const double* r = scal_read(c, S_DG, 2);   // S_DG, S_VAL
     if (!r) return WRAPD_ERR_HIP;
     const double dg = r[0];
@@ -639,13 +647,11 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
         if (it > 0) {
             double* vs[kMaxHist];
             for (int i = 0; i < it; ++i) vs[i] = c->hS[slots[i]];
-            launch_multidot(c, g, vs, it);
-            launch_lbfgs_alpha(c, slots, it);
+            launch_multidot(c, g, vs, it, 1, slots);   // and the alpha recurrence
             launch_axpy_q(c, g, slots, it);
             launch_solve(c, c->qv, c->rv);
             for (int i = 0; i < it; ++i) vs[i] = c->hY[slots[i]];
-            launch_multidot(c, c->rv, vs, it);
-            launch_lbfgs_beta(c, slots, it);
+            launch_multidot(c, c->rv, vs, it, 2, slots);   // and the beta recurrence
             launch_axpy_r(c, g, slots, it);
             const double* d = scal_read(c, S_DIR, 1);
             if (!d) return WRAPD_ERR_HIP;
@@ -691,8 +697,8 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
                 d = scal_read(c, S_RED, 1);
                 if (!d) return WRAPD_ERR_HIP;
                 const double gtp = -d[0];
+                arm_tail(c, 1, 0u, S_RED);
                 launch_defo(c, DM_VALUE, 0);
-                launch_finalize(c, elem_blocks(c), 1, 0u, S_RED);
                 d = scal_read(c, S_RED, 1);
                 if (!d) return WRAPD_ERR_HIP;
                 const double fx0 = d[0];
@@ -701,8 +707,8 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
                 int lsit = 0;
                 for (; lsit < 100000; ++lsit) {
                     launch_armijo_x(c, -alpha, stepdir, c->bv);   // x = x_save - alpha q
+                    arm_tail(c, 1, 0u, S_RED);
                     launch_defo(c, DM_VALUE, 0);
-                    launch_finalize(c, elem_blocks(c), 1, 0u, S_RED);
                     d = scal_read(c, S_RED, 1);
                     if (!d) return WRAPD_ERR_HIP;
                     if (d[0] <= fx0 + alpha * 1e-4 * gtp) break;
@@ -730,8 +736,8 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
         launch_xstep(c, rate, stepdir, c->hS[new_slot]);
         const bool early = dir > 0 && (dir < kappa * m_tol || dir < kappa * 1.e-12);
         const bool last = early || (global_iter + 1 > max_iters);
+        arm_tail(c, last ? 3 : 1, 0u, S_VAL);
         launch_defo(c, last ? (DM_GRAD | DM_VALUE | DM_WRITE_SYMF | DM_ENERGY) : (DM_GRAD | DM_VALUE), 0);
-        launch_finalize(c, elem_blocks(c), last ? 3 : 1, 0u, S_VAL);
         launch_gather_grad(c, c->grad, 2, new_pos, lslots, nlist);
         const double* d = scal_read(c, S_DENOM, 5);   // DENOM, DG, VAL, EN, FLIPS
         if (!d) return WRAPD_ERR_HIP;
@@ -760,8 +766,8 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
     c->m_k = std::min(M, c->m_k);
     o.iters = breaking_out ? global_iter : global_iter - 1;
     if (!have_final) {
+        arm_tail(c, 3, 0u, S_RED);
         launch_defo(c, DM_VALUE | DM_WRITE_SYMF | DM_ENERGY, 0);
-        launch_finalize(c, elem_blocks(c), 3, 0u, S_RED);
         const double* d = scal_read(c, S_RED, 3);
         if (!d) return WRAPD_ERR_HIP;
         o.obj_aft = d[0];
@@ -781,8 +787,8 @@ int wrapd_global_step_unaware(wrapd_ctx* c, wrapd_global_out* out) {
     launch_defo(c, DM_BVEC, 0);
     launch_gather_grad(c, c->bv, 0, 0, nullptr, 0);
     launch_solve(c, c->bv, c->xall);
+    arm_tail(c, 3, 0u, S_RED);
     launch_defo(c, DM_FT_CACHE | DM_ENERGY, 0);
-    launch_finalize(c, elem_blocks(c), 3, 0u, S_RED);
     const double* d = scal_read(c, S_RED, 3);
     if (!d) return WRAPD_ERR_HIP;
     if (std::isnan(d[1]) && solve_aborted(c)) {

@@ -11,19 +11,20 @@
 #include "../../include/wrapd_gpu.h"
 #include "chol_plan.hpp"
 #include "device/wd_mlp.hpp"
+#include "device/wd_reduce.hpp"
 #include "chol_symbolic.hpp"
 
 namespace wd {
 
 constexpr int kElemThreads = 256;     // element kernels
 constexpr int kLocalThreads = 128;    // local step (register heavy)
-constexpr int kRedStride = 24;        // reduction slots per block in `partials`
+constexpr int kRedStride = wd::kTailSlots;   // reduction slots per block in `partials`
 constexpr int kMaxHist = 8;           // largest L-BFGS memory supported on the device
-constexpr int kMaxBlocks = 1 << 16;   // capacity of the partials buffer (blocks)
+constexpr int kMaxBlocks = wd::kTailBlocks;   // capacity of the partials buffer (blocks)
 
 // device scalar slots (ctx->scal)
 enum ScalarSlot {
-    S_RED = 0,          // 0..23: result of the last finalize
+    S_RED = 0,          // 0..23: result of the last reduction
     S_ALPHA = 32,       // 8: two-loop alpha (logical order)
     S_COEF = 40,        // 8: alpha - beta
     S_DIR = 48, S_DENOM = 49, S_DG = 50, S_VAL = 51, S_EN = 52, S_FLIPS = 53, S_GTP = 54, S_GMAX = 55,
@@ -162,6 +163,15 @@ struct wrapd_ctx {
     // reductions
     double* partials = nullptr;
     double* scal = nullptr;      // device scalars
+    unsigned* d_red_flags = nullptr;   // per block of a reducing launch: epoch when its partials are stored
+    int* d_red_abort = nullptr;        // sticky: a reduction's wait ran out
+    unsigned red_epoch = 0;
+    wd::RedTail pending_tail;          // armed by arm_tail, taken by the next reducing launch
+    struct PendingFinalize {
+        int nslots = 0;
+        unsigned maxmask = 0;
+        int dst = 0, nblk = 0;   // nblk 0: none
+    } pending_fin;                     // a taken tail whose launch is too wide to finish it
     double* scal_host = nullptr; // pinned host mirror
 
     // SplitLBFGS host-side state (deps/mcloptlib/include/MCL/SplitLBFGS.hpp:48-62)
@@ -198,12 +208,22 @@ void launch_update_weights(wrapd_ctx* c);
 void launch_max_ratio(wrapd_ctx* c);
 void launch_assemble_A(wrapd_ctx* c);
 // (wd_vec.hip)
-void launch_finalize(wrapd_ctx* c, int nblk, int nslots, unsigned maxmask, int dst);
+// The next reducing launch finishes its own reduction into scal[dst, dst + nslots) in its
+// last block, summing the blocks' partials in a fixed order.
+void arm_tail(wrapd_ctx* c, int nslots, unsigned maxmask, int dst);
+// The armed tail for a launch of nblk blocks.  With may_split and more than
+// kTailMaxBlocks blocks the reduction is instead left to finish_reduction, to be called
+// right after that launch (a one-block finalize launch: one block reading that many
+// partials is slower than a launch).
+wd::RedTail take_tail(wrapd_ctx* c, int nblk, bool may_split);
+void finish_reduction(wrapd_ctx* c);
+constexpr int kTailMaxBlocks = 1024;
 void launch_gather_grad(wrapd_ctx* c, double* out, int mode, int new_slot, const int* list_slots,
                         int nlist);   // mode: 0 plain, 1 first (dg), 2 step (y + Gram)
-void launch_multidot(wrapd_ctx* c, const double* base, double* const* vecs, int nvec);
-void launch_lbfgs_alpha(wrapd_ctx* c, const int* slots, int n);
-void launch_lbfgs_beta(wrapd_ctx* c, const int* slots, int n);
+// dots of base with vecs into S_RED; post 1 then runs the alpha recurrence, 2 the beta one
+// (SlotList of `slots`), in the launch's last block
+void launch_multidot(wrapd_ctx* c, const double* base, double* const* vecs, int nvec, int post = 0,
+                     const int* slots = nullptr);
 void launch_axpy_q(wrapd_ctx* c, const double* g, const int* slots, int n);   // qv = g - sum alpha y
 void launch_axpy_r(wrapd_ctx* c, const double* g, const int* slots, int n);   // rv += sum coef s, dir = g.r
 void launch_dot(wrapd_ctx* c, const double* a, const double* b); // partial slot 0
@@ -212,7 +232,6 @@ void launch_diffnorm(wrapd_ctx* c, const double* a, const double* b);   // |a - 
 void launch_copy(wrapd_ctx* c, double* dst, const double* src, long long n);
 void launch_fill(wrapd_ctx* c, double* dst, double v, long long n);
 void launch_armijo_x(wrapd_ctx* c, double alpha, const double* p, double* xbuf); // xbuf = x + alpha p
-int finalize_blocks_n(wrapd_ctx* c);
 // (wd_solve.hip)
 int chol_prepare_device(wrapd_ctx* c);
 void launch_solve(wrapd_ctx* c, const double* b, double* x);      // n_f x 3 device arrays

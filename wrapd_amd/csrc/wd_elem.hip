@@ -145,7 +145,7 @@ template <int RA>
 __global__ void __launch_bounds__(kElemThreads)
 k_init_duals(int n_e, int n_ep, int n_f, const int4* __restrict__ vidx, const double* __restrict__ Di,
              const double* __restrict__ xall, const double* __restrict__ vol, double* __restrict__ U,
-             double* __restrict__ P, MatArgs m, int model, double* __restrict__ partials) {
+             double* __restrict__ P, MatArgs m, int model, double* __restrict__ partials, RedTail tail) {
     __shared__ double sh[kElemThreads / 64];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int e = lb * blockDim.x + threadIdx.x;
@@ -180,7 +180,8 @@ k_init_duals(int n_e, int n_ep, int n_f, const int4* __restrict__ vidx, const do
         st9(P, n_ep, e, Z);
     }
     const double t = block_sum<kElemThreads>(pe, sh);
-    if (threadIdx.x == 0) partials[lb * kRedStride + 0] = t;
+    if (threadIdx.x == 0) put_partial(partials, lb, 0, t);
+    reduce_tail<kElemThreads, 1>(tail, partials);
 }
 
 // ---------------------------------------------------------------------------------
@@ -190,7 +191,7 @@ k_defo(int n_e, int n_ep, int n_f, const int4* __restrict__ vidx, const double* 
        const double* __restrict__ xall, const double* __restrict__ P, const double* __restrict__ wsq,
        const double* __restrict__ vol, const double* __restrict__ kst, double* __restrict__ symF,
        double* __restrict__ gstage, double* __restrict__ cand, MatArgs m, int use_hessian,
-       double* __restrict__ partials) {
+       double* __restrict__ partials, RedTail tail) {
     __shared__ double sh[kElemThreads / 64];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int e = lb * blockDim.x + threadIdx.x;
@@ -283,16 +284,17 @@ k_defo(int n_e, int n_ep, int n_f, const int4* __restrict__ vidx, const double* 
     }
     if (MODE & DM_VALUE) {
         const double t = block_sum<kElemThreads>(val, sh);
-        if (threadIdx.x == 0) partials[lb * kRedStride + 0] = t;
+        if (threadIdx.x == 0) put_partial(partials, lb, 0, t);
     }
     if (MODE & DM_ENERGY) {
         const double t1 = block_sum<kElemThreads>(en, sh);
         const double t2 = block_sum<kElemThreads>(flip, sh);
         if (threadIdx.x == 0) {
-            partials[lb * kRedStride + 1] = t1;
-            partials[lb * kRedStride + 2] = t2;
+            put_partial(partials, lb, 1, t1);
+            put_partial(partials, lb, 2, t2);
         }
     }
+    if (MODE & (DM_VALUE | DM_ENERGY)) reduce_tail<kElemThreads, 3>(tail, partials);
 }
 
 // ---------------------------------------------------------------------------------
@@ -363,14 +365,33 @@ __device__ __forceinline__ void elem_prox(const M3<double>& Sin, const double sl
     for (int i = 0; i < 3; ++i) x_out[i] = static_cast<double>(x[i]);
 }
 
+// The local step's 11 reduction slots: 0 bef, 1 aft, 2 pe, 3 max ratio, 4 prox iterations,
+// 5 FP64 fallbacks, 6 inverted, 7 learned starts accepted, 8 evaluations, 9 max
+// evaluations, 10 learned starts used.
+__device__ __forceinline__ void store_local_partials(double* partials, int lb, double r0, double r1, double r2,
+                                                     double r3, double r4, double r5, double r6, double r7,
+                                                     double r8, double r9, double r10) {
+    put_partial(partials, lb, 0, r0);
+    put_partial(partials, lb, 1, r1);
+    put_partial(partials, lb, 2, r2);
+    put_partial(partials, lb, 3, r3);
+    put_partial(partials, lb, 4, r4);
+    put_partial(partials, lb, 5, r5);
+    put_partial(partials, lb, 6, r6);
+    put_partial(partials, lb, 7, r7);
+    put_partial(partials, lb, 8, r8);
+    put_partial(partials, lb, 9, r9);
+    put_partial(partials, lb, 10, r10);
+}
+
 template <int MODEL, int PREC, int PROX>
 __global__ void __launch_bounds__(kLocalThreads)
 k_local(int n_e, int n_ep, const double* __restrict__ symF, double* __restrict__ U, double* __restrict__ P,
         const double* __restrict__ wsq, const double* __restrict__ vol, const double* __restrict__ kst,
         double* __restrict__ cand, double* __restrict__ sig, MatArgs m, int prox_iters, double rtol,
         double fallback_sigma, int update_cand, int use_hessian, double* __restrict__ partials,
-        float* __restrict__ tlog) {
-    __shared__ double sh[kLocalThreads / 64];
+        float* __restrict__ tlog, RedTail tail) {
+    __shared__ double sh[4];   // the block reductions', then the tail's 4 wave sums
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int e = lb * blockDim.x + threadIdx.x;
     double bef = 0.0, aft = 0.0, pe = 0.0, ratio = 0.0, iters = 0.0, fallback = 0.0, neg = 0.0, evals = 0.0;
@@ -455,11 +476,8 @@ k_local(int n_e, int n_ep, const double* __restrict__ symF, double* __restrict__
     const double r6 = block_sum<kLocalThreads>(neg, sh);
     const double r8 = block_sum<kLocalThreads>(evals, sh);
     const double r9 = block_max<kLocalThreads>(evals, sh);
-    if (threadIdx.x == 0) {
-        double* o = partials + lb * kRedStride;
-        o[0] = r0; o[1] = r1; o[2] = r2; o[3] = r3; o[4] = r4; o[5] = r5; o[6] = r6; o[7] = 0.0;
-        o[8] = r8; o[9] = r9; o[10] = 0.0;
-    }
+    if (threadIdx.x == 0) store_local_partials(partials, lb, r0, r1, r2, r3, r4, r5, r6, 0.0, r8, r9, 0.0);
+    reduce_tail<kLocalThreads, 11>(tail, partials);
 }
 
 // Learned warm start (WRAPD_PROX_LEARNED): after the SVD of S_in, a wave in which some
@@ -485,9 +503,9 @@ k_local_learned(int n_e, int n_ep, const double* __restrict__ symF, double* __re
                 const double* __restrict__ wsq, const double* __restrict__ vol, const double* __restrict__ kst,
                 double* __restrict__ cand, double* __restrict__ sig, MatArgs m, int prox_iters, double rtol,
                 double fallback_sigma, int update_cand, int use_hessian, double* __restrict__ partials,
-                float* __restrict__ tlog, MlpDevice mlp) {
+                float* __restrict__ tlog, MlpDevice mlp, RedTail tail) {
     using T = typename std::conditional<PREC == WRAPD_FP32, float, double>::type;
-    __shared__ double shr[kLocalThreads / 64];
+    __shared__ double shr[4];   // the block reductions', then the tail's 4 wave sums
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int e = lb * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -607,11 +625,8 @@ k_local_learned(int n_e, int n_ep, const double* __restrict__ symF, double* __re
     const double r8 = block_sum<kLocalThreads>(evals, shr);
     const double r9 = block_max<kLocalThreads>(evals, shr);
     const double r10 = block_sum<kLocalThreads>(used, shr);
-    if (threadIdx.x == 0) {
-        double* o = partials + lb * kRedStride;
-        o[0] = r0; o[1] = r1; o[2] = r2; o[3] = r3; o[4] = r4; o[5] = r5; o[6] = r6; o[7] = r7;
-        o[8] = r8; o[9] = r9; o[10] = r10;
-    }
+    if (threadIdx.x == 0) store_local_partials(partials, lb, r0, r1, r2, r3, r4, r5, r6, r7, r8, r9, r10);
+    reduce_tail<kLocalThreads, 11>(tail, partials);
 }
 
 // Rotation-unaware local step (src/TetElements.cpp:193-254); the cache holds Ft = F^T.
@@ -686,7 +701,8 @@ k_update_w(int n_e, int n_ep, double* __restrict__ U, double* __restrict__ P, do
 }
 
 __global__ void __launch_bounds__(kElemThreads)
-k_max_ratio(int n_e, const double* __restrict__ wsq, const double* __restrict__ cand, double* __restrict__ partials) {
+k_max_ratio(int n_e, const double* __restrict__ wsq, const double* __restrict__ cand, double* __restrict__ partials,
+            RedTail tail) {
     __shared__ double sh[kElemThreads / 64];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int e = lb * blockDim.x + threadIdx.x;
@@ -696,7 +712,8 @@ k_max_ratio(int n_e, const double* __restrict__ wsq, const double* __restrict__ 
         r = a > b ? a : b;
     }
     const double t = block_max<kElemThreads>(r, sh);
-    if (threadIdx.x == 0) partials[lb * kRedStride + 0] = t;
+    if (threadIdx.x == 0) put_partial(partials, lb, 0, t);
+    reduce_tail<kElemThreads, 1>(tail, partials);
 }
 
 // A(slot) = sum over (element, a, b) contributions of w_e^2 d_a . d_b, in element order
@@ -742,16 +759,19 @@ void launch_init_duals(wrapd_ctx* c, int ra) {
     auto k = ra ? k_init_duals<1> : k_init_duals<0>;
     hipLaunchKernelGGL(k, dim3(nb), dim3(kElemThreads), 0, c->stream, c->n_e, c->n_ep, c->n_f,
                        reinterpret_cast<const int4*>(c->vidx), c->Di, c->xall, c->vol, c->U, c->P, mat_args(c),
-                       c->mat.model, c->partials);
+                       c->mat.model, c->partials, take_tail(c, nb, true));
+    finish_reduction(c);
 }
 
 template <int MODE>
 static void defo_mode(wrapd_ctx* c, int use_hessian) {
     const int nb = blocks_for(c->n_e, kElemThreads);
     auto k = c->mat.model == 1 ? k_defo<MODE, 1> : k_defo<MODE, 0>;
+    const RedTail t = take_tail(c, nb, true);   // used by the modes that reduce (value, energy)
     hipLaunchKernelGGL(k, dim3(nb), dim3(kElemThreads), 0, c->stream, c->n_e, c->n_ep, c->n_f,
                        reinterpret_cast<const int4*>(c->vidx), c->Di, c->xall, c->P, c->wsq, c->vol, c->kst,
-                       c->symF, c->gstage, c->cand, mat_args(c), use_hessian, c->partials);
+                       c->symF, c->gstage, c->cand, mat_args(c), use_hessian, c->partials, t);
+    finish_reduction(c);
 }
 
 void launch_defo(wrapd_ctx* c, int mode, int use_hessian) {
@@ -777,7 +797,9 @@ static void local_inst(wrapd_ctx* c, const wrapd_local_args* a, float* tlog) {
     const int nb = blocks_for(c->n_e, kLocalThreads);
     hipLaunchKernelGGL((k_local<MODEL, PREC, PROX>), dim3(nb), dim3(kLocalThreads), 0, c->stream, c->n_e, c->n_ep,
                        c->symF, c->U, c->P, c->wsq, c->vol, c->kst, c->cand, c->sig, mat_args(c), a->prox_iters,
-                       a->newton_rtol, a->fallback_sigma, a->update_cand, a->cand_use_hessian, c->partials, tlog);
+                       a->newton_rtol, a->fallback_sigma, a->update_cand, a->cand_use_hessian, c->partials, tlog,
+                       take_tail(c, nb, true));
+    finish_reduction(c);
 }
 
 template <int PREC>
@@ -786,7 +808,8 @@ static void local_learned_inst(wrapd_ctx* c, const wrapd_local_args* a, float* t
     hipLaunchKernelGGL((k_local_learned<PREC>), dim3(nb), dim3(kLocalThreads), 0, c->stream, c->n_e, c->n_ep,
                        c->symF, c->U, c->P, c->wsq, c->vol, c->kst, c->cand, c->sig, mat_args(c), a->prox_iters,
                        a->newton_rtol, a->fallback_sigma, a->update_cand, a->cand_use_hessian, c->partials, tlog,
-                       mlp_device(c));
+                       mlp_device(c), take_tail(c, nb, true));
+    finish_reduction(c);
 }
 
 void launch_local(wrapd_ctx* c, const wrapd_local_args* a) {
@@ -828,8 +851,10 @@ void launch_update_weights(wrapd_ctx* c) {
 
 void launch_max_ratio(wrapd_ctx* c) {
     ProfScope ps(c, "max_ratio");
-    hipLaunchKernelGGL(k_max_ratio, dim3(blocks_for(c->n_e, kElemThreads)), dim3(kElemThreads), 0, c->stream, c->n_e,
-                       c->wsq, c->cand, c->partials);
+    const int nb = blocks_for(c->n_e, kElemThreads);
+    hipLaunchKernelGGL(k_max_ratio, dim3(nb), dim3(kElemThreads), 0, c->stream, c->n_e, c->wsq, c->cand, c->partials,
+                       take_tail(c, nb, true));
+    finish_reduction(c);
 }
 
 void launch_assemble_A(wrapd_ctx* c) {

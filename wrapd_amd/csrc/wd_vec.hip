@@ -1,5 +1,5 @@
-// Vertex-space kernels: the atomic-free D^T z gather, the SplitLBFGS vector work and
-// the deterministic reductions.
+// Vertex-space kernels: the atomic-free D^T z gather and the SplitLBFGS vector work.
+// Their reductions finish in the launch's last block (reduce_tail, device/wd_reduce.hpp).
 //
 // The two-loop recursion of SplitLBFGS::minimize (deps/mcloptlib/include/MCL/SplitLBFGS.hpp:152-177)
 // is evaluated with a maintained Gram matrix G[i][j] = s_i . y_j:
@@ -26,18 +26,86 @@ struct VecPtrs {
     const double* v[kMaxHist];
 };
 
+// The reduction of a launch with more than kTailMaxBlocks blocks, as a launch of its own:
+// the same order as reduce_tail (lane v of 256 takes blocks v, v + 256, ...).
+// KS: the most slots (nslots <= KS); BB blocks' slots are loaded per batch.
+template <int KS>
 __global__ void __launch_bounds__(256)
 k_finalize(const double* __restrict__ partials, int nblk, int nslots, unsigned maxmask, double* __restrict__ out) {
-    __shared__ double sh[4];
-    for (int k = 0; k < nslots; ++k) {
-        const bool mx = (maxmask >> k) & 1u;
-        double acc = mx ? -1e308 : 0.0;
-        for (int b = threadIdx.x; b < nblk; b += 256) {
-            const double v = partials[b * kRedStride + k];
-            acc = mx ? fmax(acc, v) : acc + v;
+    constexpr int BB = KS > 4 ? 4 : 8;
+    __shared__ double sh[KS][4];
+    double acc[KS];
+#pragma unroll
+    for (int k = 0; k < KS; ++k) acc[k] = ((maxmask >> k) & 1u) ? -1e308 : 0.0;
+    for (int b0 = threadIdx.x; b0 < nblk; b0 += 256 * BB) {
+        double v[BB][KS];
+#pragma unroll
+        for (int u = 0; u < BB; ++u)
+#pragma unroll
+            for (int k = 0; k < KS; ++k) {
+                const int b = b0 + 256 * u;
+                v[u][k] = (b < nblk && k < nslots) ? partials[static_cast<long long>(k) * kTailBlocks + b] : 0.0;
+            }
+#pragma unroll
+        for (int u = 0; u < BB; ++u)
+#pragma unroll
+            for (int k = 0; k < KS; ++k)
+                if (b0 + 256 * u < nblk && k < nslots)
+                    acc[k] = ((maxmask >> k) & 1u) ? fmax(acc[k], v[u][k]) : acc[k] + v[u][k];
+    }
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < KS; ++k)
+        if (k < nslots) {
+            const double t = ((maxmask >> k) & 1u) ? wave_max(acc[k]) : wave_sum(acc[k]);
+            if (l == 0) sh[k][w] = t;
         }
-        const double r = mx ? block_max<256>(acc, sh) : block_sum<256>(acc, sh);
-        if (threadIdx.x == 0) out[k] = r;
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < nslots) {
+        const int k = threadIdx.x;
+        double r;
+        if ((maxmask >> k) & 1u) {
+            r = sh[k][0];
+            for (int i = 1; i < 4; ++i) r = fmax(r, sh[k][i]);
+        } else {
+            r = 0.0;
+            for (int i = 0; i < 4; ++i) r += sh[k][i];
+        }
+        out[k] = r;
+    }
+}
+
+// Store the reduced Gram entries of the new pair and the y.y denominator.
+__device__ void gram_store(double* __restrict__ scal, const SlotList& L, int ns_pos) {
+    const int sn = L.slot[ns_pos];
+    for (int j = 0; j < L.n; ++j) {
+        const int sj = L.slot[j];
+        scal[S_GRAM + sn * kMaxHist + sj] = scal[S_RED + 2 * j];       // s_new . y_j
+        scal[S_GRAM + sj * kMaxHist + sn] = scal[S_RED + 2 * j + 1];   // s_j . y_new
+    }
+    scal[S_DENOM] = scal[S_RED + 2 * L.n];
+}
+
+// alpha recurrence (first loop), logical order 0 = oldest
+__device__ void lbfgs_alpha(double* __restrict__ scal, const SlotList& L) {
+    for (int i = L.n - 1; i >= 0; --i) {
+        const int si = L.slot[i];
+        double a = scal[S_RED + i];
+        for (int j = i + 1; j < L.n; ++j) a -= scal[S_ALPHA + j] * scal[S_GRAM + si * kMaxHist + L.slot[j]];
+        const double rho = 1.0 / scal[S_GRAM + si * kMaxHist + si];
+        scal[S_ALPHA + i] = rho * a;
+    }
+}
+
+// beta recurrence (second loop); stores alpha_i - beta_i
+__device__ void lbfgs_beta(double* __restrict__ scal, const SlotList& L) {
+    for (int i = 0; i < L.n; ++i) {
+        const int si = L.slot[i];
+        double b = scal[S_RED + i];
+        for (int j = 0; j < i; ++j) b += scal[S_COEF + j] * scal[S_GRAM + L.slot[j] * kMaxHist + si];
+        const double rho = 1.0 / scal[S_GRAM + si * kMaxHist + si];
+        const double beta = rho * b;
+        scal[S_COEF + i] = scal[S_ALPHA + i] - beta;
     }
 }
 
@@ -49,7 +117,7 @@ __global__ void __launch_bounds__(kVecThreads)
 k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, const double* __restrict__ gstage,
          double* __restrict__ out, const double* __restrict__ gold, double* __restrict__ ynew,
          const double* __restrict__ snew, VecPtrs S, VecPtrs Y, SlotList L, int ns_pos,
-         double* __restrict__ partials) {
+         double* __restrict__ partials, RedTail tail, double* __restrict__ scal) {
     __shared__ double sh[kVecThreads / 64];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int v = lb * blockDim.x + threadIdx.x;
@@ -75,7 +143,7 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
             }
         }
         const double t = block_sum<kVecThreads>(a, sh);
-        if (threadIdx.x == 0) partials[lb * kRedStride + 0] = t;
+        if (threadIdx.x == 0) put_partial(partials, lb, 0, t);
     }
     if (MODE == 2) {
         double y[3] = {0, 0, 0}, s[3] = {0, 0, 0};
@@ -103,31 +171,22 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
             const double ta = block_sum<kVecThreads>(a, sh);
             const double tb = block_sum<kVecThreads>(b, sh);
             if (threadIdx.x == 0) {
-                partials[lb * kRedStride + 2 * j] = ta;
-                partials[lb * kRedStride + 2 * j + 1] = tb;
+                put_partial(partials, lb, 2 * j, ta);
+                put_partial(partials, lb, 2 * j + 1, tb);
             }
         }
         double d = y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
         const double td = block_sum<kVecThreads>(d, sh);
-        if (threadIdx.x == 0) partials[lb * kRedStride + 2 * L.n] = td;
+        if (threadIdx.x == 0) put_partial(partials, lb, 2 * L.n, td);
     }
-}
-
-// Store the reduced Gram entries of the new pair and the y.y denominator.
-__global__ void k_gram_store(double* __restrict__ scal, SlotList L, int ns_pos) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int sn = L.slot[ns_pos];
-    for (int j = 0; j < L.n; ++j) {
-        const int sj = L.slot[j];
-        scal[S_GRAM + sn * kMaxHist + sj] = scal[S_RED + 2 * j];       // s_new . y_j
-        scal[S_GRAM + sj * kMaxHist + sn] = scal[S_RED + 2 * j + 1];   // s_j . y_new
-    }
-    scal[S_DENOM] = scal[S_RED + 2 * L.n];
+    if (MODE != 0 && reduce_tail<kVecThreads, MODE == 1 ? 1 : 2 * kMaxHist + 1>(tail, partials) && MODE == 2 && threadIdx.x == 0)
+        gram_store(scal, L, ns_pos);
 }
 
 // partial dots of `base` with up to 8 vectors
 __global__ void __launch_bounds__(kVecThreads)
-k_multidot(long long n3, const double* __restrict__ base, VecPtrs V, int nvec, double* __restrict__ partials) {
+k_multidot(long long n3, const double* __restrict__ base, VecPtrs V, int nvec, double* __restrict__ partials,
+           RedTail tail, double* __restrict__ scal, SlotList L, int post) {
     __shared__ double sh[kVecThreads / 64];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const long long i0 = static_cast<long long>(lb) * blockDim.x * 4 + threadIdx.x;
@@ -146,34 +205,15 @@ k_multidot(long long n3, const double* __restrict__ base, VecPtrs V, int nvec, d
     }
     for (int j = 0; j < nvec; ++j) {
         const double t = block_sum<kVecThreads>(acc[j], sh);
-        if (threadIdx.x == 0) partials[lb * kRedStride + j] = t;
+        if (threadIdx.x == 0) put_partial(partials, lb, j, t);
+    }
+    // the last block also runs the recurrence that consumes the dots (post 1: alpha, 2: beta)
+    if (reduce_tail<kVecThreads, kMaxHist>(tail, partials) && threadIdx.x == 0) {
+        if (post == 1) lbfgs_alpha(scal, L);
+        else if (post == 2) lbfgs_beta(scal, L);
     }
 }
 
-// alpha recurrence (first loop), logical order 0 = oldest
-__global__ void k_lbfgs_alpha(double* __restrict__ scal, SlotList L) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (int i = L.n - 1; i >= 0; --i) {
-        const int si = L.slot[i];
-        double a = scal[S_RED + i];
-        for (int j = i + 1; j < L.n; ++j) a -= scal[S_ALPHA + j] * scal[S_GRAM + si * kMaxHist + L.slot[j]];
-        const double rho = 1.0 / scal[S_GRAM + si * kMaxHist + si];
-        scal[S_ALPHA + i] = rho * a;
-    }
-}
-
-// beta recurrence (second loop); stores alpha_i - beta_i
-__global__ void k_lbfgs_beta(double* __restrict__ scal, SlotList L) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (int i = 0; i < L.n; ++i) {
-        const int si = L.slot[i];
-        double b = scal[S_RED + i];
-        for (int j = 0; j < i; ++j) b += scal[S_COEF + j] * scal[S_GRAM + L.slot[j] * kMaxHist + si];
-        const double rho = 1.0 / scal[S_GRAM + si * kMaxHist + si];
-        const double beta = rho * b;
-        scal[S_COEF + i] = scal[S_ALPHA + i] - beta;
-    }
-}
 
 // q = g - sum_j alpha_j y_j
 __global__ void __launch_bounds__(kVecThreads)
@@ -189,7 +229,7 @@ k_axpy_q(long long n3, const double* __restrict__ g, VecPtrs Y, int n, const dou
 // r += sum_j coef_j s_j;  partial dir = g . r
 __global__ void __launch_bounds__(kVecThreads)
 k_axpy_r(long long n3, double* __restrict__ r, VecPtrs S, int n, const double* __restrict__ scal,
-         const double* __restrict__ g, double* __restrict__ partials) {
+         const double* __restrict__ g, double* __restrict__ partials, RedTail tail) {
     __shared__ double sh[kVecThreads / 64];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const long long i = static_cast<long long>(lb) * blockDim.x + threadIdx.x;
@@ -201,11 +241,13 @@ k_axpy_r(long long n3, double* __restrict__ r, VecPtrs S, int n, const double* _
         d = g[i] * v;
     }
     const double t = block_sum<kVecThreads>(d, sh);
-    if (threadIdx.x == 0) partials[lb * kRedStride + 0] = t;
+    if (threadIdx.x == 0) put_partial(partials, lb, 0, t);
+    reduce_tail<kVecThreads, 1>(tail, partials);
 }
 
 __global__ void __launch_bounds__(kVecThreads)
-k_dot(long long n3, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ partials) {
+k_dot(long long n3, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ partials,
+      RedTail tail) {
     __shared__ double sh[kVecThreads / 64];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const long long i = static_cast<long long>(lb) * blockDim.x + threadIdx.x;
@@ -213,7 +255,11 @@ k_dot(long long n3, const double* __restrict__ a, const double* __restrict__ b, 
     if (i < n3) { d = a[i] * b[i]; m = fabs(a[i]); }
     const double t = block_sum<kVecThreads>(d, sh);
     const double tm = block_max<kVecThreads>(m, sh);
-    if (threadIdx.x == 0) { partials[lb * kRedStride + 0] = t; partials[lb * kRedStride + 1] = tm; }
+    if (threadIdx.x == 0) {
+        put_partial(partials, lb, 0, t);
+        put_partial(partials, lb, 1, tm);
+    }
+    reduce_tail<kVecThreads, 2>(tail, partials);
 }
 
 // x <- x - rate * dir;  s = x_new - x_old
@@ -241,14 +287,16 @@ k_copy(long long n, double* __restrict__ dst, const double* __restrict__ src) {
 }
 
 __global__ void __launch_bounds__(kVecThreads)
-k_diffnorm(long long n3, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ partials) {
+k_diffnorm(long long n3, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ partials,
+           RedTail tail) {
     __shared__ double sh[kVecThreads / 64];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const long long i = static_cast<long long>(lb) * blockDim.x + threadIdx.x;
     double d = 0.0;
     if (i < n3) { const double t = a[i] - b[i]; d = t * t; }
     const double t = block_sum<kVecThreads>(d, sh);
-    if (threadIdx.x == 0) partials[lb * kRedStride + 0] = t;
+    if (threadIdx.x == 0) put_partial(partials, lb, 0, t);
+    reduce_tail<kVecThreads, 1>(tail, partials);
 }
 
 __global__ void __launch_bounds__(kVecThreads)
@@ -265,10 +313,36 @@ static SlotList mk_list(const int* slots, int n) {
     return L;
 }
 
-void launch_finalize(wrapd_ctx* c, int nblk, int nslots, unsigned maxmask, int dst) {
-    ProfScope ps(c, "finalize");
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, c->partials, nblk, nslots, maxmask,
-                       c->scal + dst);
+void arm_tail(wrapd_ctx* c, int nslots, unsigned maxmask, int dst) {
+    RedTail t;
+    t.flags = c->d_red_flags;
+    t.abortf = c->d_red_abort;
+    t.out = c->scal + dst;
+    t.nslots = nslots;
+    t.maxmask = maxmask;
+    c->pending_tail = t;
+}
+
+RedTail take_tail(wrapd_ctx* c, int nblk, bool may_split) {
+    RedTail t = c->pending_tail;
+    c->pending_tail = RedTail{};
+    if (!t.flags) return t;
+    if (may_split && nblk > kTailMaxBlocks) {   // one block would read too many partials: finalize apart
+        c->pending_fin = wrapd_ctx::PendingFinalize{t.nslots, t.maxmask, static_cast<int>(t.out - c->scal), nblk};
+        return RedTail{};
+    }
+    if (++c->red_epoch == 0) c->red_epoch = 1;   // flags start at 0
+    t.epoch = c->red_epoch;
+    return t;
+}
+
+void finish_reduction(wrapd_ctx* c) {
+    const wrapd_ctx::PendingFinalize f = c->pending_fin;
+    if (f.nblk == 0) return;
+    c->pending_fin = wrapd_ctx::PendingFinalize{};
+    auto k = f.nslots <= 1 ? k_finalize<1> : f.nslots <= 3 ? k_finalize<3> : f.nslots <= 11 ? k_finalize<11>
+                                                                                          : k_finalize<kTailSlots>;
+    hipLaunchKernelGGL(k, dim3(1), dim3(256), 0, c->stream, c->partials, f.nblk, f.nslots, f.maxmask, c->scal + f.dst);
 }
 
 void launch_gather_grad(wrapd_ctx* c, double* out, int mode, int new_pos, const int* list_slots, int nlist) {
@@ -280,39 +354,24 @@ void launch_gather_grad(wrapd_ctx* c, double* out, int mode, int new_pos, const 
     double* ynew = nullptr;
     const double* snew = nullptr;
     if (mode == 2) { ynew = c->hY[list_slots[new_pos]]; snew = c->hS[list_slots[new_pos]]; }
-    if (mode == 0)
-        hipLaunchKernelGGL(k_gather<0>, dim3(nb), dim3(kVecThreads), 0, c->stream, c->n_f, c->v2e_ptr, c->v2e_idx,
-                           c->gstage, out, c->grad_old, ynew, snew, S, Y, L, new_pos, c->partials);
-    else if (mode == 1)
-        hipLaunchKernelGGL(k_gather<1>, dim3(nb), dim3(kVecThreads), 0, c->stream, c->n_f, c->v2e_ptr, c->v2e_idx,
-                           c->gstage, out, c->grad_old, ynew, snew, S, Y, L, new_pos, c->partials);
-    else
-        hipLaunchKernelGGL(k_gather<2>, dim3(nb), dim3(kVecThreads), 0, c->stream, c->n_f, c->v2e_ptr, c->v2e_idx,
-                           c->gstage, out, c->grad_old, ynew, snew, S, Y, L, new_pos, c->partials);
-    if (mode == 2) {
-        launch_finalize(c, nb, 2 * nlist + 1, 0u, S_RED);
-        hipLaunchKernelGGL(k_gram_store, dim3(1), dim3(64), 0, c->stream, c->scal, L, new_pos);
-    }
+    // mode 1 reduces into whatever the caller armed; mode 2 into S_RED, then stores the Gram entries
+    if (mode == 2) arm_tail(c, 2 * nlist + 1, 0u, S_RED);
+    const RedTail t = mode == 0 ? RedTail{} : take_tail(c, nb, false);
+    auto k = mode == 0 ? k_gather<0> : mode == 1 ? k_gather<1> : k_gather<2>;
+    hipLaunchKernelGGL(k, dim3(nb), dim3(kVecThreads), 0, c->stream, c->n_f, c->v2e_ptr, c->v2e_idx, c->gstage, out,
+                       c->grad_old, ynew, snew, S, Y, L, new_pos, c->partials, t, c->scal);
 }
 
-int finalize_blocks_n(wrapd_ctx* c) { return blocks_for(3ll * c->n_f, kVecThreads * 4); }
-
-void launch_multidot(wrapd_ctx* c, const double* base, double* const* vecs, int nvec) {
+void launch_multidot(wrapd_ctx* c, const double* base, double* const* vecs, int nvec, int post, const int* slots) {
     ProfScope ps(c, "multidot");
     VecPtrs V{};
     for (int j = 0; j < nvec; ++j) V.v[j] = vecs[j];
     const long long n3 = 3ll * c->n_f;
     const int nb = blocks_for(n3, kVecThreads * 4);
-    hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kVecThreads), 0, c->stream, n3, base, V, nvec, c->partials);
-    launch_finalize(c, nb, nvec, 0u, S_RED);
-}
-
-void launch_lbfgs_alpha(wrapd_ctx* c, const int* slots, int n) {
-    hipLaunchKernelGGL(k_lbfgs_alpha, dim3(1), dim3(64), 0, c->stream, c->scal, mk_list(slots, n));
-}
-
-void launch_lbfgs_beta(wrapd_ctx* c, const int* slots, int n) {
-    hipLaunchKernelGGL(k_lbfgs_beta, dim3(1), dim3(64), 0, c->stream, c->scal, mk_list(slots, n));
+    arm_tail(c, nvec, 0u, S_RED);
+    const SlotList L = post ? mk_list(slots, nvec) : SlotList{};
+    hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kVecThreads), 0, c->stream, n3, base, V, nvec, c->partials,
+                       take_tail(c, nb, false), c->scal, L, post);
 }
 
 void launch_axpy_q(wrapd_ctx* c, const double* g, const int* slots, int n) {
@@ -330,24 +389,25 @@ void launch_axpy_r(wrapd_ctx* c, const double* g, const int* slots, int n) {
     for (int j = 0; j < n; ++j) S.v[j] = c->hS[slots[j]];
     const long long n3 = 3ll * c->n_f;
     const int nb = blocks_for(n3, kVecThreads);
+    arm_tail(c, 1, 0u, S_DIR);
     hipLaunchKernelGGL(k_axpy_r, dim3(nb), dim3(kVecThreads), 0, c->stream, n3, c->rv, S, n, c->scal, g,
-                       c->partials);
-    launch_finalize(c, nb, 1, 0u, S_DIR);
+                       c->partials, take_tail(c, nb, false));
 }
 
 void launch_dot(wrapd_ctx* c, const double* a, const double* b) {
     ProfScope ps(c, "dot");
     const long long n3 = 3ll * c->n_f;
     const int nb = blocks_for(n3, kVecThreads);
-    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kVecThreads), 0, c->stream, n3, a, b, c->partials);
-    launch_finalize(c, nb, 2, 2u, S_RED);
+    arm_tail(c, 2, 2u, S_RED);
+    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kVecThreads), 0, c->stream, n3, a, b, c->partials, take_tail(c, nb, false));
 }
 
 void launch_diffnorm(wrapd_ctx* c, const double* a, const double* b) {
     const long long n3 = 3ll * c->n_f;
     const int nb = blocks_for(n3, kVecThreads);
-    hipLaunchKernelGGL(k_diffnorm, dim3(nb), dim3(kVecThreads), 0, c->stream, n3, a, b, c->partials);
-    launch_finalize(c, nb, 1, 0u, S_RED);
+    arm_tail(c, 1, 0u, S_RED);
+    hipLaunchKernelGGL(k_diffnorm, dim3(nb), dim3(kVecThreads), 0, c->stream, n3, a, b, c->partials,
+                       take_tail(c, nb, false));
 }
 
 void launch_xstep(wrapd_ctx* c, double rate, const double* dirv, double* s_out) {
