@@ -74,6 +74,7 @@ struct RedTail {
     unsigned* flags = nullptr;   // per block; nullptr: the caller finalizes in a launch of its own
     int* abortf = nullptr;       // sticky: a wait ran out (the results are then NaN)
     double* out = nullptr;       // nslots results
+    double* hout = nullptr;      // their copy in host memory (pinned), or nullptr
     unsigned epoch = 0;
     int nslots = 0;
     unsigned maxmask = 0;        // slots reduced with max instead of sum
@@ -169,7 +170,9 @@ __device__ __forceinline__ bool reduce_tail(const RedTail R, const double* parti
             r = 0.0;
             for (int i = 0; i < 4; ++i) r += wsum[k][i];
         }
-        R.out[k] = bad ? __builtin_nan("") : r;
+        if (bad) r = __builtin_nan("");
+        R.out[k] = r;
+        if (R.hout) __hip_atomic_store(R.hout + k, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     __syncthreads();
     return true;

@@ -31,7 +31,8 @@ struct VecPtrs {
 // KS: the most slots (nslots <= KS); BB blocks' slots are loaded per batch.
 template <int KS>
 __global__ void __launch_bounds__(256)
-k_finalize(const double* __restrict__ partials, int nblk, int nslots, unsigned maxmask, double* __restrict__ out) {
+k_finalize(const double* __restrict__ partials, int nblk, int nslots, unsigned maxmask, double* __restrict__ out,
+           double* __restrict__ hout) {
     constexpr int BB = KS > 4 ? 4 : 8;
     __shared__ double sh[KS][4];
     double acc[KS];
@@ -72,11 +73,13 @@ k_finalize(const double* __restrict__ partials, int nblk, int nslots, unsigned m
             for (int i = 0; i < 4; ++i) r += sh[k][i];
         }
         out[k] = r;
+        __hip_atomic_store(hout + k, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
-// Store the reduced Gram entries of the new pair and the y.y denominator.
-__device__ void gram_store(double* __restrict__ scal, const SlotList& L, int ns_pos) {
+// Store the reduced Gram entries of the new pair and the y.y denominator (which the host
+// reads, so it also goes to the host copy hscal).
+__device__ void gram_store(double* __restrict__ scal, double* __restrict__ hscal, const SlotList& L, int ns_pos) {
     const int sn = L.slot[ns_pos];
     for (int j = 0; j < L.n; ++j) {
         const int sj = L.slot[j];
@@ -84,6 +87,7 @@ __device__ void gram_store(double* __restrict__ scal, const SlotList& L, int ns_
         scal[S_GRAM + sj * kMaxHist + sn] = scal[S_RED + 2 * j + 1];   // s_j . y_new
     }
     scal[S_DENOM] = scal[S_RED + 2 * L.n];
+    __hip_atomic_store(hscal + S_DENOM, scal[S_DENOM], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // alpha recurrence (first loop), logical order 0 = oldest
@@ -180,7 +184,7 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
         if (threadIdx.x == 0) put_partial(partials, lb, 2 * L.n, td);
     }
     if (MODE != 0 && reduce_tail<kVecThreads, MODE == 1 ? 1 : 2 * kMaxHist + 1>(tail, partials) && MODE == 2 && threadIdx.x == 0)
-        gram_store(scal, L, ns_pos);
+        gram_store(scal, tail.hout - S_RED, L, ns_pos);
 }
 
 // partial dots of `base` with up to 8 vectors
@@ -318,6 +322,7 @@ void arm_tail(wrapd_ctx* c, int nslots, unsigned maxmask, int dst) {
     t.flags = c->d_red_flags;
     t.abortf = c->d_red_abort;
     t.out = c->scal + dst;
+    t.hout = c->scal_host + dst;
     t.nslots = nslots;
     t.maxmask = maxmask;
     c->pending_tail = t;
@@ -342,7 +347,8 @@ void finish_reduction(wrapd_ctx* c) {
     c->pending_fin = wrapd_ctx::PendingFinalize{};
     auto k = f.nslots <= 1 ? k_finalize<1> : f.nslots <= 3 ? k_finalize<3> : f.nslots <= 11 ? k_finalize<11>
                                                                                           : k_finalize<kTailSlots>;
-    hipLaunchKernelGGL(k, dim3(1), dim3(256), 0, c->stream, c->partials, f.nblk, f.nslots, f.maxmask, c->scal + f.dst);
+    hipLaunchKernelGGL(k, dim3(1), dim3(256), 0, c->stream, c->partials, f.nblk, f.nslots, f.maxmask, c->scal + f.dst,
+                       c->scal_host + f.dst);
 }
 
 void launch_gather_grad(wrapd_ctx* c, double* out, int mode, int new_pos, const int* list_slots, int nlist) {
