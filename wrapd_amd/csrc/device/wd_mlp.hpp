@@ -71,13 +71,15 @@ __device__ inline f32x4 mfma_bf16(bf16x4 a, bf16x4 b, f32x4 c) {
 }
 
 // A operand of a weight tile: rows [16 tr, +16), k-block t, row-major matrix of width ld
+// (a multiple of 4, so a lane's 4 values are one aligned 8-byte load)
 __device__ inline bf16x4 ld_tile_a(const uint16_t* __restrict__ W, int ld, int tr, int t, int lane) {
-    const uint16_t* p = W + (16 * tr + (lane & 15)) * ld + 16 * t + 4 * (lane >> 4);
-    bf16x4 v;
-    v[0] = static_cast<short>(p[0]);
-    v[1] = static_cast<short>(p[1]);
-    v[2] = static_cast<short>(p[2]);
-    v[3] = static_cast<short>(p[3]);
+    return *reinterpret_cast<const bf16x4*>(W + (16 * tr + (lane & 15)) * ld + 16 * t + 4 * (lane >> 4));
+}
+
+__device__ inline f32x4 ld_bias(const float* __restrict__ b, int rsub) {
+    f32x4 v;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = b[rsub + i];
     return v;
 }
 
@@ -169,12 +171,56 @@ __device__ inline bf16x4 mlp_gather_input(const float f[kMlpIn], int b, int lane
 
 // Whole-wave inference: every lane passes its own features and receives its own three
 // outputs (de-standardised: prox solution minus s_hat).  All 64 lanes must call it.
+// The wave's 64 samples are 4 blocks of 16; each weight tile is loaded once and applied
+// to all 4 (the same products, in the same order, as mlp_block_forward per block).
 __device__ inline void mlp_wave_predict(const MlpDevice& M, const float f[kMlpIn], int lane, float out[kMlpOut]) {
+    const int rsub = 4 * (lane >> 4);
+    bf16x4 xb[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) xb[b] = mlp_gather_input(f, b, lane);
+    bf16x4 h1[4][4], h2[4][4];   // [block][k-block]
+#pragma unroll
+    for (int to = 0; to < 4; ++to) {
+        const bf16x4 w = ld_tile_a(M.bf + kMlpBfW1p, 16, to, 0, lane);
+        const f32x4 bias = ld_bias(M.master + kMlpOffB1 + 16 * to, rsub);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const f32x4 acc = mfma_bf16(w, xb[b], bias);
+            f32x4 g;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) g[i] = gelu(acc[i]);
+            h1[b][to] = acc_to_b(g);
+        }
+    }
+#pragma unroll
+    for (int to = 0; to < 4; ++to) {
+        bf16x4 w[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) w[t] = ld_tile_a(M.bf + kMlpBfW2, 64, to, t, lane);
+        const f32x4 bias = ld_bias(M.master + kMlpOffB2 + 16 * to, rsub);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            f32x4 acc = bias;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc = mfma_bf16(w[t], h1[b][t], acc);
+            f32x4 g;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) g[i] = gelu(acc[i]);
+            h2[b][to] = acc_to_b(g);
+        }
+    }
+    bf16x4 w3[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) w3[t] = ld_tile_a(M.bf + kMlpBfW3p, 64, 0, t, lane);
+    f32x4 b3;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) b3[i] = (rsub + i < kMlpOut) ? M.master[kMlpOffB3 + rsub + i] : 0.0f;
     out[0] = out[1] = out[2] = 0.0f;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-        const bf16x4 xb = mlp_gather_input(f, b, lane);
-        const f32x4 y = mlp_block_forward(M, xb, lane);
+        f32x4 y = b3;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) y = mfma_bf16(w3[t], h2[b][t], y);
         // lane m < 16 holds sample 16b + m in y[0..2]
         const int src = lane & 15;
         const float y0 = __shfl(y[0], src, 64), y1 = __shfl(y[1], src, 64), y2 = __shfl(y[2], src, 64);

@@ -799,7 +799,6 @@ static void local_inst(wrapd_ctx* c, const wrapd_local_args* a, float* tlog) {
                        c->symF, c->U, c->P, c->wsq, c->vol, c->kst, c->cand, c->sig, mat_args(c), a->prox_iters,
                        a->newton_rtol, a->fallback_sigma, a->update_cand, a->cand_use_hessian, c->partials, tlog,
                        take_tail(c, nb, true));
-    finish_reduction(c);
 }
 
 template <int PREC>
@@ -809,11 +808,20 @@ static void local_learned_inst(wrapd_ctx* c, const wrapd_local_args* a, float* t
                        c->symF, c->U, c->P, c->wsq, c->vol, c->kst, c->cand, c->sig, mat_args(c), a->prox_iters,
                        a->newton_rtol, a->fallback_sigma, a->update_cand, a->cand_use_hessian, c->partials, tlog,
                        mlp_device(c), take_tail(c, nb, true));
+}
+
+// The local-step phase times the element kernel; a wide-grid reduction launch that
+// finishes its sums (finish_reduction) follows outside it.
+static void launch_local_kernel(wrapd_ctx* c, const wrapd_local_args* a);
+void launch_local(wrapd_ctx* c, const wrapd_local_args* a) {
+    {
+        ProfScope ps(c, "local_step");
+        launch_local_kernel(c, a);
+    }
     finish_reduction(c);
 }
 
-void launch_local(wrapd_ctx* c, const wrapd_local_args* a) {
-    ProfScope ps(c, "local_step");
+static void launch_local_kernel(wrapd_ctx* c, const wrapd_local_args* a) {
     // tuple log: the whole step is appended when it fits
     float* tlog = nullptr;
     if (c->mlp_logging && c->mat.model == 1 && c->mlp_log_count + c->n_e <= c->mlp_log_cap) {
