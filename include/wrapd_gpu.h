@@ -10,6 +10,8 @@
 #ifndef WRAPD_GPU_H_
 #define WRAPD_GPU_H_
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -158,6 +160,17 @@ int wrapd_global_step_unaware(wrapd_ctx* ctx, wrapd_global_out* out);
 /* element state, for checking: "U","Z","wsq","cand_wsq","sigma_last","vol","Di","symF"
  * with (n_e,9)/(n_e)/(n_e,3)/(n_e,12) row-major shapes. */
 int wrapd_get_elem(wrapd_ctx* ctx, const char* name, double* out);
+
+/* Checkpoint/resume of the device-resident ADMM state: element duals and weights (U, Z-U,
+ * w^2, candidate w^2, cached F, last prox solutions), positions, the L-BFGS history with
+ * its Gram entries and gradients, and the L-BFGS bookkeeping.  No counterpart in the
+ * reference (checkpoint/resume is one of its listed extensions); for callers that stop and
+ * later continue a run bit for bit.  The mesh, material and settings are not included:
+ * load into a context set up with the same ones.  The factorization is not included
+ * either: call wrapd_refactor after a load (it depends only on w^2).
+ * save: with buf == NULL only *size (bytes) is set; otherwise cap must be >= that size. */
+int wrapd_state_save(wrapd_ctx* ctx, void* buf, size_t cap, size_t* size);
+int wrapd_state_load(wrapd_ctx* ctx, const void* buf, size_t size);
 int wrapd_set_elem(wrapd_ctx* ctx, const char* name, const double* in);
 
 /* timing of the device work, measured with HIP events on the context's stream */

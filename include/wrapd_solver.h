@@ -52,6 +52,12 @@ int wrapd_solver_iterate(wrapd_solver* s, int* converged);    /* ADMM::iterate +
 int wrapd_solver_finish(wrapd_solver* s);                     /* ADMM::finish_solve          */
 int wrapd_solver_solve(wrapd_solver* s);                      /* Solver::solve (whole loop)  */
 int wrapd_solver_num_iters(wrapd_solver* s);
+/* Checkpoint/resume (no counterpart in the reference, which lists it as an extension):
+ * checkpoint writes the whole ADMM state after the last iterate to a file; resume, on a
+ * solver created and set up with the same mesh and settings, loads it (and refactors) so
+ * that the following iterations continue the checkpointed run bit for bit. */
+int wrapd_solver_checkpoint(wrapd_solver* s, const char* path);
+int wrapd_solver_resume(wrapd_solver* s, const char* path);
 /* logs: objectives, reweighted, inner_iters, ls_dal, gs_dal, flips, x_errors, residuals,
  * accumulated_time_s, wall_time_s.  Returns the length (writes at most max_len). */
 int wrapd_solver_get_log(wrapd_solver* s, const char* name, double* out, int max_len);

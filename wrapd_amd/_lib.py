@@ -88,6 +88,8 @@ _PROTOS = {
     "wrapd_global_step": (C.c_int, [P, C.c_double, C.POINTER(GlobalOut)]),
     "wrapd_global_step_unaware": (C.c_int, [P, C.POINTER(GlobalOut)]),
     "wrapd_get_elem": (C.c_int, [P, C.c_char_p, D]),
+    "wrapd_state_save": (C.c_int, [P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "wrapd_state_load": (C.c_int, [P, P, C.c_size_t]),
     "wrapd_set_elem": (C.c_int, [P, C.c_char_p, D]),
     "wrapd_profile_enable": (C.c_int, [P, C.c_int]),
     "wrapd_profile_get": (C.c_int, [P, C.c_char_p, D, D]),
@@ -117,6 +119,8 @@ _PROTOS = {
     "wrapd_solver_finish": (C.c_int, [P]),
     "wrapd_solver_solve": (C.c_int, [P]),
     "wrapd_solver_num_iters": (C.c_int, [P]),
+    "wrapd_solver_checkpoint": (C.c_int, [P, C.c_char_p]),
+    "wrapd_solver_resume": (C.c_int, [P, C.c_char_p]),
     "wrapd_solver_get_log": (C.c_int, [P, C.c_char_p, D, C.c_int]),
     "wrapd_solver_get_x": (C.c_int, [P, D]),
     "wrapd_solver_get_x_free": (C.c_int, [P, D]),

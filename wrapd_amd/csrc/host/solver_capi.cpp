@@ -152,6 +152,22 @@ int wrapd_solver_iterate(wrapd_solver* s, int* conv) {
     });
 }
 
+int wrapd_solver_checkpoint(wrapd_solver* s, const char* path) {
+    return guarded(s, [&] {
+        if (!s->setup) throw wrapd::DeviceError(WRAPD_ERR_STATE, "setup first");
+        if (!path) throw wrapd::DeviceError(WRAPD_ERR_ARG, "checkpoint: no path");
+        s->solver->admm().checkpoint(path);
+    });
+}
+
+int wrapd_solver_resume(wrapd_solver* s, const char* path) {
+    return guarded(s, [&] {
+        if (!s->setup) throw wrapd::DeviceError(WRAPD_ERR_STATE, "setup first");
+        if (!path) throw wrapd::DeviceError(WRAPD_ERR_ARG, "resume: no path");
+        s->solver->admm().resume(path);
+    });
+}
+
 int wrapd_solver_finish(wrapd_solver* s) {
     return guarded(s, [&] {
         wrapd::math::MatX3 X = s->solver->m_system->X();

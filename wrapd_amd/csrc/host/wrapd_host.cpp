@@ -244,6 +244,32 @@ void TetElements::refactor() {
     check(wrapd_refactor(m_ctx), "factorize");
     check(wrapd_synchronize(m_ctx), "factorize");
 }
+
+void TetElements::save(ByteWriter& w) {
+    w.pod(m_curr_pe);
+    w.pod(m_cand_count);
+    w.pod(m_energy);
+    w.pod(m_flips);
+    w.pod(m_last_local);
+    w.pod(m_fp64_tail);
+    size_t n = 0;
+    check(wrapd_state_save(m_ctx, nullptr, 0, &n), "state size");
+    std::vector<char> dev(n);
+    check(wrapd_state_save(m_ctx, dev.data(), n, &n), "state save");
+    w.vec(dev);
+}
+
+void TetElements::load(ByteReader& r) {
+    r.pod(m_curr_pe);
+    r.pod(m_cand_count);
+    r.pod(m_energy);
+    r.pod(m_flips);
+    r.pod(m_last_local);
+    r.pod(m_fp64_tail);
+    std::vector<char> dev;
+    r.vec(dev);
+    check(wrapd_state_load(m_ctx, dev.data(), dev.size()), "state load");
+}
 void TetElements::reset_history() { check(wrapd_lbfgs_reset(m_ctx), "clear_history"); }
 void TetElements::global_step(double tol, wrapd_global_out& out) {
     check(wrapd_global_step(m_ctx, tol, &out), "global_step");
@@ -324,6 +350,53 @@ void AlgorithmData::update_per_iter_data() {
 void AlgorithmData::update_per_iter_runtime() {
     m_accumulated_time_s.push_back((m_runtime.local_ms + m_runtime.global_ms + m_runtime.refactor_ms) / 1000.);
     m_wall_time_s.push_back(m_wall.elapsed_ms() / 1000.);
+}
+
+void AlgorithmData::save(ByteWriter& w) const {
+    w.pod(m_runtime);
+    w.pod(m_iter);
+    w.pod(m_init_state_x_error);
+    w.pod(m_init_objective);
+    w.pod(m_init_auglag_obj);
+    w.pod(m_last_localstep_delta_auglag_obj);
+    w.vec(m_norm_obj_residuals);
+    w.vec(m_per_iter_inner_iters);
+    w.vec(m_per_iter_ls_delta_auglag);
+    w.vec(m_per_iter_gs_delta_auglag);
+    w.vec(m_objectives);
+    w.vec(m_state_x_errors);
+    w.vec(m_accumulated_time_s);
+    w.vec(m_wall_time_s);
+    w.vec(m_reweighted);
+    w.vec(m_flips_count);
+    w.vec(m_curr_x_free.v);
+    w.vec(m_curr_x_fix.v);
+}
+
+void AlgorithmData::load(ByteReader& r) {
+    r.pod(m_runtime);
+    r.pod(m_iter);
+    r.pod(m_init_state_x_error);
+    r.pod(m_init_objective);
+    r.pod(m_init_auglag_obj);
+    r.pod(m_last_localstep_delta_auglag_obj);
+    r.vec(m_norm_obj_residuals);
+    r.vec(m_per_iter_inner_iters);
+    r.vec(m_per_iter_ls_delta_auglag);
+    r.vec(m_per_iter_gs_delta_auglag);
+    r.vec(m_objectives);
+    r.vec(m_state_x_errors);
+    r.vec(m_accumulated_time_s);
+    r.vec(m_wall_time_s);
+    r.vec(m_reweighted);
+    r.vec(m_flips_count);
+    std::vector<double> xf, xx;
+    r.vec(xf);
+    r.vec(xx);
+    if (xf.size() != m_curr_x_free.v.size() || xx.size() != m_curr_x_fix.v.size())
+        throw std::runtime_error("checkpoint: the state belongs to another mesh");
+    m_curr_x_free.v = xf;
+    m_curr_x_fix.v = xx;
 }
 
 void AlgorithmData::print_initial_data() const {
@@ -530,6 +603,50 @@ void ADMM::finish_solve(math::MatX3& X) {
         ad.print_final_iter_data();
         std::cout << std::endl;
     }
+}
+
+// File: "WRPDCKP1", the element/iteration counts, then AlgorithmData, the global-step
+// behaviour and the element set (whose device part is wrapd_state_save's blob).
+void ADMM::checkpoint(const std::string& path) {
+    if (!m_initialized) throw std::runtime_error("checkpoint: ADMM not initialized");
+    ByteWriter w;
+    w.buf.insert(w.buf.end(), {'W', 'R', 'P', 'D', 'C', 'K', 'P', '1'});
+    const int ne = m_system->num_tet_elements(), nf = m_algorithm_data->num_free_verts();
+    w.pod(ne);
+    w.pod(nf);
+    m_algorithm_data->save(w);
+    m_global_step_behavior->save(w);
+    m_system->tet_elements()->save(w);
+    std::FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) throw std::runtime_error("checkpoint: cannot write " + path);
+    const size_t n = std::fwrite(w.buf.data(), 1, w.buf.size(), f);
+    const bool ok = std::fclose(f) == 0 && n == w.buf.size();
+    if (!ok) throw std::runtime_error("checkpoint: write failed for " + path);
+}
+
+void ADMM::resume(const std::string& path) {
+    if (!m_initialized) throw std::runtime_error("resume: ADMM not initialized");
+    std::FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) throw std::runtime_error("resume: cannot read " + path);
+    std::vector<char> buf;
+    char tmp[1 << 16];
+    size_t n;
+    while ((n = std::fread(tmp, 1, sizeof(tmp), f)) > 0) buf.insert(buf.end(), tmp, tmp + n);
+    std::fclose(f);
+    ByteReader r{buf.data(), buf.data() + buf.size()};
+    r.need(8);
+    if (std::memcmp(r.p, "WRPDCKP1", 8) != 0) throw std::runtime_error("resume: not a checkpoint: " + path);
+    r.p += 8;
+    int ne = 0, nf = 0;
+    r.pod(ne);
+    r.pod(nf);
+    if (ne != m_system->num_tet_elements() || nf != m_algorithm_data->num_free_verts())
+        throw std::runtime_error("resume: the checkpoint belongs to another mesh");
+    m_algorithm_data->load(r);
+    m_global_step_behavior->load(r);
+    m_system->tet_elements()->load(r);
+    if (r.p != r.end) throw std::runtime_error("resume: trailing data in " + path);
+    m_system->tet_elements()->refactor();   // the factorization of the restored weights
 }
 
 void ADMM::solve(math::MatX3& X) {

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -149,6 +150,44 @@ struct Settings {
 };
 
 // src/RuntimeData.hpp:30-44
+// Flat byte streams for checkpoints (native layout, read back by the same build).
+struct ByteWriter {
+    std::vector<char> buf;
+    template <typename T>
+    void pod(const T& v) {
+        const char* p = reinterpret_cast<const char*>(&v);
+        buf.insert(buf.end(), p, p + sizeof(T));
+    }
+    template <typename T>
+    void vec(const std::vector<T>& v) {
+        pod(static_cast<unsigned long long>(v.size()));
+        const char* p = reinterpret_cast<const char*>(v.data());
+        buf.insert(buf.end(), p, p + v.size() * sizeof(T));
+    }
+};
+struct ByteReader {
+    const char* p = nullptr;
+    const char* end = nullptr;
+    template <typename T>
+    void pod(T& v) {
+        need(sizeof(T));
+        std::memcpy(&v, p, sizeof(T));
+        p += sizeof(T);
+    }
+    template <typename T>
+    void vec(std::vector<T>& v) {
+        unsigned long long n = 0;
+        pod(n);
+        need(n * sizeof(T));
+        v.resize(n);
+        std::memcpy(v.data(), p, n * sizeof(T));
+        p += n * sizeof(T);
+    }
+    void need(size_t n) const {
+        if (static_cast<size_t>(end - p) < n) throw std::runtime_error("checkpoint: truncated");
+    }
+};
+
 struct RuntimeData {
     double setup_solve_ms = 0, local_ms = 0, global_ms = 0, refactor_ms = 0;
     int inner_iters = 0;
@@ -201,6 +240,9 @@ class TetElements {
     void set_x_ref(const math::MatX3& x);
     const wrapd_local_out& last_local() const { return m_last_local; }
     void set_fp64_tail(bool on) { m_fp64_tail = on; }
+    // checkpoint: the element set's host state and the device state (wrapd_state_save)
+    void save(ByteWriter& w);
+    void load(ByteReader& r);   // then refactor() before the next global step
 
     double m_curr_pe = 0.0;
     int m_cand_count = 0;       // src/TetElements.cpp:404 (function-static counter in the reference)
@@ -292,6 +334,8 @@ class AlgorithmData {
     void print_initial_data() const;
     void print_curr_iter_data() const;
     void print_final_iter_data() const;
+    void save(ByteWriter& w) const;   // the per-iteration state and logs (not the set-up lists)
+    void load(ByteReader& r);
     int num_free_verts() const { return m_n_free; }
     int num_fix_verts() const { return static_cast<int>(m_fix_list.size()); }
 
@@ -325,6 +369,8 @@ class GlobalStepBehavior {
     virtual double last_step_delta_auglag() { return 1.e8; }
     virtual void initialize_weights();
     virtual bool update_weights();
+    virtual void save(ByteWriter&) const {}
+    virtual void load(ByteReader&) {}
  protected:
     const Settings m_settings;
     std::shared_ptr<System> m_system;
@@ -344,6 +390,8 @@ class RotAwareGS : public GlobalStepBehavior {
     void initialize_weights() override;
     bool update_weights() override;
     double last_step_delta_auglag() override { return m_obj_aft - m_obj_bef; }
+    void save(ByteWriter& w) const override { w.pod(m_obj_bef); w.pod(m_obj_aft); }
+    void load(ByteReader& r) override { r.pod(m_obj_bef); r.pod(m_obj_aft); }
  private:
     double m_obj_bef = 0.0, m_obj_aft = 0.0;
 };
@@ -359,6 +407,11 @@ class ADMM {
     void iterate();
     bool converged();
     void finish_solve(math::MatX3& X);
+    // Checkpoint/resume (no counterpart in the reference): the whole ADMM state after the
+    // last iterate(), to a file; resume into an ADMM set up (setup_solve) with the same
+    // mesh and settings continues bit for bit.
+    void checkpoint(const std::string& path);
+    void resume(const std::string& path);
  private:
     Settings m_settings;
     std::shared_ptr<System> m_system;

@@ -839,6 +839,91 @@ int wrapd_get_elem(wrapd_ctx* c, const char* name, double* outp) {
     return WRAPD_ERR_ARG;
 }
 
+// ---- checkpoint / resume of the device-resident ADMM state ----------------------------
+extern "C++" {
+namespace {
+struct StateHeader {
+    char magic[8];   // "WRPDST01"
+    int n_e, n_ep, n_f, n_fix, nhist, rot_aware, m_k, last_flips, grad_old_valid;
+    int pos2slot[wd::kMaxHist];
+    double last_energy;
+    wrapd_lbfgs_cfg lcfg;
+};
+// (device pointer, doubles) of every array in the state, in blob order
+std::vector<std::pair<double*, size_t>> state_arrays(wrapd_ctx* c) {
+    const size_t ne = c->n_ep, nv = 3ull * c->n_f;
+    std::vector<std::pair<double*, size_t>> a = {
+        {c->wsq, ne}, {c->cand, ne}, {c->U, 9 * ne}, {c->P, 9 * ne}, {c->symF, 9 * ne}, {c->sig, 3 * ne},
+        {c->xall, 3ull * (c->n_f + c->n_fix)}, {c->grad, nv}, {c->grad_old, nv}, {c->scal, static_cast<size_t>(S_NUM)}};
+    for (int k = 0; k < kMaxHist; ++k) a.push_back({c->hS[k], nv});
+    for (int k = 0; k < kMaxHist; ++k) a.push_back({c->hY[k], nv});
+    return a;
+}
+}  // namespace
+}  // extern "C++"
+
+int wrapd_state_save(wrapd_ctx* c, void* buf, size_t cap, size_t* size) {
+    CTX_OK(c); MESH_OK(c);
+    if (!size) return WRAPD_ERR_ARG;
+    const auto arrs = state_arrays(c);
+    size_t total = sizeof(StateHeader);
+    for (const auto& a : arrs) total += a.second * sizeof(double);
+    *size = total;
+    if (!buf) return WRAPD_OK;
+    if (cap < total) { c->last_error = "wrapd_state_save: buffer too small"; return WRAPD_ERR_ARG; }
+    StateHeader h{};
+    std::memcpy(h.magic, "WRPDST01", 8);
+    h.n_e = c->n_e; h.n_ep = c->n_ep; h.n_f = c->n_f; h.n_fix = c->n_fix; h.nhist = kMaxHist;
+    h.rot_aware = c->rot_aware; h.m_k = c->m_k; h.last_flips = c->last_flips; h.grad_old_valid = c->grad_old_valid;
+    for (int k = 0; k < kMaxHist; ++k) h.pos2slot[k] = c->pos2slot[k];
+    h.last_energy = c->last_energy;
+    h.lcfg = c->lcfg;
+    char* p = static_cast<char*>(buf);
+    std::memcpy(p, &h, sizeof(h));
+    p += sizeof(h);
+    for (const auto& a : arrs) {
+        if (check_hip(c, hipMemcpyAsync(p, a.first, a.second * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+                      "state D2H"))
+            return WRAPD_ERR_HIP;
+        p += a.second * sizeof(double);
+    }
+    return check_hip(c, hipStreamSynchronize(c->stream), "sync") ? WRAPD_ERR_HIP : WRAPD_OK;
+}
+
+int wrapd_state_load(wrapd_ctx* c, const void* buf, size_t size) {
+    CTX_OK(c); MESH_OK(c);
+    if (!buf || size < sizeof(StateHeader)) return WRAPD_ERR_ARG;
+    StateHeader h;
+    std::memcpy(&h, buf, sizeof(h));
+    if (std::memcmp(h.magic, "WRPDST01", 8) != 0) { c->last_error = "wrapd_state_load: not a state blob"; return WRAPD_ERR_ARG; }
+    if (h.n_e != c->n_e || h.n_ep != c->n_ep || h.n_f != c->n_f || h.n_fix != c->n_fix || h.nhist != kMaxHist) {
+        c->last_error = "wrapd_state_load: the state belongs to another mesh";
+        return WRAPD_ERR_ARG;
+    }
+    const auto arrs = state_arrays(c);
+    size_t total = sizeof(StateHeader);
+    for (const auto& a : arrs) total += a.second * sizeof(double);
+    if (size != total) { c->last_error = "wrapd_state_load: size mismatch"; return WRAPD_ERR_ARG; }
+    const char* p = static_cast<const char*>(buf) + sizeof(h);
+    for (const auto& a : arrs) {
+        if (check_hip(c, hipMemcpyAsync(a.first, p, a.second * sizeof(double), hipMemcpyHostToDevice, c->stream),
+                      "state H2D"))
+            return WRAPD_ERR_HIP;
+        if (a.first == c->scal) std::memcpy(c->scal_host, p, a.second * sizeof(double));   // the host's copy too
+        p += a.second * sizeof(double);
+    }
+    if (check_hip(c, hipStreamSynchronize(c->stream), "sync")) return WRAPD_ERR_HIP;
+    c->rot_aware = h.rot_aware;
+    c->m_k = h.m_k;
+    c->last_flips = h.last_flips;
+    c->grad_old_valid = h.grad_old_valid != 0;
+    for (int k = 0; k < kMaxHist; ++k) c->pos2slot[k] = h.pos2slot[k];
+    c->last_energy = h.last_energy;
+    c->lcfg = h.lcfg;
+    c->factored = false;   // the factorization depends on w^2: wrapd_refactor before the next solve
+    return WRAPD_OK;
+}
+
 int wrapd_set_elem(wrapd_ctx* c, const char* name, const double* in) {
     CTX_OK(c); MESH_OK(c);
     if (!name || !in) return WRAPD_ERR_ARG;

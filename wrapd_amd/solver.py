@@ -102,6 +102,15 @@ class Solver:
     def finish(self):
         self._check(lib().wrapd_solver_finish(self.h), "finish")
 
+    def checkpoint(self, path):
+        """Write the whole ADMM state after the last iterate() to `path`."""
+        self._check(lib().wrapd_solver_checkpoint(self.h, str(path).encode()), "checkpoint")
+
+    def resume(self, path):
+        """After setup(): load a checkpoint of a run with the same mesh and settings; the
+        following iterate() calls continue that run bit for bit."""
+        self._check(lib().wrapd_solver_resume(self.h, str(path).encode()), "resume")
+
     def solve(self):
         self._check(lib().wrapd_solver_solve(self.h), "solve")
 
@@ -293,6 +302,18 @@ class Device:
         o = _lib.GlobalOut()
         self._check(lib().wrapd_global_step(self.ctx, float(tol), C.byref(o)), "global_step")
         return {k: getattr(o, k) for k, _ in _lib.GlobalOut._fields_}
+
+    def state_save(self):
+        """The device-resident ADMM state as bytes (wrapd_state_save)."""
+        n = C.c_size_t(0)
+        self._check(lib().wrapd_state_save(self.ctx, None, 0, C.byref(n)), "state_save")
+        buf = (C.c_char * n.value)()
+        self._check(lib().wrapd_state_save(self.ctx, buf, n.value, C.byref(n)), "state_save")
+        return bytes(buf)
+
+    def state_load(self, blob):
+        buf = C.create_string_buffer(blob, len(blob))
+        self._check(lib().wrapd_state_load(self.ctx, buf, len(blob)), "state_load")
 
     def elem(self, name):
         n_e = self.sizes()[1]
