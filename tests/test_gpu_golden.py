@@ -105,3 +105,23 @@ def test_gpu_uniform_per_element_material_is_the_homogeneous_run():
     assert np.array_equal(s.log("reweighted").astype(int), z["reweighted"])
     assert np.max(np.abs(e - z["objectives"])) <= bar_e * abs(z["objectives"][0])
     assert np.linalg.norm(s.x()[z["x_rows"]] - z["x"]) <= bar_x * np.linalg.norm(z["x"])
+
+
+def test_soln_x_errors_and_early_exit():
+    """-soln (src/AlgorithmData.cpp:63,70, src/ADMM.cpp:164): the log of |x - x*| / |x0 - x*|
+    over the free vertices, and the stop at the first iterate below 1e-4.  x* is the
+    beam's converged solution; the log's first iterations are checked against the oracle."""
+    import oracle_lib as ol
+    prob = meshgen.small_beam(12, 3, 3, seed=5, youngs=1e6)
+    ref = Solver(prob, tol=1e-12)
+    assert ref.run(3000) < 3000
+    xstar = ref.x()
+    s = Solver(prob, soln=xstar, tol=0.0)
+    n = s.run(3000)
+    xe = s.log("x_errors")
+    assert n < 3000 and len(xe) == n + 1
+    assert xe[-1] < 1e-4 and np.all(xe[:-1] >= 1e-4)   # stopped at the first iterate below 1e-4
+    o = ol.Oracle(prob, soln=xstar, tol=0.0)
+    o.run(10)
+    xo = o.log("x_errors")
+    assert np.allclose(xe[:11], xo[:11], rtol=1e-7, atol=0.0)
