@@ -75,6 +75,7 @@ struct RedTail {
     int* abortf = nullptr;       // sticky: a wait ran out (the results are then NaN)
     double* out = nullptr;       // nslots results
     double* hout = nullptr;      // their copy in host memory (pinned), or nullptr
+    unsigned* hseq = nullptr;    // host memory: set to epoch once the copies (and any work after) are stored
     unsigned epoch = 0;
     int nslots = 0;
     unsigned maxmask = 0;        // slots reduced with max instead of sum
@@ -85,9 +86,19 @@ __device__ __forceinline__ void put_partial(double* partials, int lb, int k, dou
                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Tells the host that the host copies of this launch are stored: thread 0 of the block that
+// finished the reduction, after every store the host will read (each thread waited for its
+// own before the block's last barrier).
+__device__ __forceinline__ void tail_signal(const RedTail R) {
+    if (threadIdx.x != 0 || !R.hseq) return;
+    __builtin_amdgcn_s_waitcnt(0);
+    __hip_atomic_store(R.hseq, R.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Returns true in the block that finished the reduction (the results are in R.out).
-// KS: the most slots this call site reduces (R.nslots <= KS).
-template <int NT, int KS>
+// KS: the most slots this call site reduces (R.nslots <= KS).  SIGNAL: call tail_signal
+// here (false: the caller does, after work of its own that the host reads).
+template <int NT, int KS, bool SIGNAL = true>
 __device__ __forceinline__ bool reduce_tail(const RedTail R, const double* partials) {
     static_assert(NT == 128 || NT == 256, "reduce_tail: 128 or 256 threads");
     static_assert(KS >= 1 && KS <= kTailSlots, "reduce_tail: slot count");
@@ -173,8 +184,10 @@ __device__ __forceinline__ bool reduce_tail(const RedTail R, const double* parti
         if (bad) r = __builtin_nan("");
         R.out[k] = r;
         if (R.hout) __hip_atomic_store(R.hout + k, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __builtin_amdgcn_s_waitcnt(0);
     }
     __syncthreads();
+    if (SIGNAL) tail_signal(R);
     return true;
 }
 

@@ -3,6 +3,7 @@
 // (deps/mcloptlib/include/MCL/SplitLBFGS.hpp:114-256) decision for decision; only the
 // vector arithmetic runs on the device.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -28,6 +29,7 @@ wrapd_ctx::~wrapd_ctx() {
     f(mlp_lpart); f(mlp_log);
     f(partials); f(scal); f(d_red_flags); f(d_red_abort);
     if (scal_host) (void)hipHostFree(scal_host);
+    if (host_seq) (void)hipHostFree(host_seq);
     for (auto& p : prof.pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (auto e : prof.pool) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -101,11 +103,25 @@ static int d2h(wrapd_ctx* c, T* h, const T* d, size_t n) {
     return check_hip(c, hipStreamSynchronize(c->stream), "sync");
 }
 
-// read n device scalars starting at slot `at`: every slot the host reads is written, by
-// the reduction that produces it, to the pinned copy too, so a stream synchronize suffices
+// read n device scalars starting at slot `at`.  Every slot the host reads is written, by
+// the reduction that produces it, to the pinned copy too, followed by that reduction's
+// epoch in host_seq: the host waits for the latest epoch by polling host memory, and falls
+// back to a stream synchronize (which also reports a failed launch) after about 50 ms.
 static const double* scal_read(wrapd_ctx* c, int at, int n) {
     (void)n;
+    const unsigned want = c->red_epoch;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned spins = 0;; ++spins) {
+        if (__atomic_load_n(c->host_seq, __ATOMIC_ACQUIRE) == want) return c->scal_host + at;
+        if ((spins & 1023) == 1023 &&
+            std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50))
+            break;
+    }
     if (check_hip(c, hipStreamSynchronize(c->stream), "sync")) return nullptr;
+    if (__atomic_load_n(c->host_seq, __ATOMIC_ACQUIRE) != want) {
+        c->last_error = "device scalars: the reduction's host copy did not arrive";
+        return nullptr;
+    }
     return c->scal_host + at;
 }
 
@@ -130,11 +146,17 @@ int wrapd_create(wrapd_ctx** out, int device) {
     if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return WRAPD_ERR_NO_DEVICE;
     wrapd_ctx* c = new wrapd_ctx();
     c->device = device;
+    // The host waits for a few scalars several times per ADMM iteration: spin rather than
+    // yield, which shortens each wait's wake-up.  Only possible before the device's
+    // context exists (otherwise the process keeps its setting).
+    (void)hipSetDeviceFlags(hipDeviceScheduleSpin);
+    (void)hipGetLastError();
     if (check_hip(c, hipSetDevice(device), "hipSetDevice") ||
         check_hip(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate") ||
         dalloc(c, c->partials, static_cast<size_t>(kMaxBlocks) * kRedStride) || dalloc(c, c->scal, S_NUM) ||
         dalloc(c, c->d_red_flags, kMaxBlocks) || dalloc(c, c->d_red_abort, 1) ||
-        check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&c->scal_host), S_NUM * sizeof(double)), "hipHostMalloc")) {
+        check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&c->scal_host), S_NUM * sizeof(double)), "hipHostMalloc") ||
+        check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&c->host_seq), 64), "hipHostMalloc")) {
         delete c;
         return WRAPD_ERR_HIP;
     }
@@ -145,6 +167,7 @@ int wrapd_create(wrapd_ctx** out, int device) {
     }
     launch_fill(c, c->scal, 0.0, S_NUM);
     std::fill(c->scal_host, c->scal_host + S_NUM, 0.0);
+    *c->host_seq = 0;
     *out = c;
     return WRAPD_OK;
 }

@@ -173,6 +173,7 @@ struct wrapd_ctx {
         int dst = 0, nblk = 0;   // nblk 0: none
     } pending_fin;                     // a taken tail whose launch is too wide to finish it
     double* scal_host = nullptr; // pinned host mirror
+    unsigned* host_seq = nullptr; // pinned: the epoch of the last reduction whose host copy is stored
 
     // SplitLBFGS host-side state (deps/mcloptlib/include/MCL/SplitLBFGS.hpp:48-62)
     wrapd_lbfgs_cfg lcfg{6, 100, 1.0, 0};

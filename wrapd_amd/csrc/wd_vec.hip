@@ -32,7 +32,7 @@ struct VecPtrs {
 template <int KS>
 __global__ void __launch_bounds__(256)
 k_finalize(const double* __restrict__ partials, int nblk, int nslots, unsigned maxmask, double* __restrict__ out,
-           double* __restrict__ hout) {
+           double* __restrict__ hout, unsigned* __restrict__ hseq, unsigned epoch) {
     constexpr int BB = KS > 4 ? 4 : 8;
     __shared__ double sh[KS][4];
     double acc[KS];
@@ -74,7 +74,10 @@ k_finalize(const double* __restrict__ partials, int nblk, int nslots, unsigned m
         }
         out[k] = r;
         __hip_atomic_store(hout + k, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __builtin_amdgcn_s_waitcnt(0);
     }
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(hseq, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Store the reduced Gram entries of the new pair and the y.y denominator (which the host
@@ -183,8 +186,10 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
         const double td = block_sum<kVecThreads>(d, sh);
         if (threadIdx.x == 0) put_partial(partials, lb, 2 * L.n, td);
     }
-    if (MODE != 0 && reduce_tail<kVecThreads, MODE == 1 ? 1 : 2 * kMaxHist + 1>(tail, partials) && MODE == 2 && threadIdx.x == 0)
-        gram_store(scal, tail.hout - S_RED, L, ns_pos);
+    if (MODE != 0 && reduce_tail<kVecThreads, MODE == 1 ? 1 : 2 * kMaxHist + 1, false>(tail, partials)) {
+        if (MODE == 2 && threadIdx.x == 0) gram_store(scal, tail.hout - S_RED, L, ns_pos);
+        tail_signal(tail);
+    }
 }
 
 // partial dots of `base` with up to 8 vectors
@@ -212,9 +217,12 @@ k_multidot(long long n3, const double* __restrict__ base, VecPtrs V, int nvec, d
         if (threadIdx.x == 0) put_partial(partials, lb, j, t);
     }
     // the last block also runs the recurrence that consumes the dots (post 1: alpha, 2: beta)
-    if (reduce_tail<kVecThreads, kMaxHist>(tail, partials) && threadIdx.x == 0) {
-        if (post == 1) lbfgs_alpha(scal, L);
-        else if (post == 2) lbfgs_beta(scal, L);
+    if (reduce_tail<kVecThreads, kMaxHist, false>(tail, partials)) {
+        if (threadIdx.x == 0) {
+            if (post == 1) lbfgs_alpha(scal, L);
+            else if (post == 2) lbfgs_beta(scal, L);
+        }
+        tail_signal(tail);
     }
 }
 
@@ -323,6 +331,7 @@ void arm_tail(wrapd_ctx* c, int nslots, unsigned maxmask, int dst) {
     t.abortf = c->d_red_abort;
     t.out = c->scal + dst;
     t.hout = c->scal_host + dst;
+    t.hseq = c->host_seq;
     t.nslots = nslots;
     t.maxmask = maxmask;
     c->pending_tail = t;
@@ -332,12 +341,12 @@ RedTail take_tail(wrapd_ctx* c, int nblk, bool may_split) {
     RedTail t = c->pending_tail;
     c->pending_tail = RedTail{};
     if (!t.flags) return t;
+    if (++c->red_epoch == 0) c->red_epoch = 1;   // flags start at 0
+    t.epoch = c->red_epoch;
     if (may_split && nblk > kTailMaxBlocks) {   // one block would read too many partials: finalize apart
         c->pending_fin = wrapd_ctx::PendingFinalize{t.nslots, t.maxmask, static_cast<int>(t.out - c->scal), nblk};
         return RedTail{};
     }
-    if (++c->red_epoch == 0) c->red_epoch = 1;   // flags start at 0
-    t.epoch = c->red_epoch;
     return t;
 }
 
@@ -348,7 +357,7 @@ void finish_reduction(wrapd_ctx* c) {
     auto k = f.nslots <= 1 ? k_finalize<1> : f.nslots <= 3 ? k_finalize<3> : f.nslots <= 11 ? k_finalize<11>
                                                                                           : k_finalize<kTailSlots>;
     hipLaunchKernelGGL(k, dim3(1), dim3(256), 0, c->stream, c->partials, f.nblk, f.nslots, f.maxmask, c->scal + f.dst,
-                       c->scal_host + f.dst);
+                       c->scal_host + f.dst, c->host_seq, c->red_epoch);
 }
 
 void launch_gather_grad(wrapd_ctx* c, double* out, int mode, int new_pos, const int* list_slots, int nlist) {
