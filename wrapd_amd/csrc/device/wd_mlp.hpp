@@ -173,7 +173,9 @@ __device__ inline bf16x4 mlp_gather_input(const float f[kMlpIn], int b, int lane
 // outputs (de-standardised: prox solution minus s_hat).  All 64 lanes must call it.
 // The wave's 64 samples are 4 blocks of 16; each weight tile is loaded once and applied
 // to all 4 (the same products, in the same order, as mlp_block_forward per block).
-__device__ inline void mlp_wave_predict(const MlpDevice& M, const float f[kMlpIn], int lane, float out[kMlpOut]) {
+// Blocks with no lane in `want` (a wave-uniform lane mask) are skipped; their outputs are 0.
+__device__ inline void mlp_wave_predict(const MlpDevice& M, const float f[kMlpIn], int lane, float out[kMlpOut],
+                                        unsigned long long want = ~0ull) {
     const int rsub = 4 * (lane >> 4);
     bf16x4 xb[4];
 #pragma unroll
@@ -185,6 +187,7 @@ __device__ inline void mlp_wave_predict(const MlpDevice& M, const float f[kMlpIn
         const f32x4 bias = ld_bias(M.master + kMlpOffB1 + 16 * to, rsub);
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
+            if (!((want >> (16 * b)) & 0xFFFFull)) continue;
             const f32x4 acc = mfma_bf16(w, xb[b], bias);
             f32x4 g;
 #pragma unroll
@@ -200,6 +203,7 @@ __device__ inline void mlp_wave_predict(const MlpDevice& M, const float f[kMlpIn
         const f32x4 bias = ld_bias(M.master + kMlpOffB2 + 16 * to, rsub);
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
+            if (!((want >> (16 * b)) & 0xFFFFull)) continue;
             f32x4 acc = bias;
 #pragma unroll
             for (int t = 0; t < 4; ++t) acc = mfma_bf16(w[t], h1[b][t], acc);
@@ -218,6 +222,7 @@ __device__ inline void mlp_wave_predict(const MlpDevice& M, const float f[kMlpIn
     out[0] = out[1] = out[2] = 0.0f;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
+        if (!((want >> (16 * b)) & 0xFFFFull)) continue;
         f32x4 y = b3;
 #pragma unroll
         for (int t = 0; t < 4; ++t) y = mfma_bf16(w3[t], h2[b][t], y);

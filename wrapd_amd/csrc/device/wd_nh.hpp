@@ -288,11 +288,13 @@ WD_HD bool spd_solve3(const M3<T>& H, const T g[3], T d[3]) {
 // direction.  Convergence: |grad| <= rtol * (mu + w^2/vol) * max(1, |x|).
 // Returns the iteration count, or -1 when the iterate would leave the feasible region
 // without making progress.
+// f0, g0: the objective and gradient at the start x (P.gradient), when the caller has them.
 template <typename T>
-WD_HD int prox_newton(const ProxProblem<T>& P, T x[3], int max_iters, T rtol, T& gnorm_out) {
+WD_HD int prox_newton_from(const ProxProblem<T>& P, T x[3], T f0, const T g0[3], int max_iters, T rtol,
+                           T& gnorm_out) {
     const T scale = (P.p.mu + P.wov);
-    T g[3];
-    T f = P.gradient(x, g);
+    T g[3] = {g0[0], g0[1], g0[2]};
+    T f = f0;
     int it = 0;
     for (; it < max_iters; ++it) {
         const T gn = wsqrt(dot3(g, g));
@@ -362,6 +364,13 @@ WD_HD int prox_newton(const ProxProblem<T>& P, T x[3], int max_iters, T rtol, T&
     }
     gnorm_out = wsqrt(dot3(g, g));
     return it;
+}
+
+template <typename T>
+WD_HD int prox_newton(const ProxProblem<T>& P, T x[3], int max_iters, T rtol, T& gnorm_out) {
+    T g[3];
+    const T f = P.gradient(x, g);
+    return prox_newton_from(P, x, f, g, max_iters, rtol, gnorm_out);
 }
 
 }  // namespace wd

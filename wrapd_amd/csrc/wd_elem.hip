@@ -488,11 +488,14 @@ k_local(int n_e, int n_ep, const double* __restrict__ symF, double* __restrict__
 // and a prediction already within the Newton tolerance is taken as is.  Everything else
 // (reductions, FP64 fallback, candidate weights) is k_local's.
 constexpr double kLearnedGate = 1e-3;
+// |grad| of the prox objective at x (infinite outside x > 0 or when not finite); f and g
+// are kept for the Newton prox that may start there.
 template <typename T>
-__device__ __forceinline__ T prox_residual(const ProxProblem<T>& P, const T x[3]) {
+__device__ __forceinline__ T prox_residual(const ProxProblem<T>& P, const T x[3], T& f, T g[3]) {
+    g[0] = g[1] = g[2] = T(0);
+    f = T(0);
     if (!(x[0] > T(0) && x[1] > T(0) && x[2] > T(0))) return T(INFINITY);
-    T g[3];
-    P.gradient(x, g);
+    f = P.gradient(x, g);
     const T r = wsqrt(dot3(g, g));
     return r == r ? r : T(INFINITY);
 }
@@ -546,28 +549,35 @@ k_local_learned(int n_e, int n_ep, const double* __restrict__ symF, double* __re
     PP.x0[0] = shat[0]; PP.x0[1] = shat[1]; PP.x0[2] = shat[2];
     PP.wov = static_cast<T>(w2 / v);
     const T xl[3] = {static_cast<T>(sl[0]), static_cast<T>(sl[1]), static_cast<T>(sl[2])};
-    const T rl = prox_residual(PP, xl);
+    T fl, gl[3];
+    const T rl = prox_residual(PP, xl, fl, gl);
     const T xln = wsqrt(dot3(xl, xl));
     const bool poor = valid && !(rl <= static_cast<T>(kLearnedGate) * (PP.p.mu + PP.wov) * (xln > T(1) ? xln : T(1)));
     float dpred[kMlpOut] = {0.0f, 0.0f, 0.0f};
-    const bool consult = __any(poor);   // wave-uniform: the MLP is a whole-wave product
-    if (consult) {
+    // wave-uniform: the MLP is a whole-wave product, over the 16-element blocks with a poor start
+    const unsigned long long poor_lanes = __ballot(poor);
+    if (poor_lanes) {
         float feat[kMlpIn];
         mlp_features(mlp.norm, static_cast<float>(shat[0]), static_cast<float>(shat[1]), static_cast<float>(shat[2]),
                      static_cast<float>(w2 / v), static_cast<float>(pd.mu), static_cast<float>(pd.lambda), feat);
-        mlp_wave_predict(mlp, feat, lane, dpred);   // every lane of the wave takes part
+        mlp_wave_predict(mlp, feat, lane, dpred, poor_lanes);   // every lane of the wave takes part
     }
     if (valid) {
         const T xp[3] = {shat[0] + static_cast<T>(dpred[0]), shat[1] + static_cast<T>(dpred[1]),
                          shat[2] + static_cast<T>(dpred[2])};
-        const T rp = poor ? prox_residual(PP, xp) : T(INFINITY);
+        T fp = T(0), gp[3] = {T(0), T(0), T(0)};
+        const T rp = poor ? prox_residual(PP, xp, fp, gp) : T(INFINITY);
         const bool use_pred = rp < rl;
         T x[3] = {use_pred ? xp[0] : xl[0], use_pred ? xp[1] : xl[1], use_pred ? xp[2] : xl[2]};
         const T xn = wsqrt(dot3(x, x));
         used = use_pred ? 1.0 : 0.0;
         accepted = (use_pred && rp <= static_cast<T>(rtol) * (PP.p.mu + PP.wov) * (xn > T(1) ? xn : T(1))) ? 1.0 : 0.0;
         T gn = T(0);
-        const int it = prox_newton(PP, x, prox_iters, static_cast<T>(rtol), gn);
+        // the start's objective and gradient are the residual's (when the start is valid)
+        const bool have = use_pred || rl < T(INFINITY);
+        const T g0[3] = {use_pred ? gp[0] : gl[0], use_pred ? gp[1] : gl[1], use_pred ? gp[2] : gl[2]};
+        const int it = have ? prox_newton_from(PP, x, use_pred ? fp : fl, g0, prox_iters, static_cast<T>(rtol), gn)
+                            : prox_newton(PP, x, prox_iters, static_cast<T>(rtol), gn);
         iters = it < 0 ? 0.0 : static_cast<double>(it);
         evals = iters;
         const M3<T> Zt = udvt(Vt, x, Ut);
