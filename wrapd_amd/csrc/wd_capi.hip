@@ -22,7 +22,7 @@ wrapd_ctx::~wrapd_ctx() {
     f(v2e_ptr); f(v2e_idx); f(a_cptr); f(a_cidx); f(Ax);
     f(G); f(ysol); f(xsol); f(ubuf); f(d_perm); f(d_iperm); f(d_col0); f(d_fsize); f(d_rowptr); f(d_rows);
     f(d_goff); f(d_uoff); f(d_asm_ptr); f(d_asm_idx); f(d_udest); f(d_leaf_ptr); f(d_leaf_sn); f(d_flow_f); f(d_flow_b);
-    f(d_flow_deps); f(d_flow_flags); f(d_flow_abort);
+    f(d_flow_abort); f(d_bcopy); f(bperm);
     f(pool); f(d_foff); f(d_cbase); f(d_acol_ptr); f(d_acol); f(d_ccol_ptr); f(d_ccol); f(d_relptr); f(d_rel);
     f(d_ftasks); f(d_ffail);
     f(mlp_master); f(mlp_m1); f(mlp_m2); f(mlp_gsum); f(mlp_norm); f(mlp_bf); f(mlp_loss); f(mlp_partial);

@@ -103,8 +103,8 @@ struct wrapd_ctx {
     bool analyzed = false, factored = false;
     double* G = nullptr;
     double* ysol = nullptr;     // n_f x 3 forward result (new numbering)
-    double* xsol = nullptr;     // n_f x 3 backward result (new numbering)
-    double* ubuf = nullptr;     // forward update rows x 3, each at its slot (d_udest)
+    double* xsol = nullptr;     // n_f x 3 backward result (new numbering); pending between solves above the leaves
+    double* ubuf = nullptr;     // forward update rows x 3, each at its slot (d_udest); pending between solves
     int* d_perm = nullptr;
     int* d_iperm = nullptr;
     int* d_col0 = nullptr;
@@ -123,12 +123,12 @@ struct wrapd_ctx {
     // direction, tasks ordered so that each waits only on earlier tasks
     void* d_flow_f = nullptr;                    // forward tasks
     void* d_flow_b = nullptr;                    // backward tasks
-    int* d_flow_deps = nullptr;                  // forward: the tasks each task waits for
-    unsigned* d_flow_flags = nullptr;            // per task (forward, then backward): epoch when done
     int* d_flow_abort = nullptr;                 // sticky: a bounded wait ran out
+    int* d_bcopy = nullptr;                      // (row, caller's row) pairs of the dataflow supernodes' columns
+    int n_bcopy = 0;
+    double* bperm = nullptr;                     // their right-hand-side rows, copied by the leaf launch
     int n_flow_f = 0, n_flow_b = 0;
     std::vector<std::pair<int, int>> bwd_segs;   // backward launches: (first task, columns per wave)
-    unsigned flow_epoch = 0;
     // numeric factorization (plan from chol_plan, fronts in one pool)
     wrapd::FactorPlan fplan;
     double* pool = nullptr;

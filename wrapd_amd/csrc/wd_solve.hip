@@ -110,10 +110,19 @@ __device__ __forceinline__ void load_leaf_meta(const SolveMaps& M, int first, in
     }
     __syncthreads();
 }
+// Each workgroup also copies its share of the dataflow supernodes' right-hand-side rows
+// into bperm (b in the solve's numbering; bcopy holds (destination, source) row pairs),
+// for the dataflow launch that follows.  The first row's loads go out with the subtree's
+// first loads.
+__device__ __forceinline__ void copy_row3(const double* __restrict__ src, double* __restrict__ dst) {
+    dst[0] = src[0];
+    dst[1] = src[1];
+    dst[2] = src[2];
+}
 __global__ void __launch_bounds__(kSolveThreads)
 k_fwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, SolveMaps M,
            const double* __restrict__ b, const double* __restrict__ G, double* __restrict__ ysol,
-           double* __restrict__ slots) {
+           double* __restrict__ slots, const int* __restrict__ bcopy, int nbcopy, double* __restrict__ bperm) {
     constexpr int B = kLeafPrefetch;
     constexpr int KA = 4;   // update rows per front row read ahead (more are read in a loop)
     extern __shared__ double dyn[];
@@ -121,9 +130,29 @@ k_fwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, So
     double* ul = dyn + kSmallFront * 3;   // the subtree's update rows x 3
     __shared__ LeafMeta L;
     const int t = blockIdx.x;
+    const int per = (nbcopy + static_cast<int>(gridDim.x) - 1) / static_cast<int>(gridDim.x);
+    const int cq0 = t * per, cq1 = min(nbcopy, cq0 + per);
+    int cdst = -1;
+    double cb[3] = {0.0, 0.0, 0.0};
+    if (cq0 + static_cast<int>(threadIdx.x) < cq1) {
+        const int q = cq0 + threadIdx.x;
+        cdst = bcopy[2 * q];
+        const double* src = b + 3ll * bcopy[2 * q + 1];
+        cb[0] = src[0];
+        cb[1] = src[1];
+        cb[2] = src[2];
+    }
     const int q0 = leaf_ptr[t], q1 = leaf_ptr[t + 1];
     const int first = leaf_sn[q0], nsn = q1 - q0;
     load_leaf_meta(M, first, nsn, L);
+    if (cdst >= 0) {
+        double* dst = bperm + 3ll * cdst;
+        dst[0] = cb[0];
+        dst[1] = cb[1];
+        dst[2] = cb[2];
+    }
+    for (int q = cq0 + kSolveThreads + threadIdx.x; q < cq1; q += kSolveThreads)
+        copy_row3(b + 3ll * bcopy[2 * q + 1], bperm + 3ll * bcopy[2 * q]);
     const int64_t ubase = L.uoff[0];
     const int r = threadIdx.x;
     for (int i = 0; i < nsn; ++i) {
@@ -193,33 +222,48 @@ k_fwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, So
     }
 }
 
+// The same copy when there are no leaf subtrees.
+__global__ void k_bperm(const int* __restrict__ bcopy, int nbcopy, const double* __restrict__ b,
+                        double* __restrict__ bperm) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nbcopy) copy_row3(b + 3ll * bcopy[2 * q + 1], bperm + 3ll * bcopy[2 * q]);
+}
+
 // ---------------------------------------------------------------------------------
 // Dataflow schedule above the leaf subtrees: all forward tasks form one launch, and the
-// backward tasks one launch per task width.  A task waits for the tasks it reads from,
-// not for the end of a level: a level boundary costs a launch (~2.3 us) plus the
-// dependent loads of its first tasks (~1.2 us each on data the previous launch wrote).
-//  - Task i is workgroup i.  Forward tasks are ordered bottom-up and wait for the tasks
-//    of their non-leaf children; backward tasks are ordered top-down and wait for the
-//    tasks of their parent, whose completion implies its ancestors'.  So a task only
-//    waits on lower-numbered workgroups, which are dispatched before it.
-//  - A finished task stores the solve's epoch in its own flag.  There is no shared
-//    counter or ticket: atomics on one address serialize at ~25 ns each, which on
-//    thousands of tasks costs more than the levels themselves.
+// backward tasks one launch per task width.  A task waits for the rows it reads, not for
+// the end of a level: a level boundary costs a launch (~2.3 us) plus the dependent loads
+// of its first tasks (~1.2 us each on data the previous launch wrote).
+//  - Task i is workgroup i.  Forward tasks are ordered bottom-up and read update rows
+//    of their children; backward tasks are ordered top-down and read solution rows of
+//    their ancestors.  So a task only waits on lower-numbered workgroups, which are
+//    dispatched before it.
+//  - The rows handed between tasks (update rows in `slots`, backward solution rows in
+//    `xsol`) hold kPending until written.  A task polls the words it needs until none is
+//    pending: one round trip after the producer's store, where a separate done-flag
+//    would add the producer's store completion and a dependent load after the flag.
+//    Every pending word is written once per solve, and NaN results stay quiet NaNs, so a
+//    word that is not pending is final.
+//  - Whoever reads a word last sets it back to kPending for the next solve: a forward
+//    task its update rows' slots (it is their only reader) and its pivot rows' xsol; a
+//    backward task its pivot rows' slots (read by every forward task of its supernode,
+//    all finished in the previous launch).  The set-up fills both arrays with kPending.
+//  - No shared counter or ticket: atomics on one address serialize at ~25 ns each,
+//    which on thousands of tasks costs more than the levels themselves.
 //  - Rows handed from one task to another in the same launch are stored and loaded at
-//    agent scope (written through to memory, read past this XCD's L2).  A task waits for
-//    its stores to complete before it sets its flag.
+//    agent scope (written through to memory, read past this XCD's L2).
 //  - Every wait is bounded.  When one expires it sets a sticky abort flag, and from then
 //    on tasks write NaN instead of waiting; the host reports the flag.
 // Operands that no task of the launch writes (G, b, the maps) are loaded before the
 // wait, so their latency overlaps the children's work.
 struct SolveTask {
     int s, first, width, ns;   // forward: rows [first, first + width); backward: columns, width = 4 x cpw
-    int f, c0, dep0, ndep;     // forward: tasks deps[dep0, dep0 + ndep); backward: tasks [dep0, dep0 + ndep)
+    int f, c0, np, nu;         // forward: np pivot-row slots from qa, nu update-row slots from uq0
     long long goff, rowptr;
-    long long uoff;
-    int pad0, pad1;
+    long long uoff, qa;
+    long long uq0, pad;
 };
-static_assert(sizeof(SolveTask) == 64, "SolveTask layout");
+static_assert(sizeof(SolveTask) == 80, "SolveTask layout");
 static_assert(kBwdChunk % kSolveThreads == 0 && kNsMax % kSolveThreads == 0, "staging layout");
 
 constexpr int kSpinLimit = 1 << 20;   // polls (about a second) before a wait gives up
@@ -231,9 +275,17 @@ __device__ __forceinline__ void st_agent(double* p, double v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Waits until *flag == epoch.  False when this wait or another one has given up.
-__device__ bool wait_flag(unsigned* flag, unsigned epoch, int* abortf) {
-    for (int polls = 1; __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch; ++polls) {
+// A signalling NaN with a payload: arithmetic only yields quiet NaNs, so no result has it.
+constexpr unsigned long long kPendingBits = 0xFFF40000FFF40000ull;
+__device__ __forceinline__ bool pending(double v) {
+    return static_cast<unsigned long long>(__double_as_longlong(v)) == kPendingBits;
+}
+__device__ __forceinline__ double pending_value() { return __longlong_as_double(static_cast<long long>(kPendingBits)); }
+
+// v already holds one load of p[0..2]; loads again until no word is pending.  False when
+// this wait or another one has given up (v may then hold pending words).
+__device__ bool poll3(const double* p, double v[3], int* abortf) {
+    for (int polls = 1; pending(v[0]) || pending(v[1]) || pending(v[2]); ++polls) {
         if ((polls & 63) == 0) {
             if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
             if (polls >= kSpinLimit) {
@@ -242,15 +294,11 @@ __device__ bool wait_flag(unsigned* flag, unsigned epoch, int* abortf) {
             }
         }
         __builtin_amdgcn_s_sleep(2);
+        v[0] = ld_agent(p + 0);
+        v[1] = ld_agent(p + 1);
+        v[2] = ld_agent(p + 2);
     }
     return true;
-}
-
-// Marks the task done once every thread's stores have completed.
-__device__ __forceinline__ void finish_task(unsigned* flag, unsigned epoch) {
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Forward task (s, rows [first, first + R)): thread (g, r) = (tid / R, tid % R) sums
@@ -258,56 +306,47 @@ __device__ __forceinline__ void finish_task(unsigned* flag, unsigned epoch) {
 // fixed order.  Update rows travel in `slots`: row i of u_s is written once, to
 // slots[udest[i]], inside its parent front row's range [asm_ptr[p], asm_ptr[p + 1]).  A
 // task's inputs are therefore two contiguous slot ranges (those of the supernode's pivot
-// rows and those of the task's update rows), loaded in parallel rounds of kStageSlots
-// into LDS and then summed per row in slot order.  B: G terms per thread loaded before
-// the wait.
+// rows and those of the task's update rows, both in the task record), polled in parallel
+// rounds of kFwdStage into LDS and then summed per row in slot order.  The pivot rows'
+// right-hand side comes from bperm, b in the solve's numbering, so every load of the
+// task depends on the task record alone.  B: G terms per thread loaded before the wait.
+constexpr int kFwdStage = 2 * kSolveThreads;
 template <int B>
 __global__ void __launch_bounds__(kSolveThreads)
-k_fwd_flow(const SolveTask* __restrict__ tasks, const int* __restrict__ deps, unsigned* flags, unsigned epoch,
-           int* abortf, SolveMaps M, const double* __restrict__ b, const double* __restrict__ G,
-           double* __restrict__ ysol, double* __restrict__ slots) {
+k_fwd_flow(const SolveTask* __restrict__ tasks, int* abortf, SolveMaps M, const double* __restrict__ bperm,
+           const double* __restrict__ G, double* __restrict__ ysol, double* __restrict__ xsol,
+           double* __restrict__ slots) {
     constexpr int kRowsPerThread = kNsMax / kSolveThreads;
-    constexpr int kStageSlots = 4 * kSolveThreads;
+    constexpr int kSlotsPerThread = kFwdStage / kSolveThreads;
     __shared__ double cv[kNsMax * 3];
-    __shared__ double sred[kStageSlots * 3];   // staged slots, then the column-group partial sums
+    __shared__ double sred[kFwdStage * 3];   // staged slots, then the column-group partial sums
     __shared__ int sh_ok;
+    static_assert(kFwdStage >= kSolveThreads, "the partial sums reuse the staging buffer");
     if (threadIdx.x == 0) sh_ok = __hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 0 : 1;
     const SolveTask T = tasks[blockIdx.x];
     const int ns = T.ns, f = T.f, c0 = T.c0;
     const int R = T.width, ng = kSolveThreads / R;
     const int g = threadIdx.x / R, rl = threadIdx.x - g * R;
     const int r = T.first + rl;
-    // Before the wait: b of the pivot rows into LDS, the slot ranges, this thread's
-    // first G terms and, for an update row, where its result goes.
-    int64_t qlo[kRowsPerThread], qhi[kRowsPerThread];
-    int old[kRowsPerThread];
+    const int np = T.np, ntot = T.np + T.nu;
+    auto slot_ptr = [&](int i) { return slots + 3 * (i < np ? T.qa + i : T.uq0 + (i - np)); };
+    double v[kSlotsPerThread][3];
+    auto load_window = [&](int base) {
 #pragma unroll
-    for (int k = 0; k < kRowsPerThread; ++k) {   // independent loads first, then their uses
-        const int j = threadIdx.x + k * kSolveThreads;
-        const bool in = j < ns;
-        old[k] = in ? M.perm[c0 + j] : 0;
-        qlo[k] = in ? M.asm_ptr[T.rowptr + j] : 0;
-        qhi[k] = in ? M.asm_ptr[T.rowptr + j + 1] : 0;
-    }
-#pragma unroll
-    for (int k = 0; k < kRowsPerThread; ++k) {
-        const int j = threadIdx.x + k * kSolveThreads;
-        double b0 = 0.0, b1 = 0.0, b2 = 0.0;
-        if (j < ns) {
-            b0 = b[3 * old[k] + 0];
-            b1 = b[3 * old[k] + 1];
-            b2 = b[3 * old[k] + 2];
+        for (int k = 0; k < kSlotsPerThread; ++k) {
+            const int i = base + threadIdx.x + k * kSolveThreads;
+            v[k][0] = v[k][1] = v[k][2] = 0.0;
+            if (i < ntot) {
+                const double* sp = slot_ptr(i);
+                v[k][0] = ld_agent(sp + 0);
+                v[k][1] = ld_agent(sp + 1);
+                v[k][2] = ld_agent(sp + 2);
+            }
         }
-        if (j < ns) {
-            cv[3 * j + 0] = b0;
-            cv[3 * j + 1] = b1;
-            cv[3 * j + 2] = b2;
-        }
-    }
-    const int64_t qa = M.asm_ptr[T.rowptr], np = M.asm_ptr[T.rowptr + ns] - qa;
-    const int ur0 = max(T.first, ns), ur1 = min(T.first + R, f);
-    const int64_t uq0 = ur0 < ur1 ? M.asm_ptr[T.rowptr + ur0] : 0;
-    const int64_t nu = ur0 < ur1 ? M.asm_ptr[T.rowptr + ur1] - uq0 : 0;
+    };
+    // Before the wait: the first window of inputs, this thread's first G terms, b of the
+    // pivot rows into LDS, the rows' slot ranges and, for an update row, where its result goes.
+    load_window(0);
     const int jend = r < ns ? r + 1 : ns;   // L11^{-1} is lower triangular
     const int nterm = (r < f && g < jend) ? (jend - g + ng - 1) / ng : 0;
     const double* gp = G + T.goff + r + static_cast<int64_t>(g) * f;
@@ -315,50 +354,61 @@ k_fwd_flow(const SolveTask* __restrict__ tasks, const int* __restrict__ deps, un
     double gv[B];
 #pragma unroll
     for (int u = 0; u < B; ++u) gv[u] = u < nterm ? gp[u * gstride] : 0.0;
+    int qlo[kRowsPerThread], qhi[kRowsPerThread];   // relative to the pivot slots' start
+#pragma unroll
+    for (int k = 0; k < kRowsPerThread; ++k) {
+        const int j = threadIdx.x + k * kSolveThreads;
+        const bool in = j < ns;
+        qlo[k] = in ? static_cast<int>(M.asm_ptr[T.rowptr + j] - T.qa) : 0;
+        qhi[k] = in ? static_cast<int>(M.asm_ptr[T.rowptr + j + 1] - T.qa) : 0;
+        if (in) {
+            const double* bp = bperm + 3ll * (c0 + j);
+            cv[3 * j + 0] = bp[0];
+            cv[3 * j + 1] = bp[1];
+            cv[3 * j + 2] = bp[2];
+        }
+    }
     const bool urow = g == 0 && r >= ns && r < f;
-    int64_t ulo = 0, uhi = 0, udst = 0;
+    int ulo = 0, uhi = 0;
+    int64_t udst = 0;
     if (urow) {   // as indices into the staged sequence: pivot slots first, then update-row slots
-        ulo = np + M.asm_ptr[T.rowptr + r] - uq0;
-        uhi = np + M.asm_ptr[T.rowptr + r + 1] - uq0;
+        ulo = np + static_cast<int>(M.asm_ptr[T.rowptr + r] - T.uq0);
+        uhi = np + static_cast<int>(M.asm_ptr[T.rowptr + r + 1] - T.uq0);
         udst = M.udest[T.uoff + (r - ns)];
     }
-    __syncthreads();   // sh_ok is set
-    if (sh_ok)
-        for (int k = threadIdx.x; k < T.ndep; k += kSolveThreads)
-            if (!wait_flag(flags + deps[T.dep0 + k], epoch, abortf)) sh_ok = 0;
-    __syncthreads();
-    // After the wait: the children's updates, staged and summed in slot order.
+    __syncthreads();   // sh_ok is set, cv holds b
+    const bool live = sh_ok != 0;
+    // The children's updates, polled, staged and summed in slot order.
     double ua0 = 0.0, ua1 = 0.0, ua2 = 0.0;
-    const int64_t ntot = np + nu;
-    for (int64_t base = 0; base < ntot; base += kStageSlots) {
-        double v[kStageSlots / kSolveThreads][3];
+    for (int base = 0; base < ntot; base += kFwdStage) {
+        if (base > 0) load_window(base);
 #pragma unroll
-        for (int k = 0; k < kStageSlots / kSolveThreads; ++k) {
-            const int64_t i = base + threadIdx.x + k * kSolveThreads;
-            v[k][0] = v[k][1] = v[k][2] = 0.0;
+        for (int k = 0; k < kSlotsPerThread; ++k) {
+            const int i = base + threadIdx.x + k * kSolveThreads;
             if (i < ntot) {
-                const double* sp = slots + 3 * (i < np ? qa + i : uq0 + (i - np));
-                v[k][0] = ld_agent(sp + 0);
-                v[k][1] = ld_agent(sp + 1);
-                v[k][2] = ld_agent(sp + 2);
+                double* sp = slot_ptr(i);
+                if (live && !poll3(sp, v[k], abortf)) sh_ok = 0;
+                if (i >= np) {   // this task is the update rows' only reader
+                    const double pv = pending_value();
+                    sp[0] = pv;
+                    sp[1] = pv;
+                    sp[2] = pv;
+                }
             }
-        }
-#pragma unroll
-        for (int k = 0; k < kStageSlots / kSolveThreads; ++k) {
-            const int i = threadIdx.x + k * kSolveThreads;
-            sred[3 * i + 0] = v[k][0];
-            sred[3 * i + 1] = v[k][1];
-            sred[3 * i + 2] = v[k][2];
+            const int il = threadIdx.x + k * kSolveThreads;
+            sred[3 * il + 0] = v[k][0];
+            sred[3 * il + 1] = v[k][1];
+            sred[3 * il + 2] = v[k][2];
         }
         __syncthreads();
-        const int64_t wend = base + kStageSlots;
+        const int wend = base + kFwdStage;
 #pragma unroll
         for (int k = 0; k < kRowsPerThread; ++k) {
             const int j = threadIdx.x + k * kSolveThreads;
-            const int64_t i0 = max(qlo[k] - qa, base), i1 = min(qhi[k] - qa, wend);
+            const int i0 = max(qlo[k], base), i1 = min(qhi[k], wend);
             if (j < ns && i0 < i1) {
                 double a0 = cv[3 * j + 0], a1 = cv[3 * j + 1], a2 = cv[3 * j + 2];
-                for (int64_t i = i0; i < i1; ++i) {
+                for (int i = i0; i < i1; ++i) {
                     const double* sv = sred + 3 * (i - base);
                     a0 += sv[0];
                     a1 += sv[1];
@@ -369,7 +419,7 @@ k_fwd_flow(const SolveTask* __restrict__ tasks, const int* __restrict__ deps, un
                 cv[3 * j + 2] = a2;
             }
         }
-        for (int64_t i = max(ulo, base); i < min(uhi, wend); ++i) {
+        for (int i = max(ulo, base); i < min(uhi, wend); ++i) {
             const double* sv = sred + 3 * (i - base);
             ua0 += sv[0];
             ua1 += sv[1];
@@ -402,29 +452,34 @@ k_fwd_flow(const SolveTask* __restrict__ tasks, const int* __restrict__ deps, un
     }
     const double nan = __builtin_nan("");
     const bool ok = sh_ok != 0;
-    if (urow) {   // read by the parent's tasks in this launch: complete before the flag
+    if (urow) {   // read by the parent's tasks in this launch
         double* u = slots + 3 * udst;
         st_agent(u + 0, ok ? ua0 + t0 : nan);
         st_agent(u + 1, ok ? ua1 + t1 : nan);
         st_agent(u + 2, ok ? ua2 + t2 : nan);
     }
-    finish_task(flags + blockIdx.x, epoch);
     if (g == 0 && r < ns) {   // read by the next launch only
         double* y = ysol + 3ll * (c0 + r);
         y[0] = ok ? t0 : nan;
         y[1] = ok ? t1 : nan;
         y[2] = ok ? t2 : nan;
+        double* x = xsol + 3ll * (c0 + r);   // pending until the backward launch writes it
+        const double pv = pending_value();
+        x[0] = pv;
+        x[1] = pv;
+        x[2] = pv;
     }
 }
 
 // Backward task (s, columns [first, first + 4 CPW)): x_J = G^T [y_J; x_R], CPW columns
 // per wave, rows staged through LDS in chunks.  BT / CPW G terms per column (the first
-// chunk's) are loaded before the wait.
+// chunk's) are loaded before the wait.  The task also sets its pivot rows' forward slots
+// back to pending.
 template <int CPW, int BT>
 __global__ void __launch_bounds__(kSolveThreads)
-k_bwd_flow(const SolveTask* __restrict__ tasks, int task0, unsigned* flags, unsigned epoch, int* abortf, SolveMaps M,
-           const double* __restrict__ G, const double* __restrict__ ysol, double* __restrict__ xsol,
-           double* __restrict__ out) {
+k_bwd_flow(const SolveTask* __restrict__ tasks, int task0, int* abortf, SolveMaps M, const double* __restrict__ G,
+           const double* __restrict__ ysol, double* __restrict__ xsol, double* __restrict__ out,
+           double* __restrict__ slots) {
     __shared__ double vv[kBwdChunk * 3];
     __shared__ int sh_ok;
     const int ti = task0 + static_cast<int>(blockIdx.x);
@@ -455,6 +510,14 @@ k_bwd_flow(const SolveTask* __restrict__ tasks, int task0, unsigned* flags, unsi
             for (int u = 0; u < B; ++u) gv[jj][u] = u < n ? gc[64 * u] : 0.0;
         }
     }
+    // Where this wave's columns' results go, and the task's forward slots (one range) to reset.
+    int pj[CPW];
+#pragma unroll
+    for (int jj = 0; jj < CPW; ++jj) {
+        const int j = jfirst + w * CPW + jj;
+        pj[jj] = j < ns ? M.perm[c0 + j] : 0;
+    }
+    const int64_t sq0 = M.asm_ptr[T.rowptr + jfirst], sq1 = M.asm_ptr[T.rowptr + min(jfirst + T.width, ns)];
     // Before the wait, the first chunk's rows of this supernode (its forward result,
     // written by the previous launch) go to LDS, and the ancestor rows' indices to registers.
     constexpr int kStage = kBwdChunk / kSolveThreads;
@@ -478,10 +541,7 @@ k_bwd_flow(const SolveTask* __restrict__ tasks, int task0, unsigned* flags, unsi
         }
     }
     __syncthreads();   // sh_ok is set
-    if (sh_ok)
-        for (int k = threadIdx.x; k < T.ndep; k += kSolveThreads)
-            if (!wait_flag(flags + T.dep0 + k, epoch, abortf)) sh_ok = 0;
-    __syncthreads();
+    const bool live = sh_ok != 0;
     double acc[CPW][3];
 #pragma unroll
     for (int jj = 0; jj < CPW; ++jj) acc[jj][0] = acc[jj][1] = acc[jj][2] = 0.0;
@@ -505,9 +565,11 @@ k_bwd_flow(const SolveTask* __restrict__ tasks, int task0, unsigned* flags, unsi
                 }
                 if (sr >= 0) {   // an ancestor's solution, written in this launch
                     const double* src = xsol + 3ll * sr;
-                    vv[3 * r + 0] = ld_agent(src + 0);
-                    vv[3 * r + 1] = ld_agent(src + 1);
-                    vv[3 * r + 2] = ld_agent(src + 2);
+                    double v[3] = {ld_agent(src + 0), ld_agent(src + 1), ld_agent(src + 2)};
+                    if (live && !poll3(src, v, abortf)) sh_ok = 0;
+                    vv[3 * r + 0] = v[0];
+                    vv[3 * r + 1] = v[1];
+                    vv[3 * r + 2] = v[2];
                 }
             }
         }
@@ -575,15 +637,22 @@ k_bwd_flow(const SolveTask* __restrict__ tasks, int task0, unsigned* flags, unsi
             st_agent(x + 2, ok ? a2 : nan);
         }
     }
-    finish_task(flags + ti, epoch);
 #pragma unroll
     for (int jj = 0; jj < CPW; ++jj) {   // the caller's copy, read after this launch only
         const int j = jfirst + w * CPW + jj;
         if (lane == 0 && j < ns) {
-            double* xo = out + 3ll * M.perm[c0 + j];
+            double* xo = out + 3ll * pj[jj];
             xo[0] = ok ? acc[jj][0] : nan;
             xo[1] = ok ? acc[jj][1] : nan;
             xo[2] = ok ? acc[jj][2] : nan;
+        }
+    }
+    {   // the task's columns' forward slots: every forward task finished in the previous launch
+        const double pv = pending_value();
+        for (int64_t q = sq0 + threadIdx.x; q < sq1; q += kSolveThreads) {
+            slots[3 * q + 0] = pv;
+            slots[3 * q + 1] = pv;
+            slots[3 * q + 2] = pv;
         }
     }
 }
@@ -794,16 +863,18 @@ int chol_prepare_device(wrapd_ctx* c) {
             nft[sn] = (S.fsize[sn] + lev_rows[l] - 1) / lev_rows[l];
             nbt[sn] = (ns + 4 * lev_cpw[l] - 1) / (4 * lev_cpw[l]);
         }
-    std::vector<std::vector<int>> kids(S.nsuper);
+    // a task may only wait on lower-numbered tasks: a level supernode's parent is on a later level
+    std::vector<int> lev_of(S.nsuper, -1);
+    for (int l = 0; l < nlev; ++l)
+        for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) lev_of[S.level_sn[i]] = l;
     for (int i = S.level_ptr[0]; i < S.level_ptr[nlev]; ++i) {
         const int sn = S.level_sn[i], p = S.parent[sn];
-        if (p >= 0 && !in_level[p]) {
-            c->last_error = "supernode schedule: a level supernode's parent is in a leaf subtree";
+        if (p >= 0 && (!in_level[p] || lev_of[p] <= lev_of[sn])) {
+            c->last_error = "supernode schedule: a level supernode's parent is in a leaf subtree or not above it";
             return WRAPD_ERR_STATE;
         }
-        if (p >= 0) kids[p].push_back(sn);
     }
-    auto task_of = [&](int sn, int first, int width, int dep0, int ndep) {
+    auto task_of = [&](int sn, int first, int width) {
         SolveTask t{};
         t.s = sn;
         t.first = first;
@@ -811,51 +882,62 @@ int chol_prepare_device(wrapd_ctx* c) {
         t.ns = S.col0[sn + 1] - S.col0[sn];
         t.f = S.fsize[sn];
         t.c0 = S.col0[sn];
-        t.dep0 = dep0;
-        t.ndep = ndep;
         t.goff = S.goff[sn];
         t.rowptr = S.rowptr[sn];
         t.uoff = S.uoff[sn];
         return t;
     };
-    // a forward task waits for every task of its non-leaf children (their indices listed
-    // in deps); a backward task for every task of its parent (a contiguous range)
+    // forward tasks bottom-up (children before parents), backward tasks top-down
+    // a forward task's inputs: the slots of its supernode's pivot rows and of its own update rows
     std::vector<SolveTask> ft, bt;
-    std::vector<int> deps, first_task(S.nsuper, -1);
+    std::vector<int> bcopy;   // (row, caller's row) of every dataflow supernode column
     for (int l = 0; l < nlev; ++l)
         for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) {
             const int sn = S.level_sn[i];
-            const int dep0 = static_cast<int>(deps.size());
-            for (int k : kids[sn])
-                for (int q = 0; q < nft[k]; ++q) deps.push_back(first_task[k] + q);
-            const int ndep = static_cast<int>(deps.size()) - dep0;
-            first_task[sn] = static_cast<int>(ft.size());
-            for (int rb = 0; rb < nft[sn]; ++rb) ft.push_back(task_of(sn, rb * lev_rows[l], lev_rows[l], dep0, ndep));
+            for (int rb = 0; rb < nft[sn]; ++rb) {
+                SolveTask t = task_of(sn, rb * lev_rows[l], lev_rows[l]);
+                const int64_t* ap = S.asm_ptr.data() + t.rowptr;
+                t.qa = ap[0];
+                const int64_t np = ap[t.ns] - ap[0];
+                const int ur0 = std::max(t.first, t.ns), ur1 = std::min(t.first + t.width, t.f);
+                t.uq0 = ur0 < ur1 ? ap[ur0] : 0;
+                const int64_t nu = ur0 < ur1 ? ap[ur1] - ap[ur0] : 0;
+                if (np + nu > INT32_MAX / 4) {
+                    c->last_error = "supernode with more child update rows than the solve kernels support";
+                    return WRAPD_ERR_ARG;
+                }
+                t.np = static_cast<int>(np);
+                t.nu = static_cast<int>(nu);
+                ft.push_back(t);
+            }
+            for (int j = S.col0[sn]; j < S.col0[sn + 1]; ++j) {
+                bcopy.push_back(j);
+                bcopy.push_back(S.perm[j]);
+            }
         }
     c->bwd_segs.clear();
     for (int l = nlev - 1; l >= 0; --l) {
         if (c->bwd_segs.empty() || c->bwd_segs.back().second != lev_cpw[l])
             c->bwd_segs.push_back({static_cast<int>(bt.size()), lev_cpw[l]});
         for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) {
-            const int sn = S.level_sn[i], p = S.parent[sn];
-            const int dep0 = p >= 0 ? first_task[p] : 0, ndep = p >= 0 ? nbt[p] : 0;
-            first_task[sn] = static_cast<int>(bt.size());
-            for (int cb = 0; cb < nbt[sn]; ++cb) bt.push_back(task_of(sn, cb * 4 * lev_cpw[l], 4 * lev_cpw[l], dep0, ndep));
+            const int sn = S.level_sn[i];
+            for (int cb = 0; cb < nbt[sn]; ++cb) bt.push_back(task_of(sn, cb * 4 * lev_cpw[l], 4 * lev_cpw[l]));
         }
     }
     c->n_flow_f = static_cast<int>(ft.size());
     c->n_flow_b = static_cast<int>(bt.size());
     rc |= up(c, reinterpret_cast<SolveTask*&>(c->d_flow_f), ft);
     rc |= up(c, reinterpret_cast<SolveTask*&>(c->d_flow_b), bt);
-    rc |= up(c, c->d_flow_deps, deps);
-    rc |= up(c, c->d_flow_flags, std::vector<unsigned>(static_cast<size_t>(c->n_flow_f) + c->n_flow_b, 0u));
     rc |= up(c, c->d_flow_abort, std::vector<int>(1, 0));
-    c->flow_epoch = 0;
-    std::vector<double> z3(3ull * std::max(1, c->n_f), 0.0);
-    rc |= up(c, c->ysol, z3);
-    rc |= up(c, c->xsol, z3);
-    std::vector<double> zu(3ull * std::max<int64_t>(1, S.uoff[S.nsuper]), 0.0);
-    rc |= up(c, c->ubuf, zu);
+    c->n_bcopy = static_cast<int>(bcopy.size() / 2);
+    rc |= up(c, c->d_bcopy, bcopy);
+    rc |= up(c, c->bperm, std::vector<double>(3ull * std::max(1, c->n_f), 0.0));
+    // the rows the dataflow tasks hand over start pending (their bit pattern, copied as integers)
+    std::vector<uint64_t> p3(3ull * std::max(1, c->n_f), kPendingBits);
+    rc |= up(c, c->ysol, std::vector<double>(p3.size(), 0.0));
+    rc |= up(c, reinterpret_cast<uint64_t*&>(c->xsol), p3);
+    rc |= up(c, reinterpret_cast<uint64_t*&>(c->ubuf),
+             std::vector<uint64_t>(3ull * std::max<int64_t>(1, S.uoff[S.nsuper]), kPendingBits));
     if (c->G) { (void)hipFree(c->G); c->G = nullptr; }
     rc |= check_hip(c, hipMalloc(&c->G, std::max<size_t>(8, S.goff[S.nsuper] * sizeof(double))), "hipMalloc(G)");
     return rc ? WRAPD_ERR_HIP : WRAPD_OK;
@@ -866,18 +948,19 @@ void launch_solve(wrapd_ctx* c, const double* b, double* x) {
     SolveMaps M{c->d_col0, c->d_fsize, c->d_rowptr, c->d_rows, c->d_goff, c->d_uoff, c->d_asm_ptr, c->d_asm_idx,
                 c->d_perm, c->d_udest};
     const int nleaf = static_cast<int>(S.leaf_ptr.size()) - 1;
-    if (++c->flow_epoch == 0) c->flow_epoch = 1;   // flags start at 0
-    const unsigned epoch = c->flow_epoch;
     const SolveTask* ft = static_cast<const SolveTask*>(c->d_flow_f);
     const SolveTask* bt = static_cast<const SolveTask*>(c->d_flow_b);
     {
         ProfScope ps(c, "solve_fwd");
         if (nleaf > 0)
             hipLaunchKernelGGL(k_fwd_leaf, dim3(nleaf), dim3(kSolveThreads), c->leaf_lds_fwd, c->stream, c->d_leaf_ptr,
-                               c->d_leaf_sn, M, b, c->G, c->ysol, c->ubuf);
+                               c->d_leaf_sn, M, b, c->G, c->ysol, c->ubuf, c->d_bcopy, c->n_bcopy, c->bperm);
+        else if (c->n_bcopy > 0)
+            hipLaunchKernelGGL(k_bperm, dim3((c->n_bcopy + 255) / 256), dim3(256), 0, c->stream, c->d_bcopy, c->n_bcopy,
+                               b, c->bperm);
         if (c->n_flow_f > 0) {
             hipLaunchKernelGGL(k_fwd_flow<kFwdPrefetch>, dim3(c->n_flow_f), dim3(kSolveThreads), 0, c->stream, ft,
-                               c->d_flow_deps, c->d_flow_flags, epoch, c->d_flow_abort, M, b, c->G, c->ysol, c->ubuf);
+                               c->d_flow_abort, M, c->bperm, c->G, c->ysol, c->xsol, c->ubuf);
         }
     }
     {
@@ -888,8 +971,8 @@ void launch_solve(wrapd_ctx* c, const double* b, double* x) {
             if (n <= 0) continue;
             auto kern = cpw == 4 ? k_bwd_flow<4, kBwdPrefetch> : cpw == 2 ? k_bwd_flow<2, kBwdPrefetch>
                                                                          : k_bwd_flow<1, kBwdPrefetch>;
-            hipLaunchKernelGGL(kern, dim3(n), dim3(kSolveThreads), 0, c->stream, bt, t0, c->d_flow_flags + c->n_flow_f,
-                               epoch, c->d_flow_abort, M, c->G, c->ysol, c->xsol, x);
+            hipLaunchKernelGGL(kern, dim3(n), dim3(kSolveThreads), 0, c->stream, bt, t0, c->d_flow_abort, M, c->G,
+                               c->ysol, c->xsol, x, c->ubuf);
         }
         if (nleaf > 0)
             hipLaunchKernelGGL(k_bwd_leaf, dim3(nleaf), dim3(kSolveThreads), c->leaf_lds_bwd, c->stream, c->d_leaf_ptr,
