@@ -43,6 +43,27 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
     return r;
 }
 
+// Block sums of n <= KS values per thread, each in the order of block_sum, with one
+// barrier for all of them; the result of value k is returned in thread k (k < n).
+// sh: KS x NT/64 doubles.
+template <int NT, int KS>
+__device__ __forceinline__ double block_sums(const double (&v)[KS], int n, double (*sh)[NT / 64]) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < KS; ++k)
+        if (k < n) {
+            const double t = wave_sum(v[k]);
+            if (l == 0) sh[k][w] = t;
+        }
+    __syncthreads();
+    double r = 0.0;
+    if (static_cast<int>(threadIdx.x) < n) {
+#pragma unroll
+        for (int i = 0; i < NT / 64; ++i) r += sh[threadIdx.x][i];
+    }
+    return r;
+}
+
 template <int NT>
 __device__ __forceinline__ double block_max(double v, double* sh) {
     v = wave_max(v);
@@ -74,8 +95,8 @@ struct RedTail {
     unsigned* flags = nullptr;   // per block; nullptr: the caller finalizes in a launch of its own
     int* abortf = nullptr;       // sticky: a wait ran out (the results are then NaN)
     double* out = nullptr;       // nslots results
-    double* hout = nullptr;      // their copy in host memory (pinned), or nullptr
-    unsigned* hseq = nullptr;    // host memory: set to epoch once the copies (and any work after) are stored
+    double* hout = nullptr;      // their copy in host memory (pinned), or nullptr when the host does not read them
+    unsigned* hseq = nullptr;    // host memory: set to epoch once the copies (and any work after) are stored, or nullptr
     unsigned epoch = 0;
     int nslots = 0;
     unsigned maxmask = 0;        // slots reduced with max instead of sum
@@ -95,11 +116,12 @@ __device__ __forceinline__ void tail_signal(const RedTail R) {
     __hip_atomic_store(R.hseq, R.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// Returns true in the block that finished the reduction (the results are in R.out).
-// KS: the most slots this call site reduces (R.nslots <= KS).  SIGNAL: call tail_signal
-// here (false: the caller does, after work of its own that the host reads).
+// Returns true in the block that finished the reduction (the results are in R.out, and
+// in shres when given: LDS of at least KS doubles).  KS: the most slots this call site
+// reduces (R.nslots <= KS).  SIGNAL: call tail_signal here (false: the caller does, after
+// work of its own that the host reads).
 template <int NT, int KS, bool SIGNAL = true>
-__device__ __forceinline__ bool reduce_tail(const RedTail R, const double* partials) {
+__device__ __forceinline__ bool reduce_tail(const RedTail R, const double* partials, double* shres = nullptr) {
     static_assert(NT == 128 || NT == 256, "reduce_tail: 128 or 256 threads");
     static_assert(KS >= 1 && KS <= kTailSlots, "reduce_tail: slot count");
     if (!R.flags) return false;
@@ -183,8 +205,11 @@ __device__ __forceinline__ bool reduce_tail(const RedTail R, const double* parti
         }
         if (bad) r = __builtin_nan("");
         R.out[k] = r;
-        if (R.hout) __hip_atomic_store(R.hout + k, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        __builtin_amdgcn_s_waitcnt(0);
+        if (shres) shres[k] = r;
+        if (R.hout) {
+            __hip_atomic_store(R.hout + k, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __builtin_amdgcn_s_waitcnt(0);
+        }
     }
     __syncthreads();
     if (SIGNAL) tail_signal(R);

@@ -109,7 +109,7 @@ static int d2h(wrapd_ctx* c, T* h, const T* d, size_t n) {
 // back to a stream synchronize (which also reports a failed launch) after about 50 ms.
 static const double* scal_read(wrapd_ctx* c, int at, int n) {
     (void)n;
-    const unsigned want = c->red_epoch;
+    const unsigned want = c->host_epoch;
     const auto t0 = std::chrono::steady_clock::now();
     for (unsigned spins = 0;; ++spins) {
         if (__atomic_load_n(c->host_seq, __ATOMIC_ACQUIRE) == want) return c->scal_host + at;

@@ -166,11 +166,13 @@ struct wrapd_ctx {
     unsigned* d_red_flags = nullptr;   // per block of a reducing launch: epoch when its partials are stored
     int* d_red_abort = nullptr;        // sticky: a reduction's wait ran out
     unsigned red_epoch = 0;
+    unsigned host_epoch = 0;           // the epoch of the last reduction with a host copy (what the host waits for)
     wd::RedTail pending_tail;          // armed by arm_tail, taken by the next reducing launch
     struct PendingFinalize {
         int nslots = 0;
         unsigned maxmask = 0;
         int dst = 0, nblk = 0;   // nblk 0: none
+        bool host = true;        // with the host copy
     } pending_fin;                     // a taken tail whose launch is too wide to finish it
     double* scal_host = nullptr; // pinned host mirror
     unsigned* host_seq = nullptr; // pinned: the epoch of the last reduction whose host copy is stored
@@ -210,8 +212,9 @@ void launch_max_ratio(wrapd_ctx* c);
 void launch_assemble_A(wrapd_ctx* c);
 // (wd_vec.hip)
 // The next reducing launch finishes its own reduction into scal[dst, dst + nslots) in its
-// last block, summing the blocks' partials in a fixed order.
-void arm_tail(wrapd_ctx* c, int nslots, unsigned maxmask, int dst);
+// last block, summing the blocks' partials in a fixed order.  host: also into the pinned
+// host copy, followed by the reduction's epoch (false for results only later launches read).
+void arm_tail(wrapd_ctx* c, int nslots, unsigned maxmask, int dst, bool host = true);
 // The armed tail for a launch of nblk blocks.  With may_split and more than
 // kTailMaxBlocks blocks the reduction is instead left to finish_reduction, to be called
 // right after that launch (a one-block finalize launch: one block reading that many

@@ -73,46 +73,55 @@ k_finalize(const double* __restrict__ partials, int nblk, int nslots, unsigned m
             for (int i = 0; i < 4; ++i) r += sh[k][i];
         }
         out[k] = r;
-        __hip_atomic_store(hout + k, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        __builtin_amdgcn_s_waitcnt(0);
+        if (hout) {
+            __hip_atomic_store(hout + k, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __builtin_amdgcn_s_waitcnt(0);
+        }
     }
     __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(hseq, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (threadIdx.x == 0 && hseq) __hip_atomic_store(hseq, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// Store the reduced Gram entries of the new pair and the y.y denominator (which the host
-// reads, so it also goes to the host copy hscal).
-__device__ void gram_store(double* __restrict__ scal, double* __restrict__ hscal, const SlotList& L, int ns_pos) {
+// Store the reduced Gram entries of the new pair (red: the reduction's results in LDS,
+// s_new.y_j and s_j.y_new per list entry j, then y.y) and the y.y denominator, which the
+// host reads, so it also goes to the host copy hscal.  Thread j < n stores entry j.
+__device__ void gram_store(double* __restrict__ scal, double* __restrict__ hscal, const double* red,
+                           const SlotList& L, int ns_pos) {
     const int sn = L.slot[ns_pos];
-    for (int j = 0; j < L.n; ++j) {
+    const int j = threadIdx.x;
+    if (j < L.n) {
         const int sj = L.slot[j];
-        scal[S_GRAM + sn * kMaxHist + sj] = scal[S_RED + 2 * j];       // s_new . y_j
-        scal[S_GRAM + sj * kMaxHist + sn] = scal[S_RED + 2 * j + 1];   // s_j . y_new
+        scal[S_GRAM + sn * kMaxHist + sj] = red[2 * j];       // s_new . y_j
+        scal[S_GRAM + sj * kMaxHist + sn] = red[2 * j + 1];   // s_j . y_new
     }
-    scal[S_DENOM] = scal[S_RED + 2 * L.n];
-    __hip_atomic_store(hscal + S_DENOM, scal[S_DENOM], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-
-// alpha recurrence (first loop), logical order 0 = oldest
-__device__ void lbfgs_alpha(double* __restrict__ scal, const SlotList& L) {
-    for (int i = L.n - 1; i >= 0; --i) {
-        const int si = L.slot[i];
-        double a = scal[S_RED + i];
-        for (int j = i + 1; j < L.n; ++j) a -= scal[S_ALPHA + j] * scal[S_GRAM + si * kMaxHist + L.slot[j]];
-        const double rho = 1.0 / scal[S_GRAM + si * kMaxHist + si];
-        scal[S_ALPHA + i] = rho * a;
+    if (j == 0) {
+        scal[S_DENOM] = red[2 * L.n];
+        __hip_atomic_store(hscal + S_DENOM, red[2 * L.n], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
-// beta recurrence (second loop); stores alpha_i - beta_i
-__device__ void lbfgs_beta(double* __restrict__ scal, const SlotList& L) {
-    for (int i = 0; i < L.n; ++i) {
-        const int si = L.slot[i];
-        double b = scal[S_RED + i];
-        for (int j = 0; j < i; ++j) b += scal[S_COEF + j] * scal[S_GRAM + L.slot[j] * kMaxHist + si];
-        const double rho = 1.0 / scal[S_GRAM + si * kMaxHist + si];
+// The two-loop recurrences on one block's LDS: red holds the multi-dot results (s_i.g for
+// loop 1, y_i.r for loop 2), gm the list's Gram entries gm[i * n + j] = G[slot i][slot j],
+// ab the alphas (loaded for loop 2).  Thread 0 runs the recurrence; the results go to
+// scal (S_ALPHA, or S_COEF = alpha - beta), read by the next launch.  Logical order 0 = oldest.
+__device__ void lbfgs_alpha(double* __restrict__ scal, const double* red, const double* gm, double* ab, int n) {
+    for (int i = n - 1; i >= 0; --i) {
+        double a = red[i];
+        for (int j = i + 1; j < n; ++j) a -= ab[j] * gm[i * n + j];
+        const double rho = 1.0 / gm[i * n + i];
+        ab[i] = rho * a;
+        scal[S_ALPHA + i] = ab[i];
+    }
+}
+__device__ void lbfgs_beta(double* __restrict__ scal, const double* red, const double* gm, const double* alpha,
+                           double* coef, int n) {
+    for (int i = 0; i < n; ++i) {
+        double b = red[i];
+        for (int j = 0; j < i; ++j) b += coef[j] * gm[j * n + i];
+        const double rho = 1.0 / gm[i * n + i];
         const double beta = rho * b;
-        scal[S_COEF + i] = scal[S_ALPHA + i] - beta;
+        coef[i] = alpha[i] - beta;
+        scal[S_COEF + i] = coef[i];
     }
 }
 
@@ -125,7 +134,10 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
          double* __restrict__ out, const double* __restrict__ gold, double* __restrict__ ynew,
          const double* __restrict__ snew, VecPtrs S, VecPtrs Y, SlotList L, int ns_pos,
          double* __restrict__ partials, RedTail tail, double* __restrict__ scal) {
+    constexpr int KG = 2 * kMaxHist + 1;
     __shared__ double sh[kVecThreads / 64];
+    __shared__ double shk[KG][kVecThreads / 64];
+    __shared__ double sres[KG];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int v = lb * blockDim.x + threadIdx.x;
     double g[3] = {0.0, 0.0, 0.0};
@@ -162,32 +174,36 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
                 s[c] = snew[3 * v + c];
             }
         }
-        for (int j = 0; j < L.n; ++j) {
+        // slot 2j: s_new . y_j, slot 2j + 1: s_j . y_new, slot 2n: y_new . y_new
+        double part[KG];
+#pragma unroll
+        for (int j = 0; j < kMaxHist; ++j) {
             double a = 0.0, b = 0.0;
-            if (v < n_f) {
+            if (j < L.n && v < n_f) {
                 const double* yj = (j == ns_pos) ? nullptr : Y.v[j] + 3 * v;
                 const double* sj = (j == ns_pos) ? nullptr : S.v[j] + 3 * v;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const double yy = yj ? yj[c] : y[c];
                     const double ss = sj ? sj[c] : s[c];
-                    a += s[c] * yy;   // s_new . y_j
-                    b += ss * y[c];   // s_j . y_new
+                    a += s[c] * yy;
+                    b += ss * y[c];
                 }
             }
-            const double ta = block_sum<kVecThreads>(a, sh);
-            const double tb = block_sum<kVecThreads>(b, sh);
-            if (threadIdx.x == 0) {
-                put_partial(partials, lb, 2 * j, ta);
-                put_partial(partials, lb, 2 * j + 1, tb);
-            }
+            part[2 * j] = a;
+            part[2 * j + 1] = b;
         }
-        double d = y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
-        const double td = block_sum<kVecThreads>(d, sh);
-        if (threadIdx.x == 0) put_partial(partials, lb, 2 * L.n, td);
+        part[KG - 1] = 0.0;
+        const double d = y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
+#pragma unroll
+        for (int k = 0; k < KG; ++k)
+            if (k == 2 * L.n) part[k] = d;
+        const double t = block_sums<kVecThreads, KG>(part, 2 * L.n + 1, shk);
+        if (static_cast<int>(threadIdx.x) < 2 * L.n + 1) put_partial(partials, lb, threadIdx.x, t);
     }
-    if (MODE != 0 && reduce_tail<kVecThreads, MODE == 1 ? 1 : 2 * kMaxHist + 1, false>(tail, partials)) {
-        if (MODE == 2 && threadIdx.x == 0) gram_store(scal, tail.hout - S_RED, L, ns_pos);
+    if (MODE != 0 && reduce_tail<kVecThreads, MODE == 1 ? 1 : KG, false>(tail, partials, sres)) {
+        if (MODE == 2) gram_store(scal, tail.hout - S_RED, sres, L, ns_pos);
+        __syncthreads();   // every thread's stores are issued before the signal
         tail_signal(tail);
     }
 }
@@ -196,7 +212,8 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
 __global__ void __launch_bounds__(kVecThreads)
 k_multidot(long long n3, const double* __restrict__ base, VecPtrs V, int nvec, double* __restrict__ partials,
            RedTail tail, double* __restrict__ scal, SlotList L, int post) {
-    __shared__ double sh[kVecThreads / 64];
+    __shared__ double shk[kMaxHist][kVecThreads / 64];
+    __shared__ double sres[kMaxHist], gm[kMaxHist * kMaxHist], sa[kMaxHist], sc[kMaxHist];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const long long i0 = static_cast<long long>(lb) * blockDim.x * 4 + threadIdx.x;
     double acc[kMaxHist];
@@ -212,15 +229,21 @@ k_multidot(long long n3, const double* __restrict__ base, VecPtrs V, int nvec, d
                 if (j < nvec) acc[j] += b * V.v[j][i];
         }
     }
-    for (int j = 0; j < nvec; ++j) {
-        const double t = block_sum<kVecThreads>(acc[j], sh);
-        if (threadIdx.x == 0) put_partial(partials, lb, j, t);
-    }
-    // the last block also runs the recurrence that consumes the dots (post 1: alpha, 2: beta)
-    if (reduce_tail<kVecThreads, kMaxHist, false>(tail, partials)) {
-        if (threadIdx.x == 0) {
-            if (post == 1) lbfgs_alpha(scal, L);
-            else if (post == 2) lbfgs_beta(scal, L);
+    const double t = block_sums<kVecThreads, kMaxHist>(acc, nvec, shk);
+    if (static_cast<int>(threadIdx.x) < nvec) put_partial(partials, lb, threadIdx.x, t);
+    // the last block also runs the recurrence that consumes the dots (post 1: alpha, 2: beta),
+    // with the list's Gram entries (and for beta the alphas) loaded into LDS together
+    if (reduce_tail<kVecThreads, kMaxHist, false>(tail, partials, sres)) {
+        if (post) {
+            const int n = L.n;
+            for (int q = threadIdx.x; q < n * n; q += kVecThreads)
+                gm[q] = scal[S_GRAM + L.slot[q / n] * kMaxHist + L.slot[q % n]];
+            if (post == 2 && static_cast<int>(threadIdx.x) < n) sa[threadIdx.x] = scal[S_ALPHA + threadIdx.x];
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                if (post == 1) lbfgs_alpha(scal, sres, gm, sa, n);
+                else lbfgs_beta(scal, sres, gm, sa, sc, n);
+            }
         }
         tail_signal(tail);
     }
@@ -325,13 +348,13 @@ static SlotList mk_list(const int* slots, int n) {
     return L;
 }
 
-void arm_tail(wrapd_ctx* c, int nslots, unsigned maxmask, int dst) {
+void arm_tail(wrapd_ctx* c, int nslots, unsigned maxmask, int dst, bool host) {
     RedTail t;
     t.flags = c->d_red_flags;
     t.abortf = c->d_red_abort;
     t.out = c->scal + dst;
-    t.hout = c->scal_host + dst;
-    t.hseq = c->host_seq;
+    t.hout = host ? c->scal_host + dst : nullptr;
+    t.hseq = host ? c->host_seq : nullptr;
     t.nslots = nslots;
     t.maxmask = maxmask;
     c->pending_tail = t;
@@ -343,8 +366,10 @@ RedTail take_tail(wrapd_ctx* c, int nblk, bool may_split) {
     if (!t.flags) return t;
     if (++c->red_epoch == 0) c->red_epoch = 1;   // flags start at 0
     t.epoch = c->red_epoch;
+    if (t.hseq) c->host_epoch = t.epoch;
     if (may_split && nblk > kTailMaxBlocks) {   // one block would read too many partials: finalize apart
-        c->pending_fin = wrapd_ctx::PendingFinalize{t.nslots, t.maxmask, static_cast<int>(t.out - c->scal), nblk};
+        c->pending_fin = wrapd_ctx::PendingFinalize{t.nslots, t.maxmask, static_cast<int>(t.out - c->scal), nblk,
+                                                    t.hseq != nullptr};
         return RedTail{};
     }
     return t;
@@ -357,7 +382,7 @@ void finish_reduction(wrapd_ctx* c) {
     auto k = f.nslots <= 1 ? k_finalize<1> : f.nslots <= 3 ? k_finalize<3> : f.nslots <= 11 ? k_finalize<11>
                                                                                           : k_finalize<kTailSlots>;
     hipLaunchKernelGGL(k, dim3(1), dim3(256), 0, c->stream, c->partials, f.nblk, f.nslots, f.maxmask, c->scal + f.dst,
-                       c->scal_host + f.dst, c->host_seq, c->red_epoch);
+                       f.host ? c->scal_host + f.dst : nullptr, f.host ? c->host_seq : nullptr, c->red_epoch);
 }
 
 void launch_gather_grad(wrapd_ctx* c, double* out, int mode, int new_pos, const int* list_slots, int nlist) {
@@ -383,7 +408,7 @@ void launch_multidot(wrapd_ctx* c, const double* base, double* const* vecs, int 
     for (int j = 0; j < nvec; ++j) V.v[j] = vecs[j];
     const long long n3 = 3ll * c->n_f;
     const int nb = blocks_for(n3, kVecThreads * 4);
-    arm_tail(c, nvec, 0u, S_RED);
+    arm_tail(c, nvec, 0u, S_RED, false);   // read by the next launches only
     const SlotList L = post ? mk_list(slots, nvec) : SlotList{};
     hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kVecThreads), 0, c->stream, n3, base, V, nvec, c->partials,
                        take_tail(c, nb, false), c->scal, L, post);
