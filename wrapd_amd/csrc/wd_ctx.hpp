@@ -118,9 +118,9 @@ struct wrapd_ctx {
     int64_t* d_udest = nullptr;
     int* d_leaf_ptr = nullptr;
     int* d_leaf_sn = nullptr;
-    size_t leaf_lds_fwd = 0, leaf_lds_bwd = 0;   // dynamic LDS of the leaf-subtree kernels
+    size_t leaf_lds_fwd = 0, leaf_lds_bwd = 0;   // LDS of a leaf-subtree workgroup
     // dataflow schedule of the supernodes above the leaf subtrees: one launch per
-    // direction, tasks ordered so that each waits only on earlier tasks
+    // direction with the leaf walks, ordered so that each workgroup waits only on earlier ones
     void* d_flow_f = nullptr;                    // forward tasks
     void* d_flow_b = nullptr;                    // backward tasks
     int* d_flow_abort = nullptr;                 // sticky: a bounded wait ran out
@@ -128,7 +128,6 @@ struct wrapd_ctx {
     int n_bcopy = 0;
     double* bperm = nullptr;                     // their right-hand-side rows, copied by the leaf launch
     int n_flow_f = 0, n_flow_b = 0;
-    std::vector<std::pair<int, int>> bwd_segs;   // backward launches: (first task, columns per wave)
     // numeric factorization (plan from chol_plan, fronts in one pool)
     wrapd::FactorPlan fplan;
     double* pool = nullptr;

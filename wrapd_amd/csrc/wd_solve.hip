@@ -15,9 +15,9 @@
 //   forward  tasks (supernode, block of R rows), R in {256, ..., 16}: the 256 threads
 //            are 256/R column groups, reduced in LDS in a fixed order;
 //   backward tasks (supernode, block of 4 cpw columns), cpw columns per wave in {4, 2, 1}.
-// These tasks run as one dataflow launch per direction (see k_fwd_flow): a task waits for
-// the supernodes it reads, not for the end of its level.  Sums are taken in a fixed
-// order, so results do not depend on timing.
+// Each direction is one launch (k_fwd_solve, k_bwd_solve) holding the leaf subtrees and
+// the tasks: a workgroup waits for the rows it reads, not for the end of its level or of
+// the leaf walks.  Sums are taken in a fixed order, so results do not depend on timing.
 #include "wd_ctx.hpp"
 #include "device/wd_reduce.hpp"
 
@@ -80,6 +80,41 @@ __device__ __forceinline__ void strided_dot3(const double* __restrict__ g, int64
     }
 }
 
+constexpr int kSpinLimit = 1 << 20;   // polls (about a second) before a wait gives up
+
+__device__ __forceinline__ double ld_agent(const double* p) {
+    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A signalling NaN with a payload: arithmetic only yields quiet NaNs, so no result has it.
+constexpr unsigned long long kPendingBits = 0xFFF40000FFF40000ull;
+__device__ __forceinline__ bool pending(double v) {
+    return static_cast<unsigned long long>(__double_as_longlong(v)) == kPendingBits;
+}
+__device__ __forceinline__ double pending_value() { return __longlong_as_double(static_cast<long long>(kPendingBits)); }
+
+// v already holds one load of p[0..2]; loads again until no word is pending.  False when
+// this wait or another one has given up (v may then hold pending words).
+__device__ bool poll3(const double* p, double v[3], int* abortf) {
+    for (int polls = 1; pending(v[0]) || pending(v[1]) || pending(v[2]); ++polls) {
+        if ((polls & 63) == 0) {
+            if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+            if (polls >= kSpinLimit) {
+                __hip_atomic_store(abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return false;
+            }
+        }
+        __builtin_amdgcn_s_sleep(2);
+        v[0] = ld_agent(p + 0);
+        v[1] = ld_agent(p + 1);
+        v[2] = ld_agent(p + 2);
+    }
+    return true;
+}
+
 // Leaf subtrees: one workgroup walks a whole subtree in postorder.  A subtree is a
 // contiguous range of supernodes [first, root] (postorder, checked at set-up), and every
 // update row it produces except its root's is consumed inside it, so those rows stay in
@@ -112,25 +147,33 @@ __device__ __forceinline__ void load_leaf_meta(const SolveMaps& M, int first, in
 }
 // Each workgroup also copies its share of the dataflow supernodes' right-hand-side rows
 // into bperm (b in the solve's numbering; bcopy holds (destination, source) row pairs),
-// for the dataflow launch that follows.  The first row's loads go out with the subtree's
-// first loads.
+// read by the dataflow tasks of the same launch.  The first row's loads go out with the
+// subtree's first loads.
 __device__ __forceinline__ void copy_row3(const double* __restrict__ src, double* __restrict__ dst) {
     dst[0] = src[0];
     dst[1] = src[1];
     dst[2] = src[2];
 }
-__global__ void __launch_bounds__(kSolveThreads)
-k_fwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, SolveMaps M,
-           const double* __restrict__ b, const double* __restrict__ G, double* __restrict__ ysol,
-           double* __restrict__ slots, const int* __restrict__ bcopy, int nbcopy, double* __restrict__ bperm) {
+__device__ __forceinline__ void copy_row3_agent(const double* __restrict__ src, double* __restrict__ dst) {
+    const double a = src[0], b = src[1], c = src[2];
+    st_agent(dst + 0, a);
+    st_agent(dst + 1, b);
+    st_agent(dst + 2, c);
+}
+// LDS of a leaf-subtree workgroup: its LeafMeta, then the rows (kLeafRowsOff doubles in).
+constexpr int kLeafRowsOff = static_cast<int>((sizeof(LeafMeta) + 15) / 16 * 2);
+__device__ __forceinline__ void fwd_leaf_body(int t, int nleaf, const int* __restrict__ leaf_ptr,
+                                              const int* __restrict__ leaf_sn, const SolveMaps& M,
+                                              const double* __restrict__ b, const double* __restrict__ G,
+                                              double* __restrict__ ysol, double* __restrict__ slots,
+                                              const int* __restrict__ bcopy, int nbcopy, double* __restrict__ bperm,
+                                              double* dyn) {
     constexpr int B = kLeafPrefetch;
     constexpr int KA = 4;   // update rows per front row read ahead (more are read in a loop)
-    extern __shared__ double dyn[];
-    double* cv = dyn;                     // kSmallFront x 3
-    double* ul = dyn + kSmallFront * 3;   // the subtree's update rows x 3
-    __shared__ LeafMeta L;
-    const int t = blockIdx.x;
-    const int per = (nbcopy + static_cast<int>(gridDim.x) - 1) / static_cast<int>(gridDim.x);
+    LeafMeta& L = *reinterpret_cast<LeafMeta*>(dyn);
+    double* cv = dyn + kLeafRowsOff;      // kSmallFront x 3
+    double* ul = cv + kSmallFront * 3;    // the subtree's update rows x 3
+    const int per = (nbcopy + nleaf - 1) / nleaf;
     const int cq0 = t * per, cq1 = min(nbcopy, cq0 + per);
     int cdst = -1;
     double cb[3] = {0.0, 0.0, 0.0};
@@ -145,14 +188,14 @@ k_fwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, So
     const int q0 = leaf_ptr[t], q1 = leaf_ptr[t + 1];
     const int first = leaf_sn[q0], nsn = q1 - q0;
     load_leaf_meta(M, first, nsn, L);
-    if (cdst >= 0) {
+    if (cdst >= 0) {   // read by the dataflow tasks of this launch
         double* dst = bperm + 3ll * cdst;
-        dst[0] = cb[0];
-        dst[1] = cb[1];
-        dst[2] = cb[2];
+        st_agent(dst + 0, cb[0]);
+        st_agent(dst + 1, cb[1]);
+        st_agent(dst + 2, cb[2]);
     }
     for (int q = cq0 + kSolveThreads + threadIdx.x; q < cq1; q += kSolveThreads)
-        copy_row3(b + 3ll * bcopy[2 * q + 1], bperm + 3ll * bcopy[2 * q]);
+        copy_row3_agent(b + 3ll * bcopy[2 * q + 1], bperm + 3ll * bcopy[2 * q]);
     const int64_t ubase = L.uoff[0];
     const int r = threadIdx.x;
     for (int i = 0; i < nsn; ++i) {
@@ -212,9 +255,11 @@ k_fwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, So
                 const double u0 = cv[3 * r + 0] + t0, u1 = cv[3 * r + 1] + t1, u2 = cv[3 * r + 2] + t2;
                 double* u = ul + 3 * (ui - ubase);
                 u[0] = u0; u[1] = u1; u[2] = u2;
-                if (i == nsn - 1) {   // the subtree's root
+                if (i == nsn - 1) {   // the subtree's root: read by the dataflow tasks of this launch
                     double* ug = slots + 3 * M.udest[ui];
-                    ug[0] = u0; ug[1] = u1; ug[2] = u2;
+                    st_agent(ug + 0, u0);
+                    st_agent(ug + 1, u1);
+                    st_agent(ug + 2, u2);
                 }
             }
         }
@@ -230,14 +275,15 @@ __global__ void k_bperm(const int* __restrict__ bcopy, int nbcopy, const double*
 }
 
 // ---------------------------------------------------------------------------------
-// Dataflow schedule above the leaf subtrees: all forward tasks form one launch, and the
-// backward tasks one launch per task width.  A task waits for the rows it reads, not for
-// the end of a level: a level boundary costs a launch (~2.3 us) plus the dependent loads
-// of its first tasks (~1.2 us each on data the previous launch wrote).
-//  - Task i is workgroup i.  Forward tasks are ordered bottom-up and read update rows
-//    of their children; backward tasks are ordered top-down and read solution rows of
-//    their ancestors.  So a task only waits on lower-numbered workgroups, which are
-//    dispatched before it.
+// Dataflow schedule above the leaf subtrees: the forward launch holds the leaf walks and
+// then the forward tasks, the backward launch the backward tasks and then the leaf walks.
+// A workgroup waits for the rows it reads, not for the end of a level: a level boundary
+// costs a launch (~2.3 us) plus the dependent loads of its first tasks (~1.2 us each on
+// data the previous launch wrote).
+//  - Forward tasks are ordered bottom-up after the leaf walks and read update rows of
+//    their children; backward tasks are ordered top-down and read solution rows of their
+//    ancestors, and the leaf walks after them read those of theirs.  So a workgroup only
+//    waits on lower-numbered workgroups, which are dispatched before it.
 //  - The rows handed between tasks (update rows in `slots`, backward solution rows in
 //    `xsol`) hold kPending until written.  A task polls the words it needs until none is
 //    pending: one round trip after the producer's store, where a separate done-flag
@@ -266,41 +312,6 @@ struct SolveTask {
 static_assert(sizeof(SolveTask) == 80, "SolveTask layout");
 static_assert(kBwdChunk % kSolveThreads == 0 && kNsMax % kSolveThreads == 0, "staging layout");
 
-constexpr int kSpinLimit = 1 << 20;   // polls (about a second) before a wait gives up
-
-__device__ __forceinline__ double ld_agent(const double* p) {
-    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_agent(double* p, double v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// A signalling NaN with a payload: arithmetic only yields quiet NaNs, so no result has it.
-constexpr unsigned long long kPendingBits = 0xFFF40000FFF40000ull;
-__device__ __forceinline__ bool pending(double v) {
-    return static_cast<unsigned long long>(__double_as_longlong(v)) == kPendingBits;
-}
-__device__ __forceinline__ double pending_value() { return __longlong_as_double(static_cast<long long>(kPendingBits)); }
-
-// v already holds one load of p[0..2]; loads again until no word is pending.  False when
-// this wait or another one has given up (v may then hold pending words).
-__device__ bool poll3(const double* p, double v[3], int* abortf) {
-    for (int polls = 1; pending(v[0]) || pending(v[1]) || pending(v[2]); ++polls) {
-        if ((polls & 63) == 0) {
-            if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
-            if (polls >= kSpinLimit) {
-                __hip_atomic_store(abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                return false;
-            }
-        }
-        __builtin_amdgcn_s_sleep(2);
-        v[0] = ld_agent(p + 0);
-        v[1] = ld_agent(p + 1);
-        v[2] = ld_agent(p + 2);
-    }
-    return true;
-}
-
 // Forward task (s, rows [first, first + R)): thread (g, r) = (tid / R, tid % R) sums
 // columns j = g, g + 256/R, ... of row r; the 256/R partial sums are added in LDS in a
 // fixed order.  Update rows travel in `slots`: row i of u_s is written once, to
@@ -308,22 +319,24 @@ __device__ bool poll3(const double* p, double v[3], int* abortf) {
 // task's inputs are therefore two contiguous slot ranges (those of the supernode's pivot
 // rows and those of the task's update rows, both in the task record), polled in parallel
 // rounds of kFwdStage into LDS and then summed per row in slot order.  The pivot rows'
-// right-hand side comes from bperm, b in the solve's numbering, so every load of the
-// task depends on the task record alone.  B: G terms per thread loaded before the wait.
+// right-hand side comes from bperm, b in the solve's numbering (pending until the leaf
+// workgroups of the launch have copied it), so every load of the task depends on the task
+// record alone.  B: G terms per thread loaded before the wait.  LDS (dyn): cv, then sred.
 constexpr int kFwdStage = 2 * kSolveThreads;
+constexpr int kFwdFlowLds = (kNsMax + kFwdStage) * 3 * static_cast<int>(sizeof(double));
 template <int B>
-__global__ void __launch_bounds__(kSolveThreads)
-k_fwd_flow(const SolveTask* __restrict__ tasks, int* abortf, SolveMaps M, const double* __restrict__ bperm,
-           const double* __restrict__ G, double* __restrict__ ysol, double* __restrict__ xsol,
-           double* __restrict__ slots) {
+__device__ __forceinline__ void fwd_flow_body(int ti, const SolveTask* __restrict__ tasks, int* abortf,
+                                              const SolveMaps& M, const double* __restrict__ bperm,
+                                              const double* __restrict__ G, double* __restrict__ ysol,
+                                              double* __restrict__ xsol, double* __restrict__ slots, double* dyn,
+                                              int& sh_ok) {
     constexpr int kRowsPerThread = kNsMax / kSolveThreads;
     constexpr int kSlotsPerThread = kFwdStage / kSolveThreads;
-    __shared__ double cv[kNsMax * 3];
-    __shared__ double sred[kFwdStage * 3];   // staged slots, then the column-group partial sums
-    __shared__ int sh_ok;
+    double* cv = dyn;                    // kNsMax x 3
+    double* sred = dyn + kNsMax * 3;     // staged slots, then the column-group partial sums
     static_assert(kFwdStage >= kSolveThreads, "the partial sums reuse the staging buffer");
     if (threadIdx.x == 0) sh_ok = __hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 0 : 1;
-    const SolveTask T = tasks[blockIdx.x];
+    const SolveTask T = tasks[ti];
     const int ns = T.ns, f = T.f, c0 = T.c0;
     const int R = T.width, ng = kSolveThreads / R;
     const int g = threadIdx.x / R, rl = threadIdx.x - g * R;
@@ -355,6 +368,7 @@ k_fwd_flow(const SolveTask* __restrict__ tasks, int* abortf, SolveMaps M, const 
 #pragma unroll
     for (int u = 0; u < B; ++u) gv[u] = u < nterm ? gp[u * gstride] : 0.0;
     int qlo[kRowsPerThread], qhi[kRowsPerThread];   // relative to the pivot slots' start
+    unsigned bpend = 0;                              // this thread's b rows still pending
 #pragma unroll
     for (int k = 0; k < kRowsPerThread; ++k) {
         const int j = threadIdx.x + k * kSolveThreads;
@@ -363,9 +377,11 @@ k_fwd_flow(const SolveTask* __restrict__ tasks, int* abortf, SolveMaps M, const 
         qhi[k] = in ? static_cast<int>(M.asm_ptr[T.rowptr + j + 1] - T.qa) : 0;
         if (in) {
             const double* bp = bperm + 3ll * (c0 + j);
-            cv[3 * j + 0] = bp[0];
-            cv[3 * j + 1] = bp[1];
-            cv[3 * j + 2] = bp[2];
+            const double b0 = ld_agent(bp + 0), b1 = ld_agent(bp + 1), b2 = ld_agent(bp + 2);
+            cv[3 * j + 0] = b0;
+            cv[3 * j + 1] = b1;
+            cv[3 * j + 2] = b2;
+            if (pending(b0) || pending(b1) || pending(b2)) bpend |= 1u << k;
         }
     }
     const bool urow = g == 0 && r >= ns && r < f;
@@ -376,8 +392,19 @@ k_fwd_flow(const SolveTask* __restrict__ tasks, int* abortf, SolveMaps M, const 
         uhi = np + static_cast<int>(M.asm_ptr[T.rowptr + r + 1] - T.uq0);
         udst = M.udest[T.uoff + (r - ns)];
     }
-    __syncthreads();   // sh_ok is set, cv holds b
+    __syncthreads();   // sh_ok is set
     const bool live = sh_ok != 0;
+#pragma unroll
+    for (int k = 0; k < kRowsPerThread; ++k)
+        if ((bpend >> k) & 1u) {
+            const int j = threadIdx.x + k * kSolveThreads;
+            double t3[3] = {cv[3 * j + 0], cv[3 * j + 1], cv[3 * j + 2]};
+            if (live && !poll3(bperm + 3ll * (c0 + j), t3, abortf)) sh_ok = 0;
+            cv[3 * j + 0] = t3[0];
+            cv[3 * j + 1] = t3[1];
+            cv[3 * j + 2] = t3[2];
+        }
+    __syncthreads();   // cv holds b
     // The children's updates, polled, staged and summed in slot order.
     double ua0 = 0.0, ua1 = 0.0, ua2 = 0.0;
     for (int base = 0; base < ntot; base += kFwdStage) {
@@ -471,18 +498,33 @@ k_fwd_flow(const SolveTask* __restrict__ tasks, int* abortf, SolveMaps M, const 
     }
 }
 
+// The forward solve in one launch: workgroups [0, nleaf) walk the leaf subtrees (and copy
+// bperm), the rest run the dataflow tasks, which only wait on lower-numbered workgroups.
+template <int B>
+__global__ void __launch_bounds__(kSolveThreads)
+k_fwd_solve(int nleaf, const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn,
+            const int* __restrict__ bcopy, int nbcopy, const SolveTask* __restrict__ tasks, int* abortf, SolveMaps M,
+            const double* __restrict__ b, const double* __restrict__ G, double* __restrict__ ysol,
+            double* __restrict__ xsol, double* __restrict__ slots, double* __restrict__ bperm) {
+    extern __shared__ double dyn[];
+    __shared__ int sh_ok;
+    if (static_cast<int>(blockIdx.x) < nleaf)
+        fwd_leaf_body(blockIdx.x, nleaf, leaf_ptr, leaf_sn, M, b, G, ysol, slots, bcopy, nbcopy, bperm, dyn);
+    else
+        fwd_flow_body<B>(blockIdx.x - nleaf, tasks, abortf, M, bperm, G, ysol, xsol, slots, dyn, sh_ok);
+}
+
 // Backward task (s, columns [first, first + 4 CPW)): x_J = G^T [y_J; x_R], CPW columns
 // per wave, rows staged through LDS in chunks.  BT / CPW G terms per column (the first
 // chunk's) are loaded before the wait.  The task also sets its pivot rows' forward slots
-// back to pending.
+// and bperm rows back to pending.  LDS (dyn): the staged rows.
+constexpr int kBwdFlowLds = kBwdChunk * 3 * static_cast<int>(sizeof(double));
 template <int CPW, int BT>
-__global__ void __launch_bounds__(kSolveThreads)
-k_bwd_flow(const SolveTask* __restrict__ tasks, int task0, int* abortf, SolveMaps M, const double* __restrict__ G,
-           const double* __restrict__ ysol, double* __restrict__ xsol, double* __restrict__ out,
-           double* __restrict__ slots) {
-    __shared__ double vv[kBwdChunk * 3];
-    __shared__ int sh_ok;
-    const int ti = task0 + static_cast<int>(blockIdx.x);
+__device__ __forceinline__ void bwd_flow_body(int ti, const SolveTask* __restrict__ tasks, int* abortf,
+                                              const SolveMaps& M, const double* __restrict__ G,
+                                              const double* __restrict__ ysol, double* __restrict__ xsol,
+                                              double* __restrict__ out, double* __restrict__ slots,
+                                              double* __restrict__ bperm, double* vv, int& sh_ok) {
     if (threadIdx.x == 0) sh_ok = __hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 0 : 1;
     const SolveTask T = tasks[ti];
     const int ns = T.ns, f = T.f, c0 = T.c0;
@@ -647,12 +689,19 @@ k_bwd_flow(const SolveTask* __restrict__ tasks, int task0, int* abortf, SolveMap
             xo[2] = ok ? acc[jj][2] : nan;
         }
     }
-    {   // the task's columns' forward slots: every forward task finished in the previous launch
+    {   // the task's columns' forward slots and b rows: every forward task finished in the previous launch
         const double pv = pending_value();
         for (int64_t q = sq0 + threadIdx.x; q < sq1; q += kSolveThreads) {
             slots[3 * q + 0] = pv;
             slots[3 * q + 1] = pv;
             slots[3 * q + 2] = pv;
+        }
+        const int j = jfirst + static_cast<int>(threadIdx.x);
+        if (static_cast<int>(threadIdx.x) < T.width && j < ns) {
+            double* bp = bperm + 3ll * (c0 + j);
+            bp[0] = pv;
+            bp[1] = pv;
+            bp[2] = pv;
         }
     }
 }
@@ -662,17 +711,16 @@ k_bwd_flow(const SolveTask* __restrict__ tasks, int task0, int* abortf, SolveMap
 // They are copied to out, in the caller's numbering, at the end.  Wave w takes
 // columns w, w + 4, ...; lane l rows j + l + 64 m.  The G terms of KC columns of a wave
 // are loaded together, the first KC before the step's rows are staged.
-__global__ void __launch_bounds__(kSolveThreads)
-k_bwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, SolveMaps M,
-           const double* __restrict__ G, const double* __restrict__ ysol, double* __restrict__ xsol,
-           double* __restrict__ out) {
+__device__ __forceinline__ void bwd_leaf_body(int t, const int* __restrict__ leaf_ptr,
+                                              const int* __restrict__ leaf_sn, const SolveMaps& M,
+                                              const double* __restrict__ G, const double* __restrict__ ysol,
+                                              const double* __restrict__ xsol, double* __restrict__ out,
+                                              int* abortf, double* dyn) {
     constexpr int KC = 4;
     constexpr int MR = kSmallFront / 64;
-    extern __shared__ double dyn[];
-    double* vv = dyn;                     // kSmallFront x 3
-    double* xl = dyn + kSmallFront * 3;   // the subtree's columns x 3
-    __shared__ LeafMeta L;
-    const int t = blockIdx.x;
+    LeafMeta& L = *reinterpret_cast<LeafMeta*>(dyn);
+    double* vv = dyn + kLeafRowsOff;      // kSmallFront x 3
+    double* xl = vv + kSmallFront * 3;    // the subtree's columns x 3
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r = threadIdx.x;
     const int q0 = leaf_ptr[t], q1 = leaf_ptr[t + 1];
@@ -700,12 +748,20 @@ k_bwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, So
         }
         int prow = -1;   // the next (lower) supernode's row source, loaded ahead
         if (r < f) {
-            const double* src;
-            if (r < ns) src = ysol + 3ll * (c0 + r);
-            else src = (row >= cfirst && row < cend) ? xl + 3 * (row - cfirst) : xsol + 3ll * row;
-            vv[3 * r + 0] = src[0];
-            vv[3 * r + 1] = src[1];
-            vv[3 * r + 2] = src[2];
+            const bool above = r >= ns && !(row >= cfirst && row < cend);
+            if (above) {   // an ancestor's solution, written by a dataflow task of this launch
+                const double* src = xsol + 3ll * row;
+                double v[3] = {ld_agent(src + 0), ld_agent(src + 1), ld_agent(src + 2)};
+                poll3(src, v, abortf);   // a wait that gave up leaves pending words: NaN results
+                vv[3 * r + 0] = v[0];
+                vv[3 * r + 1] = v[1];
+                vv[3 * r + 2] = v[2];
+            } else {
+                const double* src = r < ns ? ysol + 3ll * (c0 + r) : xl + 3 * (row - cfirst);
+                vv[3 * r + 0] = src[0];
+                vv[3 * r + 1] = src[1];
+                vv[3 * r + 2] = src[2];
+            }
         }
         if (i > 0 && r < L.fsize[i - 1] && r >= c0 - L.col0[i - 1]) prow = M.rows[L.rowptr[i - 1] + r];
         __syncthreads();
@@ -752,6 +808,28 @@ k_bwd_leaf(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn, So
         const double* x = xl + 3 * (c - cfirst);
         double* xo = out + 3ll * M.perm[c];
         xo[0] = x[0]; xo[1] = x[1]; xo[2] = x[2];
+    }
+}
+
+// The backward solve in one launch: workgroups [0, ntask) run the dataflow tasks (each
+// with its level's columns per wave, 4 x cpw columns per task), the rest walk the leaf
+// subtrees, each once the ancestor rows it reads are written.
+template <int BT>
+__global__ void __launch_bounds__(kSolveThreads)
+k_bwd_solve(const SolveTask* __restrict__ tasks, int ntask, const int* __restrict__ leaf_ptr,
+            const int* __restrict__ leaf_sn, int* abortf, SolveMaps M, const double* __restrict__ G,
+            const double* __restrict__ ysol, double* __restrict__ xsol, double* __restrict__ out,
+            double* __restrict__ slots, double* __restrict__ bperm) {
+    extern __shared__ double dyn[];
+    __shared__ int sh_ok;
+    const int ti = blockIdx.x;
+    if (ti < ntask) {
+        const int width = tasks[ti].width;   // uniform across the workgroup
+        if (width == 16) bwd_flow_body<4, BT>(ti, tasks, abortf, M, G, ysol, xsol, out, slots, bperm, dyn, sh_ok);
+        else if (width == 8) bwd_flow_body<2, BT>(ti, tasks, abortf, M, G, ysol, xsol, out, slots, bperm, dyn, sh_ok);
+        else bwd_flow_body<1, BT>(ti, tasks, abortf, M, G, ysol, xsol, out, slots, bperm, dyn, sh_ok);
+    } else {
+        bwd_leaf_body(ti - ntask, leaf_ptr, leaf_sn, M, G, ysol, xsol, out, abortf, dyn);
     }
 }
 
@@ -819,8 +897,8 @@ int chol_prepare_device(wrapd_ctx* c) {
         urows = std::max<int64_t>(urows, S.uoff[root + 1] - S.uoff[first]);
         cols = std::max<int64_t>(cols, S.col0[root + 1] - S.col0[first]);
     }
-    c->leaf_lds_fwd = static_cast<size_t>(kSmallFront + urows) * 3 * sizeof(double);
-    c->leaf_lds_bwd = static_cast<size_t>(kSmallFront + cols) * 3 * sizeof(double);
+    c->leaf_lds_fwd = (kLeafRowsOff + static_cast<size_t>(kSmallFront + urows) * 3) * sizeof(double);
+    c->leaf_lds_bwd = (kLeafRowsOff + static_cast<size_t>(kSmallFront + cols) * 3) * sizeof(double);
     if (c->leaf_lds_fwd > 65536 || c->leaf_lds_bwd > 65536) {
         c->last_error = "leaf subtree too large for the solve kernels' LDS";
         return WRAPD_ERR_ARG;
@@ -828,10 +906,9 @@ int chol_prepare_device(wrapd_ctx* c) {
     rc |= up(c, c->d_leaf_sn, S.leaf_sn);
     // Task widths are the widest that still give a level about kLevelTasks tasks:
     // forward (s, block of R rows), R in {256, ..., 16}; backward (s, block of 4 cpw
-    // columns), cpw in {4, 2, 1}, made non-decreasing from the root down so that each
-    // width is one launch.  Wide tasks keep G's reads long and the per-task costs (the
-    // right-hand side is gathered by every task of a supernode) few; narrow ones split the
-    // few supernodes near the root.  Forward tasks go bottom-up, backward tasks top-down.
+    // columns), cpw in {4, 2, 1}.  Wide tasks keep G's reads long and the per-task costs
+    // (the right-hand side is gathered by every task of a supernode) few; narrow ones split
+    // the few supernodes near the root.  Forward tasks go bottom-up, backward tasks top-down.
     const int nlev = static_cast<int>(S.level_ptr.size()) - 1;
     std::vector<char> in_level(S.nsuper, 0);
     for (int i = S.level_ptr[0]; i < S.level_ptr[nlev]; ++i) in_level[S.level_sn[i]] = 1;
@@ -855,7 +932,6 @@ int chol_prepare_device(wrapd_ctx* c) {
         }
         lev_cpw[l] = cpw;
     }
-    for (int l = nlev - 2; l >= 0; --l) lev_cpw[l] = std::max(lev_cpw[l], lev_cpw[l + 1]);
     for (int l = 0; l < nlev; ++l)
         for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) {
             const int sn = S.level_sn[i];
@@ -915,10 +991,7 @@ int chol_prepare_device(wrapd_ctx* c) {
                 bcopy.push_back(S.perm[j]);
             }
         }
-    c->bwd_segs.clear();
     for (int l = nlev - 1; l >= 0; --l) {
-        if (c->bwd_segs.empty() || c->bwd_segs.back().second != lev_cpw[l])
-            c->bwd_segs.push_back({static_cast<int>(bt.size()), lev_cpw[l]});
         for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) {
             const int sn = S.level_sn[i];
             for (int cb = 0; cb < nbt[sn]; ++cb) bt.push_back(task_of(sn, cb * 4 * lev_cpw[l], 4 * lev_cpw[l]));
@@ -931,7 +1004,7 @@ int chol_prepare_device(wrapd_ctx* c) {
     rc |= up(c, c->d_flow_abort, std::vector<int>(1, 0));
     c->n_bcopy = static_cast<int>(bcopy.size() / 2);
     rc |= up(c, c->d_bcopy, bcopy);
-    rc |= up(c, c->bperm, std::vector<double>(3ull * std::max(1, c->n_f), 0.0));
+    rc |= up(c, reinterpret_cast<uint64_t*&>(c->bperm), std::vector<uint64_t>(3ull * std::max(1, c->n_f), kPendingBits));
     // the rows the dataflow tasks hand over start pending (their bit pattern, copied as integers)
     std::vector<uint64_t> p3(3ull * std::max(1, c->n_f), kPendingBits);
     rc |= up(c, c->ysol, std::vector<double>(p3.size(), 0.0));
@@ -952,31 +1025,22 @@ void launch_solve(wrapd_ctx* c, const double* b, double* x) {
     const SolveTask* bt = static_cast<const SolveTask*>(c->d_flow_b);
     {
         ProfScope ps(c, "solve_fwd");
-        if (nleaf > 0)
-            hipLaunchKernelGGL(k_fwd_leaf, dim3(nleaf), dim3(kSolveThreads), c->leaf_lds_fwd, c->stream, c->d_leaf_ptr,
-                               c->d_leaf_sn, M, b, c->G, c->ysol, c->ubuf, c->d_bcopy, c->n_bcopy, c->bperm);
-        else if (c->n_bcopy > 0)
+        if (nleaf == 0 && c->n_bcopy > 0)
             hipLaunchKernelGGL(k_bperm, dim3((c->n_bcopy + 255) / 256), dim3(256), 0, c->stream, c->d_bcopy, c->n_bcopy,
                                b, c->bperm);
-        if (c->n_flow_f > 0) {
-            hipLaunchKernelGGL(k_fwd_flow<kFwdPrefetch>, dim3(c->n_flow_f), dim3(kSolveThreads), 0, c->stream, ft,
-                               c->d_flow_abort, M, c->bperm, c->G, c->ysol, c->xsol, c->ubuf);
-        }
+        const size_t lds = std::max<size_t>(c->n_flow_f > 0 ? kFwdFlowLds : 0, nleaf > 0 ? c->leaf_lds_fwd : 0);
+        if (nleaf + c->n_flow_f > 0)
+            hipLaunchKernelGGL(k_fwd_solve<kFwdPrefetch>, dim3(nleaf + c->n_flow_f), dim3(kSolveThreads), lds, c->stream,
+                               nleaf, c->d_leaf_ptr, c->d_leaf_sn, c->d_bcopy, c->n_bcopy, ft, c->d_flow_abort, M, b,
+                               c->G, c->ysol, c->xsol, c->ubuf, c->bperm);
     }
     {
         ProfScope ps(c, "solve_bwd");
-        for (size_t k = 0; k < c->bwd_segs.size(); ++k) {
-            const int t0 = c->bwd_segs[k].first, cpw = c->bwd_segs[k].second;
-            const int n = (k + 1 < c->bwd_segs.size() ? c->bwd_segs[k + 1].first : c->n_flow_b) - t0;
-            if (n <= 0) continue;
-            auto kern = cpw == 4 ? k_bwd_flow<4, kBwdPrefetch> : cpw == 2 ? k_bwd_flow<2, kBwdPrefetch>
-                                                                         : k_bwd_flow<1, kBwdPrefetch>;
-            hipLaunchKernelGGL(kern, dim3(n), dim3(kSolveThreads), 0, c->stream, bt, t0, c->d_flow_abort, M, c->G,
-                               c->ysol, c->xsol, x, c->ubuf);
-        }
-        if (nleaf > 0)
-            hipLaunchKernelGGL(k_bwd_leaf, dim3(nleaf), dim3(kSolveThreads), c->leaf_lds_bwd, c->stream, c->d_leaf_ptr,
-                               c->d_leaf_sn, M, c->G, c->ysol, c->xsol, x);
+        const size_t lds = std::max<size_t>(c->n_flow_b > 0 ? kBwdFlowLds : 0, nleaf > 0 ? c->leaf_lds_bwd : 0);
+        if (nleaf + c->n_flow_b > 0)
+            hipLaunchKernelGGL(k_bwd_solve<kBwdPrefetch>, dim3(c->n_flow_b + nleaf), dim3(kSolveThreads), lds, c->stream,
+                               bt, c->n_flow_b, c->d_leaf_ptr, c->d_leaf_sn, c->d_flow_abort, M, c->G, c->ysol, c->xsol,
+                               x, c->ubuf, c->bperm);
     }
 }
 
