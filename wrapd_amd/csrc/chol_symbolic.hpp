@@ -58,7 +58,7 @@ struct SymbolicOptions {
     int ns_max = 1024;           // widest supernode after splitting
     int relax_small = 16;        // always merge a child into its parent below this width
     double relax_frac = 0.06;    // allowed fraction of explicit zeros for larger merges
-    int relax_max = 192;         // no relaxed merge beyond this width
+    int relax_max = 1024;        // no relaxed merge beyond this width (wide merges near the root shorten the solves' dependency chains)
     int small_front = 256;       // leaf-subtree supernodes must have f_s <= this
     int leaf_tasks = 2048;       // target number of leaf subtrees
 };
