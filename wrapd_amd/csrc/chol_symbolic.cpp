@@ -387,19 +387,20 @@ void chol_analyze(int n, const std::vector<int>& Ap, const std::vector<int>& Aj,
         total += work[s];
         small[s] = S.fsize[s] <= opt.small_front && ns <= 128;
     }
+    std::vector<int> subn(ns_total, 1);   // supernodes in the subtree
     for (int s = 0; s < ns_total; ++s) { sub[s] = work[s]; allsmall[s] = small[s]; }
     for (int s = 0; s < ns_total; ++s) {   // children precede parents
         int p = S.parent[s];
-        if (p >= 0) { sub[p] += sub[s]; if (!allsmall[s]) allsmall[p] = 0; }
+        if (p >= 0) { sub[p] += sub[s]; subn[p] += subn[s]; if (!allsmall[s]) allsmall[p] = 0; }
     }
     double wleaf = std::max(4096.0, total / opt.leaf_tasks);
     std::vector<char> inleaf(ns_total, 0);
     std::vector<int> level(ns_total, 0);
     S.leaf_ptr.assign(1, 0);
     for (int s = 0; s < ns_total; ++s) {
-        bool ok = allsmall[s] && sub[s] <= wleaf;
+        bool ok = allsmall[s] && sub[s] <= wleaf && subn[s] <= opt.leaf_max_sn;
         int p = S.parent[s];
-        bool pok = p >= 0 && allsmall[p] && sub[p] <= wleaf;
+        bool pok = p >= 0 && allsmall[p] && sub[p] <= wleaf && subn[p] <= opt.leaf_max_sn;
         if (ok && !pok) {
             // s is a leaf-subtree root: collect its subtree in postorder (contiguous range)
             std::vector<int> st = {s}, out;

@@ -61,6 +61,7 @@ struct SymbolicOptions {
     int relax_max = 1024;        // no relaxed merge beyond this width (wide merges near the root shorten the solves' dependency chains)
     int small_front = 256;       // leaf-subtree supernodes must have f_s <= this
     int leaf_tasks = 2048;       // target number of leaf subtrees
+    int leaf_max_sn = 6;         // most supernodes in one leaf subtree (walked one after another by one workgroup)
 };
 
 // Ap/Aj: upper-triangular CSR (row i holds columns j >= i) over n free vertices;
