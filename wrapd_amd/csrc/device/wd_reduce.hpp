@@ -64,6 +64,31 @@ __device__ __forceinline__ double block_sums(const double (&v)[KS], int n, doubl
     return r;
 }
 
+// The same with some values reduced by max (bit k of maxmask), in the order of block_max.
+template <int NT, int KS>
+__device__ __forceinline__ double block_sums_max(const double (&v)[KS], unsigned maxmask, double (*sh)[NT / 64]) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+        const double t = ((maxmask >> k) & 1u) ? wave_max(v[k]) : wave_sum(v[k]);
+        if (l == 0) sh[k][w] = t;
+    }
+    __syncthreads();
+    double r = 0.0;
+    const int k = threadIdx.x;
+    if (k < KS) {
+        if ((maxmask >> k) & 1u) {
+            r = sh[k][0];
+#pragma unroll
+            for (int i = 1; i < NT / 64; ++i) r = fmax(r, sh[k][i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < NT / 64; ++i) r += sh[k][i];
+        }
+    }
+    return r;
+}
+
 template <int NT>
 __device__ __forceinline__ double block_max(double v, double* sh) {
     v = wave_max(v);

@@ -17,7 +17,7 @@
 namespace wd {
 
 constexpr int kElemThreads = 256;     // element kernels
-constexpr int kLocalThreads = 128;    // local step (register heavy)
+constexpr int kLocalThreads = 256;    // local step (register heavy: one wave per SIMD either way; 256 keeps C2's grid under the in-launch reduction limit)
 constexpr int kRedStride = wd::kTailSlots;   // reduction slots per block in `partials`
 constexpr int kMaxHist = 8;           // largest L-BFGS memory supported on the device
 constexpr int kMaxBlocks = wd::kTailBlocks;   // capacity of the partials buffer (blocks)

@@ -368,20 +368,16 @@ __device__ __forceinline__ void elem_prox(const M3<double>& Sin, const double sl
 // The local step's 11 reduction slots: 0 bef, 1 aft, 2 pe, 3 max ratio, 4 prox iterations,
 // 5 FP64 fallbacks, 6 inverted, 7 learned starts accepted, 8 evaluations, 9 max
 // evaluations, 10 learned starts used.
+// The local step's 11 block partials (slots 3 and 9 by max), reduced with one barrier and
+// stored by threads 0..10.
+constexpr unsigned kLocalMaxMask = (1u << 3) | (1u << 9);
 __device__ __forceinline__ void store_local_partials(double* partials, int lb, double r0, double r1, double r2,
                                                      double r3, double r4, double r5, double r6, double r7,
-                                                     double r8, double r9, double r10) {
-    put_partial(partials, lb, 0, r0);
-    put_partial(partials, lb, 1, r1);
-    put_partial(partials, lb, 2, r2);
-    put_partial(partials, lb, 3, r3);
-    put_partial(partials, lb, 4, r4);
-    put_partial(partials, lb, 5, r5);
-    put_partial(partials, lb, 6, r6);
-    put_partial(partials, lb, 7, r7);
-    put_partial(partials, lb, 8, r8);
-    put_partial(partials, lb, 9, r9);
-    put_partial(partials, lb, 10, r10);
+                                                     double r8, double r9, double r10,
+                                                     double (*shk)[kLocalThreads / 64]) {
+    const double v[11] = {r0, r1, r2, r3, r4, r5, r6, r7, r8, r9, r10};
+    const double t = block_sums_max<kLocalThreads, 11>(v, kLocalMaxMask, shk);
+    if (threadIdx.x < 11) put_partial(partials, lb, threadIdx.x, t);
 }
 
 template <int MODEL, int PREC, int PROX>
@@ -391,7 +387,7 @@ k_local(int n_e, int n_ep, const double* __restrict__ symF, double* __restrict__
         double* __restrict__ cand, double* __restrict__ sig, MatArgs m, int prox_iters, double rtol,
         double fallback_sigma, int update_cand, int use_hessian, double* __restrict__ partials,
         float* __restrict__ tlog, RedTail tail) {
-    __shared__ double sh[4];   // the block reductions', then the tail's 4 wave sums
+    __shared__ double shk[11][kLocalThreads / 64];   // the block reductions' wave results
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int e = lb * blockDim.x + threadIdx.x;
     double bef = 0.0, aft = 0.0, pe = 0.0, ratio = 0.0, iters = 0.0, fallback = 0.0, neg = 0.0, evals = 0.0;
@@ -467,16 +463,7 @@ k_local(int n_e, int n_ep, const double* __restrict__ symF, double* __restrict__
             ratio = a > b ? a : b;
         }
     }
-    const double r0 = block_sum<kLocalThreads>(bef, sh);
-    const double r1 = block_sum<kLocalThreads>(aft, sh);
-    const double r2 = block_sum<kLocalThreads>(pe, sh);
-    const double r3 = block_max<kLocalThreads>(ratio, sh);
-    const double r4 = block_sum<kLocalThreads>(iters, sh);
-    const double r5 = block_sum<kLocalThreads>(fallback, sh);
-    const double r6 = block_sum<kLocalThreads>(neg, sh);
-    const double r8 = block_sum<kLocalThreads>(evals, sh);
-    const double r9 = block_max<kLocalThreads>(evals, sh);
-    if (threadIdx.x == 0) store_local_partials(partials, lb, r0, r1, r2, r3, r4, r5, r6, 0.0, r8, r9, 0.0);
+    store_local_partials(partials, lb, bef, aft, pe, ratio, iters, fallback, neg, 0.0, evals, evals, 0.0, shk);
     reduce_tail<kLocalThreads, 11>(tail, partials);
 }
 
@@ -508,7 +495,7 @@ k_local_learned(int n_e, int n_ep, const double* __restrict__ symF, double* __re
                 double fallback_sigma, int update_cand, int use_hessian, double* __restrict__ partials,
                 float* __restrict__ tlog, MlpDevice mlp, RedTail tail) {
     using T = typename std::conditional<PREC == WRAPD_FP32, float, double>::type;
-    __shared__ double shr[4];   // the block reductions', then the tail's 4 wave sums
+    __shared__ double shk[11][kLocalThreads / 64];   // the block reductions' wave results
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int e = lb * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -624,18 +611,7 @@ k_local_learned(int n_e, int n_ep, const double* __restrict__ symF, double* __re
             ratio = a > b ? a : b;
         }
     }
-    const double r0 = block_sum<kLocalThreads>(bef, shr);
-    const double r1 = block_sum<kLocalThreads>(aft, shr);
-    const double r2 = block_sum<kLocalThreads>(pe, shr);
-    const double r3 = block_max<kLocalThreads>(ratio, shr);
-    const double r4 = block_sum<kLocalThreads>(iters, shr);
-    const double r5 = block_sum<kLocalThreads>(fallback, shr);
-    const double r6 = block_sum<kLocalThreads>(neg, shr);
-    const double r7 = block_sum<kLocalThreads>(accepted, shr);
-    const double r8 = block_sum<kLocalThreads>(evals, shr);
-    const double r9 = block_max<kLocalThreads>(evals, shr);
-    const double r10 = block_sum<kLocalThreads>(used, shr);
-    if (threadIdx.x == 0) store_local_partials(partials, lb, r0, r1, r2, r3, r4, r5, r6, r7, r8, r9, r10);
+    store_local_partials(partials, lb, bef, aft, pe, ratio, iters, fallback, neg, accepted, evals, evals, used, shk);
     reduce_tail<kLocalThreads, 11>(tail, partials);
 }
 
