@@ -16,6 +16,7 @@
 namespace wd {
 
 constexpr int kVecThreads = 256;
+constexpr int kGatherBatch = 8;   // corners per batch of loads in the D^T z gather
 
 struct SlotList {
     int n;
@@ -142,11 +143,31 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
     const int v = lb * blockDim.x + threadIdx.x;
     double g[3] = {0.0, 0.0, 0.0};
     if (v < n_f) {
-        for (int q = ptr[v]; q < ptr[v + 1]; ++q) {
-            const double* p = gstage + 3ll * idx[q];
-            g[0] += p[0];
-            g[1] += p[1];
-            g[2] += p[2];
+        // kGatherBatch corners at a time: their indices, then their rows, are loaded together
+        // (a vertex has ~20 corners); the sums are taken in corner order
+        const int q1 = ptr[v + 1];
+        for (int qb = ptr[v]; qb < q1; qb += kGatherBatch) {
+            int id[kGatherBatch];
+#pragma unroll
+            for (int u = 0; u < kGatherBatch; ++u) id[u] = qb + u < q1 ? idx[qb + u] : -1;
+            double a[kGatherBatch][3];
+#pragma unroll
+            for (int u = 0; u < kGatherBatch; ++u) {
+                a[u][0] = a[u][1] = a[u][2] = 0.0;
+                if (id[u] >= 0) {
+                    const double* p = gstage + 3ll * id[u];
+                    a[u][0] = p[0];
+                    a[u][1] = p[1];
+                    a[u][2] = p[2];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kGatherBatch; ++u)
+                if (id[u] >= 0) {
+                    g[0] += a[u][0];
+                    g[1] += a[u][1];
+                    g[2] += a[u][2];
+                }
         }
         out[3 * v + 0] = g[0];
         out[3 * v + 1] = g[1];
