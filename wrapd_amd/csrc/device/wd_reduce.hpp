@@ -113,7 +113,7 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
 // the order of a 256-thread finalize: lane v takes blocks v, v + 256, ...; lanes are
 // summed in waves of 64 and the 4 waves in order.  So the result is the one a separate
 // one-block finalize launch would produce, without the launch.
-constexpr int kTailSlots = 24;        // reduction slots per block
+constexpr int kTailSlots = 32;        // reduction slots per block
 constexpr int kTailBlocks = 1 << 16;  // blocks per reducing launch (the partials' row length)
 
 struct RedTail {
@@ -124,6 +124,7 @@ struct RedTail {
     unsigned* hseq = nullptr;    // host memory: set to epoch once the copies (and any work after) are stored, or nullptr
     unsigned epoch = 0;
     int nslots = 0;
+    int nstore = 0;              // the leading slots stored to out (and hout); the rest only to shres
     unsigned maxmask = 0;        // slots reduced with max instead of sum
 };
 
@@ -229,9 +230,9 @@ __device__ __forceinline__ bool reduce_tail(const RedTail R, const double* parti
             for (int i = 0; i < 4; ++i) r += wsum[k][i];
         }
         if (bad) r = __builtin_nan("");
-        R.out[k] = r;
         if (shres) shres[k] = r;
-        if (R.hout) {
+        if (k < R.nstore) R.out[k] = r;
+        if (R.hout && k < R.nstore) {
             __hip_atomic_store(R.hout + k, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             __builtin_amdgcn_s_waitcnt(0);
         }

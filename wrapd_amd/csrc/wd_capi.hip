@@ -640,7 +640,12 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
     arm_tail(c, 1, 0u, S_VAL);
     launch_defo(c, DM_GRAD | DM_VALUE, 0);
     arm_tail(c, 1, 0u, S_DG);
-    launch_gather_grad(c, c->grad, 1, 0, nullptr, 0);
+    {   // with the first two-loop's alphas for the current history
+        int hs[kMaxHist];
+        const int h = std::min(M, c->m_k);
+        for (int i = 0; i < h; ++i) hs[i] = c->pos2slot[i];
+        launch_gather_grad(c, c->grad, 1, 0, nullptr, 0, hs, h);
+    }
     const double* r = scal_read(c, S_DG, 2);   // S_DG, S_VAL
     if (!r) return WRAPD_ERR_HIP;
     const double dg = r[0];
@@ -670,7 +675,10 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
         if (it > 0) {
             double* vs[kMaxHist];
             for (int i = 0; i < it; ++i) vs[i] = c->hS[slots[i]];
-            launch_multidot(c, g, vs, it, 1, slots);   // and the alpha recurrence
+            // the gather that produced g has left the alphas of this list in S_ALPHA
+            const bool have_alpha = c->alpha_n == it && std::equal(slots, slots + it, c->alpha_slots);
+            if (!have_alpha) launch_multidot(c, g, vs, it, 1, slots);   // and the alpha recurrence
+            c->alpha_n = -1;
             launch_axpy_q(c, g, slots, it);
             launch_solve(c, c->qv, c->rv);
             for (int i = 0; i < it; ++i) vs[i] = c->hY[slots[i]];
@@ -761,7 +769,7 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
         const bool last = early || (global_iter + 1 > max_iters);
         arm_tail(c, last ? 3 : 1, 0u, S_VAL);
         launch_defo(c, last ? (DM_GRAD | DM_VALUE | DM_WRITE_SYMF | DM_ENERGY) : (DM_GRAD | DM_VALUE), 0);
-        launch_gather_grad(c, c->grad, 2, new_pos, lslots, nlist);
+        launch_gather_grad(c, c->grad, 2, new_pos, lslots, nlist, lslots, nlist);   // and the next alphas
         const double* d = scal_read(c, S_DENOM, 5);   // DENOM, DG, VAL, EN, FLIPS
         if (!d) return WRAPD_ERR_HIP;
         const double denom = d[0];

@@ -165,6 +165,8 @@ struct wrapd_ctx {
     unsigned* d_red_flags = nullptr;   // per block of a reducing launch: epoch when its partials are stored
     int* d_red_abort = nullptr;        // sticky: a reduction's wait ran out
     unsigned red_epoch = 0;
+    int alpha_n = -1;                  // S_ALPHA holds the alphas of this list for the gradient in grad (-1: none)
+    int alpha_slots[wd::kMaxHist] = {};
     unsigned host_epoch = 0;           // the epoch of the last reduction with a host copy (what the host waits for)
     wd::RedTail pending_tail;          // armed by arm_tail, taken by the next reducing launch
     struct PendingFinalize {
@@ -221,8 +223,11 @@ void arm_tail(wrapd_ctx* c, int nslots, unsigned maxmask, int dst, bool host = t
 wd::RedTail take_tail(wrapd_ctx* c, int nblk, bool may_split);
 void finish_reduction(wrapd_ctx* c);
 constexpr int kTailMaxBlocks = 1024;
-void launch_gather_grad(wrapd_ctx* c, double* out, int mode, int new_slot, const int* list_slots,
-                        int nlist);   // mode: 0 plain, 1 first (dg), 2 step (y + Gram)
+// mode: 0 plain, 1 first (dg), 2 step (y + Gram).  alpha_slots: the next two-loop's list
+// (mode 1, or mode 2 with the list itself); its alphas are then left in S_ALPHA, recorded in
+// alpha_n / alpha_slots
+void launch_gather_grad(wrapd_ctx* c, double* out, int mode, int new_slot, const int* list_slots, int nlist,
+                        const int* alpha_slots = nullptr, int nalpha = 0);
 // dots of base with vecs into S_RED; post 1 then runs the alpha recurrence, 2 the beta one
 // (SlotList of `slots`), in the launch's last block
 void launch_multidot(wrapd_ctx* c, const double* base, double* const* vecs, int nvec, int post = 0,

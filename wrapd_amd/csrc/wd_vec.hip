@@ -129,16 +129,19 @@ __device__ void lbfgs_beta(double* __restrict__ scal, const double* red, const d
 // grad[v] = sum of the staged element-corner gradients of vertex v, in element order.
 // mode 1: also |grad - grad_old|^2 (SplitLBFGS.hpp:138).
 // mode 2: y_new = grad - grad_old into slot `ns`, and the Gram entries of the new pair.
-template <int MODE>
+// With a list LA (LA.n > 0; in mode 2 the list L itself), also the dots grad . s_j of the
+// next two-loop and its alpha recurrence (what a multi-dot launch with post 1 computes
+// next), reduced in the slots after the mode's own (mode 1: from 1, mode 2: from 2n + 1)
+// and kept out of scal.  KG: the most slots of the launch.
+template <int MODE, int KG>
 __global__ void __launch_bounds__(kVecThreads)
 k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, const double* __restrict__ gstage,
          double* __restrict__ out, const double* __restrict__ gold, double* __restrict__ ynew,
-         const double* __restrict__ snew, VecPtrs S, VecPtrs Y, SlotList L, int ns_pos,
+         const double* __restrict__ snew, VecPtrs S, VecPtrs Y, SlotList L, int ns_pos, SlotList LA, VecPtrs SA,
          double* __restrict__ partials, RedTail tail, double* __restrict__ scal) {
-    constexpr int KG = 2 * kMaxHist + 1;
-    __shared__ double sh[kVecThreads / 64];
+    const int kd = MODE == 1 ? 1 : 2 * L.n + 1;   // the first alpha-dot slot
     __shared__ double shk[KG][kVecThreads / 64];
-    __shared__ double sres[KG];
+    __shared__ double sres[KG], gm[kMaxHist * kMaxHist], sa[kMaxHist];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int v = lb * blockDim.x + threadIdx.x;
     double g[3] = {0.0, 0.0, 0.0};
@@ -174,6 +177,15 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
         out[3 * v + 2] = g[2];
     }
     if (MODE == 1) {
+        double gs[kMaxHist];   // grad . s_j of the next two-loop's list
+#pragma unroll
+        for (int j = 0; j < kMaxHist; ++j) {
+            gs[j] = 0.0;
+            if (j < LA.n && v < n_f) {
+                const double* sj = SA.v[j] + 3 * v;
+                gs[j] = g[0] * sj[0] + g[1] * sj[1] + g[2] * sj[2];
+            }
+        }
         double a = 0.0;
         if (v < n_f) {
 #pragma unroll
@@ -182,8 +194,12 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
                 a += d * d;
             }
         }
-        const double t = block_sum<kVecThreads>(a, sh);
-        if (threadIdx.x == 0) put_partial(partials, lb, 0, t);
+        double part[KG];
+        part[0] = a;
+#pragma unroll
+        for (int j = 0; j + 1 < KG; ++j) part[1 + j] = j < kMaxHist ? gs[j] : 0.0;
+        const double t = block_sums<kVecThreads, KG>(part, 1 + LA.n, shk);
+        if (static_cast<int>(threadIdx.x) < 1 + LA.n) put_partial(partials, lb, threadIdx.x, t);
     }
     if (MODE == 2) {
         double y[3] = {0, 0, 0}, s[3] = {0, 0, 0};
@@ -195,11 +211,12 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
                 s[c] = snew[3 * v + c];
             }
         }
-        // slot 2j: s_new . y_j, slot 2j + 1: s_j . y_new, slot 2n: y_new . y_new
-        double part[KG];
+        // slot 2j: s_new . y_j, slot 2j + 1: s_j . y_new, slot 2n: y_new . y_new;
+        // slots 2n + 1 + j: grad . s_j (with LA, which is L)
+        double ab[2 * kMaxHist], gs[kMaxHist];
 #pragma unroll
         for (int j = 0; j < kMaxHist; ++j) {
-            double a = 0.0, b = 0.0;
+            double a = 0.0, b = 0.0, e = 0.0;
             if (j < L.n && v < n_f) {
                 const double* yj = (j == ns_pos) ? nullptr : Y.v[j] + 3 * v;
                 const double* sj = (j == ns_pos) ? nullptr : S.v[j] + 3 * v;
@@ -209,21 +226,45 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
                     const double ss = sj ? sj[c] : s[c];
                     a += s[c] * yy;
                     b += ss * y[c];
+                    e += g[c] * ss;
                 }
             }
-            part[2 * j] = a;
-            part[2 * j + 1] = b;
+            ab[2 * j] = a;
+            ab[2 * j + 1] = b;
+            gs[j] = LA.n > 0 ? e : 0.0;
         }
-        part[KG - 1] = 0.0;
         const double d = y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
+        double part[KG];
 #pragma unroll
-        for (int k = 0; k < KG; ++k)
-            if (k == 2 * L.n) part[k] = d;
-        const double t = block_sums<kVecThreads, KG>(part, 2 * L.n + 1, shk);
-        if (static_cast<int>(threadIdx.x) < 2 * L.n + 1) put_partial(partials, lb, threadIdx.x, t);
+        for (int k = 0; k < KG; ++k) {
+            double pk = k < 2 * kMaxHist ? ab[k < 2 * kMaxHist ? k : 0] : 0.0;
+            if (k >= 2 * L.n) pk = 0.0;
+            if (k == 2 * L.n) pk = d;
+#pragma unroll
+            for (int j = 0; j < kMaxHist; ++j)
+                if (k == kd + j) pk = gs[j];
+            part[k] = pk;
+        }
+        const int nk = kd + LA.n;
+        const double t = block_sums<kVecThreads, KG>(part, nk, shk);
+        if (static_cast<int>(threadIdx.x) < nk) put_partial(partials, lb, threadIdx.x, t);
     }
-    if (MODE != 0 && reduce_tail<kVecThreads, MODE == 1 ? 1 : KG, false>(tail, partials, sres)) {
+    if (MODE != 0 && reduce_tail<kVecThreads, KG, false>(tail, partials, sres)) {
+        static_assert(KG <= kTailSlots, "gather: slot count");
         if (MODE == 2) gram_store(scal, tail.hout - S_RED, sres, L, ns_pos);
+        if (LA.n > 0) {   // the next two-loop's alphas; mode 2's new Gram row and column come from sres
+            const int n = LA.n;
+            for (int q = threadIdx.x; q < n * n; q += kVecThreads) {
+                const int i = q / n, j = q - (q / n) * n;
+                double gij;
+                if (MODE == 2 && i == ns_pos) gij = sres[2 * j];
+                else if (MODE == 2 && j == ns_pos) gij = sres[2 * i + 1];
+                else gij = scal[S_GRAM + LA.slot[i] * kMaxHist + LA.slot[j]];
+                gm[q] = gij;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) lbfgs_alpha(scal, sres + kd, gm, sa, n);
+        }
         __syncthreads();   // every thread's stores are issued before the signal
         tail_signal(tail);
     }
@@ -377,6 +418,7 @@ void arm_tail(wrapd_ctx* c, int nslots, unsigned maxmask, int dst, bool host) {
     t.hout = host ? c->scal_host + dst : nullptr;
     t.hseq = host ? c->host_seq : nullptr;
     t.nslots = nslots;
+    t.nstore = nslots;
     t.maxmask = maxmask;
     c->pending_tail = t;
 }
@@ -406,21 +448,35 @@ void finish_reduction(wrapd_ctx* c) {
                        f.host ? c->scal_host + f.dst : nullptr, f.host ? c->host_seq : nullptr, c->red_epoch);
 }
 
-void launch_gather_grad(wrapd_ctx* c, double* out, int mode, int new_pos, const int* list_slots, int nlist) {
+void launch_gather_grad(wrapd_ctx* c, double* out, int mode, int new_pos, const int* list_slots, int nlist,
+                        const int* alpha_slots, int nalpha) {
     ProfScope ps(c, "gather_grad");
     const int nb = blocks_for(c->n_f, kVecThreads);
-    VecPtrs S{}, Y{};
+    VecPtrs S{}, Y{}, SA{};
     SlotList L = mk_list(list_slots, nlist);
     for (int j = 0; j < nlist; ++j) { S.v[j] = c->hS[list_slots[j]]; Y.v[j] = c->hY[list_slots[j]]; }
     double* ynew = nullptr;
     const double* snew = nullptr;
     if (mode == 2) { ynew = c->hY[list_slots[new_pos]]; snew = c->hS[list_slots[new_pos]]; }
-    // mode 1 reduces into whatever the caller armed; mode 2 into S_RED, then stores the Gram entries
+    // the next two-loop's alphas (mode 1 or 2; in mode 2 only for the list itself)
+    if (mode == 0 || nalpha <= 0 || (mode == 2 && (nalpha != nlist || !std::equal(alpha_slots, alpha_slots + nalpha,
+                                                                                    list_slots))))
+        nalpha = 0;
+    SlotList LA = mk_list(alpha_slots, nalpha);
+    for (int j = 0; j < nalpha; ++j) SA.v[j] = c->hS[alpha_slots[j]];
+    c->alpha_n = nalpha > 0 ? nalpha : -1;
+    for (int j = 0; j < nalpha; ++j) c->alpha_slots[j] = alpha_slots[j];
+    // mode 1 reduces into whatever the caller armed; mode 2 into S_RED, then stores the Gram
+    // entries; the alpha dots are reduced after the stored slots
     if (mode == 2) arm_tail(c, 2 * nlist + 1, 0u, S_RED);
+    if (mode != 0 && nalpha > 0) c->pending_tail.nslots = (mode == 1 ? 1 : 2 * nlist + 1) + nalpha;
     const RedTail t = mode == 0 ? RedTail{} : take_tail(c, nb, false);
-    auto k = mode == 0 ? k_gather<0> : mode == 1 ? k_gather<1> : k_gather<2>;
+    constexpr int kSmall = 16;   // mode 2 slots up to a memory of 5: 3 x 5 + 1
+    const int need = mode == 2 ? 2 * nlist + 1 + nalpha : 0;
+    auto k = mode == 0 ? k_gather<0, 1> : mode == 1 ? k_gather<1, 1 + kMaxHist>
+                                                    : (need <= kSmall ? k_gather<2, kSmall> : k_gather<2, 3 * kMaxHist + 1>);
     hipLaunchKernelGGL(k, dim3(nb), dim3(kVecThreads), 0, c->stream, c->n_f, c->v2e_ptr, c->v2e_idx, c->gstage, out,
-                       c->grad_old, ynew, snew, S, Y, L, new_pos, c->partials, t, c->scal);
+                       c->grad_old, ynew, snew, S, Y, L, new_pos, LA, SA, c->partials, t, c->scal);
 }
 
 void launch_multidot(wrapd_ctx* c, const double* base, double* const* vecs, int nvec, int post, const int* slots) {
