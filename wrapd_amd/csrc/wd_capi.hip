@@ -672,6 +672,9 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
         int slots[kMaxHist];
         for (int i = 0; i < it; ++i) slots[i] = c->pos2slot[i];
         double dir;
+        // without the line search, r = rv + sum coef_j s_j is only needed by the x-step, which
+        // forms it; g . r then comes from the beta launch
+        bool lazy_r = false;
         if (it > 0) {
             double* vs[kMaxHist];
             for (int i = 0; i < it; ++i) vs[i] = c->hS[slots[i]];
@@ -682,8 +685,13 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
             launch_axpy_q(c, g, slots, it);
             launch_solve(c, c->qv, c->rv);
             for (int i = 0; i < it; ++i) vs[i] = c->hY[slots[i]];
-            launch_multidot(c, c->rv, vs, it, 2, slots);   // and the beta recurrence
-            launch_axpy_r(c, g, slots, it);
+            if (c->lcfg.linesearch == 1) {
+                launch_multidot(c, c->rv, vs, it, 2, slots);   // and the beta recurrence
+                launch_axpy_r(c, g, slots, it);
+            } else {
+                launch_multidot(c, c->rv, vs, it, 3, slots, g);   // the beta recurrence and g . r
+                lazy_r = true;
+            }
             const double* d = scal_read(c, S_DIR, 1);
             if (!d) return WRAPD_ERR_HIP;
             dir = d[0];
@@ -704,6 +712,7 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
         }
         const double* stepdir = c->rv;
         if (dir <= 0) {
+            lazy_r = false;
             stepdir = g;   // q = grad
             max_iters -= c->m_k;
             c->m_k = 0;
@@ -764,7 +773,8 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
         const int nlist = new_pos + 1;
         int lslots[kMaxHist];
         for (int i = 0; i < nlist; ++i) lslots[i] = c->pos2slot[i];
-        launch_xstep(c, rate, stepdir, c->hS[new_slot]);
+        if (lazy_r) launch_xstep_r(c, rate, slots, it, c->hS[new_slot]);
+        else launch_xstep(c, rate, stepdir, c->hS[new_slot]);
         const bool early = dir > 0 && (dir < kappa * m_tol || dir < kappa * 1.e-12);
         const bool last = early || (global_iter + 1 > max_iters);
         arm_tail(c, last ? 3 : 1, 0u, S_VAL);

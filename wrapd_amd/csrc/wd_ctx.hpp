@@ -28,6 +28,7 @@ enum ScalarSlot {
     S_ALPHA = 32,       // 8: two-loop alpha (logical order)
     S_COEF = 40,        // 8: alpha - beta
     S_DIR = 48, S_DENOM = 49, S_DG = 50, S_VAL = 51, S_EN = 52, S_FLIPS = 53, S_GTP = 54, S_GMAX = 55,
+    S_GS = 56,          // 8: g . s_j of the current two-loop (logical order), kept by its alpha step
     S_GRAM = 64,        // 64: s_i . y_j by slot (8 x 8)
     S_NUM = 128
 };
@@ -229,13 +230,16 @@ constexpr int kTailMaxBlocks = 1024;
 void launch_gather_grad(wrapd_ctx* c, double* out, int mode, int new_slot, const int* list_slots, int nlist,
                         const int* alpha_slots = nullptr, int nalpha = 0);
 // dots of base with vecs into S_RED; post 1 then runs the alpha recurrence, 2 the beta one
-// (SlotList of `slots`), in the launch's last block
+// (SlotList of `slots`), in the launch's last block.  post 3: the beta recurrence and then
+// g . r = base . g + sum coef_j g . s_j (extra = g) into S_DIR for the host, without forming r
 void launch_multidot(wrapd_ctx* c, const double* base, double* const* vecs, int nvec, int post = 0,
-                     const int* slots = nullptr);
+                     const int* slots = nullptr, const double* extra = nullptr);
 void launch_axpy_q(wrapd_ctx* c, const double* g, const int* slots, int n);   // qv = g - sum alpha y
 void launch_axpy_r(wrapd_ctx* c, const double* g, const int* slots, int n);   // rv += sum coef s, dir = g.r
 void launch_dot(wrapd_ctx* c, const double* a, const double* b); // partial slot 0
 void launch_xstep(wrapd_ctx* c, double rate, const double* dirv, double* s_out);
+// the same with the direction r = rv + sum coef_j s_j formed per element (as launch_axpy_r would)
+void launch_xstep_r(wrapd_ctx* c, double rate, const int* slots, int n, double* s_out);
 void launch_diffnorm(wrapd_ctx* c, const double* a, const double* b);   // |a - b|^2 -> S_RED
 void launch_copy(wrapd_ctx* c, double* dst, const double* src, long long n);
 void launch_fill(wrapd_ctx* c, double* dst, double v, long long n);

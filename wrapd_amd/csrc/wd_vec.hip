@@ -262,6 +262,7 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
                 else gij = scal[S_GRAM + LA.slot[i] * kMaxHist + LA.slot[j]];
                 gm[q] = gij;
             }
+            if (static_cast<int>(threadIdx.x) < n) scal[S_GS + threadIdx.x] = sres[kd + threadIdx.x];
             __syncthreads();
             if (threadIdx.x == 0) lbfgs_alpha(scal, sres + kd, gm, sa, n);
         }
@@ -270,17 +271,19 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
     }
 }
 
-// partial dots of `base` with up to 8 vectors
+// partial dots of `base` with up to 8 vectors, and with `extra` (slot nvec) when given
 __global__ void __launch_bounds__(kVecThreads)
-k_multidot(long long n3, const double* __restrict__ base, VecPtrs V, int nvec, double* __restrict__ partials,
-           RedTail tail, double* __restrict__ scal, SlotList L, int post) {
-    __shared__ double shk[kMaxHist][kVecThreads / 64];
-    __shared__ double sres[kMaxHist], gm[kMaxHist * kMaxHist], sa[kMaxHist], sc[kMaxHist];
+k_multidot(long long n3, const double* __restrict__ base, VecPtrs V, int nvec, const double* __restrict__ extra,
+           double* __restrict__ partials, RedTail tail, double* __restrict__ scal, SlotList L, int post) {
+    constexpr int KM = kMaxHist + 1;
+    __shared__ double shk[KM][kVecThreads / 64];
+    __shared__ double sres[KM], gm[kMaxHist * kMaxHist], sa[kMaxHist], sc[kMaxHist], sg[kMaxHist];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const long long i0 = static_cast<long long>(lb) * blockDim.x * 4 + threadIdx.x;
-    double acc[kMaxHist];
+    double acc[KM];
 #pragma unroll
-    for (int j = 0; j < kMaxHist; ++j) acc[j] = 0.0;
+    for (int j = 0; j < KM; ++j) acc[j] = 0.0;
+    double ae = 0.0;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
         const long long i = i0 + static_cast<long long>(u) * blockDim.x;
@@ -289,22 +292,38 @@ k_multidot(long long n3, const double* __restrict__ base, VecPtrs V, int nvec, d
 #pragma unroll
             for (int j = 0; j < kMaxHist; ++j)
                 if (j < nvec) acc[j] += b * V.v[j][i];
+            if (extra) ae += b * extra[i];
         }
     }
-    const double t = block_sums<kVecThreads, kMaxHist>(acc, nvec, shk);
-    if (static_cast<int>(threadIdx.x) < nvec) put_partial(partials, lb, threadIdx.x, t);
+#pragma unroll
+    for (int j = 0; j < KM; ++j)
+        if (j == nvec) acc[j] = ae;
+    const int nk = nvec + (extra ? 1 : 0);
+    const double t = block_sums<kVecThreads, KM>(acc, nk, shk);
+    if (static_cast<int>(threadIdx.x) < nk) put_partial(partials, lb, threadIdx.x, t);
     // the last block also runs the recurrence that consumes the dots (post 1: alpha, 2: beta),
     // with the list's Gram entries (and for beta the alphas) loaded into LDS together
-    if (reduce_tail<kVecThreads, kMaxHist, false>(tail, partials, sres)) {
+    if (reduce_tail<kVecThreads, KM, false>(tail, partials, sres)) {
         if (post) {
             const int n = L.n;
             for (int q = threadIdx.x; q < n * n; q += kVecThreads)
                 gm[q] = scal[S_GRAM + L.slot[q / n] * kMaxHist + L.slot[q % n]];
-            if (post == 2 && static_cast<int>(threadIdx.x) < n) sa[threadIdx.x] = scal[S_ALPHA + threadIdx.x];
+            if (post >= 2 && static_cast<int>(threadIdx.x) < n) sa[threadIdx.x] = scal[S_ALPHA + threadIdx.x];
+            if (post == 3 && static_cast<int>(threadIdx.x) < n) sg[threadIdx.x] = scal[S_GS + threadIdx.x];
+            if (post == 1 && static_cast<int>(threadIdx.x) < n) scal[S_GS + threadIdx.x] = sres[threadIdx.x];
             __syncthreads();
             if (threadIdx.x == 0) {
                 if (post == 1) lbfgs_alpha(scal, sres, gm, sa, n);
                 else lbfgs_beta(scal, sres, gm, sa, sc, n);
+            }
+            if (post == 3) {
+                __syncthreads();
+                if (threadIdx.x == 0) {   // g . r without r: base . g + sum_j coef_j (g . s_j), in order
+                    double dir = sres[n];
+                    for (int j = 0; j < n; ++j) dir += sc[j] * sg[j];
+                    scal[S_DIR] = dir;
+                    __hip_atomic_store(tail.hout - S_RED + S_DIR, dir, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                }
             }
         }
         tail_signal(tail);
@@ -368,6 +387,20 @@ k_xstep(long long n3, double* __restrict__ x, double rate, const double* __restr
     const double xn = xo - rate * dir[i];
     x[i] = xn;
     if (s) s[i] = xn - xo;
+}
+
+// x -= rate r, s = x_new - x_old, with r = rv + sum_j coef_j s_j (k_axpy_r's expression)
+__global__ void __launch_bounds__(kVecThreads)
+k_xstep_r(long long n3, double* __restrict__ x, double rate, const double* __restrict__ rv, VecPtrs S, int n,
+          const double* __restrict__ scal, double* __restrict__ s) {
+    const long long i = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    double v = rv[i];
+    for (int j = 0; j < n; ++j) v += S.v[j][i] * scal[S_COEF + j];   // s may be one of S: read before it is written
+    const double xo = x[i];
+    const double xn = xo - rate * v;
+    x[i] = xn;
+    s[i] = xn - xo;
 }
 
 __global__ void __launch_bounds__(kVecThreads)
@@ -479,15 +512,19 @@ void launch_gather_grad(wrapd_ctx* c, double* out, int mode, int new_pos, const 
                        c->grad_old, ynew, snew, S, Y, L, new_pos, LA, SA, c->partials, t, c->scal);
 }
 
-void launch_multidot(wrapd_ctx* c, const double* base, double* const* vecs, int nvec, int post, const int* slots) {
+void launch_multidot(wrapd_ctx* c, const double* base, double* const* vecs, int nvec, int post, const int* slots,
+                     const double* extra) {
     ProfScope ps(c, "multidot");
     VecPtrs V{};
     for (int j = 0; j < nvec; ++j) V.v[j] = vecs[j];
     const long long n3 = 3ll * c->n_f;
     const int nb = blocks_for(n3, kVecThreads * 4);
-    arm_tail(c, nvec, 0u, S_RED, false);   // read by the next launches only
+    if (post != 3) extra = nullptr;
+    // read by the next launches only, except post 3's S_DIR (stored by the launch itself)
+    arm_tail(c, nvec + (extra ? 1 : 0), 0u, S_RED, post == 3);
+    if (post == 3) c->pending_tail.nstore = 0;
     const SlotList L = post ? mk_list(slots, nvec) : SlotList{};
-    hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kVecThreads), 0, c->stream, n3, base, V, nvec, c->partials,
+    hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kVecThreads), 0, c->stream, n3, base, V, nvec, extra, c->partials,
                        take_tail(c, nb, false), c->scal, L, post);
 }
 
@@ -532,6 +569,15 @@ void launch_xstep(wrapd_ctx* c, double rate, const double* dirv, double* s_out) 
     const long long n3 = 3ll * c->n_f;
     hipLaunchKernelGGL(k_xstep, dim3(blocks_for(n3, kVecThreads)), dim3(kVecThreads), 0, c->stream, n3, c->xall,
                        rate, dirv, s_out);
+}
+
+void launch_xstep_r(wrapd_ctx* c, double rate, const int* slots, int n, double* s_out) {
+    ProfScope ps(c, "xstep");
+    VecPtrs S{};
+    for (int j = 0; j < n; ++j) S.v[j] = c->hS[slots[j]];
+    const long long n3 = 3ll * c->n_f;
+    hipLaunchKernelGGL(k_xstep_r, dim3(blocks_for(n3, kVecThreads)), dim3(kVecThreads), 0, c->stream, n3, c->xall,
+                       rate, c->rv, S, n, c->scal, s_out);
 }
 
 void launch_armijo_x(wrapd_ctx* c, double alpha, const double* p, double* xbuf) {
