@@ -682,8 +682,7 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
             const bool have_alpha = c->alpha_n == it && std::equal(slots, slots + it, c->alpha_slots);
             if (!have_alpha) launch_multidot(c, g, vs, it, 1, slots);   // and the alpha recurrence
             c->alpha_n = -1;
-            launch_axpy_q(c, g, slots, it);
-            launch_solve(c, c->qv, c->rv);
+            launch_solve(c, g, c->rv, slots, it);   // of q = g - sum alpha_j y_j, formed by the solve
             for (int i = 0; i < it; ++i) vs[i] = c->hY[slots[i]];
             if (c->lcfg.linesearch == 1) {
                 launch_multidot(c, c->rv, vs, it, 2, slots);   // and the beta recurrence

@@ -246,7 +246,9 @@ void launch_fill(wrapd_ctx* c, double* dst, double v, long long n);
 void launch_armijo_x(wrapd_ctx* c, double alpha, const double* p, double* xbuf); // xbuf = x + alpha p
 // (wd_solve.hip)
 int chol_prepare_device(wrapd_ctx* c);
-void launch_solve(wrapd_ctx* c, const double* b, double* x);      // n_f x 3 device arrays
+// x = A^{-1} b (n_f x 3 device arrays); with ny > 0 the right-hand side is
+// b - sum_j alpha_j y_{yslots[j]} (alpha in S_ALPHA), SplitLBFGS's q, formed inside the solve
+void launch_solve(wrapd_ctx* c, const double* b, double* x, const int* yslots = nullptr, int ny = 0);
 int solve_aborted(wrapd_ctx* c);   // nonzero once a solve's dependency wait has timed out (syncs)
 // (wd_mlp.hip)
 MlpDevice mlp_device(wrapd_ctx* c);

@@ -154,17 +154,33 @@ __device__ __forceinline__ void copy_row3(const double* __restrict__ src, double
     dst[1] = src[1];
     dst[2] = src[2];
 }
-__device__ __forceinline__ void copy_row3_agent(const double* __restrict__ src, double* __restrict__ dst) {
-    const double a = src[0], b = src[1], c = src[2];
-    st_agent(dst + 0, a);
-    st_agent(dst + 1, b);
-    st_agent(dst + 2, c);
+// The right-hand side as read by the forward solve: b - sum_j alpha_j y_j (n terms, alpha in
+// device memory), formed per row as SplitLBFGS's first loop leaves q (j from n - 1 down),
+// so the two-loop's q never goes through memory.  n = 0: b itself.
+struct SolveRhs {
+    const double* b;
+    const double* alpha;
+    int n;
+    const double* y[kMaxHist];
+};
+__device__ __forceinline__ void rhs_row(const SolveRhs& R, long long row, double out[3]) {
+    const double* bp = R.b + 3 * row;
+    out[0] = bp[0];
+    out[1] = bp[1];
+    out[2] = bp[2];
+    for (int j = R.n - 1; j >= 0; --j) {
+        const double a = R.alpha[j];
+        const double* yp = R.y[j] + 3 * row;
+        out[0] -= a * yp[0];
+        out[1] -= a * yp[1];
+        out[2] -= a * yp[2];
+    }
 }
 // LDS of a leaf-subtree workgroup: its LeafMeta, then the rows (kLeafRowsOff doubles in).
 constexpr int kLeafRowsOff = static_cast<int>((sizeof(LeafMeta) + 15) / 16 * 2);
 __device__ __forceinline__ void fwd_leaf_body(int t, int nleaf, const int* __restrict__ leaf_ptr,
                                               const int* __restrict__ leaf_sn, const SolveMaps& M,
-                                              const double* __restrict__ b, const double* __restrict__ G,
+                                              const SolveRhs& rhs, const double* __restrict__ G,
                                               double* __restrict__ ysol, double* __restrict__ slots,
                                               const int* __restrict__ bcopy, int nbcopy, double* __restrict__ bperm,
                                               double* dyn) {
@@ -180,10 +196,7 @@ __device__ __forceinline__ void fwd_leaf_body(int t, int nleaf, const int* __res
     if (cq0 + static_cast<int>(threadIdx.x) < cq1) {
         const int q = cq0 + threadIdx.x;
         cdst = bcopy[2 * q];
-        const double* src = b + 3ll * bcopy[2 * q + 1];
-        cb[0] = src[0];
-        cb[1] = src[1];
-        cb[2] = src[2];
+        rhs_row(rhs, bcopy[2 * q + 1], cb);
     }
     const int q0 = leaf_ptr[t], q1 = leaf_ptr[t + 1];
     const int first = leaf_sn[q0], nsn = q1 - q0;
@@ -194,8 +207,14 @@ __device__ __forceinline__ void fwd_leaf_body(int t, int nleaf, const int* __res
         st_agent(dst + 1, cb[1]);
         st_agent(dst + 2, cb[2]);
     }
-    for (int q = cq0 + kSolveThreads + threadIdx.x; q < cq1; q += kSolveThreads)
-        copy_row3_agent(b + 3ll * bcopy[2 * q + 1], bperm + 3ll * bcopy[2 * q]);
+    for (int q = cq0 + kSolveThreads + threadIdx.x; q < cq1; q += kSolveThreads) {
+        double t3[3];
+        rhs_row(rhs, bcopy[2 * q + 1], t3);
+        double* dst = bperm + 3ll * bcopy[2 * q];
+        st_agent(dst + 0, t3[0]);
+        st_agent(dst + 1, t3[1]);
+        st_agent(dst + 2, t3[2]);
+    }
     const int64_t ubase = L.uoff[0];
     const int r = threadIdx.x;
     for (int i = 0; i < nsn; ++i) {
@@ -214,9 +233,11 @@ __device__ __forceinline__ void fwd_leaf_body(int t, int nleaf, const int* __res
         if (r < f) {
             double a0 = 0.0, a1 = 0.0, a2 = 0.0;
             if (r < ns) {
-                a0 = b[3 * old + 0];
-                a1 = b[3 * old + 1];
-                a2 = b[3 * old + 2];
+                double t3[3];
+                rhs_row(rhs, old, t3);
+                a0 = t3[0];
+                a1 = t3[1];
+                a2 = t3[2];
             }
 #pragma unroll
             for (int i = 0; i < KA; ++i)
@@ -268,10 +289,13 @@ __device__ __forceinline__ void fwd_leaf_body(int t, int nleaf, const int* __res
 }
 
 // The same copy when there are no leaf subtrees.
-__global__ void k_bperm(const int* __restrict__ bcopy, int nbcopy, const double* __restrict__ b,
-                        double* __restrict__ bperm) {
+__global__ void k_bperm(const int* __restrict__ bcopy, int nbcopy, SolveRhs rhs, double* __restrict__ bperm) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < nbcopy) copy_row3(b + 3ll * bcopy[2 * q + 1], bperm + 3ll * bcopy[2 * q]);
+    if (q < nbcopy) {
+        double t3[3];
+        rhs_row(rhs, bcopy[2 * q + 1], t3);
+        copy_row3(t3, bperm + 3ll * bcopy[2 * q]);
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -504,12 +528,12 @@ template <int B>
 __global__ void __launch_bounds__(kSolveThreads)
 k_fwd_solve(int nleaf, const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_sn,
             const int* __restrict__ bcopy, int nbcopy, const SolveTask* __restrict__ tasks, int* abortf, SolveMaps M,
-            const double* __restrict__ b, const double* __restrict__ G, double* __restrict__ ysol,
+            SolveRhs rhs, const double* __restrict__ G, double* __restrict__ ysol,
             double* __restrict__ xsol, double* __restrict__ slots, double* __restrict__ bperm) {
     extern __shared__ double dyn[];
     __shared__ int sh_ok;
     if (static_cast<int>(blockIdx.x) < nleaf)
-        fwd_leaf_body(blockIdx.x, nleaf, leaf_ptr, leaf_sn, M, b, G, ysol, slots, bcopy, nbcopy, bperm, dyn);
+        fwd_leaf_body(blockIdx.x, nleaf, leaf_ptr, leaf_sn, M, rhs, G, ysol, slots, bcopy, nbcopy, bperm, dyn);
     else
         fwd_flow_body<B>(blockIdx.x - nleaf, tasks, abortf, M, bperm, G, ysol, xsol, slots, dyn, sh_ok);
 }
@@ -1016,22 +1040,27 @@ int chol_prepare_device(wrapd_ctx* c) {
     return rc ? WRAPD_ERR_HIP : WRAPD_OK;
 }
 
-void launch_solve(wrapd_ctx* c, const double* b, double* x) {
+void launch_solve(wrapd_ctx* c, const double* b, double* x, const int* yslots, int ny) {
     const wrapd::CholSymbolic& S = c->sym;
     SolveMaps M{c->d_col0, c->d_fsize, c->d_rowptr, c->d_rows, c->d_goff, c->d_uoff, c->d_asm_ptr, c->d_asm_idx,
                 c->d_perm, c->d_udest};
     const int nleaf = static_cast<int>(S.leaf_ptr.size()) - 1;
     const SolveTask* ft = static_cast<const SolveTask*>(c->d_flow_f);
     const SolveTask* bt = static_cast<const SolveTask*>(c->d_flow_b);
+    SolveRhs rhs{};
+    rhs.b = b;
+    rhs.alpha = c->scal + S_ALPHA;
+    rhs.n = ny;
+    for (int j = 0; j < ny; ++j) rhs.y[j] = c->hY[yslots[j]];
     {
         ProfScope ps(c, "solve_fwd");
         if (nleaf == 0 && c->n_bcopy > 0)
             hipLaunchKernelGGL(k_bperm, dim3((c->n_bcopy + 255) / 256), dim3(256), 0, c->stream, c->d_bcopy, c->n_bcopy,
-                               b, c->bperm);
+                               rhs, c->bperm);
         const size_t lds = std::max<size_t>(c->n_flow_f > 0 ? kFwdFlowLds : 0, nleaf > 0 ? c->leaf_lds_fwd : 0);
         if (nleaf + c->n_flow_f > 0)
             hipLaunchKernelGGL(k_fwd_solve<kFwdPrefetch>, dim3(nleaf + c->n_flow_f), dim3(kSolveThreads), lds, c->stream,
-                               nleaf, c->d_leaf_ptr, c->d_leaf_sn, c->d_bcopy, c->n_bcopy, ft, c->d_flow_abort, M, b,
+                               nleaf, c->d_leaf_ptr, c->d_leaf_sn, c->d_bcopy, c->n_bcopy, ft, c->d_flow_abort, M, rhs,
                                c->G, c->ysol, c->xsol, c->ubuf, c->bperm);
     }
     {
