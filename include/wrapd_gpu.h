@@ -102,6 +102,11 @@ int wrapd_get_x_free(wrapd_ctx* ctx, double* x_free);
 /* |x_free - x_ref| for the x-error log (src/AlgorithmData.cpp:63, 70); x_ref (n_free,3) */
 int wrapd_set_x_ref(wrapd_ctx* ctx, const double* x_ref);
 int wrapd_x_distance(wrapd_ctx* ctx, double* dist);
+/* The same in two halves (no reference counterpart): the launch returns at once, the result
+ * waits for that launch's reduction.  The ADMM driver logs the distance this way so that the
+ * next iteration's kernels are queued before the host waits for it. */
+int wrapd_x_distance_launch(wrapd_ctx* ctx);
+int wrapd_x_distance_result(wrapd_ctx* ctx, double* dist);
 
 /* Settings::m_rot_awareness (src/Settings.hpp:107-111): selects what the deformation cache
  * holds (the symmetric polar factor, or F^T for the rotation-unaware local step). */

@@ -70,6 +70,8 @@ _PROTOS = {
     "wrapd_get_x_free": (C.c_int, [P, D]),
     "wrapd_set_x_ref": (C.c_int, [P, D]),
     "wrapd_x_distance": (C.c_int, [P, D]),
+    "wrapd_x_distance_launch": (C.c_int, [P]),
+    "wrapd_x_distance_result": (C.c_int, [P, D]),
     "wrapd_set_rot_aware": (C.c_int, [P, C.c_int]),
     "wrapd_init_duals": (C.c_int, [P, C.c_int, D]),
     "wrapd_defo_energy": (C.c_int, [P, D, I]),

@@ -237,6 +237,8 @@ class TetElements {
     void global_step_unaware(wrapd_global_out& out);
     void reset_history();
     double x_distance();
+    void x_distance_launch();     // x_distance in two halves: the launch returns at once
+    double x_distance_result();
     void set_x_ref(const math::MatX3& x);
     const wrapd_local_out& last_local() const { return m_last_local; }
     void set_fp64_tail(bool on) { m_fp64_tail = on; }
@@ -323,12 +325,18 @@ class AlgorithmData {
     }
     const std::vector<double>& per_iter_objectives() const { return m_objectives; }
     const std::vector<int>& per_iter_reweighted() const { return m_reweighted; }
-    const std::vector<double>& per_iter_state_x_errors() const { return m_state_x_errors; }
+    const std::vector<double>& per_iter_state_x_errors() const {
+        resolve_x_error();
+        return m_state_x_errors;
+    }
     const std::vector<double>& per_iter_accumulated_time_s() const { return m_accumulated_time_s; }
     const std::vector<double>& per_iter_wall_time_s() const { return m_wall_time_s; }
     const std::vector<int>& per_iter_inner_iters() const { return m_per_iter_inner_iters; }
     const std::vector<int>& per_iter_flips() const { return m_flips_count; }
-    double current_normalized_x_error() const { return m_state_x_errors[m_iter]; }
+    double current_normalized_x_error() const {
+        resolve_x_error();
+        return m_state_x_errors[m_iter];
+    }
     double current_normalized_objective_residual() const { return m_norm_obj_residuals[m_iter - 1]; }
     bool no_recent_flips() const { return m_flips_count[m_iter] == 0 && m_flips_count[m_iter - 1] == 0; }
     void print_initial_data() const;
@@ -356,8 +364,14 @@ class AlgorithmData {
     const Settings m_settings;
     std::shared_ptr<System> m_system;
     int m_n_free = 0;
-    std::vector<double> m_objectives, m_state_x_errors, m_accumulated_time_s, m_wall_time_s;
+    std::vector<double> m_objectives, m_accumulated_time_s, m_wall_time_s;
     std::vector<int> m_reweighted, m_flips_count;
+    // Without a reference solution the x-error log only records, so each iteration's entry
+    // is read back at the next iteration (or by its first reader): the host does not wait
+    // for it before queuing the next iteration's kernels.
+    mutable std::vector<double> m_state_x_errors;
+    mutable int m_xerr_pending = -1;   // the entry still to be read, or -1
+    void resolve_x_error() const;
 };
 
 // src/GlobalStepBehavior.hpp:76-268

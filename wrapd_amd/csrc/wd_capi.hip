@@ -107,22 +107,27 @@ static int d2h(wrapd_ctx* c, T* h, const T* d, size_t n) {
 // the reduction that produces it, to the pinned copy too, followed by that reduction's
 // epoch in host_seq: the host waits for the latest epoch by polling host memory, and falls
 // back to a stream synchronize (which also reports a failed launch) after about 50 ms.
-static const double* scal_read(wrapd_ctx* c, int at, int n) {
-    (void)n;
-    const unsigned want = c->host_epoch;
+// `want`: wait for that reduction (host_seq is the epoch of the last one stored; epochs only
+// grow, so a later one implies it).
+static const double* scal_read_epoch(wrapd_ctx* c, int at, unsigned want) {
+    auto arrived = [&] { return static_cast<int>(__atomic_load_n(c->host_seq, __ATOMIC_ACQUIRE) - want) >= 0; };
     const auto t0 = std::chrono::steady_clock::now();
     for (unsigned spins = 0;; ++spins) {
-        if (__atomic_load_n(c->host_seq, __ATOMIC_ACQUIRE) == want) return c->scal_host + at;
+        if (arrived()) return c->scal_host + at;
         if ((spins & 1023) == 1023 &&
             std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50))
             break;
     }
     if (check_hip(c, hipStreamSynchronize(c->stream), "sync")) return nullptr;
-    if (__atomic_load_n(c->host_seq, __ATOMIC_ACQUIRE) != want) {
+    if (!arrived()) {
         c->last_error = "device scalars: the reduction's host copy did not arrive";
         return nullptr;
     }
     return c->scal_host + at;
+}
+static const double* scal_read(wrapd_ctx* c, int at, int n) {
+    (void)n;
+    return scal_read_epoch(c, at, c->host_epoch);
 }
 
 static int launch_status(wrapd_ctx* c) {
@@ -413,13 +418,25 @@ int wrapd_set_x_ref(wrapd_ctx* c, const double* x) {
 }
 
 int wrapd_x_distance(wrapd_ctx* c, double* dist) {
+    int st = wrapd_x_distance_launch(c);
+    if (st != WRAPD_OK) return st;
+    return wrapd_x_distance_result(c, dist);
+}
+
+int wrapd_x_distance_launch(wrapd_ctx* c) {
     CTX_OK(c); MESH_OK(c);
     if (!c->xref) { c->last_error = "no reference positions"; return WRAPD_ERR_STATE; }
-    launch_diffnorm(c, c->xall, c->xref);
-    const double* r = scal_read(c, S_RED, 1);
+    launch_diffnorm(c, c->xall, c->xref, S_XD);
+    c->xd_epoch = c->host_epoch;
+    return launch_status(c);
+}
+
+int wrapd_x_distance_result(wrapd_ctx* c, double* dist) {
+    CTX_OK(c); MESH_OK(c);
+    const double* r = scal_read_epoch(c, S_XD, c->xd_epoch);
     if (!r) return WRAPD_ERR_HIP;
     if (dist) *dist = std::sqrt(r[0]);
-    return launch_status(c);
+    return WRAPD_OK;
 }
 
 int wrapd_set_rot_aware(wrapd_ctx* c, int rot_aware) {

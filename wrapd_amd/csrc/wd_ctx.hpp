@@ -24,13 +24,14 @@ constexpr int kMaxBlocks = wd::kTailBlocks;   // capacity of the partials buffer
 
 // device scalar slots (ctx->scal)
 enum ScalarSlot {
-    S_RED = 0,          // 0..23: result of the last reduction
+    S_RED = 0,          // 0..31: result of the last reduction
     S_ALPHA = 32,       // 8: two-loop alpha (logical order)
     S_COEF = 40,        // 8: alpha - beta
     S_DIR = 48, S_DENOM = 49, S_DG = 50, S_VAL = 51, S_EN = 52, S_FLIPS = 53, S_GTP = 54, S_GMAX = 55,
     S_GS = 56,          // 8: g . s_j of the current two-loop (logical order), kept by its alpha step
     S_GRAM = 64,        // 64: s_i . y_j by slot (8 x 8)
-    S_NUM = 128
+    S_XD = 128,         // |x - x_ref|^2 of the last x-distance launch
+    S_NUM = 136
 };
 
 // defo-pass mode bits
@@ -166,6 +167,7 @@ struct wrapd_ctx {
     unsigned* d_red_flags = nullptr;   // per block of a reducing launch: epoch when its partials are stored
     int* d_red_abort = nullptr;        // sticky: a reduction's wait ran out
     unsigned red_epoch = 0;
+    unsigned xd_epoch = 0;             // the epoch of the last x-distance reduction (its host copy: S_XD)
     int alpha_n = -1;                  // S_ALPHA holds the alphas of this list for the gradient in grad (-1: none)
     int alpha_slots[wd::kMaxHist] = {};
     unsigned host_epoch = 0;           // the epoch of the last reduction with a host copy (what the host waits for)
@@ -240,7 +242,7 @@ void launch_dot(wrapd_ctx* c, const double* a, const double* b); // partial slot
 void launch_xstep(wrapd_ctx* c, double rate, const double* dirv, double* s_out);
 // the same with the direction r = rv + sum coef_j s_j formed per element (as launch_axpy_r would)
 void launch_xstep_r(wrapd_ctx* c, double rate, const int* slots, int n, double* s_out);
-void launch_diffnorm(wrapd_ctx* c, const double* a, const double* b);   // |a - b|^2 -> S_RED
+void launch_diffnorm(wrapd_ctx* c, const double* a, const double* b, int dst = S_RED);   // |a - b|^2 -> dst
 void launch_copy(wrapd_ctx* c, double* dst, const double* src, long long n);
 void launch_fill(wrapd_ctx* c, double* dst, double v, long long n);
 void launch_armijo_x(wrapd_ctx* c, double alpha, const double* p, double* xbuf); // xbuf = x + alpha p

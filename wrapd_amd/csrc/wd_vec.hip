@@ -556,10 +556,10 @@ void launch_dot(wrapd_ctx* c, const double* a, const double* b) {
     hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kVecThreads), 0, c->stream, n3, a, b, c->partials, take_tail(c, nb, false));
 }
 
-void launch_diffnorm(wrapd_ctx* c, const double* a, const double* b) {
+void launch_diffnorm(wrapd_ctx* c, const double* a, const double* b, int dst) {
     const long long n3 = 3ll * c->n_f;
     const int nb = blocks_for(n3, kVecThreads);
-    arm_tail(c, 1, 0u, S_RED);
+    arm_tail(c, 1, 0u, dst);
     hipLaunchKernelGGL(k_diffnorm, dim3(nb), dim3(kVecThreads), 0, c->stream, n3, a, b, c->partials,
                        take_tail(c, nb, false));
 }
