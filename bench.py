@@ -66,11 +66,14 @@ def algorithmic_bytes(prob, stats):
     gather = n_e * 96 + 4 * n_e * 4 + 8 * 3 * n_f
     g = 8.0 * stats["g_entries"]
     vec = 8 * 3 * n_f
+    m = int(prob.settings.get("lbfgs_m") or 0)
     return {
         "local_step": local,
         "defo_grad": defo,
         "gather_grad": gather,
-        "solve_fwd": g + 3 * vec,      # G once, b and the update rows
+        # G once, b (the gradient), y and the update rows, and the m history vectors y_j
+        # from which the forward solve forms the two-loop's q = g - sum alpha_j y_j
+        "solve_fwd": g + (3 + m) * vec,
         "solve_bwd": g + 3 * vec,      # G once, y, x
     }
 

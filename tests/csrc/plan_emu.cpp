@@ -83,7 +83,7 @@ extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
             const int64_t fo = PL.foff[s], go = S.goff[s];
             const int k = L.panel, c0 = kFactorTile * k, nb = std::min(kFactorTile, ns - c0);
             auto Fi = [&](int r, int c) { return fo + static_cast<int64_t>(c) * f + r; };
-            auto Gi = [&](int r, int c) { return go + static_cast<int64_t>(c) * f + r; };
+            auto Gi = [&](int r, int c) { return go + static_cast<int64_t>(c) * g_ld(f) + r; };
             if (L.kind == FK_ASSEMBLE) {
                 const int j = y;
                 const int64_t gc = PL.cbase[s] + j;

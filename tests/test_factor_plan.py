@@ -98,7 +98,8 @@ def test_plan_matches_dense_cholesky(emu, name):
         assert np.array_equal(rows[:ns], np.arange(c0, c1))
         Linv = np.linalg.inv(L[c0:c1, c0:c1])
         expect = np.vstack([Linv, -L[np.ix_(rows[ns:], np.arange(c0, c1))] @ Linv])
-        got = R["G"][R["goff"][s]:R["goff"][s + 1]].reshape(ns, f).T
+        ld = (f + 15) // 16 * 16   # G's leading dimension: the front size rounded up to 16 rows
+        got = R["G"][R["goff"][s]:R["goff"][s + 1]].reshape(ns, ld).T[:f]
         worst = max(worst, float(np.max(np.abs(got - expect)) / np.max(np.abs(expect))))
     assert worst < 1e-12, worst
     # every factor entry outside the supernodes' structure is zero

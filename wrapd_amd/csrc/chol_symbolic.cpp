@@ -319,7 +319,7 @@ void chol_analyze(int n, const std::vector<int>& Ap, const std::vector<int>& Aj,
         int ns = c0[s + 1] - c0[s];
         int f = S.fsize[s];
         int m = f - ns;
-        S.goff[s + 1] = S.goff[s] + static_cast<int64_t>(f) * ns;
+        S.goff[s + 1] = S.goff[s] + static_cast<int64_t>(g_ld(f)) * ns;
         S.uoff[s + 1] = S.uoff[s] + m;
         S.relptr[s + 1] = S.relptr[s] + m;
         S.max_front = std::max(S.max_front, f);

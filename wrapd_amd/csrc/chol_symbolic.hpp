@@ -53,6 +53,11 @@ struct CholSymbolic {
     std::vector<int> rel;         // position of each off-diagonal row of s inside parent's front
 };
 
+// G_s is stored column-major with its front size f rounded up to 16 rows (128 B): tasks
+// that read 16-row segments of a column then touch whole cache lines.
+constexpr int kGAlign = 16;
+constexpr int g_ld(int f) { return (f + kGAlign - 1) / kGAlign * kGAlign; }
+
 struct SymbolicOptions {
     int leaf_size = 48;          // nested-dissection leaf size
     int ns_max = 1024;           // widest supernode after splitting

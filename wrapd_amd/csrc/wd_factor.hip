@@ -68,7 +68,7 @@ k_fassemble(const int2* __restrict__ tasks, FactArgs A, const double* __restrict
     double* F = A.pool + A.foff[s] + static_cast<int64_t>(j) * f;
     for (int r = j + lane; r < f; r += 64) F[r] = 0.0;
     if (j < ns) {   // L11^{-1} rows only: G's bottom rows are assigned whole later
-        double* Gc = A.G + A.goff[s] + static_cast<int64_t>(j) * f;
+        double* Gc = A.G + A.goff[s] + static_cast<int64_t>(j) * wrapd::g_ld(f);
         for (int r = lane; r < ns; r += 64) Gc[r] = 0.0;
     }
     __syncthreads();
@@ -216,12 +216,13 @@ k_ftrsm(const int2* __restrict__ tasks, FactArgs A, int k) {
     const int nb = min(kT, ns - c0);
     double* F = A.pool + A.foff[s];
     double* Gs = A.G + A.goff[s];
+    const int lg = wrapd::g_ld(f);
     int t0, nt;
     front_tile(t, ns, f, t0, nt);
     if (kind == 1 && t == k) {   // X[k, k] = L_kk^{-1}
         for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
             const int j = idx >> 6, i = idx & 63;
-            if (j < nb && i < nb) Gs[static_cast<int64_t>(c0 + j) * f + c0 + i] = linv_at(F, f, c0, i, j);
+            if (j < nb && i < nb) Gs[static_cast<int64_t>(c0 + j) * lg + c0 + i] = linv_at(F, f, c0, i, j);
         }
         return;
     }
@@ -240,7 +241,7 @@ k_ftrsm(const int2* __restrict__ tasks, FactArgs A, int k) {
         }
         for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
             const int j = idx >> 6, q = idx & 63;
-            Bs[q * 64 + j] = (q < nb && j < nt) ? Gs[static_cast<int64_t>(t0 + j) * f + c0 + q] : 0.0;
+            Bs[q * 64 + j] = (q < nb && j < nt) ? Gs[static_cast<int64_t>(t0 + j) * lg + c0 + q] : 0.0;
         }
     }
     __syncthreads();
@@ -253,7 +254,7 @@ k_ftrsm(const int2* __restrict__ tasks, FactArgs A, int k) {
         });
     } else {
         tile_for_each(acc, [&](int i, int j, double v) {
-            if (i < nb && j < nt) Gs[static_cast<int64_t>(t0 + j) * f + c0 + i] = v;
+            if (i < nb && j < nt) Gs[static_cast<int64_t>(t0 + j) * lg + c0 + i] = v;
         });
     }
 }
@@ -278,6 +279,7 @@ k_fupdate(const int2* __restrict__ tasks, FactArgs A, int k) {
     front_tile(b, ns, f, b0, nbt);
     double* F = A.pool + A.foff[s];
     double* Gs = A.G + A.goff[s];
+    const int lg = wrapd::g_ld(f);
     for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
         const int kk = idx >> 6, i = idx & 63;
         As[idx] = (kk < nb && i < na) ? F[static_cast<int64_t>(c0 + kk) * f + a0 + i] : 0.0;
@@ -290,7 +292,7 @@ k_fupdate(const int2* __restrict__ tasks, FactArgs A, int k) {
     } else {
         for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
             const int j = idx >> 6, kk = idx & 63;
-            Bs[kk * 64 + j] = (kk < nb && j < nbt) ? Gs[static_cast<int64_t>(b0 + j) * f + c0 + kk] : 0.0;
+            Bs[kk * 64 + j] = (kk < nb && j < nbt) ? Gs[static_cast<int64_t>(b0 + j) * lg + c0 + kk] : 0.0;
         }
     }
     __syncthreads();
@@ -298,9 +300,10 @@ k_fupdate(const int2* __restrict__ tasks, FactArgs A, int k) {
     tile_zero(acc);
     tile_mma(As, Bs, (nb + 3) & ~3, acc);
     double* C = kind == 0 ? F : Gs;
+    const int ldc = kind == 0 ? f : lg;
     const bool diag = kind == 0 && a == b;
     tile_for_each(acc, [&](int i, int j, double v) {
-        if (i < na && j < nbt && !(diag && i < j)) C[static_cast<int64_t>(b0 + j) * f + a0 + i] -= v;
+        if (i < na && j < nbt && !(diag && i < j)) C[static_cast<int64_t>(b0 + j) * ldc + a0 + i] -= v;
     });
 }
 
@@ -326,6 +329,7 @@ k_ffinal(const int2* __restrict__ tasks, FactArgs A) {
     const double* F = A.pool + A.foff[s];
     double* Fw = A.pool + A.foff[s];
     double* Gs = A.G + A.goff[s];
+    const int lg = wrapd::g_ld(f);
     // chunk q+1 is fetched into registers while chunk q runs on the matrix cores
     constexpr int kPer = kT * kT / kFactThreads;   // 16 entries per thread and operand
     double ra[kPer], rb[kPer];
@@ -341,7 +345,7 @@ k_ffinal(const int2* __restrict__ tasks, FactArgs A) {
                 rb[r] = (kk < nq && i < nbt) ? F[static_cast<int64_t>(q0 + kk) * f + b0 + i] : 0.0;
             } else {   // transposed: consecutive threads read consecutive rows kk of G
                 const int j = idx >> 6, k2 = idx & 63;
-                rb[r] = (k2 < nq && j < nbt) ? Gs[static_cast<int64_t>(b0 + j) * f + q0 + k2] : 0.0;
+                rb[r] = (k2 < nq && j < nbt) ? Gs[static_cast<int64_t>(b0 + j) * lg + q0 + k2] : 0.0;
             }
         }
     };
@@ -370,7 +374,7 @@ k_ffinal(const int2* __restrict__ tasks, FactArgs A) {
         });
     } else {
         tile_for_each(acc, [&](int i, int j, double v) {
-            if (i < na && j < nbt) Gs[static_cast<int64_t>(b0 + j) * f + a0 + i] = -v;
+            if (i < na && j < nbt) Gs[static_cast<int64_t>(b0 + j) * lg + a0 + i] = -v;
         });
     }
 }

@@ -223,8 +223,9 @@ __device__ __forceinline__ void fwd_leaf_body(int t, int nleaf, const int* __res
         const int jend = r < ns ? r + 1 : (r < f ? ns : 0);   // L11^{-1} is lower triangular
         const double* gp = G + go + r;
         double gv[B];
+        const int lg = wrapd::g_ld(f);
 #pragma unroll
-        for (int u = 0; u < B; ++u) gv[u] = u < jend ? gp[static_cast<int64_t>(u) * f] : 0.0;
+        for (int u = 0; u < B; ++u) gv[u] = u < jend ? gp[static_cast<int64_t>(u) * lg] : 0.0;
         const int old = r < ns ? M.perm[c0 + r] : 0;
         const int64_t k0 = r < f ? M.asm_ptr[rp + r] : 0, k1 = r < f ? M.asm_ptr[rp + r + 1] : 0;
         int64_t ai[KA];
@@ -267,7 +268,7 @@ __device__ __forceinline__ void fwd_leaf_body(int t, int nleaf, const int* __res
                     t1 += gv[u] * cv[3 * u + 1];
                     t2 += gv[u] * cv[3 * u + 2];
                 }
-            if (jend > B) strided_dot3(gp + static_cast<int64_t>(B) * f, f, cv + 3 * B, 3, jend - B, t0, t1, t2);
+            if (jend > B) strided_dot3(gp + static_cast<int64_t>(B) * lg, lg, cv + 3 * B, 3, jend - B, t0, t1, t2);
             if (r < ns) {
                 double* y = ysol + 3ll * (c0 + r);
                 y[0] = t0; y[1] = t1; y[2] = t2;
@@ -386,8 +387,9 @@ __device__ __forceinline__ void fwd_flow_body(int ti, const SolveTask* __restric
     load_window(0);
     const int jend = r < ns ? r + 1 : ns;   // L11^{-1} is lower triangular
     const int nterm = (r < f && g < jend) ? (jend - g + ng - 1) / ng : 0;
-    const double* gp = G + T.goff + r + static_cast<int64_t>(g) * f;
-    const int64_t gstride = static_cast<int64_t>(ng) * f;
+    const int lg = wrapd::g_ld(f);
+    const double* gp = G + T.goff + r + static_cast<int64_t>(g) * lg;
+    const int64_t gstride = static_cast<int64_t>(ng) * lg;
     double gv[B];
 #pragma unroll
     for (int u = 0; u < B; ++u) gv[u] = u < nterm ? gp[u * gstride] : 0.0;
@@ -557,6 +559,7 @@ __device__ __forceinline__ void bwd_flow_body(int ti, const SolveTask* __restric
     const int jfirst = T.first;
     const int rbase = jfirst - jfirst % 64;   // rows above jfirst are zero
     const double* Gs = G + T.goff;
+    const int lg = wrapd::g_ld(f);
     double gv[CPW][B];
     {
         const int rc = min(kBwdChunk, f - rbase);
@@ -569,7 +572,7 @@ __device__ __forceinline__ void bwd_flow_body(int ti, const SolveTask* __restric
                 const int r1 = (j > rbase ? j - rbase : 0) + lane;
                 if (r1 < rc) {
                     n = (rc - r1 + 63) / 64;
-                    gc = Gs + static_cast<int64_t>(j) * f + rbase + r1;
+                    gc = Gs + static_cast<int64_t>(j) * lg + rbase + r1;
                 }
             }
 #pragma unroll
@@ -668,7 +671,7 @@ __device__ __forceinline__ void bwd_flow_body(int ti, const SolveTask* __restric
             double gb[CPW][BB];
 #pragma unroll
             for (int jj = 0; jj < CPW; ++jj) {
-                const double* gc = Gs + static_cast<int64_t>(jfirst + w * CPW + jj) * f + r0 + r1[jj];
+                const double* gc = Gs + static_cast<int64_t>(jfirst + w * CPW + jj) * lg + r0 + r1[jj];
 #pragma unroll
                 for (int u = 0; u < BB; ++u) gb[jj][u] = u0 + u < n[jj] ? gc[64 * (u0 + u)] : 0.0;
             }
@@ -760,6 +763,7 @@ __device__ __forceinline__ void bwd_leaf_body(int t, const int* __restrict__ lea
         const int c0 = L.col0[i], ns = L.col0[i + 1] - c0, f = L.fsize[i];
         const int64_t go = L.goff[i];
         const double* Gs = G + go;
+        const int lg = wrapd::g_ld(f);
         double gv[KC][MR];
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
@@ -767,7 +771,7 @@ __device__ __forceinline__ void bwd_leaf_body(int t, const int* __restrict__ lea
 #pragma unroll
             for (int m = 0; m < MR; ++m) {
                 const int rr = j + lane + 64 * m;
-                gv[k][m] = (j < ns && rr < f) ? Gs[static_cast<int64_t>(j) * f + rr] : 0.0;
+                gv[k][m] = (j < ns && rr < f) ? Gs[static_cast<int64_t>(j) * lg + rr] : 0.0;
             }
         }
         int prow = -1;   // the next (lower) supernode's row source, loaded ahead
@@ -797,7 +801,7 @@ __device__ __forceinline__ void bwd_leaf_body(int t, const int* __restrict__ lea
 #pragma unroll
                     for (int m = 0; m < MR; ++m) {
                         const int rr = j + lane + 64 * m;
-                        gv[k][m] = (j < ns && rr < f) ? Gs[static_cast<int64_t>(j) * f + rr] : 0.0;
+                        gv[k][m] = (j < ns && rr < f) ? Gs[static_cast<int64_t>(j) * lg + rr] : 0.0;
                     }
                 }
             }
