@@ -105,32 +105,52 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
 }
 
 // ---------------------------------------------------------------------------------
-// Reductions finished in the producing launch.  Every block stores its partials
-// (slot k of block lb at partials[k * kTailBlocks + lb]: slot-major, so the reading block's
-// loads are contiguous) at agent scope and then its flag;
-// the block dispatched last (blockIdx.x == gridDim.x - 1: every other block is already
-// running, and none waits for anything) waits for all flags and sums the partials in
-// the order of a 256-thread finalize: lane v takes blocks v, v + 256, ...; lanes are
-// summed in waves of 64 and the 4 waves in order.  So the result is the one a separate
-// one-block finalize launch would produce, without the launch.
+// Values handed from one workgroup to another in the same launch are stored and loaded at
+// agent scope (written through to memory, read past this XCD's L2).  Where the reader
+// polls the values themselves, they hold kPendingBits until written: a signalling NaN with
+// a payload, which arithmetic never yields (it only yields quiet NaNs).  Each 8-byte word
+// is written once per use, so a word that is not pending is final.
+constexpr unsigned long long kPendingBits = 0xFFF40000FFF40000ull;
+__device__ __forceinline__ bool pending(double v) {
+    return static_cast<unsigned long long>(__double_as_longlong(v)) == kPendingBits;
+}
+__device__ __forceinline__ double pending_value() { return __longlong_as_double(static_cast<long long>(kPendingBits)); }
+__device__ __forceinline__ double ld_agent(const double* p) {
+    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------------
+// Reductions finished in the producing launch.  Every block stores its partials (slot k
+// of block lb at partials[k * kTailBlocks + lb]: slot-major, so the reading block's loads
+// are contiguous) at agent scope, exactly the R.nslots slots the reduction reads.  The
+// block dispatched last (blockIdx.x == gridDim.x - 1: every other block is already
+// running, and none waits for anything) polls the partials until none is pending, sets
+// them back to pending for the next reduction, and sums them in the order of a 256-thread
+// finalize: lane v takes blocks v, v + 256, ...; lanes are summed in waves of 64 and the 4
+// waves in order.  So the result is the one a separate one-block finalize launch would
+// produce, without the launch (and without a done-flag per block, which would cost its
+// writer a store completion and the reader a round trip).
 constexpr int kTailSlots = 32;        // reduction slots per block
 constexpr int kTailBlocks = 1 << 16;  // blocks per reducing launch (the partials' row length)
+constexpr int kTailSpinLimit = 1 << 20;
 
 struct RedTail {
-    unsigned* flags = nullptr;   // per block; nullptr: the caller finalizes in a launch of its own
+    bool armed = false;          // finished in this launch; otherwise (with nslots) by a finalize launch
     int* abortf = nullptr;       // sticky: a wait ran out (the results are then NaN)
     double* out = nullptr;       // nslots results
     double* hout = nullptr;      // their copy in host memory (pinned), or nullptr when the host does not read them
     unsigned* hseq = nullptr;    // host memory: set to epoch once the copies (and any work after) are stored, or nullptr
     unsigned epoch = 0;
-    int nslots = 0;
+    int nslots = 0;              // the slots every block stores (0: the launch reduces nothing)
     int nstore = 0;              // the leading slots stored to out (and hout); the rest only to shres
     unsigned maxmask = 0;        // slots reduced with max instead of sum
 };
 
 __device__ __forceinline__ void put_partial(double* partials, int lb, int k, double v) {
-    __hip_atomic_store(partials + static_cast<long long>(k) * kTailBlocks + lb, v, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    st_agent(partials + static_cast<long long>(k) * kTailBlocks + lb, v);
 }
 
 // Tells the host that the host copies of this launch are stored: thread 0 of the block that
@@ -147,46 +167,23 @@ __device__ __forceinline__ void tail_signal(const RedTail R) {
 // reduces (R.nslots <= KS).  SIGNAL: call tail_signal here (false: the caller does, after
 // work of its own that the host reads).
 template <int NT, int KS, bool SIGNAL = true>
-__device__ __forceinline__ bool reduce_tail(const RedTail R, const double* partials, double* shres = nullptr) {
+__device__ __forceinline__ bool reduce_tail(const RedTail R, double* partials, double* shres = nullptr) {
     static_assert(NT == 128 || NT == 256, "reduce_tail: 128 or 256 threads");
     static_assert(KS >= 1 && KS <= kTailSlots, "reduce_tail: slot count");
-    if (!R.flags) return false;
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
+    if (!R.armed) return false;
     const int nblk = gridDim.x;
-    if (blockIdx.x != nblk - 1) {
-        if (threadIdx.x == 0) __hip_atomic_store(R.flags + blockIdx.x, R.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return false;
-    }
-    bool bad = false;
-    for (int polls = 1;; ++polls) {
-        bool mine = true;
-        for (int b0 = threadIdx.x; b0 < nblk - 1; b0 += 8 * NT) {
-            unsigned v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int b = b0 + u * NT;
-                v[u] = b < nblk - 1 ? __hip_atomic_load(R.flags + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : R.epoch;
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) mine = mine && v[u] == R.epoch;
-        }
-        if (__syncthreads_and(mine)) break;
-        const bool stop = polls >= (1 << 20) ||
-                          __hip_atomic_load(R.abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-        if (__syncthreads_or(stop)) {
-            if (threadIdx.x == 0) __hip_atomic_store(R.abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            bad = true;
-            break;
-        }
-        __builtin_amdgcn_s_sleep(2);
-    }
+    if (blockIdx.x != nblk - 1) return false;
+    __shared__ int sbad;
+    if (threadIdx.x == 0) sbad = 0;
     // The partials, at agent scope (past this XCD's L2), BB blocks' slots per batch.
     constexpr int BB = KS > 8 ? 2 : 4;
     constexpr int V = 256 / NT;   // lanes of the 256-lane order per thread
     __shared__ double wsum[KS][4];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int ns = R.nslots;
+    const double pv = pending_value();
+    bool bad = false;
+    __syncthreads();
 #pragma unroll
     for (int h = 0; h < V; ++h) {
         double acc[KS];
@@ -194,22 +191,48 @@ __device__ __forceinline__ bool reduce_tail(const RedTail R, const double* parti
         for (int k = 0; k < KS; ++k) acc[k] = ((R.maxmask >> k) & 1u) ? -1e308 : 0.0;
         for (int b0 = threadIdx.x + h * NT; b0 < nblk; b0 += 256 * BB) {
             double v[BB][KS];
+            auto load = [&](bool only_pending) {
+#pragma unroll
+                for (int u = 0; u < BB; ++u) {
+                    const int b = b0 + u * 256;
+#pragma unroll
+                    for (int k = 0; k < KS; ++k)
+                        if (b < nblk && k < ns && (!only_pending || pending(v[u][k])))
+                            v[u][k] = ld_agent(partials + static_cast<long long>(k) * kTailBlocks + b);
+                }
+            };
+#pragma unroll
+            for (int u = 0; u < BB; ++u)
+#pragma unroll
+                for (int k = 0; k < KS; ++k) v[u][k] = 0.0;
+            load(false);
+            for (int polls = 1;; ++polls) {   // until no partial of this batch is pending
+                bool any = false;
+#pragma unroll
+                for (int u = 0; u < BB; ++u)
+#pragma unroll
+                    for (int k = 0; k < KS; ++k) any = any || pending(v[u][k]);
+                if (!any || bad) break;
+                if ((polls & 63) == 0 &&
+                    (polls >= kTailSpinLimit ||
+                     __hip_atomic_load(R.abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+                    __hip_atomic_store(R.abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    bad = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+                load(true);
+            }
 #pragma unroll
             for (int u = 0; u < BB; ++u) {
                 const int b = b0 + u * 256;
 #pragma unroll
                 for (int k = 0; k < KS; ++k)
-                    v[u][k] = (b < nblk && k < ns)
-                                  ? __hip_atomic_load(const_cast<double*>(partials) + static_cast<long long>(k) * kTailBlocks + b,
-                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                  : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < BB; ++u)
-#pragma unroll
-                for (int k = 0; k < KS; ++k)
-                    if (b0 + u * 256 < nblk && k < ns)
+                    if (b < nblk && k < ns) {   // pending again for the next reduction
+                        st_agent(partials + static_cast<long long>(k) * kTailBlocks + b, pv);
                         acc[k] = ((R.maxmask >> k) & 1u) ? fmax(acc[k], v[u][k]) : acc[k] + v[u][k];
+                    }
+            }
         }
 #pragma unroll
         for (int k = 0; k < KS; ++k)
@@ -218,7 +241,9 @@ __device__ __forceinline__ bool reduce_tail(const RedTail R, const double* parti
                 if (l == 0) wsum[k][w + h * (NT / 64)] = t;
             }
     }
+    if (bad) sbad = 1;
     __syncthreads();
+    bad = sbad != 0;
     if (static_cast<int>(threadIdx.x) < ns) {
         const int k = threadIdx.x;
         double r;

@@ -27,7 +27,7 @@ wrapd_ctx::~wrapd_ctx() {
     f(d_ftasks); f(d_ffail);
     f(mlp_master); f(mlp_m1); f(mlp_m2); f(mlp_gsum); f(mlp_norm); f(mlp_bf); f(mlp_loss); f(mlp_partial);
     f(mlp_lpart); f(mlp_log);
-    f(partials); f(scal); f(d_red_flags); f(d_red_abort);
+    f(partials); f(scal); f(d_red_abort);
     if (scal_host) (void)hipHostFree(scal_host);
     if (host_seq) (void)hipHostFree(host_seq);
     for (auto& p : prof.pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -159,13 +159,16 @@ int wrapd_create(wrapd_ctx** out, int device) {
     if (check_hip(c, hipSetDevice(device), "hipSetDevice") ||
         check_hip(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate") ||
         dalloc(c, c->partials, static_cast<size_t>(kMaxBlocks) * kRedStride) || dalloc(c, c->scal, S_NUM) ||
-        dalloc(c, c->d_red_flags, kMaxBlocks) || dalloc(c, c->d_red_abort, 1) ||
+        dalloc(c, c->d_red_abort, 1) ||
         check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&c->scal_host), S_NUM * sizeof(double)), "hipHostMalloc") ||
         check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&c->host_seq), 64), "hipHostMalloc")) {
         delete c;
         return WRAPD_ERR_HIP;
     }
-    if (check_hip(c, hipMemset(c->d_red_flags, 0, kMaxBlocks * sizeof(unsigned)), "hipMemset") ||
+    // partials start pending (each 32-bit half of the pattern is 0xFFF40000)
+    static_assert(wd::kPendingBits == 0xFFF40000FFF40000ull, "pending pattern: two equal halves");
+    if (check_hip(c, hipMemsetD32(reinterpret_cast<hipDeviceptr_t>(c->partials), 0xFFF40000u,
+                                  2 * static_cast<size_t>(kMaxBlocks) * kRedStride), "hipMemsetD32") ||
         check_hip(c, hipMemset(c->d_red_abort, 0, sizeof(int)), "hipMemset")) {
         delete c;
         return WRAPD_ERR_HIP;

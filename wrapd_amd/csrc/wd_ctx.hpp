@@ -164,7 +164,6 @@ struct wrapd_ctx {
     // reductions
     double* partials = nullptr;
     double* scal = nullptr;      // device scalars
-    unsigned* d_red_flags = nullptr;   // per block of a reducing launch: epoch when its partials are stored
     int* d_red_abort = nullptr;        // sticky: a reduction's wait ran out
     unsigned red_epoch = 0;
     unsigned xd_epoch = 0;             // the epoch of the last x-distance reduction (its host copy: S_XD)

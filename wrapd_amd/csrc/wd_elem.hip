@@ -180,7 +180,7 @@ k_init_duals(int n_e, int n_ep, int n_f, const int4* __restrict__ vidx, const do
         st9(P, n_ep, e, Z);
     }
     const double t = block_sum<kElemThreads>(pe, sh);
-    if (threadIdx.x == 0) put_partial(partials, lb, 0, t);
+    if (threadIdx.x == 0 && tail.nslots > 0) put_partial(partials, lb, 0, t);
     reduce_tail<kElemThreads, 1>(tail, partials);
 }
 
@@ -192,7 +192,7 @@ k_defo(int n_e, int n_ep, int n_f, const int4* __restrict__ vidx, const double* 
        const double* __restrict__ vol, const double* __restrict__ kst, double* __restrict__ symF,
        double* __restrict__ gstage, double* __restrict__ cand, MatArgs m, int use_hessian,
        double* __restrict__ partials, RedTail tail) {
-    __shared__ double sh[kElemThreads / 64];
+    __shared__ double shk[3][kElemThreads / 64];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int e = lb * blockDim.x + threadIdx.x;
     double val = 0.0, en = 0.0, flip = 0.0;
@@ -282,19 +282,12 @@ k_defo(int n_e, int n_ep, int n_f, const int4* __restrict__ vidx, const double* 
             cand[e] = vol[e] * cs;
         }
     }
-    if (MODE & DM_VALUE) {
-        const double t = block_sum<kElemThreads>(val, sh);
-        if (threadIdx.x == 0) put_partial(partials, lb, 0, t);
+    if (MODE & (DM_VALUE | DM_ENERGY)) {   // slots value, energy, flips: the reduction's nslots of them
+        const double vals[3] = {val, en, flip};
+        const double t = block_sums<kElemThreads, 3>(vals, 3, shk);
+        if (static_cast<int>(threadIdx.x) < tail.nslots) put_partial(partials, lb, threadIdx.x, t);
+        reduce_tail<kElemThreads, 3>(tail, partials);
     }
-    if (MODE & DM_ENERGY) {
-        const double t1 = block_sum<kElemThreads>(en, sh);
-        const double t2 = block_sum<kElemThreads>(flip, sh);
-        if (threadIdx.x == 0) {
-            put_partial(partials, lb, 1, t1);
-            put_partial(partials, lb, 2, t2);
-        }
-    }
-    if (MODE & (DM_VALUE | DM_ENERGY)) reduce_tail<kElemThreads, 3>(tail, partials);
 }
 
 // ---------------------------------------------------------------------------------
@@ -374,10 +367,10 @@ constexpr unsigned kLocalMaxMask = (1u << 3) | (1u << 9);
 __device__ __forceinline__ void store_local_partials(double* partials, int lb, double r0, double r1, double r2,
                                                      double r3, double r4, double r5, double r6, double r7,
                                                      double r8, double r9, double r10,
-                                                     double (*shk)[kLocalThreads / 64]) {
+                                                     double (*shk)[kLocalThreads / 64], int nslots) {
     const double v[11] = {r0, r1, r2, r3, r4, r5, r6, r7, r8, r9, r10};
     const double t = block_sums_max<kLocalThreads, 11>(v, kLocalMaxMask, shk);
-    if (threadIdx.x < 11) put_partial(partials, lb, threadIdx.x, t);
+    if (static_cast<int>(threadIdx.x) < min(nslots, 11)) put_partial(partials, lb, threadIdx.x, t);
 }
 
 template <int MODEL, int PREC, int PROX>
@@ -463,7 +456,8 @@ k_local(int n_e, int n_ep, const double* __restrict__ symF, double* __restrict__
             ratio = a > b ? a : b;
         }
     }
-    store_local_partials(partials, lb, bef, aft, pe, ratio, iters, fallback, neg, 0.0, evals, evals, 0.0, shk);
+    store_local_partials(partials, lb, bef, aft, pe, ratio, iters, fallback, neg, 0.0, evals, evals, 0.0, shk,
+                         tail.nslots);
     reduce_tail<kLocalThreads, 11>(tail, partials);
 }
 
@@ -611,7 +605,8 @@ k_local_learned(int n_e, int n_ep, const double* __restrict__ symF, double* __re
             ratio = a > b ? a : b;
         }
     }
-    store_local_partials(partials, lb, bef, aft, pe, ratio, iters, fallback, neg, accepted, evals, evals, used, shk);
+    store_local_partials(partials, lb, bef, aft, pe, ratio, iters, fallback, neg, accepted, evals, evals, used, shk,
+                         tail.nslots);
     reduce_tail<kLocalThreads, 11>(tail, partials);
 }
 
@@ -698,7 +693,7 @@ k_max_ratio(int n_e, const double* __restrict__ wsq, const double* __restrict__ 
         r = a > b ? a : b;
     }
     const double t = block_max<kElemThreads>(r, sh);
-    if (threadIdx.x == 0) put_partial(partials, lb, 0, t);
+    if (threadIdx.x == 0 && tail.nslots > 0) put_partial(partials, lb, 0, t);
     reduce_tail<kElemThreads, 1>(tail, partials);
 }
 

@@ -81,20 +81,7 @@ __device__ __forceinline__ void strided_dot3(const double* __restrict__ g, int64
 }
 
 constexpr int kSpinLimit = 1 << 20;   // polls (about a second) before a wait gives up
-
-__device__ __forceinline__ double ld_agent(const double* p) {
-    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_agent(double* p, double v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// A signalling NaN with a payload: arithmetic only yields quiet NaNs, so no result has it.
-constexpr unsigned long long kPendingBits = 0xFFF40000FFF40000ull;
-__device__ __forceinline__ bool pending(double v) {
-    return static_cast<unsigned long long>(__double_as_longlong(v)) == kPendingBits;
-}
-__device__ __forceinline__ double pending_value() { return __longlong_as_double(static_cast<long long>(kPendingBits)); }
+// ld_agent / st_agent and the pending bit pattern are device/wd_reduce.hpp's.
 
 // v already holds one load of p[0..2]; loads again until no word is pending.  False when
 // this wait or another one has given up (v may then hold pending words).

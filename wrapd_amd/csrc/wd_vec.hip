@@ -32,7 +32,7 @@ struct VecPtrs {
 // KS: the most slots (nslots <= KS); BB blocks' slots are loaded per batch.
 template <int KS>
 __global__ void __launch_bounds__(256)
-k_finalize(const double* __restrict__ partials, int nblk, int nslots, unsigned maxmask, double* __restrict__ out,
+k_finalize(double* __restrict__ partials, int nblk, int nslots, unsigned maxmask, double* __restrict__ out,
            double* __restrict__ hout, unsigned* __restrict__ hseq, unsigned epoch) {
     constexpr int BB = KS > 4 ? 4 : 8;
     __shared__ double sh[KS][4];
@@ -52,8 +52,11 @@ k_finalize(const double* __restrict__ partials, int nblk, int nslots, unsigned m
         for (int u = 0; u < BB; ++u)
 #pragma unroll
             for (int k = 0; k < KS; ++k)
-                if (b0 + 256 * u < nblk && k < nslots)
+                if (b0 + 256 * u < nblk && k < nslots) {
                     acc[k] = ((maxmask >> k) & 1u) ? fmax(acc[k], v[u][k]) : acc[k] + v[u][k];
+                    // pending again for the next in-launch reduction
+                    partials[static_cast<long long>(k) * kTailBlocks + b0 + 256 * u] = pending_value();
+                }
     }
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
 #pragma unroll
@@ -199,7 +202,7 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
 #pragma unroll
         for (int j = 0; j + 1 < KG; ++j) part[1 + j] = j < kMaxHist ? gs[j] : 0.0;
         const double t = block_sums<kVecThreads, KG>(part, 1 + LA.n, shk);
-        if (static_cast<int>(threadIdx.x) < 1 + LA.n) put_partial(partials, lb, threadIdx.x, t);
+        if (static_cast<int>(threadIdx.x) < tail.nslots) put_partial(partials, lb, threadIdx.x, t);
     }
     if (MODE == 2) {
         double y[3] = {0, 0, 0}, s[3] = {0, 0, 0};
@@ -247,7 +250,7 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
         }
         const int nk = kd + LA.n;
         const double t = block_sums<kVecThreads, KG>(part, nk, shk);
-        if (static_cast<int>(threadIdx.x) < nk) put_partial(partials, lb, threadIdx.x, t);
+        if (static_cast<int>(threadIdx.x) < tail.nslots) put_partial(partials, lb, threadIdx.x, t);
     }
     if (MODE != 0 && reduce_tail<kVecThreads, KG, false>(tail, partials, sres)) {
         static_assert(KG <= kTailSlots, "gather: slot count");
@@ -300,7 +303,7 @@ k_multidot(long long n3, const double* __restrict__ base, VecPtrs V, int nvec, c
         if (j == nvec) acc[j] = ae;
     const int nk = nvec + (extra ? 1 : 0);
     const double t = block_sums<kVecThreads, KM>(acc, nk, shk);
-    if (static_cast<int>(threadIdx.x) < nk) put_partial(partials, lb, threadIdx.x, t);
+    if (static_cast<int>(threadIdx.x) < tail.nslots) put_partial(partials, lb, threadIdx.x, t);
     // the last block also runs the recurrence that consumes the dots (post 1: alpha, 2: beta),
     // with the list's Gram entries (and for beta the alphas) loaded into LDS together
     if (reduce_tail<kVecThreads, KM, false>(tail, partials, sres)) {
@@ -357,7 +360,7 @@ k_axpy_r(long long n3, double* __restrict__ r, VecPtrs S, int n, const double* _
         d = g[i] * v;
     }
     const double t = block_sum<kVecThreads>(d, sh);
-    if (threadIdx.x == 0) put_partial(partials, lb, 0, t);
+    if (threadIdx.x == 0 && tail.nslots > 0) put_partial(partials, lb, 0, t);
     reduce_tail<kVecThreads, 1>(tail, partials);
 }
 
@@ -371,7 +374,7 @@ k_dot(long long n3, const double* __restrict__ a, const double* __restrict__ b, 
     if (i < n3) { d = a[i] * b[i]; m = fabs(a[i]); }
     const double t = block_sum<kVecThreads>(d, sh);
     const double tm = block_max<kVecThreads>(m, sh);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && tail.nslots >= 2) {
         put_partial(partials, lb, 0, t);
         put_partial(partials, lb, 1, tm);
     }
@@ -425,7 +428,7 @@ k_diffnorm(long long n3, const double* __restrict__ a, const double* __restrict_
     double d = 0.0;
     if (i < n3) { const double t = a[i] - b[i]; d = t * t; }
     const double t = block_sum<kVecThreads>(d, sh);
-    if (threadIdx.x == 0) put_partial(partials, lb, 0, t);
+    if (threadIdx.x == 0 && tail.nslots > 0) put_partial(partials, lb, 0, t);
     reduce_tail<kVecThreads, 1>(tail, partials);
 }
 
@@ -445,7 +448,7 @@ static SlotList mk_list(const int* slots, int n) {
 
 void arm_tail(wrapd_ctx* c, int nslots, unsigned maxmask, int dst, bool host) {
     RedTail t;
-    t.flags = c->d_red_flags;
+    t.armed = true;
     t.abortf = c->d_red_abort;
     t.out = c->scal + dst;
     t.hout = host ? c->scal_host + dst : nullptr;
@@ -459,14 +462,16 @@ void arm_tail(wrapd_ctx* c, int nslots, unsigned maxmask, int dst, bool host) {
 RedTail take_tail(wrapd_ctx* c, int nblk, bool may_split) {
     RedTail t = c->pending_tail;
     c->pending_tail = RedTail{};
-    if (!t.flags) return t;
-    if (++c->red_epoch == 0) c->red_epoch = 1;   // flags start at 0
+    if (!t.armed) return t;
+    if (++c->red_epoch == 0) c->red_epoch = 1;   // the host's sequence word starts at 0
     t.epoch = c->red_epoch;
     if (t.hseq) c->host_epoch = t.epoch;
     if (may_split && nblk > kTailMaxBlocks) {   // one block would read too many partials: finalize apart
         c->pending_fin = wrapd_ctx::PendingFinalize{t.nslots, t.maxmask, static_cast<int>(t.out - c->scal), nblk,
                                                     t.hseq != nullptr};
-        return RedTail{};
+        RedTail split;   // the blocks still store their partials, for the finalize launch
+        split.nslots = t.nslots;
+        return split;
     }
     return t;
 }
