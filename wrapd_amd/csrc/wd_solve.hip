@@ -80,11 +80,13 @@ __device__ __forceinline__ void strided_dot3(const double* __restrict__ g, int64
     }
 }
 
-constexpr int kSpinLimit = 1 << 20;   // polls (about a second) before a wait gives up
+constexpr int kSpinLimit = 1 << 20;   // polls (one to two seconds) before a wait gives up
 // ld_agent / st_agent and the pending bit pattern are device/wd_reduce.hpp's.
 
-// v already holds one load of p[0..2]; loads again until no word is pending.  False when
-// this wait or another one has given up (v may then hold pending words).
+// v already holds one load of p[0..2]; loads the pending words again until none is left.
+// False when this wait or another one has given up (v may then hold pending words).  After
+// 16 polls a wait sleeps ~1 us between polls: hundreds of waiting workgroups poll the same
+// few rows near the root, and their loads otherwise slow down the stores they wait for.
 __device__ bool poll3(const double* p, double v[3], int* abortf) {
     for (int polls = 1; pending(v[0]) || pending(v[1]) || pending(v[2]); ++polls) {
         if ((polls & 63) == 0) {
@@ -94,10 +96,13 @@ __device__ bool poll3(const double* p, double v[3], int* abortf) {
                 return false;
             }
         }
-        __builtin_amdgcn_s_sleep(2);
-        v[0] = ld_agent(p + 0);
-        v[1] = ld_agent(p + 1);
-        v[2] = ld_agent(p + 2);
+        if (polls < 16)
+            __builtin_amdgcn_s_sleep(2);
+        else
+            __builtin_amdgcn_s_sleep(40);
+        if (pending(v[0])) v[0] = ld_agent(p + 0);
+        if (pending(v[1])) v[1] = ld_agent(p + 1);
+        if (pending(v[2])) v[2] = ld_agent(p + 2);
     }
     return true;
 }
@@ -330,7 +335,8 @@ static_assert(kBwdChunk % kSolveThreads == 0 && kNsMax % kSolveThreads == 0, "st
 // slots[udest[i]], inside its parent front row's range [asm_ptr[p], asm_ptr[p + 1]).  A
 // task's inputs are therefore two contiguous slot ranges (those of the supernode's pivot
 // rows and those of the task's update rows, both in the task record), polled in parallel
-// rounds of kFwdStage into LDS and then summed per row in slot order.  The pivot rows'
+// rounds of kFwdStage into LDS and then summed per row in slot order (a round's loads go
+// out before the previous round is summed).  The pivot rows'
 // right-hand side comes from bperm, b in the solve's numbering (pending until the leaf
 // workgroups of the launch have copied it), so every load of the task depends on the task
 // record alone.  B: G terms per thread loaded before the wait.  LDS (dyn): cv, then sred.
@@ -421,7 +427,6 @@ __device__ __forceinline__ void fwd_flow_body(int ti, const SolveTask* __restric
     // The children's updates, polled, staged and summed in slot order.
     double ua0 = 0.0, ua1 = 0.0, ua2 = 0.0;
     for (int base = 0; base < ntot; base += kFwdStage) {
-        if (base > 0) load_window(base);
 #pragma unroll
         for (int k = 0; k < kSlotsPerThread; ++k) {
             const int i = base + threadIdx.x + k * kSolveThreads;
@@ -440,6 +445,7 @@ __device__ __forceinline__ void fwd_flow_body(int ti, const SolveTask* __restric
             sred[3 * il + 1] = v[k][1];
             sred[3 * il + 2] = v[k][2];
         }
+        if (base + kFwdStage < ntot) load_window(base + kFwdStage);   // in flight during this window's sums
         __syncthreads();
         const int wend = base + kFwdStage;
 #pragma unroll
