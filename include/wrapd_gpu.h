@@ -131,6 +131,11 @@ int wrapd_update_actual_weights(wrapd_ctx* ctx);
 
 /* TetElements::polar_update (src/TetElements.cpp:256-356): local step on all elements. */
 int wrapd_local_step(wrapd_ctx* ctx, const wrapd_local_args* args, double pe_prev, wrapd_local_out* out);
+/* The same in two calls: the launch, then its results (which wait for it).  Between them the
+ * caller may queue wrapd_global_step_begin, so the GPU works while the host reads the local
+ * step's results.  Results without a launch before them: WRAPD_ERR_STATE. */
+int wrapd_local_step_launch(wrapd_ctx* ctx, const wrapd_local_args* args);
+int wrapd_local_step_result(wrapd_ctx* ctx, double pe_prev, wrapd_local_out* out);
 /* TetElements::update (src/TetElements.cpp:193-254): rotation-unaware local step (-ra 0). */
 int wrapd_local_step_unaware(wrapd_ctx* ctx, const wrapd_local_args* args);
 
@@ -158,6 +163,9 @@ int wrapd_lbfgs_reset(wrapd_ctx* ctx);              /* update_A0 / clear_history
 /* RotAwareGS::global_step (src/GlobalStepBehavior.hpp:246-258): value, minimize with the
  * early-exit tolerance |dAL_local|, value; energy and flips at the final x. */
 int wrapd_global_step(wrapd_ctx* ctx, double early_exit_tol, wrapd_global_out* out);
+/* Queues the global step's first value and gradient (which do not depend on the
+ * tolerance); the next wrapd_global_step continues from them instead of launching them. */
+int wrapd_global_step_begin(wrapd_ctx* ctx);
 /* UnawareGS::global_step (src/GlobalStepBehavior.hpp:164-184): x = A^-1 b with
  * b from add_to_b (src/TetElements.cpp:470-481). */
 int wrapd_global_step_unaware(wrapd_ctx* ctx, wrapd_global_out* out);
@@ -180,6 +188,9 @@ int wrapd_set_elem(wrapd_ctx* ctx, const char* name, const double* in);
 
 /* timing of the device work, measured with HIP events on the context's stream */
 int wrapd_profile_enable(wrapd_ctx* ctx, int on);
+/* only = NULL or "": every scope; otherwise the one scope named (each timed launch costs two
+ * event records in the stream, which slow a host-driven loop down noticeably) */
+int wrapd_profile_only(wrapd_ctx* ctx, const char* only);
 int wrapd_profile_get(wrapd_ctx* ctx, const char* kernel, double* total_ms, double* launches);
 int wrapd_synchronize(wrapd_ctx* ctx);
 

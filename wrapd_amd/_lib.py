@@ -80,6 +80,8 @@ _PROTOS = {
     "wrapd_update_actual_weights": (C.c_int, [P]),
     "wrapd_local_step": (C.c_int, [P, C.POINTER(LocalArgs), C.c_double, C.POINTER(LocalOut)]),
     "wrapd_local_step_unaware": (C.c_int, [P, C.POINTER(LocalArgs)]),
+    "wrapd_local_step_launch": (C.c_int, [P, C.POINTER(LocalArgs)]),
+    "wrapd_local_step_result": (C.c_int, [P, C.c_double, C.POINTER(LocalOut)]),
     "wrapd_global_value_grad": (C.c_int, [P, D, D]),
     "wrapd_refactor": (C.c_int, [P]),
     "wrapd_solve3": (C.c_int, [P, D, D]),
@@ -88,12 +90,14 @@ _PROTOS = {
     "wrapd_lbfgs_configure": (C.c_int, [P, C.POINTER(LbfgsCfg)]),
     "wrapd_lbfgs_reset": (C.c_int, [P]),
     "wrapd_global_step": (C.c_int, [P, C.c_double, C.POINTER(GlobalOut)]),
+    "wrapd_global_step_begin": (C.c_int, [P]),
     "wrapd_global_step_unaware": (C.c_int, [P, C.POINTER(GlobalOut)]),
     "wrapd_get_elem": (C.c_int, [P, C.c_char_p, D]),
     "wrapd_state_save": (C.c_int, [P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "wrapd_state_load": (C.c_int, [P, P, C.c_size_t]),
     "wrapd_set_elem": (C.c_int, [P, C.c_char_p, D]),
     "wrapd_profile_enable": (C.c_int, [P, C.c_int]),
+    "wrapd_profile_only": (C.c_int, [P, C.c_char_p]),
     "wrapd_profile_get": (C.c_int, [P, C.c_char_p, D, D]),
     "wrapd_synchronize": (C.c_int, [P]),
     "wrapd_mlp_init": (C.c_int, [P, C.c_ulonglong]),

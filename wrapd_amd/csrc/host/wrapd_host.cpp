@@ -194,6 +194,11 @@ void TetElements::initialize_dual_vars() {
 }
 
 double TetElements::polar_update(bool update_cand) {
+    polar_update_launch(update_cand);
+    return polar_update_finish();
+}
+
+void TetElements::polar_update_launch(bool update_cand) {
     // src/TetElements.cpp:256-356; the candidate-weight counter of :404-428
     wrapd_local_args a;
     a.precision = m_fp64_tail ? WRAPD_FP64 : m_settings.m_precision;
@@ -206,10 +211,12 @@ double TetElements::polar_update(bool update_cand) {
                                                  : (a.precision == WRAPD_FP32 ? 1e-6 : 1e-12);
     a.fallback_sigma = m_settings.m_fallback_sigma;
     if (a.prox_mode != WRAPD_PROX_LBFGS && a.prox_iters < 30) a.prox_iters = 30;
-    const double pe_prev = m_curr_pe;
-    const int st = wrapd_local_step(m_ctx, &a, pe_prev, &m_last_local);
-    check(st, "polar_update");
+    check(wrapd_local_step_launch(m_ctx, &a), "polar_update");
     if (update_cand) m_cand_count++;
+}
+
+double TetElements::polar_update_finish() {
+    check(wrapd_local_step_result(m_ctx, m_curr_pe, &m_last_local), "polar_update");
     m_curr_pe = m_last_local.pe;
     return m_last_local.delta_auglag;
 }
@@ -272,6 +279,8 @@ void TetElements::load(ByteReader& r) {
     check(wrapd_state_load(m_ctx, dev.data(), dev.size()), "state load");
 }
 void TetElements::reset_history() { check(wrapd_lbfgs_reset(m_ctx), "clear_history"); }
+void TetElements::global_step_begin() { check(wrapd_global_step_begin(m_ctx), "global_step"); }
+
 void TetElements::global_step(double tol, wrapd_global_out& out) {
     check(wrapd_global_step(m_ctx, tol, &out), "global_step");
     m_energy = out.energy;
@@ -503,6 +512,8 @@ bool RotAwareGS::update_weights() {
     return updated;
 }
 
+void RotAwareGS::begin_global_step() { m_system->tet_elements()->global_step_begin(); }
+
 void RotAwareGS::global_step() {
     MicroTimer t;
     wrapd_global_out o{};
@@ -583,8 +594,11 @@ void ADMM::iterate() {
     m_micro_timer.reset();
     if (m_settings.m_rot_awareness == Settings::RotAwareness::DISABLED) {
         m_system->local_update(upd);
-    } else {
-        ad.m_last_localstep_delta_auglag_obj = m_system->polar_local_update(upd);
+    } else {   // the global step's first launches are queued before the host reads the local step's results
+        auto& te = m_system->tet_elements();
+        te->polar_update_launch(upd);
+        m_global_step_behavior->begin_global_step();
+        ad.m_last_localstep_delta_auglag_obj = te->polar_update_finish();
     }
     ad.m_runtime.local_ms += m_micro_timer.elapsed_ms();
     const wrapd_local_out& lo = m_system->tet_elements()->last_local();

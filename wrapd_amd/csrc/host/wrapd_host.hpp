@@ -228,12 +228,15 @@ class TetElements {
     void update_defo_cache();                                  // energy + flips + symF at x
     void initialize_dual_vars();                               // at the current x
     double polar_update(bool update_candidate_weights);        // returns dAL of the local step
+    void polar_update_launch(bool update_candidate_weights);   // polar_update in two halves
+    double polar_update_finish();
     void update(bool update_candidate_weights);                // -ra 0
     void midupdate_weights(bool direct);
     void update_actual_weights();
     double max_wsq_ratio();
     void refactor();
     void global_step(double early_exit_tol, wrapd_global_out& out);
+    void global_step_begin();     // queues the global step's first value and gradient
     void global_step_unaware(wrapd_global_out& out);
     void reset_history();
     double x_distance();
@@ -380,6 +383,7 @@ class GlobalStepBehavior {
     GlobalStepBehavior(const Settings& s, std::shared_ptr<System> sys, std::shared_ptr<AlgorithmData> ad);
     virtual ~GlobalStepBehavior() {}
     virtual void global_step() = 0;
+    virtual void begin_global_step() {}   // queue what global_step starts with, where that does not wait for the host
     virtual double last_step_delta_auglag() { return 1.e8; }
     virtual void initialize_weights();
     virtual bool update_weights();
@@ -401,6 +405,7 @@ class RotAwareGS : public GlobalStepBehavior {
  public:
     RotAwareGS(const Settings& s, std::shared_ptr<System> sys, std::shared_ptr<AlgorithmData> ad);
     void global_step() override;
+    void begin_global_step() override;
     void initialize_weights() override;
     bool update_weights() override;
     double last_step_delta_auglag() override { return m_obj_aft - m_obj_bef; }

@@ -55,7 +55,7 @@ static hipEvent_t ev_get(wrapd_ctx* c) {
 }
 
 ProfScope::ProfScope(wrapd_ctx* ctx, const char* name) : c(ctx) {
-    if (!c->prof.on) return;
+    if (!c->prof.on || (!c->prof.only.empty() && c->prof.only != name)) return;
     Prof::Pending p;
     p.name = name;
     p.a = ev_get(c);
@@ -130,6 +130,11 @@ static const double* scal_read(wrapd_ctx* c, int at, int n) {
     return scal_read_epoch(c, at, c->host_epoch);
 }
 
+// Entry points that change what a begun global step's first launches read (x, the duals,
+// the weights, the history, the factorization) drop it: the global step then launches
+// them again itself.
+static void drop_begun(wrapd_ctx* c) { c->gs_begun = false; }
+
 static int launch_status(wrapd_ctx* c) {
     return check_hip(c, hipGetLastError(), "kernel launch") ? WRAPD_ERR_HIP : WRAPD_OK;
 }
@@ -202,6 +207,7 @@ int wrapd_sizes(wrapd_ctx* c, int* nv, int* ne, int* nf, int* nfix) {
 int wrapd_upload_mesh(wrapd_ctx* c, int n_v, const double* rest, int n_e, const int* tets, int n_pins,
                       const int* pins, const wrapd_material* mat, const double* mu_e, const double* lambda_e) {
     CTX_OK(c);
+    drop_begun(c);
     if (n_v <= 0 || n_e <= 0 || !rest || !tets || !mat || n_pins < 0 || (n_pins > 0 && !pins)) {
         c->last_error = "wrapd_upload_mesh: bad arguments";
         return WRAPD_ERR_ARG;
@@ -396,6 +402,7 @@ int wrapd_upload_mesh(wrapd_ctx* c, int n_v, const double* rest, int n_e, const 
 
 int wrapd_set_x_fix(wrapd_ctx* c, const double* x_fix) {
     CTX_OK(c); MESH_OK(c);
+    drop_begun(c);
     if (c->n_fix == 0) return WRAPD_OK;
     if (!x_fix) return WRAPD_ERR_ARG;
     return h2d(c, c->xall + 3ll * c->n_f, x_fix, 3ull * c->n_fix) ? WRAPD_ERR_HIP : WRAPD_OK;
@@ -403,6 +410,7 @@ int wrapd_set_x_fix(wrapd_ctx* c, const double* x_fix) {
 
 int wrapd_set_x_free(wrapd_ctx* c, const double* x) {
     CTX_OK(c); MESH_OK(c);
+    drop_begun(c);
     if (!x) return WRAPD_ERR_ARG;
     return h2d(c, c->xall, x, 3ull * c->n_f) ? WRAPD_ERR_HIP : WRAPD_OK;
 }
@@ -444,12 +452,14 @@ int wrapd_x_distance_result(wrapd_ctx* c, double* dist) {
 
 int wrapd_set_rot_aware(wrapd_ctx* c, int rot_aware) {
     CTX_OK(c);
+    drop_begun(c);
     c->rot_aware = rot_aware ? 1 : 0;
     return WRAPD_OK;
 }
 
 int wrapd_init_duals(wrapd_ctx* c, int rot_aware, double* polar_pe) {
     CTX_OK(c); MESH_OK(c);
+    drop_begun(c);
     c->rot_aware = rot_aware ? 1 : 0;
     arm_tail(c, 1, 0u, S_RED);
     launch_init_duals(c, c->rot_aware);
@@ -461,6 +471,7 @@ int wrapd_init_duals(wrapd_ctx* c, int rot_aware, double* polar_pe) {
 
 int wrapd_defo_energy(wrapd_ctx* c, double* energy, int* flips) {
     CTX_OK(c); MESH_OK(c);
+    drop_begun(c);
     arm_tail(c, 3, 0u, S_RED);
     launch_defo(c, c->rot_aware ? (DM_WRITE_SYMF | DM_ENERGY) : (DM_FT_CACHE | DM_ENERGY), 0);
     const double* r = scal_read(c, S_RED, 3);
@@ -474,6 +485,7 @@ int wrapd_defo_energy(wrapd_ctx* c, double* energy, int* flips) {
 
 int wrapd_candidate_weights(wrapd_ctx* c, int use_hessian, int direct) {
     CTX_OK(c); MESH_OK(c);
+    drop_begun(c);
     if (c->mat.model != 1) { c->last_error = "Adaptive weights are only supported for the NH model."; return WRAPD_ERR_STATE; }
     launch_defo(c, direct ? (DM_CAND | DM_DIRECT) : DM_CAND, use_hessian);
     return launch_status(c);
@@ -491,12 +503,14 @@ int wrapd_max_wsq_ratio(wrapd_ctx* c, double* ratio) {
 
 int wrapd_update_actual_weights(wrapd_ctx* c) {
     CTX_OK(c); MESH_OK(c);
+    drop_begun(c);
     launch_update_weights(c);
     return launch_status(c);
 }
 
-int wrapd_local_step(wrapd_ctx* c, const wrapd_local_args* a, double pe_prev, wrapd_local_out* out) {
+int wrapd_local_step_launch(wrapd_ctx* c, const wrapd_local_args* a) {
     CTX_OK(c); MESH_OK(c);
+    drop_begun(c);
     if (!a) return WRAPD_ERR_ARG;
     if (a->update_cand && c->mat.model != 1) {
         c->last_error = "Adaptive weights are only supported for the NH model.";
@@ -506,9 +520,21 @@ int wrapd_local_step(wrapd_ctx* c, const wrapd_local_args* a, double pe_prev, wr
         c->last_error = "learned prox: no network (wrapd_mlp_init / wrapd_mlp_set_params)";
         return WRAPD_ERR_STATE;
     }
-    arm_tail(c, 11, (1u << 3) | (1u << 9), S_RED);
+    arm_tail(c, 11, (1u << 3) | (1u << 9), S_LOCAL);   // a slot range of its own: read after later launches
     launch_local(c, a);
-    const double* r = scal_read(c, S_RED, 11);
+    c->local_epoch = c->host_epoch;
+    c->local_pending = true;
+    return launch_status(c);
+}
+
+int wrapd_local_step_result(wrapd_ctx* c, double pe_prev, wrapd_local_out* out) {
+    CTX_OK(c);
+    if (!c->local_pending) {
+        c->last_error = "local step results: no local step launched";
+        return WRAPD_ERR_STATE;
+    }
+    c->local_pending = false;
+    const double* r = scal_read_epoch(c, S_LOCAL, c->local_epoch);
     if (!r) return WRAPD_ERR_HIP;
     if (out) {
         out->bef = r[0];
@@ -532,8 +558,14 @@ int wrapd_local_step(wrapd_ctx* c, const wrapd_local_args* a, double pe_prev, wr
     return st;
 }
 
+int wrapd_local_step(wrapd_ctx* c, const wrapd_local_args* a, double pe_prev, wrapd_local_out* out) {
+    const int st = wrapd_local_step_launch(c, a);
+    return st != WRAPD_OK ? st : wrapd_local_step_result(c, pe_prev, out);
+}
+
 int wrapd_local_step_unaware(wrapd_ctx* c, const wrapd_local_args* a) {
     CTX_OK(c); MESH_OK(c);
+    drop_begun(c);
     if (!a) return WRAPD_ERR_ARG;
     launch_local_unaware(c, a);
     return launch_status(c);
@@ -541,6 +573,7 @@ int wrapd_local_step_unaware(wrapd_ctx* c, const wrapd_local_args* a) {
 
 int wrapd_global_value_grad(wrapd_ctx* c, double* value, double* grad) {
     CTX_OK(c); MESH_OK(c);
+    drop_begun(c);
     arm_tail(c, 1, 0u, S_VAL);
     launch_defo(c, DM_GRAD | DM_VALUE, 0);
     launch_gather_grad(c, c->grad, 0, 0, nullptr, 0);
@@ -553,6 +586,7 @@ int wrapd_global_value_grad(wrapd_ctx* c, double* value, double* grad) {
 
 int wrapd_refactor(wrapd_ctx* c) {
     CTX_OK(c); MESH_OK(c);
+    drop_begun(c);
     launch_assemble_A(c);
     if (!c->analyzed) {
         std::vector<double> xyz(3ull * c->n_f);
@@ -591,6 +625,7 @@ int wrapd_refactor(wrapd_ctx* c) {
 
 int wrapd_solve3(wrapd_ctx* c, const double* b, double* x) {
     CTX_OK(c); MESH_OK(c);
+    drop_begun(c);
     if (!c->factored) { c->last_error = "no factorization"; return WRAPD_ERR_STATE; }
     if (!b || !x) return WRAPD_ERR_ARG;
     if (h2d(c, c->bv, b, 3ull * c->n_f)) return WRAPD_ERR_HIP;
@@ -630,6 +665,7 @@ int wrapd_chol_stats(wrapd_ctx* c, double* st) {
 
 int wrapd_lbfgs_configure(wrapd_ctx* c, const wrapd_lbfgs_cfg* cfg) {
     CTX_OK(c);
+    drop_begun(c);
     if (!cfg || cfg->memory < 1 || cfg->memory > kMaxHist || cfg->max_iters < 0) {
         if (c) c->last_error = "L-BFGS memory must be in [1, 8]";
         return WRAPD_ERR_ARG;
@@ -641,31 +677,43 @@ int wrapd_lbfgs_configure(wrapd_ctx* c, const wrapd_lbfgs_cfg* cfg) {
 
 int wrapd_lbfgs_reset(wrapd_ctx* c) {
     CTX_OK(c);
+    drop_begun(c);
     c->m_k = 0;
     return WRAPD_OK;
+}
+
+// The global step's first launches: value(x) fused with the first gradient (the gradient
+// from the cache at the same x), and the first two-loop's alphas for the current history.
+int wrapd_global_step_begin(wrapd_ctx* c) {
+    CTX_OK(c); MESH_OK(c);
+    if (!c->factored) { c->last_error = "A0 is required but wasn't supplied."; return WRAPD_ERR_STATE; }
+    if (c->m_k == 0) launch_fill(c, c->grad_old, 0.0, 3ll * c->n_f);   // minimize() re-zeros its vectors when m_k == 0
+    arm_tail(c, 1, 0u, S_VAL);
+    launch_defo(c, DM_GRAD | DM_VALUE, 0);
+    arm_tail(c, 1, 0u, S_DG);
+    int hs[kMaxHist];
+    const int h = std::min(c->lcfg.memory, c->m_k);
+    for (int i = 0; i < h; ++i) hs[i] = c->pos2slot[i];
+    launch_gather_grad(c, c->grad, 1, 0, nullptr, 0, hs, h);
+    c->gs_begun = true;
+    return launch_status(c);
 }
 
 // SplitLBFGS::minimize between the two GlobalProblem::value calls of RotAwareGS::global_step.
 int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
     CTX_OK(c); MESH_OK(c);
     if (!c->factored) { c->last_error = "A0 is required but wasn't supplied."; return WRAPD_ERR_STATE; }
+    if (!c->gs_begun) {
+        const int st = wrapd_global_step_begin(c);
+        if (st != WRAPD_OK) return st;
+    }
+    c->gs_begun = false;
     const int M = c->lcfg.memory;
     const long long n3 = 3ll * c->n_f;
     const double m_tol = tol;
     const double kappa = c->lcfg.kappa;
     wrapd_global_out o{};
     o.exit_reason = 0;
-    if (c->m_k == 0) launch_fill(c, c->grad_old, 0.0, n3);   // minimize() re-zeros its vectors when m_k == 0
-    // value(x) fused with the first gradient (the gradient from the cache at the same x)
-    arm_tail(c, 1, 0u, S_VAL);
-    launch_defo(c, DM_GRAD | DM_VALUE, 0);
-    arm_tail(c, 1, 0u, S_DG);
-    {   // with the first two-loop's alphas for the current history
-        int hs[kMaxHist];
-        const int h = std::min(M, c->m_k);
-        for (int i = 0; i < h; ++i) hs[i] = c->pos2slot[i];
-        launch_gather_grad(c, c->grad, 1, 0, nullptr, 0, hs, h);
-    }
     const double* r = scal_read(c, S_DG, 2);   // S_DG, S_VAL
     if (!r) return WRAPD_ERR_HIP;
     const double dg = r[0];
@@ -843,6 +891,7 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
 // UnawareGS::global_step: b = add_to_b, x = A^-1 b, refresh the cache (F and sigma)
 int wrapd_global_step_unaware(wrapd_ctx* c, wrapd_global_out* out) {
     CTX_OK(c); MESH_OK(c);
+    drop_begun(c);
     if (!c->factored) { c->last_error = "no factorization"; return WRAPD_ERR_STATE; }
     launch_defo(c, DM_BVEC, 0);
     launch_gather_grad(c, c->bv, 0, 0, nullptr, 0);
@@ -952,6 +1001,7 @@ int wrapd_state_save(wrapd_ctx* c, void* buf, size_t cap, size_t* size) {
 
 int wrapd_state_load(wrapd_ctx* c, const void* buf, size_t size) {
     CTX_OK(c); MESH_OK(c);
+    drop_begun(c);
     if (!buf || size < sizeof(StateHeader)) return WRAPD_ERR_ARG;
     StateHeader h;
     std::memcpy(&h, buf, sizeof(h));
@@ -986,6 +1036,7 @@ int wrapd_state_load(wrapd_ctx* c, const void* buf, size_t size) {
 
 int wrapd_set_elem(wrapd_ctx* c, const char* name, const double* in) {
     CTX_OK(c); MESH_OK(c);
+    drop_begun(c);
     if (!name || !in) return WRAPD_ERR_ARG;
     const std::string n(name);
     const int ne = c->n_e, nep = c->n_ep;
@@ -1012,6 +1063,13 @@ int wrapd_profile_enable(wrapd_ctx* c, int on) {
     prof_flush(c);
     c->prof.on = on != 0;
     if (on) c->prof.acc.clear();
+    return WRAPD_OK;
+}
+
+int wrapd_profile_only(wrapd_ctx* c, const char* only) {
+    CTX_OK(c);
+    prof_flush(c);
+    c->prof.only = only ? only : "";
     return WRAPD_OK;
 }
 

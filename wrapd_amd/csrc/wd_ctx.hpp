@@ -31,7 +31,8 @@ enum ScalarSlot {
     S_GS = 56,          // 8: g . s_j of the current two-loop (logical order), kept by its alpha step
     S_GRAM = 64,        // 64: s_i . y_j by slot (8 x 8)
     S_XD = 128,         // |x - x_ref|^2 of the last x-distance launch
-    S_NUM = 136
+    S_LOCAL = 136,      // 11: the last local step's sums and maxima
+    S_NUM = 152
 };
 
 // defo-pass mode bits
@@ -42,6 +43,7 @@ enum DefoMode {
 
 struct Prof {
     bool on = false;
+    std::string only;   // the one scope timed, or empty: all
     struct Pending { std::string name; hipEvent_t a, b; };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> pool;
@@ -170,6 +172,9 @@ struct wrapd_ctx {
     int alpha_n = -1;                  // S_ALPHA holds the alphas of this list for the gradient in grad (-1: none)
     int alpha_slots[wd::kMaxHist] = {};
     unsigned host_epoch = 0;           // the epoch of the last reduction with a host copy (what the host waits for)
+    unsigned local_epoch = 0;          // the epoch of the last local step launched
+    bool local_pending = false;        // a local step launched and its results not read yet
+    bool gs_begun = false;             // the global step's first launches are queued (wrapd_global_step_begin)
     wd::RedTail pending_tail;          // armed by arm_tail, taken by the next reducing launch
     struct PendingFinalize {
         int nslots = 0;

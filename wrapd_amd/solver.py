@@ -330,7 +330,9 @@ class Device:
         a = np.ascontiguousarray(arr, dtype=np.float64)
         self._check(lib().wrapd_set_elem(self.ctx, name.encode(), dp(a)), f"set_elem {name}")
 
-    def profile(self, on=True):
+    def profile(self, on=True, only=None):
+        """Times device work with HIP events; only: one scope name (None: all scopes)."""
+        self._check(lib().wrapd_profile_only(self.ctx, only.encode() if only else None), "profile_only")
         self._check(lib().wrapd_profile_enable(self.ctx, int(on)), "profile_enable")
 
     def profile_get(self, kernel):
