@@ -714,7 +714,60 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
     const double kappa = c->lcfg.kappa;
     wrapd_global_out o{};
     o.exit_reason = 0;
-    const double* r = scal_read(c, S_DG, 2);   // S_DG, S_VAL
+    // An iteration's first half: grad_old = grad (a pointer swap; x_old is kept implicitly by
+    // the x-step), the two-loop's solve and the decrement g . r, which the host reads next.
+    // It is queued before the host reads what the launches before it produced (the first
+    // gradient, or the previous iteration's y . s), so the GPU has work while the host
+    // reads; when the host then stops the loop, it undoes the swap (the rest wrote scratch).
+    struct Stage {
+        const double* g = nullptr;
+        int it = 0;
+        int slots[kMaxHist] = {};
+        bool lazy_r = false;   // r = rv + sum coef_j s_j is formed by the x-step
+        int dir_at = S_DIR;    // where the host reads g . r
+        bool live = false;     // queued and not consumed
+    };
+    auto stage = [&](Stage& st) {
+        std::swap(c->grad, c->grad_old);
+        st.g = c->grad_old;
+        st.it = std::min(M, c->m_k);
+        for (int i = 0; i < st.it; ++i) st.slots[i] = c->pos2slot[i];
+        st.lazy_r = false;
+        st.dir_at = S_DIR;
+        if (st.it > 0) {
+            double* vs[kMaxHist];
+            for (int i = 0; i < st.it; ++i) vs[i] = c->hS[st.slots[i]];
+            // the gather that produced g has left the alphas of this list in S_ALPHA
+            const bool have_alpha = c->alpha_n == st.it && std::equal(st.slots, st.slots + st.it, c->alpha_slots);
+            if (!have_alpha) launch_multidot(c, st.g, vs, st.it, 1, st.slots);   // and the alpha recurrence
+            c->alpha_n = -1;
+            launch_solve(c, st.g, c->rv, st.slots, st.it);   // of q = g - sum alpha_j y_j, formed by the solve
+            for (int i = 0; i < st.it; ++i) vs[i] = c->hY[st.slots[i]];
+            if (c->lcfg.linesearch == 1) {
+                launch_multidot(c, c->rv, vs, st.it, 2, st.slots);   // and the beta recurrence
+                launch_axpy_r(c, st.g, st.slots, st.it);
+            } else {
+                // without the line search, r is only needed by the x-step, which forms it;
+                // g . r then comes from the beta launch
+                launch_multidot(c, c->rv, vs, st.it, 3, st.slots, st.g);   // the beta recurrence and g . r
+                st.lazy_r = true;
+            }
+        } else {
+            launch_solve(c, st.g, c->rv);
+            launch_dot(c, st.g, c->rv);
+            st.dir_at = S_RED;
+        }
+        st.live = true;
+    };
+    auto unstage = [&](Stage& st) {
+        if (!st.live) return;
+        std::swap(c->grad, c->grad_old);
+        st.live = false;
+    };
+    int max_iters = c->lcfg.max_iters;
+    Stage st;
+    if (max_iters >= 1) stage(st);
+    const double* r = scal_read(c, S_DG, 2);   // S_DG, S_VAL (and the stage's decrement)
     if (!r) return WRAPD_ERR_HIP;
     const double dg = r[0];
     o.obj_bef = r[1];
@@ -722,53 +775,26 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
     o.energy = c->last_energy;
     o.flips = c->last_flips;
     if (dg < 1.e-13) {
+        unstage(st);
         o.iters = 0;
         o.exit_reason = 3;
         if (out) *out = o;
         return launch_status(c);
     }
     int global_iter = 1;
-    int max_iters = c->lcfg.max_iters;
     bool breaking_out = false;
     bool have_final = false;
     double alpha_init = 1.0;
     for (; global_iter <= max_iters; ++global_iter) {
-        // x_old = x (kept implicitly by the x-step), grad_old = grad
-        std::swap(c->grad, c->grad_old);
-        const double* g = c->grad_old;
-        const int it = std::min(M, c->m_k);
-        int slots[kMaxHist];
-        for (int i = 0; i < it; ++i) slots[i] = c->pos2slot[i];
-        double dir;
-        // without the line search, r = rv + sum coef_j s_j is only needed by the x-step, which
-        // forms it; g . r then comes from the beta launch
-        bool lazy_r = false;
-        if (it > 0) {
-            double* vs[kMaxHist];
-            for (int i = 0; i < it; ++i) vs[i] = c->hS[slots[i]];
-            // the gather that produced g has left the alphas of this list in S_ALPHA
-            const bool have_alpha = c->alpha_n == it && std::equal(slots, slots + it, c->alpha_slots);
-            if (!have_alpha) launch_multidot(c, g, vs, it, 1, slots);   // and the alpha recurrence
-            c->alpha_n = -1;
-            launch_solve(c, g, c->rv, slots, it);   // of q = g - sum alpha_j y_j, formed by the solve
-            for (int i = 0; i < it; ++i) vs[i] = c->hY[slots[i]];
-            if (c->lcfg.linesearch == 1) {
-                launch_multidot(c, c->rv, vs, it, 2, slots);   // and the beta recurrence
-                launch_axpy_r(c, g, slots, it);
-            } else {
-                launch_multidot(c, c->rv, vs, it, 3, slots, g);   // the beta recurrence and g . r
-                lazy_r = true;
-            }
-            const double* d = scal_read(c, S_DIR, 1);
-            if (!d) return WRAPD_ERR_HIP;
-            dir = d[0];
-        } else {
-            launch_solve(c, g, c->rv);
-            launch_dot(c, g, c->rv);
-            const double* d = scal_read(c, S_RED, 1);
-            if (!d) return WRAPD_ERR_HIP;
-            dir = d[0];
-        }
+        if (!st.live) stage(st);
+        const double* dp = scal_read(c, st.dir_at, 1);
+        if (!dp) return WRAPD_ERR_HIP;
+        const double dir = dp[0];
+        st.live = false;
+        const double* g = st.g;
+        const int it = st.it;
+        const int* slots = st.slots;
+        bool lazy_r = st.lazy_r;
         if (std::isnan(dir)) {
             if (solve_aborted(c)) {
                 c->last_error = "triangular solve: a dependency wait timed out";
@@ -847,6 +873,10 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
         arm_tail(c, last ? 3 : 1, 0u, S_VAL);
         launch_defo(c, last ? (DM_GRAD | DM_VALUE | DM_WRITE_SYMF | DM_ENERGY) : (DM_GRAD | DM_VALUE), 0);
         launch_gather_grad(c, c->grad, 2, new_pos, lslots, nlist, lslots, nlist);   // and the next alphas
+        if (!last) {   // the loop goes on unless y . s is 0: the next iteration's first half goes out now
+            c->m_k++;
+            stage(st);
+        }
         const double* d = scal_read(c, S_DENOM, 5);   // DENOM, DG, VAL, EN, FLIPS
         if (!d) return WRAPD_ERR_HIP;
         const double denom = d[0];
@@ -863,13 +893,17 @@ int wrapd_global_step(wrapd_ctx* c, double tol, wrapd_global_out* out) {
             break;
         }
         if (std::fabs(denom) <= 0) {
+            if (st.live) {   // the pair is not kept: neither its count nor the next iteration's launches
+                unstage(st);
+                c->m_k--;
+            }
             breaking_out = true;
             o.exit_reason = 2;
             have_final = false;
             break;
         }
         alpha_init = 1.0;
-        c->m_k++;
+        if (!st.live) c->m_k++;   // (counted before the next iteration was queued otherwise)
     }
     c->m_k = std::min(M, c->m_k);
     o.iters = breaking_out ? global_iter : global_iter - 1;
