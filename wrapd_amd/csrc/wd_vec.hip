@@ -147,6 +147,16 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
     __shared__ double sres[KG], gm[kMaxHist * kMaxHist], sa[kMaxHist];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int v = lb * blockDim.x + threadIdx.x;
+    // the block that finishes the reduction loads the list's Gram entries (written by earlier
+    // launches; mode 2's new row and column come from this launch's sums) now, so their round
+    // trip overlaps its own gather
+    if (MODE != 0 && LA.n > 0 && tail.armed && blockIdx.x == gridDim.x - 1) {
+        const int n = LA.n;
+        for (int q = threadIdx.x; q < n * n; q += kVecThreads) {
+            const int i = q / n, j = q - (q / n) * n;
+            if (MODE != 2 || (i != ns_pos && j != ns_pos)) gm[q] = scal[S_GRAM + LA.slot[i] * kMaxHist + LA.slot[j]];
+        }
+    }
     double g[3] = {0.0, 0.0, 0.0};
     if (v < n_f) {
         // kGatherBatch corners at a time: their indices, then their rows, are loaded together
@@ -257,14 +267,12 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
         if (MODE == 2) gram_store(scal, tail.hout - S_RED, sres, L, ns_pos);
         if (LA.n > 0) {   // the next two-loop's alphas; mode 2's new Gram row and column come from sres
             const int n = LA.n;
-            for (int q = threadIdx.x; q < n * n; q += kVecThreads) {
-                const int i = q / n, j = q - (q / n) * n;
-                double gij;
-                if (MODE == 2 && i == ns_pos) gij = sres[2 * j];
-                else if (MODE == 2 && j == ns_pos) gij = sres[2 * i + 1];
-                else gij = scal[S_GRAM + LA.slot[i] * kMaxHist + LA.slot[j]];
-                gm[q] = gij;
-            }
+            if (MODE == 2)   // the rest of gm was loaded at the start
+                for (int q = threadIdx.x; q < n * n; q += kVecThreads) {
+                    const int i = q / n, j = q - (q / n) * n;
+                    if (i == ns_pos) gm[q] = sres[2 * j];
+                    else if (j == ns_pos) gm[q] = sres[2 * i + 1];
+                }
             if (static_cast<int>(threadIdx.x) < n) scal[S_GS + threadIdx.x] = sres[kd + threadIdx.x];
             __syncthreads();
             if (threadIdx.x == 0) lbfgs_alpha(scal, sres + kd, gm, sa, n);
@@ -283,6 +291,15 @@ k_multidot(long long n3, const double* __restrict__ base, VecPtrs V, int nvec, c
     __shared__ double sres[KM], gm[kMaxHist * kMaxHist], sa[kMaxHist], sc[kMaxHist], sg[kMaxHist];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const long long i0 = static_cast<long long>(lb) * blockDim.x * 4 + threadIdx.x;
+    // the block that finishes the reduction loads the recurrence's inputs (written by earlier
+    // launches) now, so their round trip overlaps its own dots
+    if (post && tail.armed && blockIdx.x == gridDim.x - 1) {
+        const int n = L.n;
+        for (int q = threadIdx.x; q < n * n; q += kVecThreads)
+            gm[q] = scal[S_GRAM + L.slot[q / n] * kMaxHist + L.slot[q % n]];
+        if (post >= 2 && static_cast<int>(threadIdx.x) < n) sa[threadIdx.x] = scal[S_ALPHA + threadIdx.x];
+        if (post == 3 && static_cast<int>(threadIdx.x) < n) sg[threadIdx.x] = scal[S_GS + threadIdx.x];
+    }
     double acc[KM];
 #pragma unroll
     for (int j = 0; j < KM; ++j) acc[j] = 0.0;
@@ -309,10 +326,6 @@ k_multidot(long long n3, const double* __restrict__ base, VecPtrs V, int nvec, c
     if (reduce_tail<kVecThreads, KM, false>(tail, partials, sres)) {
         if (post) {
             const int n = L.n;
-            for (int q = threadIdx.x; q < n * n; q += kVecThreads)
-                gm[q] = scal[S_GRAM + L.slot[q / n] * kMaxHist + L.slot[q % n]];
-            if (post >= 2 && static_cast<int>(threadIdx.x) < n) sa[threadIdx.x] = scal[S_ALPHA + threadIdx.x];
-            if (post == 3 && static_cast<int>(threadIdx.x) < n) sg[threadIdx.x] = scal[S_GS + threadIdx.x];
             if (post == 1 && static_cast<int>(threadIdx.x) < n) scal[S_GS + threadIdx.x] = sres[threadIdx.x];
             __syncthreads();
             if (threadIdx.x == 0) {
