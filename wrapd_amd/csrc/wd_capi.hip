@@ -19,7 +19,7 @@ wrapd_ctx::~wrapd_ctx() {
     f(vidx); f(Di); f(vol); f(kst); f(mu_e); f(lam_e); f(wsq); f(cand); f(U); f(P); f(symF); f(sig); f(gstage);
     f(xall); f(grad); f(grad_old); f(qv); f(rv); f(bv); f(xref);
     for (int i = 0; i < wd::kMaxHist; ++i) { f(hS[i]); f(hY[i]); }
-    f(v2e_ptr); f(v2e_idx); f(a_cptr); f(a_cidx); f(Ax);
+    f(v2e_ptr); f(cpos); f(a_cptr); f(a_cidx); f(Ax);
     f(G); f(ysol); f(xsol); f(ubuf); f(d_perm); f(d_iperm); f(d_col0); f(d_fsize); f(d_rowptr); f(d_rows);
     f(d_goff); f(d_uoff); f(d_asm_ptr); f(d_asm_idx); f(d_udest); f(d_leaf_ptr); f(d_leaf_sn); f(d_flow_f); f(d_flow_b);
     f(d_flow_abort); f(d_bcopy); f(bperm);
@@ -310,7 +310,8 @@ int wrapd_upload_mesh(wrapd_ctx* c, int n_v, const double* rest, int n_e, const 
                 }
             }
     }
-    // vertex -> element-corner gather lists (element order)
+    // vertex -> element-corner gather lists (element order); the gradient staging is laid out
+    // in this order, so a vertex's corners are one contiguous range
     std::vector<int> vptr(n_f + 1, 0), vlist;
     for (int e = 0; e < n_e; ++e)
         for (int q = 0; q < 4; ++q) {
@@ -327,6 +328,8 @@ int wrapd_upload_mesh(wrapd_ctx* c, int n_v, const double* rest, int n_e, const 
                 if (f >= 0) vlist[fill[f]++] = 4 * e + q;
             }
     }
+    std::vector<int> cpos(4ull * n_ep, -1);   // element corner -> its row of the staging
+    for (size_t q = 0; q < vlist.size(); ++q) cpos[vlist[q]] = static_cast<int>(q);
     // device allocations
     int rc = 0;
     rc |= dalloc(c, c->vidx, 4ull * n_ep);
@@ -339,7 +342,7 @@ int wrapd_upload_mesh(wrapd_ctx* c, int n_v, const double* rest, int n_e, const 
     rc |= dalloc(c, c->P, 9ull * n_ep);
     rc |= dalloc(c, c->symF, 9ull * n_ep);
     rc |= dalloc(c, c->sig, 3ull * n_ep);
-    rc |= dalloc(c, c->gstage, 12ull * n_e);
+    rc |= dalloc(c, c->gstage, 3ull * vlist.size());
     rc |= dalloc(c, c->xall, 3ull * (n_f + c->n_fix));
     rc |= dalloc(c, c->grad, 3ull * n_f);
     rc |= dalloc(c, c->grad_old, 3ull * n_f);
@@ -348,7 +351,7 @@ int wrapd_upload_mesh(wrapd_ctx* c, int n_v, const double* rest, int n_e, const 
     rc |= dalloc(c, c->bv, 3ull * n_f);
     for (int i = 0; i < kMaxHist; ++i) { rc |= dalloc(c, c->hS[i], 3ull * n_f); rc |= dalloc(c, c->hY[i], 3ull * n_f); }
     rc |= dalloc(c, c->v2e_ptr, vptr.size());
-    rc |= dalloc(c, c->v2e_idx, vlist.size());
+    rc |= dalloc(c, c->cpos, cpos.size());
     rc |= dalloc(c, c->a_cptr, ccnt.size());
     rc |= dalloc(c, c->a_cidx, cidx.size());
     rc |= dalloc(c, c->Ax, c->nnzA);
@@ -357,7 +360,7 @@ int wrapd_upload_mesh(wrapd_ctx* c, int n_v, const double* rest, int n_e, const 
     if (rc) return WRAPD_ERR_HIP;
     rc |= h2d(c, c->vidx, vidx.data(), vidx.size());
     rc |= h2d(c, c->v2e_ptr, vptr.data(), vptr.size());
-    rc |= h2d(c, c->v2e_idx, vlist.data(), vlist.size());
+    rc |= h2d(c, c->cpos, cpos.data(), cpos.size());
     rc |= h2d(c, c->a_cptr, ccnt.data(), ccnt.size());
     rc |= h2d(c, c->a_cidx, cidx.data(), cidx.size());
     std::vector<double> me, le;

@@ -80,7 +80,7 @@ struct wrapd_ctx {
     double* P = nullptr;        // 9 x n_ep, P = Z - U
     double* symF = nullptr;     // 9 x n_ep (Ft for the rotation-unaware mode)
     double* sig = nullptr;      // 3 x n_ep last prox solution
-    double* gstage = nullptr;   // n_e x 12 gradient staging (AoS)
+    double* gstage = nullptr;   // corner gradients (3 doubles) of the free corners, vertex-major: in v2e order
 
     // vertex vectors (n_f x 3 AoS)
     double* xall = nullptr;     // (n_f + n_fix) x 3
@@ -92,8 +92,8 @@ struct wrapd_ctx {
     double* xref = nullptr;     // reference positions for the x-error log
     double* hS[wd::kMaxHist] = {};
     double* hY[wd::kMaxHist] = {};
-    int* v2e_ptr = nullptr;     // free vertex -> element corners (e*4+k), element order
-    int* v2e_idx = nullptr;
+    int* v2e_ptr = nullptr;     // free vertex v's corners are gstage rows [v2e_ptr[v], v2e_ptr[v+1]), element order
+    int* cpos = nullptr;        // int4 per element: each corner's gstage row (-1: a fixed vertex)
 
     // A
     std::vector<int> Ap, Aj;    // upper CSR over free vertices (host)

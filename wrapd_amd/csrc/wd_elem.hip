@@ -190,8 +190,8 @@ __global__ void __launch_bounds__(kElemThreads)
 k_defo(int n_e, int n_ep, int n_f, const int4* __restrict__ vidx, const double* __restrict__ Di,
        const double* __restrict__ xall, const double* __restrict__ P, const double* __restrict__ wsq,
        const double* __restrict__ vol, const double* __restrict__ kst, double* __restrict__ symF,
-       double* __restrict__ gstage, double* __restrict__ cand, MatArgs m, int use_hessian,
-       double* __restrict__ partials, RedTail tail) {
+       double* __restrict__ gstage, const int4* __restrict__ cpos, double* __restrict__ cand, MatArgs m,
+       int use_hessian, double* __restrict__ partials, RedTail tail) {
     __shared__ double shk[3][kElemThreads / 64];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int e = lb * blockDim.x + threadIdx.x;
@@ -200,6 +200,9 @@ k_defo(int n_e, int n_ep, int n_f, const int4* __restrict__ vidx, const double* 
         M3<double> F, Fx;
         double d[3][4];
         const int4 vi = vidx[e];
+        // where the corner gradients go: each free corner's row of the vertex-major staging
+        const int4 cp = (MODE & (DM_GRAD | DM_BVEC)) ? cpos[e] : make_int4(-1, -1, -1, -1);
+        const int cq[4] = {cp.x, cp.y, cp.z, cp.w};
         if (MODE & DM_DIRECT) gather_F<false>(vi, Di, n_ep, e, n_f, xall, F, d, &Fx);
         else gather_F<true>(vi, Di, n_ep, e, n_f, xall, F, d, &Fx);
         if (MODE & DM_FT_CACHE) st9(symF, n_ep, e, transpose(F));
@@ -241,25 +244,29 @@ k_defo(int n_e, int n_ep, int n_f, const int4* __restrict__ vidx, const double* 
                 M3<double> Mm;
 #pragma unroll
                 for (int i = 0; i < 9; ++i) Mm.m[i] = F.m[i] - T.m[i];
-                double* g = gstage + 12ll * e;
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
+                    if (cq[k] >= 0) {
+                        double* g = gstage + 3ll * cq[k];
 #pragma unroll
-                    for (int c = 0; c < 3; ++c)
-                        g[3 * k + c] = d[0][k] * (w2 * Mm(c, 0)) + d[1][k] * (w2 * Mm(c, 1)) + d[2][k] * (w2 * Mm(c, 2));
+                        for (int c = 0; c < 3; ++c)
+                            g[c] = d[0][k] * (w2 * Mm(c, 0)) + d[1][k] * (w2 * Mm(c, 1)) + d[2][k] * (w2 * Mm(c, 2));
+                    }
             }
         }
         if (MODE & DM_BVEC) {
             // add_to_b: w^2 D^T (Z - U - D_fix x_fix), the cache here holds Ft so P = Z - U
-            double* g = gstage + 12ll * e;
 #pragma unroll
             for (int k = 0; k < 4; ++k)
+                if (cq[k] >= 0) {
+                    double* g = gstage + 3ll * cq[k];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    double acc = 0.0;
+                    for (int c = 0; c < 3; ++c) {
+                        double acc = 0.0;
 #pragma unroll
-                    for (int r = 0; r < 3; ++r) acc += (w2 * d[r][k]) * (Pm(r, c) - Fx(r, c));
-                    g[3 * k + c] = acc;
+                        for (int r = 0; r < 3; ++r) acc += (w2 * d[r][k]) * (Pm(r, c) - Fx(r, c));
+                        g[c] = acc;
+                    }
                 }
         }
         if (MODE & DM_ENERGY) {
@@ -751,7 +758,8 @@ static void defo_mode(wrapd_ctx* c, int use_hessian) {
     const RedTail t = take_tail(c, nb, true);   // used by the modes that reduce (value, energy)
     hipLaunchKernelGGL(k, dim3(nb), dim3(kElemThreads), 0, c->stream, c->n_e, c->n_ep, c->n_f,
                        reinterpret_cast<const int4*>(c->vidx), c->Di, c->xall, c->P, c->wsq, c->vol, c->kst,
-                       c->symF, c->gstage, c->cand, mat_args(c), use_hessian, c->partials, t);
+                       c->symF, c->gstage, reinterpret_cast<const int4*>(c->cpos), c->cand, mat_args(c), use_hessian,
+                       c->partials, t);
     finish_reduction(c);
 }
 

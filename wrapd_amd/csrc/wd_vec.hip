@@ -138,7 +138,7 @@ __device__ void lbfgs_beta(double* __restrict__ scal, const double* red, const d
 // and kept out of scal.  KG: the most slots of the launch.
 template <int MODE, int KG>
 __global__ void __launch_bounds__(kVecThreads)
-k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, const double* __restrict__ gstage,
+k_gather(int n_f, const int* __restrict__ ptr, const double* __restrict__ gstage,
          double* __restrict__ out, const double* __restrict__ gold, double* __restrict__ ynew,
          const double* __restrict__ snew, VecPtrs S, VecPtrs Y, SlotList L, int ns_pos, SlotList LA, VecPtrs SA,
          double* __restrict__ partials, RedTail tail, double* __restrict__ scal) {
@@ -159,19 +159,16 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
     }
     double g[3] = {0.0, 0.0, 0.0};
     if (v < n_f) {
-        // kGatherBatch corners at a time: their indices, then their rows, are loaded together
-        // (a vertex has ~20 corners); the sums are taken in corner order
+        // the vertex's corners are the staging rows [ptr[v], ptr[v+1]) (a vertex has ~20),
+        // loaded kGatherBatch at a time and summed in corner order
         const int q1 = ptr[v + 1];
         for (int qb = ptr[v]; qb < q1; qb += kGatherBatch) {
-            int id[kGatherBatch];
-#pragma unroll
-            for (int u = 0; u < kGatherBatch; ++u) id[u] = qb + u < q1 ? idx[qb + u] : -1;
             double a[kGatherBatch][3];
 #pragma unroll
             for (int u = 0; u < kGatherBatch; ++u) {
                 a[u][0] = a[u][1] = a[u][2] = 0.0;
-                if (id[u] >= 0) {
-                    const double* p = gstage + 3ll * id[u];
+                if (qb + u < q1) {
+                    const double* p = gstage + 3ll * (qb + u);
                     a[u][0] = p[0];
                     a[u][1] = p[1];
                     a[u][2] = p[2];
@@ -179,7 +176,7 @@ k_gather(int n_f, const int* __restrict__ ptr, const int* __restrict__ idx, cons
             }
 #pragma unroll
             for (int u = 0; u < kGatherBatch; ++u)
-                if (id[u] >= 0) {
+                if (qb + u < q1) {
                     g[0] += a[u][0];
                     g[1] += a[u][1];
                     g[2] += a[u][2];
@@ -526,7 +523,7 @@ void launch_gather_grad(wrapd_ctx* c, double* out, int mode, int new_pos, const 
     const int need = mode == 2 ? 2 * nlist + 1 + nalpha : 0;
     auto k = mode == 0 ? k_gather<0, 1> : mode == 1 ? k_gather<1, 1 + kMaxHist>
                                                     : (need <= kSmall ? k_gather<2, kSmall> : k_gather<2, 3 * kMaxHist + 1>);
-    hipLaunchKernelGGL(k, dim3(nb), dim3(kVecThreads), 0, c->stream, c->n_f, c->v2e_ptr, c->v2e_idx, c->gstage, out,
+    hipLaunchKernelGGL(k, dim3(nb), dim3(kVecThreads), 0, c->stream, c->n_f, c->v2e_ptr, c->gstage, out,
                        c->grad_old, ynew, snew, S, Y, L, new_pos, LA, SA, c->partials, t, c->scal);
 }
 
