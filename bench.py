@@ -60,10 +60,13 @@ def algorithmic_bytes(prob, stats):
     # local step per element: symF, U, P (3 x 72), w^2, vol, k (3 x 8), last s (24)
     # [+ mu, lambda], writes U, P (2 x 72), s (24), candidate w^2 (8)
     local = n_e * (3 * 72 + 3 * 8 + 24 + (16 if hetero else 0) + 2 * 72 + 24 + 8)
-    # value + gradient pass per element: vertex indices 16, D^-1 96, P 72, w^2 8, x gather,
-    # staged gradient 96 written; the vertex gather re-reads the staging (96) and writes x3
-    defo = n_e * (16 + 96 + 72 + 8 + 96) + 8 * 3 * n_f
-    gather = n_e * 96 + 4 * n_e * 4 + 8 * 3 * n_f
+    # value + gradient pass per element: vertex indices and staging rows (2 x 16), D^-1 96,
+    # P 72, w^2 8, x gather; each free corner's gradient (24) written to the vertex-major
+    # staging, which the vertex gather reads back as contiguous ranges (plus its row pointers)
+    # before writing the gradient
+    free_corners = int(np.isin(np.asarray(prob.tets), np.asarray(prob.pins), invert=True).sum())
+    defo = n_e * (2 * 16 + 96 + 72 + 8) + 24 * free_corners + 8 * 3 * n_f
+    gather = 24 * free_corners + 4 * (n_f + 1) + 8 * 3 * n_f
     g = 8.0 * stats["g_entries"]
     vec = 8 * 3 * n_f
     m = int(prob.settings.get("lbfgs_m") or 0)
