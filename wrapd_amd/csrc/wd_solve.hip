@@ -209,20 +209,33 @@ __device__ __forceinline__ void fwd_leaf_body(int t, int nleaf, const int* __res
     }
     const int64_t ubase = L.uoff[0];
     const int r = threadIdx.x;
+    // this row's permutation entry and assembly range of step i: loaded one step ahead, so a
+    // step's second-level loads (the right-hand side, the update indices) go out at its start
+    int old_n = 0;
+    int64_t k0_n = 0, k1_n = 0;
+    auto maps_of = [&](int i) {
+        const int c0 = L.col0[i], ns = L.col0[i + 1] - c0, f = L.fsize[i];
+        const int64_t rp = L.rowptr[i];
+        old_n = r < ns ? M.perm[c0 + r] : 0;
+        k0_n = r < f ? M.asm_ptr[rp + r] : 0;
+        k1_n = r < f ? M.asm_ptr[rp + r + 1] : 0;
+    };
+    if (nsn > 0) maps_of(0);
     for (int i = 0; i < nsn; ++i) {
         const int c0 = L.col0[i], ns = L.col0[i + 1] - c0, f = L.fsize[i];
-        const int64_t rp = L.rowptr[i], go = L.goff[i], uo = L.uoff[i];
+        const int64_t go = L.goff[i], uo = L.uoff[i];
         const int jend = r < ns ? r + 1 : (r < f ? ns : 0);   // L11^{-1} is lower triangular
         const double* gp = G + go + r;
         double gv[B];
         const int lg = wrapd::g_ld(f);
 #pragma unroll
         for (int u = 0; u < B; ++u) gv[u] = u < jend ? gp[static_cast<int64_t>(u) * lg] : 0.0;
-        const int old = r < ns ? M.perm[c0 + r] : 0;
-        const int64_t k0 = r < f ? M.asm_ptr[rp + r] : 0, k1 = r < f ? M.asm_ptr[rp + r + 1] : 0;
+        const int old = old_n;
+        const int64_t k0 = k0_n, k1 = k1_n;
         int64_t ai[KA];
 #pragma unroll
         for (int i = 0; i < KA; ++i) ai[i] = k0 + i < k1 ? M.asm_idx[k0 + i] : 0;
+        if (i + 1 < nsn) maps_of(i + 1);
         if (r < f) {
             double a0 = 0.0, a1 = 0.0, a2 = 0.0;
             if (r < ns) {
