@@ -45,13 +45,13 @@ struct SolveMaps {
 };
 
 // t += sum_k g[k * gstride] * v[k * vstride + 0..2], k < n, in order of k.  Loads are
-// issued in batches of 8 independent reads before they are used (the last batch
-// predicated), so a thread has up to 8 misses in flight instead of one: the solves'
-// dense products are latency-bound otherwise.
+// issued in batches of 12 independent reads before they are used (the last batch
+// predicated), so a thread has up to 12 misses in flight instead of one: the solves'
+// dense products are latency-bound otherwise (16 costs the forward kernel occupancy).
 __device__ __forceinline__ void strided_dot3(const double* __restrict__ g, int64_t gstride,
                                              const double* __restrict__ v, int vstride, int n, double& t0,
                                              double& t1, double& t2) {
-    constexpr int B = 8;
+    constexpr int B = 12;
     int k = 0;
     for (; k + B <= n; k += B) {
         double gv[B];
