@@ -144,50 +144,166 @@ __device__ __forceinline__ double linv_at(const double* __restrict__ F, int f, i
 }
 
 // ---------------------------------------------------------------------------------
-// Cholesky of the diagonal tile and the inverse of its factor, both in LDS
-// (column-major: D[c * 64 + r]); the inverse by substitution on the identity
+// Cholesky of the diagonal tile and the inverse of its factor, blocked by 16.
+//
+// The 64x64 tile D (LDS, column-major D[c * 64 + r]) is factored in four block steps:
+// wave 0 factors the 16x16 diagonal block in registers (lane r holds row r; columns are
+// broadcast with v_readlane, no barrier inside the block) and inverts it; the panel below
+// becomes L_IJ = A_IJ L_JJ^{-T} and the trailing blocks get A_IK -= L_IJ L_KJ^T, both as
+// 16x16x16 products on the matrix cores, one block per wave.  The inverse of the whole
+// factor is then assembled block row by block row, X_IJ = -X_II sum_K L_IK X_KJ.  Three
+// barriers per block step instead of three per column.
+__device__ __forceinline__ double lane_bcast(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// the 16x16x4 product of a 16-row and a 16-column operand, 4 k-steps (k = 16): lane l
+// supplies a[kc] = Aop[l % 16][4 kc + l / 16] and b[kc] = Bop[4 kc + l / 16][l % 16]
+__device__ __forceinline__ dbl4 mma16(const double a[4], const double b[4], dbl4 acc) {
+#pragma unroll
+    for (int kc = 0; kc < 4; ++kc) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kc], b[kc], acc, 0, 0, 0);
+    return acc;
+}
+
 __global__ void __launch_bounds__(kFactThreads)
 k_fpotrf(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ fail) {
     __shared__ double D[kT * kT];
-    __shared__ double X[kT * kT];
+    __shared__ double X[kT * kT];   // L^{-1}, column-major, lower blocks only
     const int s = tasks[blockIdx.x].x;
     const int f = A.fsize[s];
     const int ns = A.col0[s + 1] - A.col0[s];
     const int c0 = kT * k;
     const int nb = min(kT, ns - c0);
+    const int nblk = (nb + 15) >> 4;   // 16-blocks holding tile columns
     double* F = A.pool + A.foff[s];
+    // pad rows/columns beyond nb with the identity so every 16-block factors cleanly
     for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
         const int c = idx >> 6, r = idx & 63;
-        D[idx] = (c < nb && r < nb && r >= c) ? F[static_cast<int64_t>(c0 + c) * f + c0 + r] : 0.0;
-        X[idx] = (c == r && c < nb) ? 1.0 : 0.0;
+        D[idx] = (c < nb && r < nb) ? (r >= c ? F[static_cast<int64_t>(c0 + c) * f + c0 + r] : 0.0)
+                                    : (r == c ? 1.0 : 0.0);
+        X[idx] = 0.0;
     }
     __syncthreads();
-    const int r = threadIdx.x & 63, cg = threadIdx.x >> 6;
-    for (int p = 0; p < nb; ++p) {
-        if (threadIdx.x == 0) {
-            double d = D[p * 65];
-            if (!(d > 0.0)) {
-                *fail = 1;
-                d = 1.0;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int li = l & 15, lk = l >> 4;
+    int bad = 0;
+    for (int J = 0; J < nblk; ++J) {
+        const int j0 = 16 * J;
+        if (w == 0) {
+            // Cholesky of D_JJ: lane li holds row li (lanes 16..63 repeat rows, write nothing)
+            double a[16], x[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) a[c] = D[(j0 + c) * 64 + j0 + li];
+#pragma unroll
+            for (int p = 0; p < 16; ++p) {
+                double d = lane_bcast(a[p], p);
+                if (!(d > 0.0)) {
+                    bad = 1;
+                    d = 1.0;
+                }
+                const double sq = sqrt(d);
+                const double inv = 1.0 / sq;
+                a[p] = li == p ? sq : (li > p ? a[p] * inv : a[p]);
+#pragma unroll
+                for (int c = p + 1; c < 16; ++c) {
+                    const double lc = lane_bcast(a[p], c);   // L[c][p]
+                    if (li >= c) a[c] -= a[p] * lc;
+                }
             }
-            D[p * 65] = sqrt(d);
+            // X_JJ = L_JJ^{-1} by substitution on the identity (row li in x)
+#pragma unroll
+            for (int c = 0; c < 16; ++c) x[c] = c == li ? 1.0 : 0.0;
+#pragma unroll
+            for (int p = 0; p < 16; ++p) {
+                const double rp = 1.0 / lane_bcast(a[p], p);
+#pragma unroll
+                for (int c = 0; c <= p; ++c) x[c] = li == p ? x[c] * rp : x[c];
+#pragma unroll
+                for (int c = 0; c <= p; ++c) {
+                    const double xpc = lane_bcast(x[c], p);   // X[p][c]
+                    if (li > p) x[c] -= a[p] * xpc;
+                }
+            }
+            if (l < 16) {
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    if (li >= c) {
+                        D[(j0 + c) * 64 + j0 + li] = a[c];
+                        X[(j0 + c) * 64 + j0 + li] = x[c];
+                    }
+                }
+            }
         }
         __syncthreads();
-        if (threadIdx.x < nb - p - 1) D[p * 64 + p + 1 + threadIdx.x] /= D[p * 65];
+        // panel: L_IJ = A_IJ X_JJ^T for I > J, one block per wave.  Computed transposed,
+        // C^T = X_JJ A_IJ^T, so consecutive lanes hold consecutive rows of L_IJ.
+        {
+            const int I = J + 1 + w;
+            if (w < 3 && I < 4) {
+                double a[4], b[4];
+#pragma unroll
+                for (int kc = 0; kc < 4; ++kc) {
+                    const int kk = 4 * kc + lk;
+                    a[kc] = X[(j0 + kk) * 64 + j0 + li];       // X_JJ[li][kk]
+                    b[kc] = D[(j0 + kk) * 64 + 16 * I + li];   // A_IJ[li][kk]
+                }
+                dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
+                acc = mma16(a, b, acc);   // acc[q] = C^T[4q + lk][li] = L_IJ[li][4q + lk]
+#pragma unroll
+                for (int q = 0; q < 4; ++q) D[(j0 + 4 * q + lk) * 64 + 16 * I + li] = acc[q];
+            }
+        }
         __syncthreads();
-        if (r > p && r < nb) {
-            const double lr = D[p * 64 + r];
-            for (int c = p + 1 + cg; c <= r; c += 4) D[c * 64 + r] -= lr * D[p * 64 + c];
+        // trailing blocks: A_IK -= L_IJ L_KJ^T for J < K <= I < 4
+        {
+            int t = 0;
+            for (int K = J + 1; K < 4; ++K)
+                for (int I = K; I < 4; ++I, ++t) {
+                    if ((t & 3) != w) continue;
+                    double a[4], b[4];
+#pragma unroll
+                    for (int kc = 0; kc < 4; ++kc) {
+                        const int kk = 4 * kc + lk;
+                        a[kc] = D[(j0 + kk) * 64 + 16 * K + li];   // L_KJ[li][kk]
+                        b[kc] = D[(j0 + kk) * 64 + 16 * I + li];   // L_IJ[li][kk]
+                    }
+                    dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
+                    acc = mma16(a, b, acc);   // acc[q] = (L_IJ L_KJ^T)[li][4q + lk]
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) D[(16 * K + 4 * q + lk) * 64 + 16 * I + li] -= acc[q];
+                }
         }
         __syncthreads();
     }
-    // X = L^{-1}: row p of X is final after dividing by L_pp; then eliminate below
-    for (int p = 0; p < nb; ++p) {
-        if (threadIdx.x <= p) X[threadIdx.x * 64 + p] /= D[p * 65];
-        __syncthreads();
-        if (r > p && r < nb) {
-            const double l = D[p * 64 + r];
-            for (int j = cg; j <= p; j += 4) X[j * 64 + r] -= l * X[j * 64 + p];
+    if (w == 0 && l == 0 && bad) *fail = 1;
+    // off-diagonal blocks of the inverse: X_IJ = -X_II (sum_{K=J}^{I-1} L_IK X_KJ), by block rows
+    for (int I = 1; I < nblk; ++I) {
+        const int J = w;
+        if (J < I) {
+            dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};   // T = sum_K L_IK X_KJ, acc[q] = T[4q + lk][li]
+            for (int K = J; K < I; ++K) {
+                double a[4], b[4];
+#pragma unroll
+                for (int kc = 0; kc < 4; ++kc) {
+                    const int kk = 4 * kc + lk;
+                    a[kc] = D[(16 * K + kk) * 64 + 16 * I + li];   // L_IK[li][kk]
+                    b[kc] = X[(16 * J + li) * 64 + 16 * K + kk];   // X_KJ[kk][li]
+                }
+                acc = mma16(a, b, acc);
+            }
+            // acc[q] = T[4q + lk][li] is the B operand layout for k-step q
+            double a[4], b[4];
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) {
+                a[kc] = X[(16 * I + 4 * kc + lk) * 64 + 16 * I + li];   // X_II[li][4kc + lk]
+                b[kc] = acc[kc];
+            }
+            dbl4 r = dbl4{0.0, 0.0, 0.0, 0.0};
+            r = mma16(a, b, r);   // r[q] = (X_II T)[4q + lk][li]
+#pragma unroll
+            for (int q = 0; q < 4; ++q) X[(16 * J + li) * 64 + 16 * I + 4 * q + lk] = -r[q];
         }
         __syncthreads();
     }
