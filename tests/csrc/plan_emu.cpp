@@ -111,6 +111,19 @@ extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
                         for (int r = c; r < nb; ++r)
                             wrF(Fi(c0 + r, c0 + c), rdF(Fi(c0 + r, c0 + c)) - rdF(Fi(c0 + r, c0 + p)) * rdF(Fi(c0 + c, c0 + p)));
                 }
+                // the kernel also stores L_kk^{-1} in G's diagonal tile (k, k)
+                std::vector<double> X(kFactorTile * kFactorTile, 0.0);
+                for (int j = 0; j < nb; ++j) X[j * 64 + j] = 1.0;
+                for (int p = 0; p < nb; ++p) {
+                    const double d = rdF(Fi(c0 + p, c0 + p));
+                    for (int j = 0; j < nb; ++j) X[j * 64 + p] /= d;
+                    for (int r = p + 1; r < nb; ++r) {
+                        const double l = rdF(Fi(c0 + r, c0 + p));
+                        for (int j = 0; j < nb; ++j) X[j * 64 + r] -= l * X[j * 64 + p];
+                    }
+                }
+                for (int j = 0; j < nb; ++j)
+                    for (int r = 0; r < nb; ++r) wrG(Gi(c0 + r, c0 + j), X[j * 64 + r]);
             } else if (L.kind == FK_TRSM) {
                 const int kind = y >> kTaskKindShift, t = y & ((1 << kTaskKindShift) - 1);
                 int t0, nt;
@@ -125,11 +138,10 @@ extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
                                 wrF(Fi(t0 + r, c0 + c), rdF(Fi(t0 + r, c0 + c)) - rdF(Fi(t0 + r, c0 + p)) * l);
                         }
                     }
-                } else {
+                } else if (t != k) {   // the diagonal tile X[k, k] was stored by POTRF
                     std::vector<double> X(kFactorTile * kFactorTile, 0.0);
                     for (int j = 0; j < nt; ++j)
-                        for (int r = 0; r < nb; ++r)
-                            X[j * 64 + r] = (t == k) ? (r == j ? 1.0 : 0.0) : rdG(Gi(c0 + r, t0 + j));
+                        for (int r = 0; r < nb; ++r) X[j * 64 + r] = rdG(Gi(c0 + r, t0 + j));
                     for (int p = 0; p < nb; ++p) {
                         const double d = rdF(Fi(c0 + p, c0 + p));
                         for (int j = 0; j < nt; ++j) X[j * 64 + p] /= d;
@@ -141,17 +153,16 @@ extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
                     for (int j = 0; j < nt; ++j)
                         for (int r = 0; r < nb; ++r) wrG(Gi(c0 + r, t0 + j), X[j * 64 + r]);
                 }
-            } else if (L.kind == FK_FINAL) {
+            } else if (L.kind == FK_FINAL) {   // 128x128 super-tiles (kFinalTile)
                 const int kind = y >> kTaskKindShift;
-                const int a = (y >> kTaskTileShift) & kTaskTileMask, b = y & kTaskTileMask;
-                const int P = tile_count_l11(ns);
-                int a0, na, b0, nbt;
-                tile(a, ns, f, a0, na);
-                tile(b, ns, f, b0, nbt);
+                const int u = (y >> kTaskTileShift) & kTaskTileMask, v = y & kTaskTileMask;
+                const int a0 = ns + kFinalTile * u, na = std::min(kFinalTile, f - a0);
+                const int b0 = kind == 0 ? ns + kFinalTile * v : kFinalTile * v;
+                const int nbt = std::min(kFinalTile, (kind == 0 ? f : ns) - b0);
                 for (int j = 0; j < nbt; ++j)
-                    for (int i = (kind == 0 && a == b) ? j : 0; i < na; ++i) {
+                    for (int i = (kind == 0 && u == v) ? j : 0; i < na; ++i) {
                         double acc = 0;
-                        for (int q = kind == 0 ? 0 : b * kFactorTile; q < P * kFactorTile && q < ns; ++q)
+                        for (int q = kind == 0 ? 0 : b0; q < ns; ++q)
                             acc += rdF(Fi(a0 + i, q)) * (kind == 0 ? rdF(Fi(b0 + j, q)) : rdG(Gi(q, b0 + j)));
                         if (kind == 0) wrF(Fi(a0 + i, b0 + j), rdF(Fi(a0 + i, b0 + j)) - acc);
                         else wrG(Gi(a0 + i, b0 + j), -acc);

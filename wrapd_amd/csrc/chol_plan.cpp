@@ -199,18 +199,20 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P) {
             }
         }
         begin(FK_FINAL, 0);
+        const double T2 = kFinalTile;
         for (int s : L) {
-            const int p11 = tile_count_l11(ns[s]);
-            const int nt = tile_count(ns[s], f[s]);
-            for (int a = p11; a < nt; ++a)
-                for (int b = p11; b <= a; ++b) {
-                    push(s, (a << kTaskTileShift) | b);
-                    P.flops += 2 * T * T * T * p11;
+            const int U = final_row_tiles(ns[s], f[s]);
+            const int C = final_col_tiles(ns[s]);
+            const double k32 = 32.0 * ((ns[s] + 31) / 32);   // the kernel's k-chunks are 32 deep
+            for (int u = 0; u < U; ++u)
+                for (int v = 0; v <= u; ++v) {
+                    push(s, (u << kTaskTileShift) | v);
+                    P.flops += 2 * T2 * T2 * k32;
                 }
-            for (int a = p11; a < nt; ++a)
-                for (int c = 0; c < p11; ++c) {
-                    push(s, (1 << kTaskKindShift) | (a << kTaskTileShift) | c);
-                    P.flops += 2 * T * T * T * (p11 - c);
+            for (int u = 0; u < U; ++u)
+                for (int c = 0; c < C; ++c) {
+                    push(s, (1 << kTaskKindShift) | (u << kTaskTileShift) | c);
+                    P.flops += 2 * T2 * T2 * (k32 - T2 * c);
                 }
         }
     }

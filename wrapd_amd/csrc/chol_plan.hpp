@@ -15,9 +15,9 @@
 //                      supernode's remaining columns only), and X[i, c] -= L[i, k] X[k, c]
 //                      for L11 row tiles k < i < P, c <= k
 //      after which X = L11^{-1} (the top of G_s);
-//   3. one launch for the rest, each tile a single pass over all ns columns:
-//        SCHUR         F[a, b] -= sum_q L[a, q] L[b, q]^T for update tiles a >= b >= P
-//        GBOT          G[a, c] = -sum_q L[a, q] X[q, c] for update rows a >= P, c < P
+//   3. one launch for the rest, each 128x128 super-tile a single pass over all ns columns:
+//        SCHUR         F[a, b] -= sum_q L[a, q] L[b, q]^T for update super-tiles a >= b
+//        GBOT          G[a, c] = -sum_q L[a, q] X[q, c] for update rows a, L11 columns c
 //      so the Schur complement is read and written once (not once per panel).
 //      Every task of a launch writes its own tile and reads only tiles that no task of
 //      the same launch writes.  G_s = [L11^{-1}; -L21 L11^{-1}] (f_s x ns_s, column-major).
@@ -67,6 +67,12 @@ struct FactorPlan {
 };
 
 constexpr int kFactorTile = 64;
+// FK_FINAL tasks work on 128x128 output super-tiles (twice the operand reuse of 64x64):
+// SCHUR (u, v), u >= v: rows ns + 128 u.., columns ns + 128 v.. of the update block;
+// GBOT (u, c): rows ns + 128 u.. of G's bottom, columns 128 c.. (< ns).
+constexpr int kFinalTile = 128;
+inline int final_row_tiles(int ns, int f) { return (f - ns + kFinalTile - 1) / kFinalTile; }
+inline int final_col_tiles(int ns) { return (ns + kFinalTile - 1) / kFinalTile; }
 
 inline int tile_count_l11(int ns) { return (ns + kFactorTile - 1) / kFactorTile; }
 inline int tile_count(int ns, int f) { return tile_count_l11(ns) + (f - ns + kFactorTile - 1) / kFactorTile; }

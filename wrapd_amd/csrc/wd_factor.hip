@@ -8,18 +8,19 @@
 //   k_fassemble  one wave per front column: zero, store A entries, add children's
 //                Schur-complement columns in child order (deterministic, no atomics)
 //   k_fpotrf     one workgroup per supernode: Cholesky L_kk of the panel's diagonal tile
-//                and its inverse in LDS; L_kk goes to the tile's lower half, the strict
-//                lower part of L_kk^{-1} (transposed) to its otherwise unused upper half
+//                and its inverse in LDS; L_kk goes to the tile's lower half, L_kk^{-1}
+//                to G's diagonal tile (k, k), which is also the diagonal tile of L11^{-1}
 //   k_ftrsm      TRSM tiles F[t, k] <- F[t, k] L_kk^{-T}, or GDIAG tiles
 //                X[k, c] <- L_kk^{-1} X[k, c], both as products with the stored inverse
 //   k_fupdate    SYRK/GEMM updates of the supernode's own later columns, or GEMM
 //                updates of X = L11^{-1}
 //   k_ffinal     the Schur complement F[a, b] -= L21 L21^T and G's bottom -L21 L11^{-1},
-//                each output tile accumulated over all ns columns in registers
+//                each 128x128 output super-tile accumulated over all ns columns in registers
 // Every product runs on the matrix cores in 64x64 output tiles (v_mfma_f64_16x16x4:
 // each of the 4 waves owns a 32x32 quarter as 2x2 16x16 accumulators) over 64-deep
 // operand chunks staged in two 32 KB LDS tiles; the accumulator layout puts
-// consecutive lanes on consecutive rows, so tile stores are coalesced.  All FP64.
+// consecutive lanes on consecutive rows, so tile stores are coalesced.  Tiles are loaded
+// with every thread's 16 loads in flight at once (tile_load).  All FP64.
 #include "wd_ctx.hpp"
 #include "chol_plan.hpp"
 #include "device/wd_reduce.hpp"
@@ -121,26 +122,66 @@ __device__ __forceinline__ void tile_mma(const double* __restrict__ As, const do
     }
 }
 
-// visits every accumulator entry: fn(i, j, value) with (i, j) in the 64x64 tile; within a
-// wave, lanes l and l+1 hold rows i and i+1 of the same column j
-template <typename Fn>
-__device__ __forceinline__ void tile_for_each(const dbl4 acc[2][2], Fn fn) {
+// 64x64 tile of a column-major block (leading dimension ld) into registers: thread t's
+// entry r (0..15) is (row t % 64, column t / 64 + 4 r), that is LDS index t + 256 r of a
+// column-major tile.  Rows >= nr and columns >= nc read as zero.  Every address is
+// clamped into the block (nr, nc >= 1), so the 16 loads carry no branch and issue together.
+__device__ __forceinline__ void tile_load(const double* __restrict__ src, int64_t ld, int nr, int nc, double v[16]) {
+    const int row = threadIdx.x & 63, col0 = threadIdx.x >> 6;
+    const bool rok = row < nr;
+    const double* p = src + (rok ? row : 0);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int col = col0 + 4 * r;
+        const bool ok = rok && col < nc;
+        const double x = p[static_cast<int64_t>(col < nc ? col : 0) * ld];
+        v[r] = ok ? x : 0.0;
+    }
+}
+// the registers of tile_load into an LDS tile, as loaded (T[col * 64 + row]) or transposed
+__device__ __forceinline__ void tile_to_lds(double* T, const double v[16]) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) T[threadIdx.x + kFactThreads * r] = v[r];
+}
+__device__ __forceinline__ void tile_to_lds_t(double* T, const double v[16]) {
+    const int row = threadIdx.x & 63, col0 = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) T[row * 64 + col0 + 4 * r] = v[r];
+}
+
+// accumulator tile into a column-major block C (leading dimension ldc): C[i][j] -= acc
+// (kSub) or C[i][j] = scale * acc, for i < nr, j < nc, and i >= j when lower_only.  With
+// kSub the 16 old values are loaded first, all at once (clamped addresses).
+template <bool kSub>
+__device__ __forceinline__ void tile_write(double* __restrict__ C, int64_t ldc, int nr, int nc, bool lower_only,
+                                           const dbl4 acc[2][2], double scale) {
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int j0 = 32 * (w >> 1), i0 = 32 * (w & 1);
+    double old[16];
+    if (kSub) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int i = i0 + 16 * b + (l & 15), j = j0 + 16 * a + 4 * q + (l >> 4);
+                    old[(a * 2 + b) * 4 + q] =
+                        C[static_cast<int64_t>(j < nc ? j : 0) * ldc + (i < nr ? i : 0)];
+                }
+    }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) fn(i0 + 16 * b + (l & 15), j0 + 16 * a + 4 * q + (l >> 4), acc[a][b][q]);
-}
-
-// Linv[i][q] of panel (c0, nb) from the diagonal tile: 1/L_ii on the diagonal, the stored
-// transpose above it, zero below the diagonal of the inverse's upper part
-__device__ __forceinline__ double linv_at(const double* __restrict__ F, int f, int c0, int i, int q) {
-    if (q > i) return 0.0;
-    if (q == i) return 1.0 / F[static_cast<int64_t>(c0 + i) * f + c0 + i];
-    return F[static_cast<int64_t>(c0 + i) * f + c0 + q];
+            for (int q = 0; q < 4; ++q) {
+                const int i = i0 + 16 * b + (l & 15), j = j0 + 16 * a + 4 * q + (l >> 4);
+                if (i < nr && j < nc && !(lower_only && i < j)) {
+                    double* d = C + static_cast<int64_t>(j) * ldc + i;
+                    *d = kSub ? old[(a * 2 + b) * 4 + q] - acc[a][b][q] : scale * acc[a][b][q];
+                }
+            }
 }
 
 // ---------------------------------------------------------------------------------
@@ -178,12 +219,19 @@ k_fpotrf(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ fa
     const int nb = min(kT, ns - c0);
     const int nblk = (nb + 15) >> 4;   // 16-blocks holding tile columns
     double* F = A.pool + A.foff[s];
-    // pad rows/columns beyond nb with the identity so every 16-block factors cleanly
-    for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
-        const int c = idx >> 6, r = idx & 63;
-        D[idx] = (c < nb && r < nb) ? (r >= c ? F[static_cast<int64_t>(c0 + c) * f + c0 + r] : 0.0)
-                                    : (r == c ? 1.0 : 0.0);
-        X[idx] = 0.0;
+    double* Fkk = F + static_cast<int64_t>(c0) * f + c0;
+    {
+        // the lower triangle; rows/columns beyond nb padded with the identity so every
+        // 16-block factors cleanly
+        double v[16];
+        tile_load(Fkk, f, nb, nb, v);
+        const int r = threadIdx.x & 63, cb = threadIdx.x >> 6;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int c = cb + 4 * q;
+            D[threadIdx.x + kFactThreads * q] = r < c ? 0.0 : (c < nb ? v[q] : (r == c ? 1.0 : 0.0));
+            X[threadIdx.x + kFactThreads * q] = 0.0;
+        }
     }
     __syncthreads();
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -193,7 +241,7 @@ k_fpotrf(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ fa
         const int j0 = 16 * J;
         if (w == 0) {
             // Cholesky of D_JJ: lane li holds row li (lanes 16..63 repeat rows, write nothing)
-            double a[16], x[16];
+            double a[16], x[16], rinv[16];   // rinv[p] = 1 / L[p][p] (uniform)
 #pragma unroll
             for (int c = 0; c < 16; ++c) a[c] = D[(j0 + c) * 64 + j0 + li];
 #pragma unroll
@@ -205,6 +253,7 @@ k_fpotrf(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ fa
                 }
                 const double sq = sqrt(d);
                 const double inv = 1.0 / sq;
+                rinv[p] = inv;
                 a[p] = li == p ? sq : (li > p ? a[p] * inv : a[p]);
 #pragma unroll
                 for (int c = p + 1; c < 16; ++c) {
@@ -217,9 +266,8 @@ k_fpotrf(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ fa
             for (int c = 0; c < 16; ++c) x[c] = c == li ? 1.0 : 0.0;
 #pragma unroll
             for (int p = 0; p < 16; ++p) {
-                const double rp = 1.0 / lane_bcast(a[p], p);
 #pragma unroll
-                for (int c = 0; c <= p; ++c) x[c] = li == p ? x[c] * rp : x[c];
+                for (int c = 0; c <= p; ++c) x[c] = li == p ? x[c] * rinv[p] : x[c];
 #pragma unroll
                 for (int c = 0; c <= p; ++c) {
                     const double xpc = lane_bcast(x[c], p);   // X[p][c]
@@ -307,17 +355,28 @@ k_fpotrf(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ fa
         }
         __syncthreads();
     }
-    for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
-        const int c = idx >> 6, rr = idx & 63;
-        if (c >= nb || rr >= nb) continue;
-        double* dst = F + static_cast<int64_t>(c0 + c) * f + c0 + rr;
-        if (rr >= c) *dst = D[idx];                 // L_kk (lower, with the diagonal)
-        else *dst = X[rr * 64 + c];                 // (L_kk^{-1})[c][rr] above the diagonal
+    // L_kk to the front's lower triangle, L_kk^{-1} (zero above the diagonal) to G's
+    // diagonal tile (k, k), where the TRSM tasks read it and the solves use it
+    double* Gkk = A.G + A.goff[s] + static_cast<int64_t>(c0) * wrapd::g_ld(f) + c0;
+    const int64_t lg = wrapd::g_ld(f);
+    {
+        const int r = threadIdx.x & 63, cb = threadIdx.x >> 6;
+        if (r < nb) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int c = cb + 4 * q;
+                if (c < nb) {
+                    if (r >= c) Fkk[static_cast<int64_t>(c) * f + r] = D[threadIdx.x + kFactThreads * q];
+                    Gkk[c * lg + r] = X[threadIdx.x + kFactThreads * q];
+                }
+            }
+        }
     }
 }
 
 // ---------------------------------------------------------------------------------
-// TRSM (kind 0): F[t, k] <- F[t, k] L_kk^{-T};  GDIAG (kind 1): X[k, c] <- L_kk^{-1} X[k, c]
+// TRSM (kind 0): F[t, k] <- F[t, k] L_kk^{-T};  GDIAG (kind 1): X[k, c] <- L_kk^{-1} X[k, c],
+// both as products with L_kk^{-1}, which k_fpotrf left in G's diagonal tile (k, k)
 __global__ void __launch_bounds__(kFactThreads)
 k_ftrsm(const int2* __restrict__ tasks, FactArgs A, int k) {
     __shared__ double As[kT * kT];   // As[q * 64 + i]
@@ -333,46 +392,30 @@ k_ftrsm(const int2* __restrict__ tasks, FactArgs A, int k) {
     double* F = A.pool + A.foff[s];
     double* Gs = A.G + A.goff[s];
     const int lg = wrapd::g_ld(f);
+    if (kind == 1 && t == k) return;   // X[k, k] = L_kk^{-1}: stored by k_fpotrf
     int t0, nt;
     front_tile(t, ns, f, t0, nt);
-    if (kind == 1 && t == k) {   // X[k, k] = L_kk^{-1}
-        for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
-            const int j = idx >> 6, i = idx & 63;
-            if (j < nb && i < nb) Gs[static_cast<int64_t>(c0 + j) * lg + c0 + i] = linv_at(F, f, c0, i, j);
-        }
-        return;
-    }
+    const double* Linv = Gs + static_cast<int64_t>(c0) * lg + c0;   // column-major, leading dimension lg
+    double va[16], vb[16];
     if (kind == 0) {
-        // C[i][j] = sum_q B[i][q] Linv[j][q]
-        for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
-            const int q = idx >> 6, i = idx & 63;
-            As[idx] = (q < nb && i < nt) ? F[static_cast<int64_t>(c0 + q) * f + t0 + i] : 0.0;
-            Bs[idx] = (q < nb && i < nb) ? linv_at(F, f, c0, i, q) : 0.0;   // here i plays j
-        }
+        // C[i][j] = sum_q B[i][q] Linv[j][q]: As[q][i] = B[i][q], Bs[q][j] = Linv[j][q]
+        tile_load(F + static_cast<int64_t>(c0) * f + t0, f, nt, nb, va);
+        tile_load(Linv, lg, nb, nb, vb);
+        tile_to_lds(As, va);
+        tile_to_lds(Bs, vb);
     } else {
         // C[i][j] = sum_q Linv[i][q] X[q][j], X = G rows [c0, c0+nb), columns [t0, t0+nt)
-        for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
-            const int q = idx >> 6, i = idx & 63;
-            As[idx] = (q < nb && i < nb) ? linv_at(F, f, c0, i, q) : 0.0;
-        }
-        for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
-            const int j = idx >> 6, q = idx & 63;
-            Bs[q * 64 + j] = (q < nb && j < nt) ? Gs[static_cast<int64_t>(t0 + j) * lg + c0 + q] : 0.0;
-        }
+        tile_load(Linv, lg, nb, nb, va);
+        tile_load(Gs + static_cast<int64_t>(t0) * lg + c0, lg, nb, nt, vb);
+        tile_to_lds(As, va);
+        tile_to_lds_t(Bs, vb);
     }
     __syncthreads();
     dbl4 acc[2][2];
     tile_zero(acc);
     tile_mma(As, Bs, (nb + 3) & ~3, acc);
-    if (kind == 0) {
-        tile_for_each(acc, [&](int i, int j, double v) {
-            if (i < nt && j < nb) F[static_cast<int64_t>(c0 + j) * f + t0 + i] = v;
-        });
-    } else {
-        tile_for_each(acc, [&](int i, int j, double v) {
-            if (i < nb && j < nt) Gs[static_cast<int64_t>(t0 + j) * lg + c0 + i] = v;
-        });
-    }
+    if (kind == 0) tile_write<false>(F + static_cast<int64_t>(c0) * f + t0, f, nt, nb, false, acc, 1.0);
+    else tile_write<false>(Gs + static_cast<int64_t>(t0) * lg + c0, lg, nb, nt, false, acc, 1.0);
 }
 
 // ---------------------------------------------------------------------------------
@@ -396,102 +439,148 @@ k_fupdate(const int2* __restrict__ tasks, FactArgs A, int k) {
     double* F = A.pool + A.foff[s];
     double* Gs = A.G + A.goff[s];
     const int lg = wrapd::g_ld(f);
-    for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
-        const int kk = idx >> 6, i = idx & 63;
-        As[idx] = (kk < nb && i < na) ? F[static_cast<int64_t>(c0 + kk) * f + a0 + i] : 0.0;
-    }
+    double va[16], vb[16];
+    tile_load(F + static_cast<int64_t>(c0) * f + a0, f, na, nb, va);
     if (kind == 0) {
-        for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
-            const int kk = idx >> 6, j = idx & 63;
-            Bs[idx] = (kk < nb && j < nbt) ? F[static_cast<int64_t>(c0 + kk) * f + b0 + j] : 0.0;
-        }
+        tile_load(F + static_cast<int64_t>(c0) * f + b0, f, nbt, nb, vb);
+        tile_to_lds(Bs, vb);
     } else {
-        for (int idx = threadIdx.x; idx < kT * kT; idx += kFactThreads) {
-            const int j = idx >> 6, kk = idx & 63;
-            Bs[kk * 64 + j] = (kk < nb && j < nbt) ? Gs[static_cast<int64_t>(b0 + j) * lg + c0 + kk] : 0.0;
-        }
+        tile_load(Gs + static_cast<int64_t>(b0) * lg + c0, lg, nb, nbt, vb);
+        tile_to_lds_t(Bs, vb);
     }
+    tile_to_lds(As, va);
     __syncthreads();
     dbl4 acc[2][2];
     tile_zero(acc);
     tile_mma(As, Bs, (nb + 3) & ~3, acc);
-    double* C = kind == 0 ? F : Gs;
-    const int ldc = kind == 0 ? f : lg;
-    const bool diag = kind == 0 && a == b;
-    tile_for_each(acc, [&](int i, int j, double v) {
-        if (i < na && j < nbt && !(diag && i < j)) C[static_cast<int64_t>(b0 + j) * ldc + a0 + i] -= v;
-    });
+    if (kind == 0) tile_write<true>(F + static_cast<int64_t>(b0) * f + a0, f, na, nbt, a == b, acc, 1.0);
+    else tile_write<true>(Gs + static_cast<int64_t>(b0) * lg + a0, lg, na, nbt, false, acc, 1.0);
 }
 
-
 // ---------------------------------------------------------------------------------
-// SCHUR (kind 0): F[a, b] -= sum_q L[a, q] L[b, q]^T over all panels q < P, tiles a >= b >= P;
-// GBOT (kind 1): G[a, c] = -sum_{q >= c} L[a, q] X[q, c], X = L11^{-1}, rows a >= P
+// SCHUR (kind 0): F[a, b] -= sum_q L[a, q] L[b, q]^T over all ns columns, update super-tiles
+// a >= b;  GBOT (kind 1): G[a, c] = -sum_{q >= c} L[a, q] X[q, c], X = L11^{-1}.
+// 128x128 output super-tiles: each wave owns a 64x64 quarter as 4x4 16x16 accumulators, so
+// every operand value staged in LDS feeds twice as many products as with 64x64 tiles.  The
+// k range is walked in 32-deep chunks, chunk q+1 fetched into registers while chunk q runs
+// on the matrix cores.  B is staged column by column (Bs[j * 33 + k]): the padding keeps the
+// transposed stores of G's columns and the matrix-core reads free of bank conflicts.
+constexpr int kFT = wrapd::kFinalTile;   // 128
+constexpr int kFK = 32;                  // k-chunk
+constexpr int kBsLd = kFK + 1;
+
+// R x C block (R * C = 4096) of a column-major source into registers: thread t's entry r
+// is (row t % R, column t / R + (256 / R) r).  Rows >= nr and columns >= nc read as zero;
+// addresses are clamped into the block (nr, nc >= 1), so the 16 loads issue together.
+template <int R>
+__device__ __forceinline__ void blk_load(const double* __restrict__ src, int64_t ld, int nr, int nc, double v[16]) {
+    constexpr int CS = kFactThreads / R;
+    const int row = threadIdx.x % R, col0 = threadIdx.x / R;
+    const bool rok = row < nr;
+    const double* p = src + (rok ? row : 0);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int col = col0 + CS * r;
+        const bool ok = rok && col < nc;
+        const double x = p[static_cast<int64_t>(col < nc ? col : 0) * ld];
+        v[r] = ok ? x : 0.0;
+    }
+}
+
 __global__ void __launch_bounds__(kFactThreads)
 k_ffinal(const int2* __restrict__ tasks, FactArgs A) {
-    __shared__ double As[kT * kT];   // As[kk * 64 + i]
-    __shared__ double Bs[kT * kT];   // Bs[kk * 64 + j]
+    __shared__ double As[kFK * kFT];      // As[k * 128 + i]
+    __shared__ double Bs[kFT * kBsLd];    // Bs[j * 33 + k]
     const int2 task = tasks[blockIdx.x];
     const int s = task.x;
     const int kind = task.y >> wrapd::kTaskKindShift;
-    const int a = (task.y >> wrapd::kTaskTileShift) & wrapd::kTaskTileMask;
-    const int b = task.y & wrapd::kTaskTileMask;
+    const int u = (task.y >> wrapd::kTaskTileShift) & wrapd::kTaskTileMask;
+    const int v = task.y & wrapd::kTaskTileMask;
     const int f = A.fsize[s];
     const int ns = A.col0[s + 1] - A.col0[s];
-    const int P = (ns + kT - 1) / kT;
-    int a0, na, b0, nbt;
-    front_tile(a, ns, f, a0, na);
-    front_tile(b, ns, f, b0, nbt);
-    const double* F = A.pool + A.foff[s];
-    double* Fw = A.pool + A.foff[s];
+    const int a0 = ns + kFT * u, na = min(kFT, f - a0);
+    const int b0 = kind == 0 ? ns + kFT * v : kFT * v;
+    const int nbt = min(kFT, (kind == 0 ? f : ns) - b0);
+    double* F = A.pool + A.foff[s];
     double* Gs = A.G + A.goff[s];
     const int lg = wrapd::g_ld(f);
-    // chunk q+1 is fetched into registers while chunk q runs on the matrix cores
-    constexpr int kPer = kT * kT / kFactThreads;   // 16 entries per thread and operand
-    double ra[kPer], rb[kPer];
-    auto fetch = [&](int q) {
-        const int q0 = kT * q;
-        const int nq = min(kT, ns - q0);
-#pragma unroll
-        for (int r = 0; r < kPer; ++r) {
-            const int idx = threadIdx.x + kFactThreads * r;
-            const int kk = idx >> 6, i = idx & 63;
-            ra[r] = (kk < nq && i < na) ? F[static_cast<int64_t>(q0 + kk) * f + a0 + i] : 0.0;
-            if (kind == 0) {
-                rb[r] = (kk < nq && i < nbt) ? F[static_cast<int64_t>(q0 + kk) * f + b0 + i] : 0.0;
-            } else {   // transposed: consecutive threads read consecutive rows kk of G
-                const int j = idx >> 6, k2 = idx & 63;
-                rb[r] = (k2 < nq && j < nbt) ? Gs[static_cast<int64_t>(b0 + j) * lg + q0 + k2] : 0.0;
-            }
-        }
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int li = l & 15, lk = l >> 4;
+    const int i0 = 64 * (w & 1), j0 = 64 * (w >> 1);
+    // a diagonal SCHUR super-tile keeps its lower half: the quarter above the diagonal is idle
+    const bool idle = kind == 0 && u == v && i0 < j0;
+    double ra[16], rb[16];
+    auto fetch = [&](int q0) {
+        const int nq = min(kFK, ns - q0);
+        blk_load<kFT>(F + static_cast<int64_t>(q0) * f + a0, f, na, nq, ra);
+        if (kind == 0) blk_load<kFT>(F + static_cast<int64_t>(q0) * f + b0, f, nbt, nq, rb);
+        else blk_load<kFK>(Gs + static_cast<int64_t>(b0) * lg + q0, lg, nq, nbt, rb);
     };
-    dbl4 acc[2][2];
-    tile_zero(acc);
-    const int qfirst = kind == 0 ? 0 : b;
+    dbl4 acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = dbl4{0.0, 0.0, 0.0, 0.0};
+    const int qfirst = kind == 0 ? 0 : b0;   // X[q, c] = 0 for q < c
     fetch(qfirst);
-    for (int q = qfirst; q < P; ++q) {
-        const int nq = min(kT, ns - kT * q);
+    for (int q0 = qfirst; q0 < ns; q0 += kFK) {
+        const int nq = min(kFK, ns - q0);
         __syncthreads();   // the previous chunk has been consumed
 #pragma unroll
-        for (int r = 0; r < kPer; ++r) {
-            const int idx = threadIdx.x + kFactThreads * r;
-            As[idx] = ra[r];
-            if (kind == 0) Bs[idx] = rb[r];
-            else Bs[(idx & 63) * 64 + (idx >> 6)] = rb[r];
+        for (int r = 0; r < 16; ++r) As[threadIdx.x + kFactThreads * r] = ra[r];
+        if (kind == 0) {   // rb entry r: (row j = t % 128, column k = t / 128 + 2 r)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) Bs[(threadIdx.x & 127) * kBsLd + (threadIdx.x >> 7) + 2 * r] = rb[r];
+        } else {           // rb entry r: (row k = t % 32, column j = t / 32 + 8 r)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) Bs[((threadIdx.x >> 5) + 8 * r) * kBsLd + (threadIdx.x & 31)] = rb[r];
         }
         __syncthreads();
-        if (q + 1 < P) fetch(q + 1);
-        tile_mma(As, Bs, (nq + 3) & ~3, acc);
+        if (q0 + kFK < ns) fetch(q0 + kFK);
+        if (!idle) {
+            const int nk = (nq + 3) & ~3;
+#pragma unroll 2
+            for (int kk = 0; kk < nk; kk += 4) {
+                const int k = kk + lk;
+                double bj[4], ai[4];
+#pragma unroll
+                for (int x = 0; x < 4; ++x) {
+                    bj[x] = Bs[(j0 + 16 * x + li) * kBsLd + k];
+                    ai[x] = As[k * kFT + i0 + 16 * x + li];
+                }
+#pragma unroll
+                for (int x = 0; x < 4; ++x)
+#pragma unroll
+                    for (int y = 0; y < 4; ++y)
+                        acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(bj[x], ai[y], acc[x][y], 0, 0, 0);
+            }
+        }
     }
-    if (kind == 0) {
-        const bool diag = a == b;
-        tile_for_each(acc, [&](int i, int j, double v) {
-            if (i < na && j < nbt && !(diag && i < j)) Fw[static_cast<int64_t>(b0 + j) * f + a0 + i] -= v;
-        });
-    } else {
-        tile_for_each(acc, [&](int i, int j, double v) {
-            if (i < na && j < nbt) Gs[static_cast<int64_t>(b0 + j) * lg + a0 + i] = -v;
-        });
+    if (idle) return;
+    // acc[x][y][q] is output (i0 + 16 y + li, j0 + 16 x + 4 q + lk)
+    double* C = kind == 0 ? F + static_cast<int64_t>(b0) * f + a0 : Gs + static_cast<int64_t>(b0) * lg + a0;
+    const int64_t ldc = kind == 0 ? f : lg;
+    const bool lower = kind == 0 && u == v;
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+        double old[16];
+        if (kind == 0) {
+#pragma unroll
+            for (int y = 0; y < 4; ++y)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int i = i0 + 16 * y + li, j = j0 + 16 * x + 4 * q + lk;
+                    old[y * 4 + q] = C[static_cast<int64_t>(j < nbt ? j : 0) * ldc + (i < na ? i : 0)];
+                }
+        }
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = i0 + 16 * y + li, j = j0 + 16 * x + 4 * q + lk;
+                if (i < na && j < nbt && !(lower && i < j))
+                    C[static_cast<int64_t>(j) * ldc + i] = kind == 0 ? old[y * 4 + q] - acc[x][y][q] : -acc[x][y][q];
+            }
     }
 }
 
