@@ -152,10 +152,10 @@ int wrapd_refactor(wrapd_ctx* ctx);
 int wrapd_solve3(wrapd_ctx* ctx, const double* b, double* x);
 /* A values (upper triangle, CSR over free vertices), for checking. Returns nnz. */
 int wrapd_get_A(wrapd_ctx* ctx, int* row_ptr, int* cols, double* vals);
-/* factorization statistics: [nsuper, nnz(L), factor flops, max front, solve leaf tasks,
- * solve levels, G entries, factor levels, factor launches, factor tile flops,
- * front pool doubles] */
-int wrapd_chol_stats(wrapd_ctx* ctx, double* stats11);
+/* factorization statistics: [nsuper, nnz(L), Cholesky flops, max front, solve leaf tasks,
+ * solve levels, G entries, factor levels, factor launches, factor tile flops (as issued,
+ * padding included), front pool doubles, G formation flops (L11^{-1} and -L21 L11^{-1})] */
+int wrapd_chol_stats(wrapd_ctx* ctx, double* stats12);
 
 /* SplitLBFGS (deps/mcloptlib/include/MCL/SplitLBFGS.hpp:41-257) */
 int wrapd_lbfgs_configure(wrapd_ctx* ctx, const wrapd_lbfgs_cfg* cfg);

@@ -16,9 +16,18 @@
 //          "x1 x2 x3 iters" with %.17g.
 //   svd    reads 9 numbers (row-major F) per line; prints sigma(3) U(9, row-major)
 //          V(9, row-major) from wrapd::math::svd (src/Math.hpp:140-161).
+//   local  <reference quasi flags...> [-E youngs] [-n steps]
+//          Times the reference local step alone: the same binding as `solve`, then
+//          TetElements set up at the initial positions (deformation cache, dual variables
+//          and weights as ADMM::solve sets them, src/ADMM.cpp:59-84) and `steps` calls of
+//          TetElements::polar_update (src/TetElements.cpp:256-356), each after a
+//          deformation-cache refresh.  No solver is initialized, so no matrix is factored.
+//          Prints "elements n steps k local_s t" (t: the polar_update calls only).
 //   hess   reads "s1 s2 s3 mu lambda" per line; prints the largest eigenvalue of
 //          NHProx::hessian_density (src/TetElements.hpp:109-144) as the candidate weight
 //          code does (src/TetElements.cpp:414-416).
+#include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -31,16 +40,21 @@
 #include "AddMeshes.hpp"
 #include "MeshIO.hpp"
 
+static int run_local(int argc, char** argv);
+
 static int run_solve(int argc, char** argv) {
     std::vector<char*> args;
     args.push_back(argv[0]);
     double youngs = -1.0;
     std::string outdir = ".";
+    int steps = 0;
     for (int i = 2; i < argc; ++i) {
         if (std::strcmp(argv[i], "-E") == 0 && i + 1 < argc) {
             youngs = std::atof(argv[++i]);
         } else if (std::strcmp(argv[i], "-out") == 0 && i + 1 < argc) {
             outdir = argv[++i];
+        } else if (std::strcmp(argv[i], "-n") == 0 && i + 1 < argc) {
+            steps = std::atoi(argv[++i]);
         } else {
             args.push_back(argv[i]);
         }
@@ -88,6 +102,32 @@ static int run_solve(int argc, char** argv) {
         soln = rest;
     }
     binding::add_tetmesh(solver.get(), rest, init, soln, pins, settings);
+    if (steps > 0) {
+        // local-step timing (mode `local`): ADMM::solve's set-up of the element state at X_init
+        std::shared_ptr<wrapd::TetElements>& te = solver->m_system->tet_elements();
+        const wrapd::math::MatX3& Xi = solver->m_system->X_init();
+        std::vector<int> sp(pins);
+        std::sort(sp.begin(), sp.end());
+        const int nv = static_cast<int>(Xi.rows());
+        wrapd::math::MatX3 x_free(nv - static_cast<int>(sp.size()), 3), x_fix(static_cast<int>(sp.size()), 3);
+        for (int i = 0, j = 0, k = 0; i < nv; ++i) {
+            if (j < static_cast<int>(sp.size()) && i == sp[j]) x_fix.row(j++) = Xi.row(i);
+            else x_free.row(k++) = Xi.row(i);
+        }
+        te->update_fix_cache(x_fix);
+        te->update_defo_cache(x_free);
+        te->initialize_dual_vars(Xi);
+        te->midupdate_weights(Xi);
+        double local_s = 0.0;
+        for (int it = 0; it < steps; ++it) {
+            te->update_defo_cache(x_free);
+            const auto t0 = std::chrono::steady_clock::now();
+            te->polar_update(settings.m_reweighting == wrapd::Settings::Reweighting::ENABLED);
+            local_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        std::printf("elements %d steps %d local_s %.6f\n", te->num_elements(), steps, local_s);
+        return 0;
+    }
     solver->initialize(settings);
     solver->solve();
 
@@ -172,6 +212,16 @@ static int run_hess() {
     return 0;
 }
 
+static int run_local(int argc, char** argv) {
+    bool has_n = false;
+    for (int i = 2; i < argc; ++i) has_n = has_n || std::strcmp(argv[i], "-n") == 0;
+    if (!has_n) {
+        std::fprintf(stderr, "local: -n <steps> is required\n");
+        return 2;
+    }
+    return run_solve(argc, argv);
+}
+
 int main(int argc, char** argv) {
     if (argc < 2) {
         std::fprintf(stderr, "usage: ref_driver solve|prox|svd|hess ...\n");
@@ -179,6 +229,7 @@ int main(int argc, char** argv) {
     }
     std::string mode = argv[1];
     if (mode == "solve") return run_solve(argc, argv);
+    if (mode == "local") return run_local(argc, argv);
     if (mode == "prox") return run_prox();
     if (mode == "svd") return run_svd();
     if (mode == "hess") return run_hess();

@@ -326,6 +326,7 @@ void chol_analyze(int n, const std::vector<int>& Ap, const std::vector<int>& Aj,
         S.max_ns = std::max(S.max_ns, ns);
         S.nnz_L += 0.5 * ns * (ns + 1.0) + static_cast<double>(ns) * m;
         S.flops_factor += ns * (double)ns * ns / 3.0 + (double)ns * ns * m + (double)ns * m * m;
+        S.flops_g += ns * (double)ns * ns / 3.0 + (double)m * ns * ns;
         S.g_entries += static_cast<double>(f) * ns;
     }
     // 9. extend-add maps and forward assembly lists

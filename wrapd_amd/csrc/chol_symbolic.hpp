@@ -43,6 +43,7 @@ struct CholSymbolic {
     std::vector<int> level_ptr, level_sn;  // top supernodes grouped by level (bottom-up)
     int max_front = 0, max_ns = 0;
     double nnz_L = 0, flops_factor = 0, g_entries = 0;
+    double flops_g = 0;   // forming G's blocks: L11^{-1} (ns^3/3) and -L21 L11^{-1} (m ns^2) per supernode
 
     // A (upper triangle CSR over old free indices) -> lower entries in new numbering:
     // for each A slot the supernode column position, used to scatter A into fronts.

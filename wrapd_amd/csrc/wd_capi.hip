@@ -663,6 +663,7 @@ int wrapd_chol_stats(wrapd_ctx* c, double* st) {
     st[8] = static_cast<double>(c->fplan.launches.size());
     st[9] = c->fplan.flops;
     st[10] = static_cast<double>(c->fplan.pool_size);
+    st[11] = c->sym.flops_g;
     return WRAPD_OK;
 }
 

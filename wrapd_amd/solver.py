@@ -286,10 +286,11 @@ class Device:
         return rp, cols, vals
 
     def chol_stats(self):
-        s = np.zeros(11)
+        s = np.zeros(12)
         self._check(lib().wrapd_chol_stats(self.ctx, dp(s)), "chol_stats")
         return dict(zip(["nsuper", "nnz_L", "flops", "max_front", "leaf_tasks", "levels", "g_entries",
-                         "factor_levels", "factor_launches", "factor_tile_flops", "pool_doubles"], s.tolist()))
+                         "factor_levels", "factor_launches", "factor_tile_flops", "pool_doubles", "flops_g"],
+                        s.tolist()))
 
     def lbfgs_configure(self, memory=6, max_iters=100, kappa=1.0, linesearch=0):
         c = _lib.LbfgsCfg(memory, max_iters, kappa, linesearch)
