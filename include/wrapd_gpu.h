@@ -211,6 +211,14 @@ int wrapd_mlp_get_params(wrapd_ctx* ctx, float* params4803, float* norm18);
 int wrapd_mlp_infer(wrapd_ctx* ctx, const float* in, int n, float* out);
 /* mean squared error (standardised targets) and its gradient on the n given tuples */
 int wrapd_mlp_grad(wrapd_ctx* ctx, const float* tuples, int n, float* grad4803, float* loss);
+/* wrapd_mlp_grad without device allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same result, bit for bit, as wrapd_mlp_grad on the
+ * same tuples; the tuples, sample list, per-wave partial gradients and sums are staged in a
+ * static 8 MiB device array of the library instead of memory allocated per call, so at most
+ * 65536 tuples per call (WRAPD_ERR_ARG beyond).  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (from several contexts or threads) race. */
+int wrapd_mlp_grad_noalloc(wrapd_ctx* ctx, const float* tuples, int n, float* grad4803, float* loss);
 /* tuple log: while started, every NH local step appends its n_e (input, solution) tuples */
 int wrapd_mlp_log_start(wrapd_ctx* ctx, long long capacity);
 int wrapd_mlp_log_stop(wrapd_ctx* ctx);

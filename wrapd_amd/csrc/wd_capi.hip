@@ -4,6 +4,7 @@
 // vector arithmetic runs on the device.
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -142,6 +143,55 @@ static int launch_status(wrapd_ctx* c) {
 
 }  // namespace wd
 
+// ---- the static scratch array of the allocation-free entry points ------------------------
+// wrapd_mlp_grad_noalloc stages its tuples, sample list, per-wave partial gradients and sums
+// here instead of in memory it allocates.  Every GPU holds its own copy of the array until
+// the process ends, at its own device address: the host never passes `scratch` itself to a
+// kernel or a copy (in host code the name is a host-side stand-in), only the address
+// hipGetSymbolAddress returns with that device current, looked up once per device at the
+// first wrapd_create on it.  All calls on one GPU share its array, and its contents stay from
+// call to call: calls that overlap on one GPU (several contexts or host threads) race.
+namespace {
+constexpr size_t kScratchBytes = size_t(8) << 20;   // 8 MiB per GPU
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;   // per device index, null until looked up
+
+int scratch_lookup(wrapd_ctx* c) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (static_cast<int>(g_scratch_ptr.size()) <= c->device) g_scratch_ptr.resize(c->device + 1, nullptr);
+    if (!g_scratch_ptr[c->device]) {
+        void* p = nullptr;   // c->device is current (wrapd_create set it)
+        if (wd::check_hip(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)") || !p)
+            return WRAPD_ERR_HIP;
+        g_scratch_ptr[c->device] = p;
+    }
+    c->scratch = g_scratch_ptr[c->device];
+    return WRAPD_OK;
+}
+
+// wrapd_mlp_grad_noalloc's layout inside the scratch array, for n samples
+struct MlpGradScratch {
+    size_t partial, lpart, gsum, loss, tuples, idx, end;
+};
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr MlpGradScratch mlp_grad_layout(long long n) {
+    const size_t waves = static_cast<size_t>((n + 255) / 256);
+    MlpGradScratch L{};
+    L.partial = 0;
+    L.lpart = align256(L.partial + waves * wd::kMlpParams * sizeof(float));
+    L.gsum = align256(L.lpart + waves * sizeof(float));
+    L.loss = align256(L.gsum + wd::kMlpParams * sizeof(float));
+    L.tuples = align256(L.loss + sizeof(float));
+    L.idx = align256(L.tuples + 9ull * n * sizeof(float));
+    L.end = align256(L.idx + static_cast<size_t>(n) * sizeof(int));
+    return L;
+}
+constexpr long long kMlpGradNoAllocMax = 65536;
+static_assert(mlp_grad_layout(kMlpGradNoAllocMax).end <= kScratchBytes, "scratch too small for the stated maximum");
+}  // namespace
+
 using namespace wd;
 
 #define CTX_OK(c) do { if (!(c)) return WRAPD_ERR_ARG; } while (0)
@@ -175,6 +225,10 @@ int wrapd_create(wrapd_ctx** out, int device) {
     if (check_hip(c, hipMemsetD32(reinterpret_cast<hipDeviceptr_t>(c->partials), 0xFFF40000u,
                                   2 * static_cast<size_t>(kMaxBlocks) * kRedStride), "hipMemsetD32") ||
         check_hip(c, hipMemset(c->d_red_abort, 0, sizeof(int)), "hipMemset")) {
+        delete c;
+        return WRAPD_ERR_HIP;
+    }
+    if (scratch_lookup(c)) {
         delete c;
         return WRAPD_ERR_HIP;
     }
@@ -1243,6 +1297,41 @@ int wrapd_mlp_grad(wrapd_ctx* c, const float* tuples, int n, float* grad, float*
     if (dt) (void)hipFree(dt);
     if (didx) (void)hipFree(didx);
     return st ? st : launch_status(c);
+}
+
+int wrapd_mlp_grad_noalloc(wrapd_ctx* c, const float* tuples, int n, float* grad, float* loss) {
+    CTX_OK(c);
+    if (!c->mlp_ready) { c->last_error = "mlp: no network"; return WRAPD_ERR_STATE; }
+    if (n <= 0 || !tuples) return WRAPD_ERR_ARG;
+    if (n > kMlpGradNoAllocMax) {
+        c->last_error = "wrapd_mlp_grad_noalloc: at most 65536 tuples per call (the static scratch array is 8 MiB)";
+        return WRAPD_ERR_ARG;
+    }
+    if (!c->scratch) { c->last_error = "wrapd_mlp_grad_noalloc: no scratch address for this device"; return WRAPD_ERR_STATE; }
+    const MlpGradScratch L = mlp_grad_layout(n);
+    unsigned char* base = static_cast<unsigned char*>(c->scratch);
+    float* d_partial = reinterpret_cast<float*>(base + L.partial);
+    float* d_lpart = reinterpret_cast<float*>(base + L.lpart);
+    float* d_gsum = reinterpret_cast<float*>(base + L.gsum);
+    float* d_loss = reinterpret_cast<float*>(base + L.loss);
+    float* d_tuples = reinterpret_cast<float*>(base + L.tuples);
+    int* d_idx = reinterpret_cast<int*>(base + L.idx);
+    // the sample list 0..n-1, as wrapd_mlp_grad uses it; a small host vector, no device allocation
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    if (check_hip(c, hipMemcpyAsync(d_tuples, tuples, 9ull * n * sizeof(float), hipMemcpyHostToDevice, c->stream),
+                  "mlp grad (no alloc)") ||
+        check_hip(c, hipMemcpyAsync(d_idx, idx.data(), static_cast<size_t>(n) * sizeof(int), hipMemcpyHostToDevice,
+                                    c->stream), "mlp grad (no alloc)"))
+        return WRAPD_ERR_HIP;
+    launch_mlp_grad(c, d_tuples, d_idx, n, d_partial, d_lpart, d_gsum, d_loss);
+    if ((grad && check_hip(c, hipMemcpyAsync(grad, d_gsum, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+                           "mlp grad (no alloc)")) ||
+        (loss && check_hip(c, hipMemcpyAsync(loss, d_loss, sizeof(float), hipMemcpyDeviceToHost, c->stream),
+                           "mlp grad (no alloc)")) ||
+        check_hip(c, hipStreamSynchronize(c->stream), "mlp grad (no alloc)"))
+        return WRAPD_ERR_HIP;
+    return launch_status(c);
 }
 
 int wrapd_mlp_log_start(wrapd_ctx* c, long long capacity) {

@@ -182,6 +182,7 @@ struct wrapd_ctx {
         int dst = 0, nblk = 0;   // nblk 0: none
         bool host = true;        // with the host copy
     } pending_fin;                     // a taken tail whose launch is too wide to finish it
+    void* scratch = nullptr;     // this device's address of the library's static scratch array (wd_capi.hip)
     double* scal_host = nullptr; // pinned host mirror
     unsigned* host_seq = nullptr; // pinned: the epoch of the last reduction whose host copy is stored
 
@@ -263,6 +264,9 @@ int mlp_infer(wrapd_ctx* c, const float* d_in, int n, float* d_out);
 int mlp_step(wrapd_ctx* c, const float* d_tuples, long long n_data, const int* d_idx, long long n, uint64_t seed,
              float lr, int t);   // loss to c->mlp_loss[0], summed gradient to c->mlp_gsum
 void launch_mlp_augment(wrapd_ctx* c, float* d_tuples, long long n, double lo, double hi, uint64_t seed);
+inline int mlp_grad_waves(long long n) { return static_cast<int>((n + 255) / 256); }   // 4 rounds of 64 per wave
+void launch_mlp_grad(wrapd_ctx* c, const float* d_tuples, const int* d_idx, long long n, float* partial, float* lpart,
+                     float* gsum, float* loss);
 int mlp_sample(wrapd_ctx* c, long long stride, long long m, float* host);   // every stride-th logged tuple
 // (wd_factor.hip)
 int factor_prepare_device(wrapd_ctx* c);

@@ -327,6 +327,19 @@ int mlp_infer(wrapd_ctx* c, const float* d_in, int n, float* d_out) {
     return WRAPD_OK;
 }
 
+// the loss and its gradient summed over n given tuples (no Adam step), into caller-given
+// buffers: partial (waves x kMlpParams), lpart (waves), gsum (kMlpParams), loss (1), with
+// waves = mlp_grad_waves(n).  The same kernels and order as mlp_step, so the same bits.
+void launch_mlp_grad(wrapd_ctx* c, const float* d_tuples, const int* d_idx, long long n, float* partial, float* lpart,
+                     float* gsum, float* loss) {
+    const int rounds = 4;
+    const int waves = mlp_grad_waves(n);
+    TrainBatch B{d_tuples, n, d_idx, n, 0};
+    hipLaunchKernelGGL(k_mlp_train_fb, dim3(waves), dim3(64), 0, c->stream, mlp_device(c), B, rounds, partial, lpart);
+    hipLaunchKernelGGL(k_mlp_adam, dim3(blocks_for(kMlpParams, 256)), dim3(256), 0, c->stream, partial, lpart, waves,
+                       c->mlp_master, c->mlp_m1, c->mlp_m2, gsum, loss, 0.0f, 0.9f, 0.999f, 1e-8f, 1);
+}
+
 // one minibatch forward/backward (+ Adam when lr > 0); waves of `rounds` x 64 samples
 int mlp_step(wrapd_ctx* c, const float* d_tuples, long long n_data, const int* d_idx, long long n, uint64_t seed,
              float lr, int t) {

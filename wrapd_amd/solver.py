@@ -375,6 +375,14 @@ class Device:
         self._check(lib().wrapd_mlp_grad(self.ctx, fp(t), len(t), fp(g), fp(loss)), "mlp_grad")
         return float(loss[0]), g
 
+    def mlp_grad_noalloc(self, tuples):
+        """mlp_grad through the entry point that allocates no device memory (<= 65536 tuples)."""
+        t = np.ascontiguousarray(tuples, dtype=np.float32).reshape(-1, 9)
+        g = np.zeros(self.N_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(lib().wrapd_mlp_grad_noalloc(self.ctx, fp(t), len(t), fp(g), fp(loss)), "mlp_grad_noalloc")
+        return float(loss[0]), g
+
     def mlp_log_start(self, capacity):
         self._check(lib().wrapd_mlp_log_start(self.ctx, int(capacity)), "mlp_log_start")
 
