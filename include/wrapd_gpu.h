@@ -185,6 +185,11 @@ int wrapd_get_elem(wrapd_ctx* ctx, const char* name, double* out);
 int wrapd_state_save(wrapd_ctx* ctx, void* buf, size_t cap, size_t* size);
 int wrapd_state_load(wrapd_ctx* ctx, const void* buf, size_t size);
 int wrapd_set_elem(wrapd_ctx* ctx, const char* name, const double* in);
+/* math::svd (src/Math.hpp:140-161) as the device computes it, for checking: n row-major
+ * 3x3 matrices F in, sigma (n,3) sorted by magnitude with det's sign on sigma[2], U and V
+ * (n,9) row-major rotations out; precision WRAPD_FP64 or WRAPD_FP32 (the FP32 local step's
+ * arithmetic, inputs rounded to float). */
+int wrapd_svd3_batch(wrapd_ctx* ctx, const double* F, int n, int precision, double* sigma, double* U, double* V);
 
 /* timing of the device work, measured with HIP events on the context's stream */
 int wrapd_profile_enable(wrapd_ctx* ctx, int on);

@@ -1129,6 +1129,25 @@ struct Oracle {
     }
 
     // ---- ADMM (src/ADMM.cpp) --------------------------------------------------------
+    // the element state ADMM::solve sets up (fixed positions and their cache, weights at
+    // X_init, x = X_init), without the linear system: for element-level checks on meshes
+    // too large for the CPU factorization
+    int setup_elements() {
+        x_fix.assign(3 * n_fix, 0.0);
+        for (int j = 0; j < n_fix; ++j)
+            for (int c = 0; c < 3; ++c) x_fix[3 * j + c] = X_init[3 * pins[j] + c];
+        update_fix_cache();
+        midupdate_weights_X(X_init);
+        X = X_init;
+        x_free.assign(3 * n_f, 0.0);
+        for (int v = 0; v < n_v; ++v) {
+            const int f = free_idx[v];
+            if (f >= 0)
+                for (int c = 0; c < 3; ++c) x_free[3 * f + c] = X[3 * v + c];
+        }
+        return 0;
+    }
+
     int initialize() {
         x_fix.assign(3 * n_fix, 0.0);
         for (int j = 0; j < n_fix; ++j)
@@ -1347,6 +1366,7 @@ void* oracle_create(const oracle_settings_t* s, int n_v, const double* rest, con
 
 void oracle_destroy(void* h) { delete O(h); }
 int oracle_initialize(void* h) { return O(h)->initialize(); }
+int oracle_setup_elements(void* h) { return O(h)->setup_elements(); }
 int oracle_setup_solve(void* h) { return O(h)->setup_solve(); }
 int oracle_iterate(void* h) { return O(h)->iterate(); }
 int oracle_finish(void* h) { O(h)->finish(); return 0; }

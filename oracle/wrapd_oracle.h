@@ -49,6 +49,7 @@ void* oracle_create(const oracle_settings_t* s, int n_v, const double* rest, con
                     const double* mu_e, const double* lambda_e);
 void oracle_destroy(void* h);
 int oracle_initialize(void* h);          /* ADMM::initialize                               */
+int oracle_setup_elements(void* h);   /* element state at X_init, no linear system */
 int oracle_setup_solve(void* h);         /* ADMM::solve up to the iteration loop           */
 int oracle_iterate(void* h);             /* one ADMM::iterate; returns 1 when converged    */
 int oracle_finish(void* h);              /* ADMM::finish_solve                             */

@@ -1,10 +1,10 @@
 """Writes the golden vectors under tests/golden/ from the reference build (oracle/_ref).
 
 Run in a container where the reference tree is present, after `make -C oracle ref`:
-    OMP_NUM_THREADS=1 python tests/golden/make_golden.py
+    OMP_NUM_THREADS=1 python tests/golden/make_golden.py [all|base|samples|c3]
 The reference runs single-threaded (its OpenMP reductions are order dependent).  Only
 inputs and outputs are stored; the mesh inputs of the sample case come from the
-reference's own data files (samples/data/untwist).
+reference's own data files (samples/data/*).
 """
 from __future__ import annotations
 
@@ -138,6 +138,95 @@ def save_case(name, prob, flags, youngs=None, store_mesh=False, x_every=1):
           "spread_e %.2e spread_x %.2e pert_e %.2e pert_x %.2e" % (spread_e, spread_x, pert_e, pert_x), flush=True)
 
 
+SAMPLES = ("shear", "stretch", "dancer", "dino", "botijo", "homer", "cat", "centaur", "dinoskel", "rabbit")
+
+
+def sample_inputs(case):
+    """The reference's own sample data (samples/data/<case>): rest, init and the shipped
+    neo-Hookean solution, tets, pins."""
+    d = os.path.join(REF_TREE, "samples", "data", case)
+    rest, tets = meshgen.load_elenode(os.path.join(d, f"{case}_rest"))
+    init, _ = meshgen.load_elenode(os.path.join(d, f"{case}_init"))
+    soln, _ = meshgen.load_elenode(os.path.join(d, f"{case}_nh_soln"))
+    pins = np.sort(meshgen.load_handles(os.path.join(d, f"{case}_handles.txt")))
+    return rest, init, soln, tets.astype(np.int32), pins.astype(np.int32)
+
+
+def run_soln(case, threads):
+    """The reference with its defaults and -soln: stops at the first x-error below 1e-4
+    (src/ADMM.cpp:164, src/AlgorithmData.cpp:70)."""
+    d = os.path.join(REF_TREE, "samples", "data", case)
+    with tempfile.TemporaryDirectory() as out:
+        cmd = [REF, "solve", "-rest", os.path.join(d, f"{case}_rest"), "-init", os.path.join(d, f"{case}_init"),
+               "-soln", os.path.join(d, f"{case}_nh_soln"), "-handles", os.path.join(d, f"{case}_handles.txt"),
+               "-p", "0", "-s", "0", "-out", out]
+        subprocess.run(cmd, env=dict(ENV, OMP_NUM_THREADS=str(threads)), check=True, capture_output=True)
+        rd = lambda n: np.atleast_1d(np.loadtxt(os.path.join(out, n)))  # noqa: E731
+        return dict(objectives=rd("objectives.txt"), reweighted=rd("reweighted.txt").astype(int),
+                    inner_iters=rd("inner_iters.txt").astype(int), x_errors=rd("x_errors.txt"),
+                    x=np.loadtxt(os.path.join(out, "final_x.txt")))
+
+
+def samples():
+    """One fixture per sample case: its input meshes and the reference's run to the -soln
+    exit, single-threaded, with the 8-thread run's exit iteration and deviation."""
+    for case in SAMPLES:
+        if os.path.exists(os.path.join(HERE, f"sample_{case}.npz")) and "--redo" not in sys.argv:
+            continue
+        rest, init, soln, tets, pins = sample_inputs(case)
+        t1 = run_soln(case, 1)
+        t8 = run_soln(case, 8)
+        n = min(len(t1["objectives"]), len(t8["objectives"]))
+        spread_e = float(np.max(np.abs(t1["objectives"][:n] - t8["objectives"][:n])) / abs(t1["objectives"][0]))
+        spread_x = float(np.linalg.norm(t1["x"] - t8["x"]) / np.linalg.norm(t1["x"]))
+        np.savez_compressed(os.path.join(HERE, f"sample_{case}.npz"), rest=rest, init=init, soln=soln, tets=tets,
+                            pins=pins, objectives=t1["objectives"], reweighted=t1["reweighted"],
+                            inner_iters=t1["inner_iters"], x_errors=t1["x_errors"], x=t1["x"],
+                            iters=len(t1["objectives"]) - 1, iters_8threads=len(t8["objectives"]) - 1,
+                            spread_e=spread_e, spread_x=spread_x)
+        print(case, "iters", len(t1["objectives"]) - 1, "(8 threads:", len(t8["objectives"]) - 1, ")",
+              "spread_e %.2e spread_x %.2e" % (spread_e, spread_x), flush=True)
+
+
+def c3():
+    """The 32x32x240 beam (40% push + 180 degree twist, E = 1e6), 10 iterations at the
+    reference's defaults (its L-BFGS memory 6)."""
+    save_case("c3_it10", meshgen.config_problem(3), ["-it", "10"], youngs=1e6, x_every=101)
+
+
+def voronoi_problem():
+    """8x8x64 cells, 16 Voronoi regions of E in {1e5, 1e7}, 360 degree twist and 20% push:
+    the 3.07M-tet benchmark beam's load on a 24,576-tet beam."""
+    return meshgen.small_voronoi_beam(64, 8, 8, regions=16, twist_deg=360.0, axial=-0.2)
+
+
+def voronoi():
+    """The heterogeneous beam has no reference counterpart (the reference has one
+    material): its fixture is the oracle's run, 100 iterations, with the oracle's own
+    sensitivity to round-off (the run from initial positions moved by 1e-15 relative)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    prob = voronoi_problem()
+    runs = []
+    for pr in (prob, perturbed(prob)):
+        o = ol.Oracle(pr, tol=0.0)
+        o.run(100)
+        runs.append(dict(objectives=o.log("objectives"), reweighted=o.log("reweighted").astype(int),
+                         inner_iters=o.log("inner_iters").astype(int), flips=o.log("flips").astype(int), x=o.x()))
+    t, tp = runs
+    # the run is chaotic: the iterations over which the perturbed run stays within 1e-9 E0
+    dev = np.abs(t["objectives"] - tp["objectives"]) / abs(t["objectives"][0])
+    k_det = int(np.argmax(dev > 1e-9)) if np.any(dev > 1e-9) else len(dev)
+    pert_e = float(np.max(np.abs(t["objectives"] - tp["objectives"])) / abs(t["objectives"][0]))
+    pert_x = float(np.linalg.norm(t["x"] - tp["x"]) / np.linalg.norm(t["x"]))
+    np.savez_compressed(os.path.join(HERE, "trace_voronoi_oracle.npz"), youngs_per_elem=prob.youngs_per_elem,
+                        pert_e=pert_e, pert_x=pert_x, k_det=k_det, objectives_perturbed=tp["objectives"],
+                        pert_inner_equal=bool(np.array_equal(t["inner_iters"], tp["inner_iters"])), **t)
+    e = t["objectives"]
+    print("voronoi E0 %.4e max %.4e end %.4e reweights %d pert_e %.2e pert_x %.2e, within 1e-9 E0 for %d iterations" % (
+        e[0], e.max(), e[-1], t["reweighted"].sum(), pert_e, pert_x, k_det))
+
+
 def main():
     if not os.path.exists(REF):
         sys.exit("build the reference first: make -C oracle ref")
@@ -173,4 +262,12 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if what in ("all", "base"):
+        main()
+    if what in ("all", "samples"):
+        samples()
+    if what in ("all", "c3"):
+        c3()
+    if what in ("all", "voronoi"):
+        voronoi()

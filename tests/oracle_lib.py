@@ -50,7 +50,7 @@ def lib():
                                     C.c_int, _I, _D, _D]
         L.oracle_create.restype = C.c_void_p
         L.oracle_destroy.argtypes = [C.c_void_p]
-        for fn in ("oracle_initialize", "oracle_setup_solve", "oracle_iterate", "oracle_finish",
+        for fn in ("oracle_initialize", "oracle_setup_elements", "oracle_setup_solve", "oracle_iterate", "oracle_finish",
                    "oracle_solve", "oracle_num_iters", "oracle_n_free", "oracle_inversion_count"):
             getattr(L, fn).argtypes = [C.c_void_p]
             getattr(L, fn).restype = C.c_int
@@ -172,6 +172,12 @@ class Oracle:
         r = lib().oracle_initialize(self.h)
         if r != 0:
             raise RuntimeError(f"oracle_initialize failed: {r}")
+
+    def setup_elements(self):
+        """Fixed positions, weights and x at X_init without the linear system (element-level checks)."""
+        r = lib().oracle_setup_elements(self.h)
+        if r != 0:
+            raise RuntimeError(f"oracle_setup_elements failed: {r}")
 
     def setup_solve(self):
         r = lib().oracle_setup_solve(self.h)

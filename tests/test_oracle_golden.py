@@ -112,3 +112,16 @@ def test_uniform_per_element_material_is_the_homogeneous_run():
     prob = meshgen.small_beam(20, 4, 4, seed=1, pin_far_end=False, youngs=1e6)
     prob.youngs_per_elem = np.full(prob.n_elems, 1e6)
     compare_trace("c1geom_nh_E1e6", prob, {})
+
+
+def test_heterogeneous_oracle_fixture_reproduces():
+    """The oracle regenerates the first 30 iterations of its heterogeneous-beam fixture
+    (tests/golden/make_golden.py voronoi) within the fixture's own round-off envelope."""
+    z = load("trace_voronoi_oracle.npz")
+    prob = meshgen.small_voronoi_beam(64, 8, 8, regions=16, twist_deg=360.0, axial=-0.2)
+    assert np.array_equal(prob.youngs_per_elem, z["youngs_per_elem"])
+    o = ol.Oracle(prob, tol=0.0)
+    o.run(30)
+    e, eo = o.log("objectives"), z["objectives"][:31]
+    env = np.maximum.accumulate(np.abs(z["objectives"] - z["objectives_perturbed"]) / abs(eo[0]))[:31]
+    assert np.all(np.abs(e - eo) / abs(eo[0]) <= np.maximum(5e-9, 8.0 * env))

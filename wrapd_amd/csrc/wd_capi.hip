@@ -6,6 +6,7 @@
 #include <chrono>
 #include <mutex>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -231,6 +232,11 @@ int wrapd_create(wrapd_ctx** out, int device) {
     if (scratch_lookup(c)) {
         delete c;
         return WRAPD_ERR_HIP;
+    }
+    c->tail_max_blocks = kTailMaxBlocks;
+    if (const char* e = std::getenv("WRAPD_TAIL_MAX_BLOCKS")) {
+        const int v = std::atoi(e);
+        if (v >= 1 && v <= kTailMaxBlocks) c->tail_max_blocks = v;
     }
     launch_fill(c, c->scal, 0.0, S_NUM);
     std::fill(c->scal_host, c->scal_host + S_NUM, 0.0);
@@ -1124,6 +1130,25 @@ int wrapd_state_load(wrapd_ctx* c, const void* buf, size_t size) {
     c->lcfg = h.lcfg;
     c->factored = false;   // the factorization depends on w^2: wrapd_refactor before the next solve
     return WRAPD_OK;
+}
+
+int wrapd_svd3_batch(wrapd_ctx* c, const double* F, int n, int precision, double* sigma, double* U, double* V) {
+    CTX_OK(c);
+    if (n < 0 || (n > 0 && (!F || !sigma || !U || !V)) || (precision != WRAPD_FP64 && precision != WRAPD_FP32))
+        return WRAPD_ERR_ARG;
+    if (n == 0) return WRAPD_OK;
+    double* d = nullptr;
+    int st = WRAPD_OK;
+    if (dalloc(c, d, 30ull * n)) return WRAPD_ERR_HIP;
+    if (h2d(c, d, F, 9ull * n)) st = WRAPD_ERR_HIP;
+    if (!st) {
+        launch_svd_batch(c, n, precision, d, d + 9ll * n, d + 12ll * n, d + 21ll * n);
+        if (launch_status(c) || d2h(c, sigma, d + 9ll * n, 3ull * n) || d2h(c, U, d + 12ll * n, 9ull * n) ||
+            d2h(c, V, d + 21ll * n, 9ull * n))
+            st = WRAPD_ERR_HIP;
+    }
+    (void)hipFree(d);
+    return st;
 }
 
 int wrapd_set_elem(wrapd_ctx* c, const char* name, const double* in) {

@@ -183,6 +183,10 @@ struct wrapd_ctx {
         bool host = true;        // with the host copy
     } pending_fin;                     // a taken tail whose launch is too wide to finish it
     void* scratch = nullptr;     // this device's address of the library's static scratch array (wd_capi.hip)
+    // launches wider than this many blocks finish their reductions in a separate one-block
+    // launch (kTailMaxBlocks; WRAPD_TAIL_MAX_BLOCKS in the environment at wrapd_create
+    // lowers it, so small meshes can exercise that path in tests)
+    int tail_max_blocks = 1024;
     double* scal_host = nullptr; // pinned host mirror
     unsigned* host_seq = nullptr; // pinned: the epoch of the last reduction whose host copy is stored
 
@@ -219,13 +223,14 @@ void launch_local_unaware(wrapd_ctx* c, const wrapd_local_args* a);
 void launch_update_weights(wrapd_ctx* c);
 void launch_max_ratio(wrapd_ctx* c);
 void launch_assemble_A(wrapd_ctx* c);
+void launch_svd_batch(wrapd_ctx* c, int n, int precision, const double* F, double* sig, double* U, double* V);
 // (wd_vec.hip)
 // The next reducing launch finishes its own reduction into scal[dst, dst + nslots) in its
 // last block, summing the blocks' partials in a fixed order.  host: also into the pinned
 // host copy, followed by the reduction's epoch (false for results only later launches read).
 void arm_tail(wrapd_ctx* c, int nslots, unsigned maxmask, int dst, bool host = true);
 // The armed tail for a launch of nblk blocks.  With may_split and more than
-// kTailMaxBlocks blocks the reduction is instead left to finish_reduction, to be called
+// c->tail_max_blocks (kTailMaxBlocks) blocks the reduction is instead left to finish_reduction, to be called
 // right after that launch (a one-block finalize launch: one block reading that many
 // partials is slower than a launch).
 wd::RedTail take_tail(wrapd_ctx* c, int nblk, bool may_split);

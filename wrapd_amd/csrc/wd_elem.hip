@@ -734,6 +734,36 @@ static MatArgs mat_args(wrapd_ctx* c) {
     return m;
 }
 
+// the device SVD on n given matrices (row-major), in T arithmetic; for checking svd3
+template <typename T>
+__global__ void __launch_bounds__(kElemThreads)
+k_svd_batch(int n, const double* __restrict__ F, double* __restrict__ sig, double* __restrict__ U,
+            double* __restrict__ V) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    M3<T> f, u, v;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f.m[i] = static_cast<T>(F[9ll * e + i]);
+    T s[3];
+    svd3(f, s, u, v);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) sig[3ll * e + i] = static_cast<double>(s[i]);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        U[9ll * e + i] = static_cast<double>(u.m[i]);
+        V[9ll * e + i] = static_cast<double>(v.m[i]);
+    }
+}
+
+void launch_svd_batch(wrapd_ctx* c, int n, int precision, const double* F, double* sig, double* U, double* V) {
+    if (precision == WRAPD_FP32)
+        hipLaunchKernelGGL(k_svd_batch<float>, dim3(blocks_for(n, kElemThreads)), dim3(kElemThreads), 0, c->stream, n,
+                           F, sig, U, V);
+    else
+        hipLaunchKernelGGL(k_svd_batch<double>, dim3(blocks_for(n, kElemThreads)), dim3(kElemThreads), 0, c->stream,
+                           n, F, sig, U, V);
+}
+
 void launch_setup(wrapd_ctx* c, const int* d_tv, const double* d_rest, int* d_bad) {
     ProfScope ps(c, "setup");
     hipLaunchKernelGGL(k_setup, dim3(blocks_for(c->n_ep, kElemThreads)), dim3(kElemThreads), 0, c->stream, c->n_e,

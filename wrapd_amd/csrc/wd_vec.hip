@@ -476,7 +476,7 @@ RedTail take_tail(wrapd_ctx* c, int nblk, bool may_split) {
     if (++c->red_epoch == 0) c->red_epoch = 1;   // the host's sequence word starts at 0
     t.epoch = c->red_epoch;
     if (t.hseq) c->host_epoch = t.epoch;
-    if (may_split && nblk > kTailMaxBlocks) {   // one block would read too many partials: finalize apart
+    if (may_split && nblk > c->tail_max_blocks) {   // one block would read too many partials: finalize apart
         c->pending_fin = wrapd_ctx::PendingFinalize{t.nslots, t.maxmask, static_cast<int>(t.out - c->scal), nblk,
                                                     t.hseq != nullptr};
         RedTail split;   // the blocks still store their partials, for the finalize launch

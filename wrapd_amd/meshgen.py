@@ -176,6 +176,15 @@ def config_problem(cfg: int) -> Problem:
     raise ValueError(f"unknown configuration {cfg}")
 
 
+def small_voronoi_beam(n_long=24, n_y=4, n_z=4, *, regions=8, seed=21, twist_deg=240.0, axial=-0.15,
+                       **settings) -> Problem:
+    """A small heterogeneous beam: per-region E in {1e5, 1e7} over `regions` seeded Voronoi
+    cells, twisted and pushed like the 3.07M-tet benchmark beam."""
+    r, i0, t, p, ype = beam(n_long, n_y, n_z, seed=seed, twist_deg=twist_deg, axial=axial, voronoi_regions=regions)
+    return Problem(f"voronoi beam {n_y}x{n_z}x{n_long}", r, i0, t, p, model="nh", youngs=1e6, youngs_per_elem=ype,
+                   settings=dict(settings))
+
+
 def small_beam(n_long=8, n_y=2, n_z=2, *, twist_deg=90.0, axial=0.0, seed=7, model="nh",
                youngs=1e7, pin_far_end=True, **settings) -> Problem:
     r, i0, t, p, _ = beam(n_long, n_y, n_z, seed=seed, twist_deg=twist_deg, axial=axial,

@@ -331,6 +331,14 @@ class Device:
         a = np.ascontiguousarray(arr, dtype=np.float64)
         self._check(lib().wrapd_set_elem(self.ctx, name.encode(), dp(a)), f"set_elem {name}")
 
+    def svd3(self, F, precision="fp64"):
+        """The device SVD of (n, 3, 3) matrices: sigma (n,3), U, V (n,3,3)."""
+        F = np.ascontiguousarray(F, dtype=np.float64).reshape(-1, 9)
+        n = len(F)
+        s, U, V = np.zeros((n, 3)), np.zeros((n, 9)), np.zeros((n, 9))
+        self._check(lib().wrapd_svd3_batch(self.ctx, dp(F), n, PRECISION[precision], dp(s), dp(U), dp(V)), "svd3")
+        return s, U.reshape(n, 3, 3), V.reshape(n, 3, 3)
+
     def profile(self, on=True, only=None):
         """Times device work with HIP events; only: one scope name (None: all scopes)."""
         self._check(lib().wrapd_profile_only(self.ctx, only.encode() if only else None), "profile_only")
