@@ -295,31 +295,35 @@ def main():
     # the roofline kernel: the refactorization (the largest device phase per iteration)
     pl = prof_runs["learned_fp32"]["profile"]
     dom = max(pl, key=lambda k2: pl[k2]["ms"])
-    rf = runs["learned_fp32"]["profile"]["refactor"]
-    avg_rf = rf["ms"] / rf["launches"] * 1e-3
-    alg_flops = stats["flops"] + stats["flops_g"]
-    traffic = None
-    tr_path = os.path.join(ROOT, "profiles", f"traffic_refactor_c{args.config}.json")
-    if os.path.exists(tr_path):
-        with open(tr_path) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-    ach = alg_flops / avg_rf / 1e12
-    roofline = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / FP64_PEAK_TFLOPS, "traffic": traffic, "kernel": "refactor",
-                "largest_phase_per_iter": dom,
-                "flops_per_launch": alg_flops,
-                "flops_definition": "Cholesky of every supernode (ns^3/3 + ns^2 m + ns m^2) plus forming G's blocks "
-                                    "(L11^{-1}: ns^3/3, -L21 L11^{-1}: m ns^2), unpadded; issued tile flops "
-                                    f"{stats['factor_tile_flops']:.3e}",
-                "avg_launch_ms": avg_rf * 1e3, "launches": int(rf["launches"]),
-                "timed": "HIP events on the solver's stream around each whole refactorization (all its kernel "
-                         "launches) in the measured run"}
     lpl = pl["local_step"]
     avg_l = lpl["ms"] / lpl["launches"] * 1e-3
     lb = local_step_bytes(prob, 1)
     roofline_local = {"bound": "hbm", "achieved": lb / avg_l / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": lb / avg_l / 1e9 / HBM_PEAK_GBS, "bytes_per_launch": lb,
                       "avg_launch_ms": avg_l * 1e3, "kernel": "local_step (learned path)"}
+    rf = runs["learned_fp32"]["profile"].get("refactor")
+    if rf:   # the refactorization: the largest phase wherever reweighting fires
+        avg_rf = rf["ms"] / rf["launches"] * 1e-3
+        alg_flops = stats["flops"] + stats["flops_g"]
+        traffic = None
+        tr_path = os.path.join(ROOT, "profiles", f"traffic_refactor_c{args.config}.json")
+        if os.path.exists(tr_path):
+            with open(tr_path) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+        ach = alg_flops / avg_rf / 1e12
+        roofline = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": ach / FP64_PEAK_TFLOPS, "traffic": traffic, "kernel": "refactor",
+                    "largest_phase_per_iter": dom,
+                    "flops_per_launch": alg_flops,
+                    "flops_definition": "Cholesky of every supernode (ns^3/3 + ns^2 m + ns m^2) plus forming G's "
+                                        "blocks (L11^{-1}: ns^3/3, -L21 L11^{-1}: m ns^2), unpadded; issued tile "
+                                        f"flops {stats['factor_tile_flops']:.3e}",
+                    "avg_launch_ms": avg_rf * 1e3, "launches": int(rf["launches"]),
+                    "timed": "HIP events on the solver's stream around each whole refactorization (all its kernel "
+                             "launches) in the measured run"}
+    else:    # no reweighting in the timed iterations (a converged beam): the local step's roofline
+        roofline = dict(roofline_local, traffic=None, largest_phase_per_iter=dom,
+                        note="no refactorization in the timed iterations")
 
     c2 = None
     if not args.no_c2 and args.config != 2:

@@ -235,6 +235,9 @@ int wrapd_mlp_fit_norm(wrapd_ctx* ctx);
 /* w^2/vol redrawn log-uniformly over [w_lo, w_hi] (<= 0: the logged range) and the
  * targets re-solved by the FP64 Newton prox */
 int wrapd_mlp_augment(wrapd_ctx* ctx, double w_lo, double w_hi, unsigned long long seed);
+/* full-batch Adam steps on the n given tuples, in order (the Adam state continues from the
+ * previous steps; wrapd_mlp_init / set_params restart it); loss_hist (steps floats) may be NULL */
+int wrapd_mlp_train_full(wrapd_ctx* ctx, const float* tuples, int n, int steps, float lr, float* loss_hist);
 /* Adam steps on minibatches drawn from the log; loss_hist (steps floats) may be NULL */
 int wrapd_mlp_train(wrapd_ctx* ctx, int steps, int batch, float lr, unsigned long long seed, float* loss_hist);
 

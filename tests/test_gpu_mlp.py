@@ -154,3 +154,28 @@ def test_learned_mixed_run_reaches_fp64_fixed_point(trained):
     err = np.linalg.norm(xf - xr) / np.linalg.norm(xr)
     print(f"FP64 {n_ref} iterations, learned + FP32 {n_fast}: x rel err {err:.2e}")
     assert err <= 1e-6
+
+
+def test_adam_steps_match_oracle():
+    """Three full-batch Adam steps on the GPU (k_mlp_adam after the forward/backward)
+    against oracle/mlp_oracle.py's adam applied to the same gradients: each step's gradient
+    is read first with wrapd_mlp_grad at the current parameters (the same kernels the step
+    runs, so the same bits), and the oracle carries the moments in float64."""
+    p, norm = _net()
+    d = Device()
+    d.mlp_set_params(p, norm)
+    t = _tuples(3000)
+    lr = 1e-2
+    pc = p.astype(np.float64)
+    m1 = np.zeros_like(pc)
+    m2 = np.zeros_like(pc)
+    for step in range(1, 4):
+        _, g = d.mlp_grad(t)                       # the gradient this step will use
+        pc, m1, m2 = mo.adam(pc, g.astype(np.float64), m1, m2, step, lr)
+        d.mlp_train_full(t, 1, lr=lr)
+        pg, _ = d.mlp_get_params()
+        dev = np.abs(pg - pc)
+        print(f"step {step}: max |p_gpu - p_oracle| {dev.max():.2e} (update ~{lr:g})")
+        # float32 Adam: the update's relative rounding is a few 1e-7, the parameters round to
+        # float32 (|p| < 1) each step
+        assert dev.max() <= 2e-6 * lr * step + 3e-7

@@ -106,6 +106,7 @@ _PROTOS = {
     "wrapd_mlp_infer": (C.c_int, [P, F, C.c_int, F]),
     "wrapd_mlp_grad": (C.c_int, [P, F, C.c_int, F, F]),
     "wrapd_mlp_grad_noalloc": (C.c_int, [P, F, C.c_int, F, F]),
+    "wrapd_mlp_train_full": (C.c_int, [P, F, C.c_int, C.c_int, C.c_float, F]),
     "wrapd_svd3_batch": (C.c_int, [P, D, C.c_int, C.c_int, D, D, D]),
     "wrapd_mlp_log_start": (C.c_int, [P, C.c_longlong]),
     "wrapd_mlp_log_stop": (C.c_int, [P]),

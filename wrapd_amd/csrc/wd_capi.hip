@@ -1449,6 +1449,35 @@ int wrapd_mlp_augment(wrapd_ctx* c, double w_lo, double w_hi, unsigned long long
     return launch_status(c);
 }
 
+int wrapd_mlp_train_full(wrapd_ctx* c, const float* tuples, int n, int steps, float lr, float* loss_hist) {
+    CTX_OK(c);
+    if (!c->mlp_ready) { c->last_error = "mlp: no network"; return WRAPD_ERR_STATE; }
+    if (steps < 0 || n <= 0 || !tuples || !(lr > 0.0f)) return WRAPD_ERR_ARG;
+    float* dt = nullptr;
+    int* didx = nullptr;
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    int st = WRAPD_OK;
+    if (check_hip(c, hipMalloc(&dt, 9ull * n * sizeof(float)), "mlp train") ||
+        check_hip(c, hipMalloc(&didx, static_cast<size_t>(n) * sizeof(int)), "mlp train") ||
+        check_hip(c, hipMemcpyAsync(dt, tuples, 9ull * n * sizeof(float), hipMemcpyHostToDevice, c->stream), "mlp train") ||
+        check_hip(c, hipMemcpyAsync(didx, idx.data(), static_cast<size_t>(n) * sizeof(int), hipMemcpyHostToDevice,
+                                    c->stream), "mlp train"))
+        st = WRAPD_ERR_HIP;
+    for (int k = 0; k < steps && !st; ++k) {
+        st = mlp_step(c, dt, n, didx, n, 0, lr, ++c->mlp_t);
+        if (!st && loss_hist &&
+            (check_hip(c, hipMemcpyAsync(loss_hist + k, c->mlp_loss, sizeof(float), hipMemcpyDeviceToHost, c->stream),
+                       "mlp train") ||
+             check_hip(c, hipStreamSynchronize(c->stream), "mlp train")))
+            st = WRAPD_ERR_HIP;
+    }
+    if (!st) st = check_hip(c, hipStreamSynchronize(c->stream), "mlp train") ? WRAPD_ERR_HIP : WRAPD_OK;
+    if (dt) (void)hipFree(dt);
+    if (didx) (void)hipFree(didx);
+    return st ? st : launch_status(c);
+}
+
 int wrapd_mlp_train(wrapd_ctx* c, int steps, int batch, float lr, unsigned long long seed, float* loss_hist) {
     CTX_OK(c);
     if (!c->mlp_ready) { c->last_error = "mlp: no network"; return WRAPD_ERR_STATE; }

@@ -417,6 +417,14 @@ class Device:
     def mlp_augment(self, w_lo=0.0, w_hi=0.0, seed=0):
         self._check(lib().wrapd_mlp_augment(self.ctx, float(w_lo), float(w_hi), seed), "mlp_augment")
 
+    def mlp_train_full(self, tuples, steps, lr=1e-3):
+        """Full-batch Adam steps on the given tuples; returns the loss of each step."""
+        t = np.ascontiguousarray(tuples, dtype=np.float32).reshape(-1, 9)
+        hist = np.zeros(max(steps, 1), np.float32)
+        self._check(lib().wrapd_mlp_train_full(self.ctx, fp(t), len(t), int(steps), float(lr), fp(hist)),
+                    "mlp_train_full")
+        return hist[:steps]
+
     def mlp_train(self, steps, batch=8192, lr=1e-3, seed=0):
         hist = np.zeros(max(steps, 1), np.float32)
         self._check(lib().wrapd_mlp_train(self.ctx, int(steps), int(batch), float(lr), seed, fp(hist)), "mlp_train")
