@@ -150,6 +150,15 @@ int wrapd_refactor(wrapd_ctx* ctx);
 /* LinearSolver::solve for the three coordinate columns (SplitLBFGS.hpp:159-171):
  * b, x (n_free,3) host arrays. */
 int wrapd_solve3(wrapd_ctx* ctx, const double* b, double* x);
+/* LinearSolver::compute / factorize (deps/mcloptlib/include/MCL/LinearSolver.hpp:279-298) for
+ * a matrix the caller gives, on a context without a mesh.  analyze: the pattern of the upper
+ * triangle as CSR (row i holds sorted columns j >= i, the diagonal first), n <= 2^31; xyz
+ * (n,3) coordinates of the unknowns for a geometric nested dissection, or NULL for one by
+ * breadth-first level structures of the matrix graph.  factorize: the values in that
+ * pattern's order; WRAPD_ERR_NOT_SPD if A is not positive definite.  wrapd_solve3 then
+ * solves with three right-hand sides (n,3). */
+int wrapd_linsolve_analyze(wrapd_ctx* ctx, int n, const int* row_ptr, const int* cols, const double* xyz);
+int wrapd_linsolve_factorize(wrapd_ctx* ctx, const double* vals);
 /* A values (upper triangle, CSR over free vertices), for checking. Returns nnz. */
 int wrapd_get_A(wrapd_ctx* ctx, int* row_ptr, int* cols, double* vals);
 /* factorization statistics: [nsuper, nnz(L), Cholesky flops, max front, solve leaf tasks,
@@ -169,6 +178,11 @@ int wrapd_global_step_begin(wrapd_ctx* ctx);
 /* UnawareGS::global_step (src/GlobalStepBehavior.hpp:164-184): x = A^-1 b with
  * b from add_to_b (src/TetElements.cpp:470-481). */
 int wrapd_global_step_unaware(wrapd_ctx* ctx, wrapd_global_out* out);
+
+/* TetElements::add_to_b (src/TetElements.cpp:470-481): b = sum_e w_e^2 D_e^T (Z_e - U_e - Dfix x_fix)
+ * over the free vertices, (n_free,3), from the duals as they are (the right-hand side of the
+ * rotation-unaware global step). */
+int wrapd_add_to_b(wrapd_ctx* ctx, double* b);
 
 /* element state, for checking: "U","Z","wsq","cand_wsq","sigma_last","vol","Di","symF"
  * with (n_e,9)/(n_e)/(n_e,3)/(n_e,12) row-major shapes. */

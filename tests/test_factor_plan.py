@@ -45,11 +45,12 @@ def _upper_csr(prob):
     return n, np.cumsum(Ap).astype(np.int32), np.ascontiguousarray(c.astype(np.int32)), np.ascontiguousarray(v)
 
 
-def _run(emu, prob, leaf_size, negate=False):
+def _run(emu, prob, leaf_size, negate=False, coords=True):
     n, Ap, Aj, Ax = _upper_csr(prob)
     free = np.setdiff1d(np.arange(prob.n_verts), prob.pins)
     st = np.zeros(7)
-    emu.pe_analyze(n, ol.ip(Ap), ol.ip(Aj), ol.dp(np.ascontiguousarray(prob.rest[free])), leaf_size, ol.dp(st))
+    xyz = ol.dp(np.ascontiguousarray(prob.rest[free])) if coords else None
+    emu.pe_analyze(n, ol.ip(Ap), ol.ip(Aj), xyz, leaf_size, ol.dp(st))
     nsup = int(st[0])
     perm = np.zeros(n, np.int32)
     col0 = np.zeros(nsup + 1, np.int32)
@@ -77,13 +78,17 @@ CASES = {
     "deep_tree": (lambda: meshgen.small_beam(24, 5, 4, seed=3), 8),
     "two_panel_separators": (lambda: meshgen.small_beam(16, 10, 10, seed=4), 48),
     "ragged": (lambda: meshgen.small_beam(9, 7, 5, seed=5, twist_deg=30.0), 16),
+    # the ordering without vertex coordinates (breadth-first level structures of the graph),
+    # as LinearSolver uses it for a matrix it is given
+    "graph_order": (lambda: meshgen.small_beam(16, 10, 10, seed=4), 48),
+    "graph_order_deep": (lambda: meshgen.small_beam(24, 5, 4, seed=3), 8),
 }
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_plan_matches_dense_cholesky(emu, name):
     make, leaf = CASES[name]
-    R = _run(emu, make(), leaf)
+    R = _run(emu, make(), leaf, coords=not name.startswith("graph_order"))
     assert R["fail"] == 0
     assert R["conflicts"] == 0
     if name == "two_panel_separators":

@@ -108,6 +108,9 @@ _PROTOS = {
     "wrapd_mlp_grad_noalloc": (C.c_int, [P, F, C.c_int, F, F]),
     "wrapd_mlp_train_full": (C.c_int, [P, F, C.c_int, C.c_int, C.c_float, F]),
     "wrapd_svd3_batch": (C.c_int, [P, D, C.c_int, C.c_int, D, D, D]),
+    "wrapd_linsolve_analyze": (C.c_int, [P, C.c_int, I, I, D]),
+    "wrapd_linsolve_factorize": (C.c_int, [P, D]),
+    "wrapd_add_to_b": (C.c_int, [P, D]),
     "wrapd_mlp_log_start": (C.c_int, [P, C.c_longlong]),
     "wrapd_mlp_log_stop": (C.c_int, [P]),
     "wrapd_mlp_log_count": (C.c_longlong, [P]),

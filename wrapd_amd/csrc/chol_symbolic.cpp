@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <numeric>
 #include <stdexcept>
 
@@ -35,17 +36,44 @@ Graph symmetric_graph(int n, const std::vector<int>& Ap, const std::vector<int>&
     return g;
 }
 
-// Geometric nested dissection: split the longest bounding-box axis at the median,
-// take the smaller one-sided vertex separator, recurse, order the separator last.
+// Nested dissection: split the part in two halves, take the smaller one-sided vertex
+// separator, recurse, order the separator last.  With vertex coordinates the halves are
+// the two sides of the median along the longest bounding-box axis (geometric nested
+// dissection); without them, the two sides of the median level of a breadth-first level
+// structure rooted at a pseudo-peripheral vertex of the part (George 1973).
 struct Dissector {
     const Graph& g;
     const double* xyz;
     int leaf;
     std::vector<int> side;    // -1 outside the current call
     std::vector<int> order;
+    std::vector<int> level;   // breadth-first level inside the current part (no coordinates)
+    std::vector<int> queue;
 
     Dissector(const Graph& gg, const double* p, int n, int lf) : g(gg), xyz(p), leaf(lf), side(n, -1) {
         order.reserve(n);
+        if (!xyz) level.assign(n, -1);
+    }
+
+    // breadth-first levels from `root` over the vertices marked side == 3; returns the last
+    // vertex reached (one of the farthest)
+    int bfs(int root) {
+        queue.clear();
+        queue.push_back(root);
+        level[root] = 0;
+        int last = root;
+        for (size_t h = 0; h < queue.size(); ++h) {
+            const int v = queue[h];
+            last = v;
+            for (int p = g.ptr[v]; p < g.ptr[v + 1]; ++p) {
+                const int w = g.adj[p];
+                if (side[w] == 3 && level[w] < 0) {
+                    level[w] = level[v] + 1;
+                    queue.push_back(w);
+                }
+            }
+        }
+        return last;
     }
 
     void run(std::vector<int>& ids) {
@@ -53,19 +81,31 @@ struct Dissector {
             order.insert(order.end(), ids.begin(), ids.end());
             return;
         }
-        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-        for (int v : ids)
-            for (int c = 0; c < 3; ++c) {
-                lo[c] = std::min(lo[c], xyz[3 * v + c]);
-                hi[c] = std::max(hi[c], xyz[3 * v + c]);
-            }
-        int ax = 0;
-        for (int c = 1; c < 3; ++c) if (hi[c] - lo[c] > hi[ax] - lo[ax]) ax = c;
         size_t mid = ids.size() / 2;
-        std::nth_element(ids.begin(), ids.begin() + mid, ids.end(), [&](int a, int b) {
-            double xa = xyz[3 * a + ax], xb = xyz[3 * b + ax];
-            return xa < xb || (xa == xb && a < b);
-        });
+        if (xyz) {
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            for (int v : ids)
+                for (int c = 0; c < 3; ++c) {
+                    lo[c] = std::min(lo[c], xyz[3 * v + c]);
+                    hi[c] = std::max(hi[c], xyz[3 * v + c]);
+                }
+            int ax = 0;
+            for (int c = 1; c < 3; ++c) if (hi[c] - lo[c] > hi[ax] - lo[ax]) ax = c;
+            std::nth_element(ids.begin(), ids.begin() + mid, ids.end(), [&](int a, int b) {
+                double xa = xyz[3 * a + ax], xb = xyz[3 * b + ax];
+                return xa < xb || (xa == xb && a < b);
+            });
+        } else {
+            for (int v : ids) { side[v] = 3; level[v] = -1; }
+            const int far = bfs(*std::min_element(ids.begin(), ids.end()));
+            for (int v : ids) level[v] = -1;
+            bfs(far);
+            // vertices the search did not reach (another component) go last
+            for (int v : ids) if (level[v] < 0) level[v] = std::numeric_limits<int>::max();
+            std::nth_element(ids.begin(), ids.begin() + mid, ids.end(), [&](int a, int b) {
+                return level[a] < level[b] || (level[a] == level[b] && a < b);
+            });
+        }
         for (size_t i = 0; i < ids.size(); ++i) side[ids[i]] = i < mid ? 0 : 1;
         std::vector<int> sepL, sepR;
         for (int v : ids) {
@@ -99,9 +139,10 @@ struct Dissector {
             run(R);
         }
         std::sort(S.begin(), S.end(), [&](int a, int b) {
-            for (int c = 0; c < 3; ++c) {
-                if (xyz[3 * a + c] != xyz[3 * b + c]) return xyz[3 * a + c] < xyz[3 * b + c];
-            }
+            if (xyz)
+                for (int c = 0; c < 3; ++c) {
+                    if (xyz[3 * a + c] != xyz[3 * b + c]) return xyz[3 * a + c] < xyz[3 * b + c];
+                }
             return a < b;
         });
         order.insert(order.end(), S.begin(), S.end());

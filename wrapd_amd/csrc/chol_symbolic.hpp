@@ -71,7 +71,8 @@ struct SymbolicOptions {
 };
 
 // Ap/Aj: upper-triangular CSR (row i holds columns j >= i) over n free vertices;
-// xyz: (n,3) coordinates used for the dissection.
+// xyz: (n,3) coordinates used for the dissection, or null for a dissection by
+// breadth-first level structures of the matrix graph.
 void chol_analyze(int n, const std::vector<int>& Ap, const std::vector<int>& Aj, const double* xyz,
                   const SymbolicOptions& opt, CholSymbolic& out);
 

@@ -140,8 +140,18 @@ void TetElements::check(int st, const char* what) const {
 }
 
 TetElements::TetElements(const std::vector<double>& rest, const std::vector<int>& tets, const std::vector<int>& pins,
-                         const Settings& settings)
-    : m_settings(settings) {
+                         const Settings& settings, int vertex_offset)
+    : Elements(static_cast<int>(tets.size() / 4)), m_settings(settings) {
+    m_tet.resize(tets.size() / 4);
+    for (size_t e = 0; e < m_tet.size(); ++e)
+        for (int q = 0; q < 4; ++q) m_tet[e].x[q] = tets[4 * e + q] + vertex_offset;
+    {
+        std::vector<char> pinned(rest.size() / 3, 0);
+        for (int p : pins)
+            if (p >= 0 && p < static_cast<int>(pinned.size())) pinned[p] = 1;
+        for (size_t v = 0; v < pinned.size(); ++v)
+            if (!pinned[v]) m_free_rows.push_back(static_cast<int>(v));
+    }
     int st = wrapd_create(&m_ctx, settings.m_device);
     if (st != WRAPD_OK) throw DeviceError(st, "wrapd_create: no usable HIP device");
     const ModelSettings& ms = settings.m_model_settings;
@@ -248,6 +258,174 @@ void TetElements::midupdate_weights(bool direct) {
 
 void TetElements::update_actual_weights() { check(wrapd_update_actual_weights(m_ctx), "update_actual_weights"); }
 double TetElements::max_wsq_ratio() { return m_last_local.max_wsq_ratio; }
+
+// ---- the rest of the Elements interface (src/TetElements.cpp), for callers outside ADMM ----
+std::vector<double> TetElements::elem(const char* name, int width) const {
+    std::vector<double> v(static_cast<size_t>(m_n_e) * width);
+    check(wrapd_get_elem(m_ctx, name, v.data()), name);
+    return v;
+}
+
+math::MatX3 TetElements::all_to_free(const math::MatX3& X) const {
+    math::MatX3 x(static_cast<int>(m_free_rows.size()));
+    for (size_t i = 0; i < m_free_rows.size(); ++i)
+        for (int c = 0; c < 3; ++c) x(static_cast<int>(i), c) = X(m_free_rows[i], c);
+    return x;
+}
+
+void TetElements::update_defo_cache(const math::MatX3& x_free) {
+    set_x_free(x_free);
+    update_defo_cache();
+}
+
+void TetElements::initialize_dual_vars(const math::MatX3& X) {
+    // src/TetElements.cpp:145-183 reads F(X) without moving x: evaluate at X, then return to x
+    math::MatX3 x;
+    get_x_free(x);
+    set_x_free(all_to_free(X));
+    initialize_dual_vars();
+    update_defo_cache(x);
+}
+
+void TetElements::midupdate_weights(const math::MatX3& X) {
+    // src/TetElements.cpp:360-379: the candidate weights from F(X), X all vertices
+    math::MatX3 x;
+    get_x_free(x);
+    set_x_free(all_to_free(X));
+    midupdate_weights(true);
+    update_defo_cache(x);
+}
+
+math::MatX3 TetElements::add_to_b() {
+    math::MatX3 b(m_n_f);
+    check(wrapd_add_to_b(m_ctx, b.data()), "add_to_b");
+    return b;
+}
+
+void TetElements::get_A_triplets(math::Triplets& triplets) {
+    // src/TetElements.cpp:483-496 gives one triplet per element entry; the sums per (i, j)
+    // are the same matrix after SpMat::setFromTriplets
+    const int nnz = wrapd_get_A(m_ctx, nullptr, nullptr, nullptr);
+    if (nnz < 0) check(-nnz, "get_A_triplets");
+    std::vector<int> rp(m_n_f + 1), cols(nnz);
+    std::vector<double> vals(nnz);
+    const int r = wrapd_get_A(m_ctx, rp.data(), cols.data(), vals.data());
+    if (r < 0) check(-r, "get_A_triplets");
+    triplets.reserve(triplets.size() + static_cast<size_t>(nnz));
+    for (int i = 0; i < m_n_f; ++i)
+        for (int p = rp[i]; p < rp[i + 1]; ++p) triplets.emplace_back(i, cols[p], vals[p]);
+}
+
+void TetElements::update_A_coeffs(math::SpMat& A) {
+    // src/TetElements.cpp:498-510 adds every entry into A (inserting it into an empty A)
+    math::Triplets t;
+    get_A_triplets(t);
+    if (A.nonZeros() == 0) {
+        if (A.rows() != m_n_f || A.cols() != m_n_f) A.resize(m_n_f, m_n_f);
+        A.setFromTriplets(t.begin(), t.end());
+        return;
+    }
+    for (const math::Triplet& e : t) A.coeffRef(e.row(), e.col()) += e.value();
+}
+
+double TetElements::penalty_energy(bool lagged_u) const {
+    // src/TetElements.cpp:512-525
+    if (lagged_u) throw std::runtime_error("penalty_energy(lagged_u): the previous duals are not kept on the device");
+    const std::vector<double> S = elem("symF", 9), P = elem("P", 9), w = elem("wsq", 1);
+    double se = 0.0;
+    for (int e = 0; e < m_n_e; ++e) {
+        double n2 = 0.0;
+        for (int k = 0; k < 9; ++k) {
+            const double d = S[9ull * e + k] - P[9ull * e + k];
+            n2 += d * d;
+        }
+        se += 0.5 * w[e] * n2;
+    }
+    return se;
+}
+
+double TetElements::primal_residual_sq(Settings::RotAwareness ra) const {
+    // src/TetElements.cpp:527-544: sum w^2 |S - Z|^2, S the cached symmetric factor (rotation
+    // aware) or F^T (unaware), which is what the device's cache holds in each mode
+    if (ra != m_settings.m_rot_awareness)
+        throw std::runtime_error("primal_residual_sq: the deformation cache holds the other rotation mode's factor");
+    const std::vector<double> S = elem("symF", 9), P = elem("P", 9), U = elem("U", 9), w = elem("wsq", 1);
+    double r = 0.0;
+    for (int e = 0; e < m_n_e; ++e) {
+        double n2 = 0.0;
+        for (int k = 0; k < 9; ++k) {
+            const double d = S[9ull * e + k] - (P[9ull * e + k] + U[9ull * e + k]);
+            n2 += d * d;
+        }
+        r += w[e] * n2;
+    }
+    return r;
+}
+
+double TetElements::polar_potential_energy() const {
+    // src/TetElements.cpp:638-675: the energy of Z.  sigma(Z^T) enters only as |sigma|^2 =
+    // |Z|_F^2 and J = prod sigma = det Z (the SVD's sign convention carries det's sign)
+    const std::vector<double> P = elem("P", 9), U = elem("U", 9), vol = elem("vol", 1);
+    const ModelSettings& ms = m_settings.m_model_settings;
+    const bool arap = ms.elastic_model() == ModelSettings::ElasticModel::ARAP;
+    double pe = 0.0;
+    for (int e = 0; e < m_n_e; ++e) {
+        double Z[9];
+        for (int k = 0; k < 9; ++k) Z[k] = P[9ull * e + k] + U[9ull * e + k];
+        if (arap) {
+            double n2 = 0.0;
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) {
+                    const double d = Z[3 * r + c] - (r == c ? 1.0 : 0.0);
+                    n2 += d * d;
+                }
+            pe += 0.5 * vol[e] * n2;
+            continue;
+        }
+        const double mu = m_settings.m_mu_e.empty() ? ms.elastic_lame().mu() : m_settings.m_mu_e[e];
+        const double lambda = m_settings.m_lambda_e.empty() ? ms.elastic_lame().lambda() : m_settings.m_lambda_e[e];
+        double I1 = 0.0;
+        for (int k = 0; k < 9; ++k) I1 += Z[k] * Z[k];
+        const double J = Z[0] * (Z[4] * Z[8] - Z[5] * Z[7]) - Z[1] * (Z[3] * Z[8] - Z[5] * Z[6]) +
+                         Z[2] * (Z[3] * Z[7] - Z[4] * Z[6]);
+        const double J0 = 0.1;
+        double val = 0.5 * mu * (I1 - 3.0);
+        if (J > J0) {
+            const double lj = std::log(J);
+            val += -mu * lj + 0.5 * lambda * lj * lj;
+        } else {
+            const double fJ = std::log(J0) + (J - J0) / J0 - 0.5 * std::pow(J / J0 - 1.0, 2.0);
+            val += -mu * fJ + 0.5 * lambda * fJ * fJ;
+        }
+        pe += vol[e] * val;
+    }
+    return pe;
+}
+
+double TetElements::global_obj_value(const math::MatX3& x_free) {
+    // src/TetElements.cpp:678-688; the deformation cache ends at x_free, as there
+    set_x_free(x_free);
+    double v = 0.0;
+    check(wrapd_global_value_grad(m_ctx, &v, nullptr), "global_obj_value");
+    update_defo_cache();
+    return v;
+}
+
+double TetElements::global_obj_grad(const math::MatX3& x_free, math::MatX3& grad) {
+    set_x_free(x_free);
+    grad = math::MatX3(m_n_f);
+    double v = 0.0;
+    check(wrapd_global_value_grad(m_ctx, &v, grad.data()), "global_obj_grad");
+    update_defo_cache();
+    return v;
+}
+
+math::MatX3 TetElements::global_obj_grad() {
+    math::MatX3 g(m_n_f);
+    double v = 0.0;
+    check(wrapd_global_value_grad(m_ctx, &v, g.data()), "global_obj_grad");
+    return g;
+}
 void TetElements::refactor() {
     check(wrapd_refactor(m_ctx), "factorize");
     check(wrapd_synchronize(m_ctx), "factorize");
@@ -322,6 +500,20 @@ void System::set_pins(const std::vector<int>& inds, const std::vector<math::Vec3
     if ((m_X.rows() == 0 && pin_in_place) || (pin_in_place && !points.empty()))
         throw DeviceError(WRAPD_ERR_ARG, "**Solver::set_pins Error: Bad input.");
     m_constraints->m_pin_inds = inds;
+}
+
+void System::get_A(math::SpMat& A) {
+    // src/System.cpp:78-90
+    math::Triplets t;
+    if (num_tet_elements() > 0) m_tet_elements->get_A_triplets(t);
+    const int n = m_tet_elements ? m_tet_elements->num_free() : 0;
+    A.resize(n, n);
+    A.setFromTriplets(t.begin(), t.end());
+}
+
+void System::update_A(math::SpMat& A) {
+    // src/System.cpp:93-100
+    if (num_tet_elements() > 0) m_tet_elements->update_A_coeffs(A);
 }
 
 bool System::possibly_update_weights(double gamma) {

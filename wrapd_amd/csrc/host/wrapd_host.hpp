@@ -33,6 +33,80 @@ struct MatX3 {
 };
 struct Vec3 { double x[3]; };
 struct Vec4i { int x[4]; };
+using VectorXd = std::vector<double>;
+
+// src/Math.hpp:47-53: Triplet / Triplets / SpMat (a row-major sparse matrix; here compressed
+// rows with sorted columns, duplicates summed by setFromTriplets)
+struct Triplet {
+    int r = 0, c = 0;
+    double v = 0.0;
+    Triplet() = default;
+    Triplet(int row, int col, double value) : r(row), c(col), v(value) {}
+    int row() const { return r; }
+    int col() const { return c; }
+    double value() const { return v; }
+};
+using Triplets = std::vector<Triplet>;
+
+class SpMat {
+ public:
+    SpMat() = default;
+    SpMat(int rows, int cols) { resize(rows, cols); }
+    int rows() const { return m_rows; }
+    int cols() const { return m_cols; }
+    long long nonZeros() const { return static_cast<long long>(m_inner.size()); }
+    void resize(int rows, int cols) {
+        m_rows = rows;
+        m_cols = cols;
+        m_outer.assign(static_cast<size_t>(rows) + 1, 0);
+        m_inner.clear();
+        m_values.clear();
+    }
+    template <typename It>
+    void setFromTriplets(It begin, It end) {
+        std::vector<Triplet> t(begin, end);
+        std::stable_sort(t.begin(), t.end(), [](const Triplet& a, const Triplet& b) {
+            return a.r < b.r || (a.r == b.r && a.c < b.c);
+        });
+        resize(m_rows, m_cols);
+        for (size_t k = 0; k < t.size(); ++k) {
+            if (t[k].r < 0 || t[k].r >= m_rows || t[k].c < 0 || t[k].c >= m_cols)
+                throw std::runtime_error("SpMat::setFromTriplets: index out of range");
+            if (k > 0 && t[k].r == t[k - 1].r && t[k].c == t[k - 1].c) {
+                m_values.back() += t[k].v;
+                continue;
+            }
+            m_inner.push_back(t[k].c);
+            m_values.push_back(t[k].v);
+            m_outer[static_cast<size_t>(t[k].r) + 1]++;
+        }
+        for (int i = 0; i < m_rows; ++i) m_outer[i + 1] += m_outer[i];
+    }
+    // the stored entry (i, j); throws if the pattern has no such entry
+    double& coeffRef(int i, int j) {
+        const int* b = m_inner.data() + m_outer[i];
+        const int* e = m_inner.data() + m_outer[i + 1];
+        const int* p = std::lower_bound(b, e, j);
+        if (p == e || *p != j) throw std::runtime_error("SpMat::coeffRef: entry not in the pattern");
+        return m_values[static_cast<size_t>(p - m_inner.data())];
+    }
+    double coeff(int i, int j) const {
+        const int* b = m_inner.data() + m_outer[i];
+        const int* e = m_inner.data() + m_outer[i + 1];
+        const int* p = std::lower_bound(b, e, j);
+        return (p == e || *p != j) ? 0.0 : m_values[static_cast<size_t>(p - m_inner.data())];
+    }
+    void makeCompressed() {}
+    void setZero() { std::fill(m_values.begin(), m_values.end(), 0.0); }
+    const std::vector<int>& outerIndex() const { return m_outer; }   // row pointers (rows + 1)
+    const std::vector<int>& innerIndex() const { return m_inner; }   // column of each entry
+    const std::vector<double>& valuesVec() const { return m_values; }
+    std::vector<double>& valuesVec() { return m_values; }
+ private:
+    int m_rows = 0, m_cols = 0;
+    std::vector<int> m_outer{0}, m_inner;
+    std::vector<double> m_values;
+};
 }  // namespace math
 
 // src/Lame.hpp:36-58 (Young's modulus is a parameter here)
@@ -207,33 +281,120 @@ class MicroTimer {
     std::chrono::high_resolution_clock::time_point t0 = std::chrono::high_resolution_clock::now();
 };
 
-// src/TetElements.hpp:165-247: the element set, resident on the GPU.
-class TetElements {
+// src/Elements.hpp:36-157: the reference's element interface.  The element data of the one
+// implementation, TetElements, lives on the GPU, so the base class keeps only the count;
+// every operation an implementation does not provide throws, as the reference's do.
+class Elements {
  public:
+    explicit Elements(int num_elements) : m_num_elements(num_elements) {}
+    virtual ~Elements() {}
+    int num_elements() const { return m_num_elements; }
+    virtual int is_inverted(const math::MatX3& x_free, const math::MatX3& x_fix) {
+        (void)x_free;
+        (void)x_fix;
+        return 0;
+    }
+    // src/Elements.hpp:56-69 (max over elements of w^2/cand and its reciprocal)
+    virtual double max_wsq_ratio() { throw std::runtime_error("Error: Need to implement max_wsq_ratio"); }
+    virtual void update_actual_weights() { throw std::runtime_error("Error: Need to implement update_actual_weights"); }
+    virtual void initialize_dual_vars(const math::MatX3& X) {
+        (void)X;
+        throw std::runtime_error("Error: Need to implement initialize_dual_vars");
+    }
+    virtual void advance_dual_vars() { throw std::runtime_error("Error: Need to implement advance_dual_vars"); }
+    virtual void midupdate_weights(const math::MatX3& X) {
+        (void)X;
+        throw std::runtime_error("Error: Need to implement midupdate_weights(X)");
+    }
+    virtual void midupdate_weights() { throw std::runtime_error("Error: Need to implement midupdate_weights()"); }
+    virtual void update(bool update_candidate_weights) {
+        (void)update_candidate_weights;
+        throw std::runtime_error("Error: Need to implement EnergyTerm::update(MatX3, bool)");
+    }
+    virtual double polar_update(bool update_candidate_weights) {
+        (void)update_candidate_weights;
+        throw std::runtime_error("Error: Implement polar_update(bool)");
+    }
+    virtual math::MatX3 add_to_b() { throw std::runtime_error("Error: Need to implement add_to_b()"); }
+    virtual void get_A_triplets(math::Triplets& triplets) {
+        (void)triplets;
+        throw std::runtime_error("Error: Need to implement get_A_triplets(triplets)");
+    }
+    virtual void update_A_coeffs(math::SpMat& A) {
+        (void)A;
+        throw std::runtime_error("Error: Need to implement update_A_coeffs(A)");
+    }
+    virtual double penalty_energy(bool lagged_u) const {
+        (void)lagged_u;
+        throw std::runtime_error("Error: Need to implement penalty_energy(lagged_u)");
+    }
+    virtual double gradient(const math::MatX3& Ft, math::MatX3& grad) {
+        (void)Ft;
+        (void)grad;
+        throw std::runtime_error("Error: Need to implement gradient");
+    }
+    virtual double primal_residual_sq(Settings::RotAwareness rot_awareness) const {
+        (void)rot_awareness;
+        throw std::runtime_error("Error: Need to implement primal_residual_sq");
+    }
+    virtual double potential_energy_x() const { throw std::runtime_error("Error: Need to implement potential_energy_x"); }
+    virtual double polar_potential_energy() const {
+        throw std::runtime_error("Error: Need to implement polar_potential_energy");
+    }
+
+ protected:
+    virtual double potential_energy(const math::MatX3& Ft) {
+        (void)Ft;
+        throw std::runtime_error("Error: Need to implement EnergyTerm::potential_energy(const MatX3 &Ft)");
+    }
+    const int m_num_elements;
+};
+
+// src/TetElements.hpp:165-247: the element set, resident on the GPU.
+class TetElements : public Elements {
+ public:
+    // tets: indices into rest (n_v x 3); vertex_offset is added to the indices inds() reports
+    // (create_tets_from_mesh, src/TetElements.hpp:285)
     TetElements(const std::vector<double>& rest, const std::vector<int>& tets, const std::vector<int>& pins,
-                const Settings& settings);
-    ~TetElements();
+                const Settings& settings, int vertex_offset = 0);
+    ~TetElements() override;
     TetElements(const TetElements&) = delete;
     TetElements& operator=(const TetElements&) = delete;
 
-    int num_elements() const { return m_n_e; }
     int num_free() const { return m_n_f; }
     wrapd_ctx* ctx() { return m_ctx; }
+    const std::vector<math::Vec4i>& inds() const { return m_tet; }   // src/TetElements.hpp:207
 
     void update_fix_cache(const math::MatX3& x_fix);
     void set_x_free(const math::MatX3& x_free);
     void get_x_free(math::MatX3& x_free);
-    double potential_energy_x() const { return m_energy; }
+    double potential_energy_x() const override { return m_energy; }
     int inversion_count() const { return m_flips; }
     void update_defo_cache();                                  // energy + flips + symF at x
+    void update_defo_cache(const math::MatX3& x_free);         // the same at the given x (it stays the current x)
     void initialize_dual_vars();                               // at the current x
-    double polar_update(bool update_candidate_weights);        // returns dAL of the local step
+    void initialize_dual_vars(const math::MatX3& X) override;  // at X (all vertices); the current x is kept
+    // The device keeps no copies of the previous iterate's Z and U: the reference reads them
+    // only in penalty_energy(true) and scales them with the weights in update_actual_weights.
+    void advance_dual_vars() override {}
+    double polar_update(bool update_candidate_weights) override;   // returns dAL of the local step
     void polar_update_launch(bool update_candidate_weights);   // polar_update in two halves
     double polar_update_finish();
-    void update(bool update_candidate_weights);                // -ra 0
+    void update(bool update_candidate_weights) override;       // -ra 0
     void midupdate_weights(bool direct);
-    void update_actual_weights();
-    double max_wsq_ratio();
+    void midupdate_weights(const math::MatX3& X) override;     // from F(X), X all vertices
+    void midupdate_weights() override { midupdate_weights(false); }
+    void update_actual_weights() override;
+    double max_wsq_ratio() override;
+    math::MatX3 add_to_b() override;                          // (n_free, 3)
+    void get_A_triplets(math::Triplets& triplets) override;   // upper triangle, one triplet per entry
+    void update_A_coeffs(math::SpMat& A) override;            // A(i, j) += entry, on A's pattern
+    double penalty_energy(bool lagged_u) const override;      // lagged_u: not kept (throws)
+    double primal_residual_sq(Settings::RotAwareness rot_awareness) const override;
+    double polar_potential_energy() const override;
+    double global_obj_value(const math::MatX3& x_free);       // src/TetElements.cpp:678-688 (x_free becomes current)
+    double global_obj_grad(const math::MatX3& x_free, math::MatX3& grad);   // :691-738
+    math::MatX3 global_obj_grad();                            // :742-787, at the current x
     void refactor();
     void global_step(double early_exit_tol, wrapd_global_out& out);
     void global_step_begin();     // queues the global step's first value and gradient
@@ -254,9 +415,13 @@ class TetElements {
 
  private:
     void check(int st, const char* what) const;
+    std::vector<double> elem(const char* name, int width) const;   // device element array, row-major
+    math::MatX3 all_to_free(const math::MatX3& X) const;
     const Settings m_settings;
     wrapd_ctx* m_ctx = nullptr;
     int m_n_e = 0, m_n_f = 0;
+    std::vector<math::Vec4i> m_tet;   // vertex indices (+ vertex_offset)
+    std::vector<int> m_free_rows;     // vertex of each free index
     double m_energy = 0.0;
     int m_flips = 0;
     wrapd_local_out m_last_local{};
@@ -268,17 +433,18 @@ inline void create_tets_from_mesh(std::shared_ptr<TYPE>& tet_elements, const IN_
                                   int n_tets, const Settings& settings, const std::vector<int>& free_index_list,
                                   const std::vector<int>& fix_index_list, const int vertex_offset) {
     // src/TetElements.hpp:250-288; the GPU keeps whole-mesh arrays instead of per-tet copies
+    // the vertices of the mesh are verts[0 .. n_v); the tets index them, and the stored
+    // indices (inds()) are shifted by vertex_offset as the reference's are
     const int n_v = static_cast<int>(free_index_list.size());
     std::vector<double> rest(3ull * n_v);
     for (long long i = 0; i < 3ll * n_v; ++i) rest[i] = static_cast<double>(verts[i]);
     std::vector<int> tets(inds, inds + 4ll * n_tets);
-    (void)vertex_offset;
     std::vector<int> pins;
     std::vector<std::pair<int, int>> fx;
     for (int v = 0; v < n_v; ++v) if (fix_index_list[v] >= 0) fx.emplace_back(fix_index_list[v], v);
     std::sort(fx.begin(), fx.end());
     for (auto& p : fx) pins.push_back(p.second);
-    tet_elements = std::make_shared<TYPE>(rest, tets, pins, settings);
+    tet_elements = std::make_shared<TYPE>(rest, tets, pins, settings, vertex_offset);
 }
 
 class ConstraintSet {
@@ -305,6 +471,20 @@ class System {
     void local_update(bool update_candidate_weights) { m_tet_elements->update(update_candidate_weights); }
     double problem_objective() const { return m_tet_elements->potential_energy_x(); }
     int inversion_count() const { return m_tet_elements->inversion_count(); }
+    // src/System.cpp:66-100 and System.hpp:120-156
+    math::MatX3 get_b() { return m_tet_elements->add_to_b(); }
+    void get_A(math::SpMat& A);
+    void update_A(math::SpMat& A);
+    double penalty_energy(bool lagged_u) const { return m_tet_elements->penalty_energy(lagged_u); }
+    double primal_residual_sq() const { return m_tet_elements->primal_residual_sq(m_settings.m_rot_awareness); }
+    void midupdate_WtW(const math::MatX3& X) { m_tet_elements->midupdate_weights(X); }
+    void midupdate_WtW() { m_tet_elements->midupdate_weights(); }
+    double global_obj_value(const math::MatX3& x_free) { return m_tet_elements->global_obj_value(x_free); }
+    double global_obj_grad(const math::MatX3& x_free, math::MatX3& grad, bool positions_have_changed) {
+        if (positions_have_changed) return m_tet_elements->global_obj_grad(x_free, grad);
+        grad = m_tet_elements->global_obj_grad();
+        return 0.0;
+    }
     void set_initial_pose() { m_X = m_X_init; }
     void initialize() {}
  private:

@@ -686,8 +686,71 @@ int wrapd_refactor(wrapd_ctx* c) {
     return launch_status(c);
 }
 
+int wrapd_linsolve_analyze(wrapd_ctx* c, int n, const int* row_ptr, const int* cols, const double* xyz) {
+    CTX_OK(c);
+    if (c->mesh_ready) { c->last_error = "wrapd_linsolve_analyze: this context holds a mesh"; return WRAPD_ERR_STATE; }
+    if (n <= 0 || !row_ptr || !cols || row_ptr[0] != 0) { c->last_error = "wrapd_linsolve_analyze: bad arguments"; return WRAPD_ERR_ARG; }
+    for (int i = 0; i < n; ++i) {
+        // upper triangle by rows, columns strictly increasing, the diagonal first
+        if (row_ptr[i + 1] <= row_ptr[i] || cols[row_ptr[i]] != i) {
+            c->last_error = "wrapd_linsolve_analyze: row " + std::to_string(i) + " lacks its diagonal entry";
+            return WRAPD_ERR_ARG;
+        }
+        for (int p = row_ptr[i] + 1; p < row_ptr[i + 1]; ++p)
+            if (cols[p] <= cols[p - 1] || cols[p] >= n) {
+                c->last_error = "wrapd_linsolve_analyze: row " + std::to_string(i) +
+                                " is not sorted upper-triangular with columns < n";
+                return WRAPD_ERR_ARG;
+            }
+    }
+    (void)hipSetDevice(c->device);
+    drop_begun(c);
+    c->n_f = n;
+    c->n_v = n;
+    c->Ap.assign(row_ptr, row_ptr + n + 1);
+    c->Aj.assign(cols, cols + row_ptr[n]);
+    c->nnzA = row_ptr[n];
+    if (dalloc(c, c->Ax, c->nnzA) || dalloc(c, c->bv, 3ull * n) || dalloc(c, c->rv, 3ull * n)) return WRAPD_ERR_HIP;
+    wrapd::SymbolicOptions opt;
+    try {
+        wrapd::chol_analyze(n, c->Ap, c->Aj, xyz, opt, c->sym);
+    } catch (const std::exception& ex) {
+        c->last_error = ex.what();
+        return WRAPD_ERR_STATE;
+    }
+    int st = chol_prepare_device(c);
+    if (!st) st = factor_prepare_device(c);
+    if (st) return st;
+    c->linsolve = true;
+    c->analyzed = true;
+    c->factored = false;
+    return launch_status(c);
+}
+
+int wrapd_linsolve_factorize(wrapd_ctx* c, const double* vals) {
+    CTX_OK(c);
+    if (!c->linsolve || !c->analyzed) { c->last_error = "wrapd_linsolve_factorize: no analyzed pattern"; return WRAPD_ERR_STATE; }
+    if (!vals) return WRAPD_ERR_ARG;
+    if (h2d(c, c->Ax, vals, static_cast<size_t>(c->nnzA))) return WRAPD_ERR_HIP;
+    {
+        ProfScope ps(c, "refactor");
+        launch_factor(c);
+    }
+    int fail = 0;
+    if (check_hip(c, hipMemcpyAsync(&fail, c->d_ffail, sizeof(int), hipMemcpyDeviceToHost, c->stream), "factor status") ||
+        check_hip(c, hipStreamSynchronize(c->stream), "factor"))
+        return WRAPD_ERR_HIP;
+    c->factored = !fail;
+    if (fail) {
+        c->last_error = "factorization: non-positive pivot (A not positive definite)";
+        return WRAPD_ERR_NOT_SPD;
+    }
+    return launch_status(c);
+}
+
 int wrapd_solve3(wrapd_ctx* c, const double* b, double* x) {
-    CTX_OK(c); MESH_OK(c);
+    CTX_OK(c);
+    if (!c->mesh_ready && !c->linsolve) { c->last_error = "mesh not uploaded"; return WRAPD_ERR_STATE; }
     drop_begun(c);
     if (!c->factored) { c->last_error = "no factorization"; return WRAPD_ERR_STATE; }
     if (!b || !x) return WRAPD_ERR_ARG;
@@ -1009,6 +1072,16 @@ int wrapd_global_step_unaware(wrapd_ctx* c, wrapd_global_out* out) {
     c->last_energy = o.energy;
     c->last_flips = o.flips;
     if (out) *out = o;
+    return launch_status(c);
+}
+
+int wrapd_add_to_b(wrapd_ctx* c, double* b) {
+    CTX_OK(c); MESH_OK(c);
+    if (!b) return WRAPD_ERR_ARG;
+    drop_begun(c);
+    launch_defo(c, DM_BVEC, 0);
+    launch_gather_grad(c, c->bv, 0, 0, nullptr, 0);
+    if (d2h(c, b, c->bv, 3ull * c->n_f)) return WRAPD_ERR_HIP;
     return launch_status(c);
 }
 

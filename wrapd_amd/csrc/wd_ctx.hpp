@@ -61,6 +61,7 @@ struct wrapd_ctx {
     // sizes
     int n_v = 0, n_e = 0, n_ep = 0, n_f = 0, n_fix = 0;
     bool mesh_ready = false;
+    bool linsolve = false;      // a matrix-only context (wrapd_linsolve_analyze): A given by the caller
     // host copies
     std::vector<int> tets, pins, free_idx, fix_idx;
     std::vector<double> rest;

@@ -539,7 +539,6 @@ k_ffinal(const int2* __restrict__ tasks, FactArgs A) {
         if (q0 + kFK < ns) fetch(q0 + kFK);
         if (!idle) {
             const int nk = (nq + 3) & ~3;
-#pragma unroll 2
             for (int kk = 0; kk < nk; kk += 4) {
                 const int k = kk + lk;
                 double bj[4], ai[4];

@@ -267,6 +267,25 @@ class Device:
         self._check(lib().wrapd_global_value_grad(self.ctx, C.byref(v), dp(g)), "value_grad")
         return v.value, g
 
+    def add_to_b(self):
+        n_f = self.sizes()[2]
+        b = np.zeros((n_f, 3))
+        self._check(lib().wrapd_add_to_b(self.ctx, dp(b)), "add_to_b")
+        return b
+
+    def linsolve_analyze(self, row_ptr, cols, xyz=None):
+        """Symbolic analysis of a caller-given SPD matrix's upper-triangle pattern (no mesh)."""
+        rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        cl = np.ascontiguousarray(cols, dtype=np.int32)
+        x = None if xyz is None else np.ascontiguousarray(xyz, dtype=np.float64)
+        self._keep = [rp, cl, x]
+        self._check(lib().wrapd_linsolve_analyze(self.ctx, len(rp) - 1, ip(rp), ip(cl), None if x is None else dp(x)),
+                    "linsolve_analyze")
+
+    def linsolve_factorize(self, vals):
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        self._check(lib().wrapd_linsolve_factorize(self.ctx, dp(v)), "linsolve_factorize")
+
     def refactor(self):
         self._check(lib().wrapd_refactor(self.ctx), "refactor")
 
