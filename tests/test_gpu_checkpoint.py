@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from wrapd_amd import meshgen
-from wrapd_amd.solver import Device, Solver
+from wrapd_amd.solver import Device, Solver, WrapdError
 
 pytestmark = pytest.mark.gpu
 
@@ -104,3 +104,28 @@ def test_device_state_round_trip():
     assert np.array_equal(d.elem("U"), u0) and np.array_equal(d.elem("Z"), z0)
     with pytest.raises(Exception):
         d.state_load(blob[:-8])    # wrong size
+
+
+def test_state_load_rejects_corrupt_bookkeeping():
+    """wrapd_state_load checks the L-BFGS bookkeeping it will index device buffers with: the
+    slot map must be a permutation of the 8 slots, the pair count within the memory."""
+    prob = meshgen.small_beam(8, 2, 2, seed=3)
+    s = Solver(prob)
+    s.initialize()
+    s.setup()
+    _iterate(s, 3)
+    d = s.device()
+    blob = bytearray(d.state_save())
+    # header: 8-byte magic, 9 ints (n_e, n_ep, n_f, n_fix, nhist, rot_aware, m_k, last_flips,
+    # grad_old_valid), then the 8-int slot map
+    bad = bytearray(blob)
+    bad[44:48] = (9).to_bytes(4, "little", signed=True)      # slot 9 of 8
+    with pytest.raises(WrapdError) as e:
+        d.state_load(bytes(bad))
+    assert e.value.status == 1
+    bad = bytearray(blob)
+    bad[32:36] = (100).to_bytes(4, "little", signed=True)    # m_k beyond the memory
+    with pytest.raises(WrapdError) as e:
+        d.state_load(bytes(bad))
+    assert e.value.status == 1
+    d.state_load(bytes(blob))                                 # the genuine blob still loads

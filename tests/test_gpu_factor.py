@@ -73,3 +73,22 @@ def test_factor_reports_indefinite_matrix():
     assert e.value.status == 4   # WRAPD_ERR_NOT_SPD
     d.set_elem("wsq", w)
     _check_solve(d, 4)
+
+
+def test_assembly_in_lds_and_in_memory_agree(monkeypatch):
+    """Front columns of fronts with 2048 to 8192 rows are assembled in LDS, the others in
+    memory; with WRAPD_ASM_LDS_MAX=0 every front takes the memory path.  Both add A's entries and then the
+    children's columns in the same order, so the factorizations are the same bits."""
+    prob = meshgen.config_problem(3)   # fronts up to ~4000 rows
+    b = np.random.default_rng(7).standard_normal((prob.n_verts - len(prob.pins), 3))
+    xs = []
+    for lim in (None, "0"):
+        if lim is None:
+            monkeypatch.delenv("WRAPD_ASM_LDS_MAX", raising=False)
+        else:
+            monkeypatch.setenv("WRAPD_ASM_LDS_MAX", lim)
+        d = _device(prob)
+        d.refactor()
+        xs.append(d.solve3(b))
+        d.close()
+    assert np.array_equal(xs[0], xs[1])

@@ -155,7 +155,7 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P) {
 
     // launches
     auto begin = [&](int kind, int panel) {
-        P.launches.push_back(FactorLaunch{kind, panel, static_cast<int64_t>(P.tasks.size() / 2), 0});
+        P.launches.push_back(FactorLaunch{kind, panel, static_cast<int64_t>(P.tasks.size() / 2), 0, 0});
     };
     auto push = [&](int x, int y) {
         P.tasks.push_back(x);
@@ -166,8 +166,10 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P) {
     for (int l = 0; l < nlev; ++l) {
         const std::vector<int>& L = by_level[l];
         begin(FK_ASSEMBLE, 0);
-        for (int s : L)
+        for (int s : L) {
             for (int j = 0; j < f[s]; ++j) push(s, j);
+            P.launches.back().maxf = std::max(P.launches.back().maxf, f[s]);
+        }
         int pmax = 0;
         for (int s : L) pmax = std::max(pmax, tile_count_l11(ns[s]));
         for (int k = 0; k < pmax; ++k) {

@@ -49,6 +49,7 @@ struct FactorLaunch {
     int panel;          // k (unused for FK_ASSEMBLE)
     int64_t offset;     // first task (int pairs) in tasks
     int count;          // tasks
+    int maxf = 0;       // FK_ASSEMBLE: the largest front of the launch (its LDS column buffer)
 };
 
 struct FactorPlan {

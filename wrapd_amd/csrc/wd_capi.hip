@@ -127,9 +127,26 @@ static const double* scal_read_epoch(wrapd_ctx* c, int at, unsigned want) {
     }
     return c->scal_host + at;
 }
+// A reduction whose wait for a block's partials ran out leaves NaN results and sets the
+// sticky abort word; the partials are then in no defined state, so the context refuses all
+// further work (wrapd_destroy still works).
+static bool reduction_aborted(wrapd_ctx* c) {
+    int v = 0;
+    if (hipMemcpyAsync(&v, c->d_red_abort, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess || v != 0) {
+        c->dead = true;
+        c->last_error = "reduction: a block's partial results did not arrive (wait timed out); the context is unusable";
+        return true;
+    }
+    return false;
+}
+
 static const double* scal_read(wrapd_ctx* c, int at, int n) {
-    (void)n;
-    return scal_read_epoch(c, at, c->host_epoch);
+    const double* r = scal_read_epoch(c, at, c->host_epoch);
+    if (!r) return r;
+    for (int k = 0; k < n; ++k)
+        if (std::isnan(r[k]) && reduction_aborted(c)) return nullptr;
+    return r;
 }
 
 // Entry points that change what a begun global step's first launches read (x, the duals,
@@ -195,7 +212,13 @@ static_assert(mlp_grad_layout(kMlpGradNoAllocMax).end <= kScratchBytes, "scratch
 
 using namespace wd;
 
-#define CTX_OK(c) do { if (!(c)) return WRAPD_ERR_ARG; } while (0)
+#define CTX_OK(c) do {                                                                        \
+        if (!(c)) return WRAPD_ERR_ARG;                                                       \
+        if ((c)->dead) {                                                                      \
+            (c)->last_error = "the context is unusable after a device wait timed out; destroy it"; \
+            return WRAPD_ERR_STATE;                                                           \
+        }                                                                                     \
+    } while (0)
 #define MESH_OK(c) do { if (!(c)->mesh_ready) { (c)->last_error = "mesh not uploaded"; return WRAPD_ERR_STATE; } } while (0)
 
 extern "C" {
@@ -237,6 +260,10 @@ int wrapd_create(wrapd_ctx** out, int device) {
     if (const char* e = std::getenv("WRAPD_TAIL_MAX_BLOCKS")) {
         const int v = std::atoi(e);
         if (v >= 1 && v <= kTailMaxBlocks) c->tail_max_blocks = v;
+    }
+    if (const char* e = std::getenv("WRAPD_ASM_LDS_MAX")) {
+        const int v = std::atoi(e);
+        if (v >= 0 && v <= c->asm_lds_max) c->asm_lds_max = v;
     }
     launch_fill(c, c->scal, 0.0, S_NUM);
     std::fill(c->scal_host, c->scal_host + S_NUM, 0.0);
@@ -508,7 +535,7 @@ int wrapd_x_distance_launch(wrapd_ctx* c) {
 int wrapd_x_distance_result(wrapd_ctx* c, double* dist) {
     CTX_OK(c); MESH_OK(c);
     const double* r = scal_read_epoch(c, S_XD, c->xd_epoch);
-    if (!r) return WRAPD_ERR_HIP;
+    if (!r || (std::isnan(r[0]) && reduction_aborted(c))) return WRAPD_ERR_HIP;
     if (dist) *dist = std::sqrt(r[0]);
     return WRAPD_OK;
 }
@@ -1185,6 +1212,21 @@ int wrapd_state_load(wrapd_ctx* c, const void* buf, size_t size) {
     size_t total = sizeof(StateHeader);
     for (const auto& a : arrs) total += a.second * sizeof(double);
     if (size != total) { c->last_error = "wrapd_state_load: size mismatch"; return WRAPD_ERR_ARG; }
+    {
+        // the L-BFGS bookkeeping indexes device buffers: reject what no save can produce
+        bool seen[kMaxHist] = {};
+        bool perm = true;
+        for (int k = 0; k < kMaxHist; ++k) {
+            const int v = h.pos2slot[k];
+            if (v < 0 || v >= kMaxHist || seen[v]) { perm = false; break; }
+            seen[v] = true;
+        }
+        if (!perm || h.lcfg.memory < 1 || h.lcfg.memory > kMaxHist || h.m_k < 0 || h.m_k > h.lcfg.memory ||
+            (h.rot_aware != 0 && h.rot_aware != 1)) {
+            c->last_error = "wrapd_state_load: corrupt L-BFGS state";
+            return WRAPD_ERR_ARG;
+        }
+    }
     const char* p = static_cast<const char*>(buf) + sizeof(h);
     for (const auto& a : arrs) {
         if (check_hip(c, hipMemcpyAsync(a.first, p, a.second * sizeof(double), hipMemcpyHostToDevice, c->stream),
@@ -1195,7 +1237,7 @@ int wrapd_state_load(wrapd_ctx* c, const void* buf, size_t size) {
     }
     if (check_hip(c, hipStreamSynchronize(c->stream), "sync")) return WRAPD_ERR_HIP;
     c->rot_aware = h.rot_aware;
-    c->m_k = h.m_k;
+    c->m_k = h.m_k;   // checked below, before any device copy
     c->last_flips = h.last_flips;
     c->grad_old_valid = h.grad_old_valid != 0;
     for (int k = 0; k < kMaxHist; ++k) c->pos2slot[k] = h.pos2slot[k];

@@ -62,6 +62,7 @@ struct wrapd_ctx {
     int n_v = 0, n_e = 0, n_ep = 0, n_f = 0, n_fix = 0;
     bool mesh_ready = false;
     bool linsolve = false;      // a matrix-only context (wrapd_linsolve_analyze): A given by the caller
+    bool dead = false;          // a device wait timed out: every entry point but destroy refuses
     // host copies
     std::vector<int> tets, pins, free_idx, fix_idx;
     std::vector<double> rest;
@@ -188,6 +189,9 @@ struct wrapd_ctx {
     // launch (kTailMaxBlocks; WRAPD_TAIL_MAX_BLOCKS in the environment at wrapd_create
     // lowers it, so small meshes can exercise that path in tests)
     int tail_max_blocks = 1024;
+    // front assembly builds a column in LDS up to this many rows (64 KiB), larger fronts in
+    // memory (WRAPD_ASM_LDS_MAX in the environment at wrapd_create lowers it, for tests)
+    int asm_lds_max = 8192;
     double* scal_host = nullptr; // pinned host mirror
     unsigned* host_seq = nullptr; // pinned: the epoch of the last reduction whose host copy is stored
 

@@ -5,8 +5,8 @@
 //
 // The schedule (levels, 64x64 tiles, task lists, pool offsets) comes from chol_plan on
 // the host.  Four kernels, one launch per (level, step):
-//   k_fassemble  one wave per front column: zero, store A entries, add children's
-//                Schur-complement columns in child order (deterministic, no atomics)
+//   k_fassemble  one workgroup per front column, built in LDS: A entries, then children's
+//                Schur-complement columns in child order (deterministic, no atomics), stored once
 //   k_fpotrf     one workgroup per supernode: Cholesky L_kk of the panel's diagonal tile
 //                and its inverse in LDS; L_kk goes to the tile's lower half, L_kk^{-1}
 //                to G's diagonal tile (k, k), which is also the diagonal tile of L11^{-1}
@@ -56,11 +56,60 @@ __global__ void k_fill_int(int* __restrict__ p, int n, int v) {
 }
 
 // ---------------------------------------------------------------------------------
-__global__ void __launch_bounds__(64)
+// One workgroup per front column j: the column's lower part [j, f) is built in LDS (A's
+// entries, then each child's Schur-complement column added in child order: deterministic,
+// no atomics) and written to the front once, so the front is neither zeroed in memory nor
+// read back.  Rows < ns of G's column j (L11^{-1}'s) are zeroed for the TRSM/update steps.
+constexpr int kAsmThreads = 256;
+constexpr int kAsmLdsMin = 2048;   // fronts from this size on are assembled in LDS
+__global__ void __launch_bounds__(kAsmThreads)
 k_fassemble(const int2* __restrict__ tasks, FactArgs A, const double* __restrict__ Ax,
             const int64_t* __restrict__ cbase, const int64_t* __restrict__ acol_ptr, const int2* __restrict__ acol,
             const int64_t* __restrict__ ccol_ptr, const int2* __restrict__ ccol, const int64_t* __restrict__ relptr,
             const int* __restrict__ rel) {
+    extern __shared__ double col[];   // rows j .. f-1 of the column
+    const int2 task = tasks[blockIdx.x];
+    const int s = task.x, j = task.y;
+    const int t = threadIdx.x;
+    const int f = A.fsize[s];
+    const int ns = A.col0[s + 1] - A.col0[s];
+    const int len = f - j;
+    for (int r = t; r < len; r += kAsmThreads) col[r] = 0.0;
+    if (j < ns) {   // L11^{-1} rows only: G's bottom rows are assigned whole later
+        double* Gc = A.G + A.goff[s] + static_cast<int64_t>(j) * wrapd::g_ld(f);
+        for (int r = t; r < ns; r += kAsmThreads) Gc[r] = 0.0;
+    }
+    __syncthreads();
+    const int64_t gc = cbase[s] + j;
+    for (int64_t q = acol_ptr[gc] + t; q < acol_ptr[gc + 1]; q += kAsmThreads) {
+        const int2 e = acol[q];
+        col[e.x - j] = Ax[e.y];
+    }
+    const int64_t c1 = ccol_ptr[gc + 1];
+    for (int64_t q = ccol_ptr[gc]; q < c1; ++q) {
+        __syncthreads();   // the previous contribution (or A's entries) is in
+        const int2 e = ccol[q];
+        const int ch = e.x, jj = e.y;
+        const int fc = A.fsize[ch];
+        const int nsc = A.col0[ch + 1] - A.col0[ch];
+        const int mc = fc - nsc;
+        const double* src = A.pool + A.foff[ch] + static_cast<int64_t>(nsc + jj) * fc + nsc;
+        const int* rl = rel + relptr[ch];
+        for (int ii = jj + t; ii < mc; ii += kAsmThreads) col[rl[ii] - j] += src[ii];
+    }
+    __syncthreads();
+    double* F = A.pool + A.foff[s] + static_cast<int64_t>(j) * f + j;
+    for (int r = t; r < len; r += kAsmThreads) F[r] = col[r];
+}
+
+// The same in memory, one wave per column, zeroing the column and adding into it there: for
+// fronts below kAsmLdsMin rows (measured faster there: the leaf levels hold hundreds of
+// thousands of short columns) and above what a workgroup's LDS holds (64 KiB).
+__global__ void __launch_bounds__(64)
+k_fassemble_mem(const int2* __restrict__ tasks, FactArgs A, const double* __restrict__ Ax,
+                const int64_t* __restrict__ cbase, const int64_t* __restrict__ acol_ptr, const int2* __restrict__ acol,
+                const int64_t* __restrict__ ccol_ptr, const int2* __restrict__ ccol, const int64_t* __restrict__ relptr,
+                const int* __restrict__ rel) {
     const int2 task = tasks[blockIdx.x];
     const int s = task.x, j = task.y;
     const int lane = threadIdx.x;
@@ -68,7 +117,7 @@ k_fassemble(const int2* __restrict__ tasks, FactArgs A, const double* __restrict
     const int ns = A.col0[s + 1] - A.col0[s];
     double* F = A.pool + A.foff[s] + static_cast<int64_t>(j) * f;
     for (int r = j + lane; r < f; r += 64) F[r] = 0.0;
-    if (j < ns) {   // L11^{-1} rows only: G's bottom rows are assigned whole later
+    if (j < ns) {
         double* Gc = A.G + A.goff[s] + static_cast<int64_t>(j) * wrapd::g_ld(f);
         for (int r = lane; r < ns; r += 64) Gc[r] = 0.0;
     }
@@ -631,9 +680,17 @@ void launch_factor(wrapd_ctx* c) {
         const int2* t = tasks + L.offset;
         switch (L.kind) {
         case wrapd::FK_ASSEMBLE:
-            hipLaunchKernelGGL(k_fassemble, dim3(L.count), dim3(64), 0, c->stream, t, A, c->Ax, c->d_cbase,
-                               c->d_acol_ptr, reinterpret_cast<const int2*>(c->d_acol), c->d_ccol_ptr,
-                               reinterpret_cast<const int2*>(c->d_ccol), c->d_relptr, c->d_rel);
+            // small fronts (the many leaf-level columns): one wave per column in memory is
+            // faster than a workgroup per column; large ones are built in LDS
+            if (L.maxf >= kAsmLdsMin && L.maxf <= c->asm_lds_max)
+                hipLaunchKernelGGL(k_fassemble, dim3(L.count), dim3(kAsmThreads),
+                                   static_cast<size_t>(L.maxf) * sizeof(double), c->stream, t, A, c->Ax, c->d_cbase,
+                                   c->d_acol_ptr, reinterpret_cast<const int2*>(c->d_acol), c->d_ccol_ptr,
+                                   reinterpret_cast<const int2*>(c->d_ccol), c->d_relptr, c->d_rel);
+            else
+                hipLaunchKernelGGL(k_fassemble_mem, dim3(L.count), dim3(64), 0, c->stream, t, A, c->Ax, c->d_cbase,
+                                   c->d_acol_ptr, reinterpret_cast<const int2*>(c->d_acol), c->d_ccol_ptr,
+                                   reinterpret_cast<const int2*>(c->d_ccol), c->d_relptr, c->d_rel);
             break;
         case wrapd::FK_POTRF:
             hipLaunchKernelGGL(k_fpotrf, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A, L.panel, c->d_ffail);

@@ -1088,7 +1088,8 @@ int solve_aborted(wrapd_ctx* c) {
     if (!c->d_flow_abort) return 0;
     if (hipMemcpyAsync(&v, c->d_flow_abort, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
-        return 1;
+        v = 1;
+    if (v) c->dead = true;   // the solves' rows are in no defined state after an expired wait
     return v;
 }
 
