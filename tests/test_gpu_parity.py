@@ -45,6 +45,58 @@ def test_setup_matches_oracle():
     assert _rel(d.elem("wsq"), o.elem("wsq")) < 1e-13
 
 
+def _pattern_numpy(tets, pins, n_v):
+    """A's upper-triangle pattern over the free vertices by brute force: every corner pair
+    of every element with both vertices free, as (row, col) with row <= col."""
+    free = np.ones(n_v, bool)
+    free[pins] = False
+    fidx = np.cumsum(free) - 1
+    pairs = set()
+    for t in tets:
+        f = [int(fidx[v]) for v in t if free[v]]
+        for a in f:
+            for b in f:
+                if a <= b:
+                    pairs.add((a, b))
+    rc = np.array(sorted(pairs), dtype=np.int64).reshape(-1, 2)
+    return rc[:, 0], rc[:, 1], int(free.sum())
+
+
+@pytest.mark.parametrize("case", ["end_pins", "scattered_pins", "isolated_vertex"])
+def test_mesh_maps_pattern(case):
+    """The device-built A pattern (row pointers, columns) against a brute-force pattern, on
+    pins at the ends, pins scattered through the mesh (some elements wholly pinned), and a
+    free vertex that no element uses (an empty row)."""
+    prob = meshgen.small_beam(9, 3, 3, seed=21)
+    rest, init, tets, pins = prob.rest, prob.init, prob.tets, prob.pins
+    if case != "end_pins":
+        rng = np.random.default_rng(4)
+        pins = np.unique(np.concatenate([pins, rng.choice(prob.n_verts, prob.n_verts // 5, replace=False)]))
+    if case == "isolated_vertex":
+        rest = np.vstack([rest[:7], rest[:1] + 5.0, rest[7:]])
+        init = np.vstack([init[:7], init[:1] + 5.0, init[7:]])
+        tets = np.where(tets >= 7, tets + 1, tets).astype(np.int32)
+        pins = np.where(pins >= 7, pins + 1, pins)
+    prob = meshgen.Problem(prob.name, rest, init, tets.astype(np.int32), np.sort(pins).astype(np.int32))
+    d = Device()
+    d.upload_mesh(prob)
+    rp, cols, _ = d.A_csr()
+    r_ref, c_ref, n_f = _pattern_numpy(prob.tets, prob.pins, prob.n_verts)
+    assert len(rp) == n_f + 1 and rp[0] == 0 and np.all(np.diff(rp) >= 0)
+    rows = np.repeat(np.arange(n_f), np.diff(rp))
+    assert np.array_equal(rows, r_ref) and np.array_equal(cols, c_ref)
+    if case == "isolated_vertex":
+        f7 = int(np.count_nonzero(~np.isin(np.arange(7), prob.pins)))   # vertex 7's free index
+        assert rp[f7] == rp[f7 + 1]   # vertex 7 is in no element: its row is empty
+    if case == "scattered_pins":
+        # the vertex -> corner gathers: the gradient with pins spread through the mesh
+        o, d2 = _matched_state(prob)
+        vo, go = o.global_grad(o.x_free())
+        vd, gd = d2.value_grad()
+        assert abs(vd - vo) <= 1e-11 * abs(vo)
+        assert _rel(gd, go) < 1e-10
+
+
 def test_assembled_A_and_solve3():
     prob = meshgen.small_beam(10, 3, 2, seed=12)
     d = _device_for(prob)

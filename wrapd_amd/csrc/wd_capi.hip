@@ -332,94 +332,34 @@ int wrapd_upload_mesh(wrapd_ctx* c, int n_v, const double* rest, int n_e, const 
     c->n_fix = j;
     if (c->n_f == 0) { c->last_error = "no free vertices"; return WRAPD_ERR_ARG; }
     const int n_ep = c->n_ep, n_f = c->n_f;
-    // element -> combined position index
-    std::vector<int> vidx(4ll * n_ep, 0);
-    for (int e = 0; e < n_e; ++e)
-        for (int q = 0; q < 4; ++q) {
-            const int v = tets[4 * e + q];
-            vidx[4 * e + q] = c->free_idx[v] >= 0 ? c->free_idx[v] : n_f + c->fix_idx[v];
-        }
-    // A pattern (upper triangle over free vertices) and its per-entry contributions
-    std::vector<long long> keys;
-    keys.reserve(10ll * n_e);
-    for (int e = 0; e < n_e; ++e)
-        for (int a = 0; a < 4; ++a) {
-            const int fa = c->free_idx[tets[4 * e + a]];
-            if (fa < 0) continue;
-            for (int b = 0; b < 4; ++b) {
-                const int fb = c->free_idx[tets[4 * e + b]];
-                if (fb >= 0 && fa <= fb) keys.push_back(static_cast<long long>(fa) * n_f + fb);
-            }
-        }
-    std::vector<long long> ukeys = keys;
-    std::sort(ukeys.begin(), ukeys.end());
-    ukeys.erase(std::unique(ukeys.begin(), ukeys.end()), ukeys.end());
-    c->nnzA = static_cast<int>(ukeys.size());
-    c->Ap.assign(n_f + 1, 0);
-    c->Aj.resize(c->nnzA);
-    for (int p = 0; p < c->nnzA; ++p) {
-        c->Ap[ukeys[p] / n_f + 1]++;
-        c->Aj[p] = static_cast<int>(ukeys[p] % n_f);
-    }
-    for (int i = 0; i < n_f; ++i) c->Ap[i + 1] += c->Ap[i];
-    std::vector<int> ccnt(c->nnzA + 1, 0), cidx;
-    std::vector<int> slot_of;   // per (e, a, b) in key order
-    slot_of.reserve(keys.size());
-    {
-        for (int e = 0; e < n_e; ++e)
-            for (int a = 0; a < 4; ++a) {
-                const int fa = c->free_idx[tets[4 * e + a]];
-                if (fa < 0) continue;
-                for (int b = 0; b < 4; ++b) {
-                    const int fb = c->free_idx[tets[4 * e + b]];
-                    if (!(fb >= 0 && fa <= fb)) continue;
-                    const int* row = c->Aj.data() + c->Ap[fa];
-                    const int* rend = c->Aj.data() + c->Ap[fa + 1];
-                    const int p = c->Ap[fa] + static_cast<int>(std::lower_bound(row, rend, fb) - row);
-                    slot_of.push_back(p);
-                    ccnt[p + 1]++;
-                }
-            }
-    }
-    for (int p = 0; p < c->nnzA; ++p) ccnt[p + 1] += ccnt[p];
-    cidx.assign(ccnt[c->nnzA], 0);
-    {
-        std::vector<int> fill(ccnt.begin(), ccnt.end() - 1);
-        size_t t = 0;
-        for (int e = 0; e < n_e; ++e)
-            for (int a = 0; a < 4; ++a) {
-                const int fa = c->free_idx[tets[4 * e + a]];
-                if (fa < 0) continue;
-                for (int b = 0; b < 4; ++b) {
-                    const int fb = c->free_idx[tets[4 * e + b]];
-                    if (!(fb >= 0 && fa <= fb)) continue;
-                    cidx[fill[slot_of[t++]]++] = e * 16 + a * 4 + b;
-                }
-            }
-    }
-    // vertex -> element-corner gather lists (element order); the gradient staging is laid out
-    // in this order, so a vertex's corners are one contiguous range
-    std::vector<int> vptr(n_f + 1, 0), vlist;
-    for (int e = 0; e < n_e; ++e)
-        for (int q = 0; q < 4; ++q) {
-            const int f = c->free_idx[tets[4 * e + q]];
-            if (f >= 0) vptr[f + 1]++;
-        }
-    for (int i = 0; i < n_f; ++i) vptr[i + 1] += vptr[i];
-    vlist.assign(vptr[n_f], 0);
-    {
-        std::vector<int> fill(vptr.begin(), vptr.end() - 1);
-        for (int e = 0; e < n_e; ++e)
-            for (int q = 0; q < 4; ++q) {
-                const int f = c->free_idx[tets[4 * e + q]];
-                if (f >= 0) vlist[fill[f]++] = 4 * e + q;
-            }
-    }
-    std::vector<int> cpos(4ull * n_ep, -1);   // element corner -> its row of the staging
-    for (size_t q = 0; q < vlist.size(); ++q) cpos[vlist[q]] = static_cast<int>(q);
-    // device allocations
+    // the tets (padding rows: vertex 0) and each vertex's combined position index go to the
+    // device, which builds vidx, A's pattern and contribution lists and the vertex -> corner
+    // lists (build_mesh_maps)
+    int* d_tv = nullptr;
+    int* d_comb = nullptr;
+    double* d_rest = nullptr;
+    int* d_bad = nullptr;
+    auto free_tmp = [&] {
+        (void)hipFree(d_tv);
+        (void)hipFree(d_comb);
+        (void)hipFree(d_rest);
+        (void)hipFree(d_bad);
+    };
+    std::vector<int> comb(n_v);
+    for (int i = 0; i < n_v; ++i) comb[i] = c->free_idx[i] >= 0 ? c->free_idx[i] : n_f + c->fix_idx[i];
     int rc = 0;
-    rc |= dalloc(c, c->vidx, 4ull * n_ep);
+    rc |= dalloc(c, d_tv, 4ull * n_ep);
+    rc |= dalloc(c, d_comb, n_v);
+    rc |= dalloc(c, d_rest, 3ull * n_v);
+    rc |= dalloc(c, d_bad, 1);
+    if (rc) { free_tmp(); return WRAPD_ERR_HIP; }
+    rc |= check_hip(c, hipMemsetAsync(d_tv + 4ll * n_e, 0, 4ull * (n_ep - n_e) * sizeof(int), c->stream), "memset");
+    rc |= h2d(c, d_tv, tets, 4ull * n_e);
+    rc |= h2d(c, d_comb, comb.data(), comb.size());
+    rc |= h2d(c, d_rest, rest, 3ull * n_v);
+    long long n_corners = 0;
+    if (rc || build_mesh_maps(c, d_tv, d_comb, &n_corners) != WRAPD_OK) { free_tmp(); return WRAPD_ERR_HIP; }
+    // device allocations
     rc |= dalloc(c, c->Di, 12ull * n_ep);
     rc |= dalloc(c, c->vol, n_ep);
     rc |= dalloc(c, c->kst, n_ep);
@@ -429,7 +369,7 @@ int wrapd_upload_mesh(wrapd_ctx* c, int n_v, const double* rest, int n_e, const 
     rc |= dalloc(c, c->P, 9ull * n_ep);
     rc |= dalloc(c, c->symF, 9ull * n_ep);
     rc |= dalloc(c, c->sig, 3ull * n_ep);
-    rc |= dalloc(c, c->gstage, 3ull * vlist.size());
+    rc |= dalloc(c, c->gstage, 3ull * n_corners);
     rc |= dalloc(c, c->xall, 3ull * (n_f + c->n_fix));
     rc |= dalloc(c, c->grad, 3ull * n_f);
     rc |= dalloc(c, c->grad_old, 3ull * n_f);
@@ -437,44 +377,23 @@ int wrapd_upload_mesh(wrapd_ctx* c, int n_v, const double* rest, int n_e, const 
     rc |= dalloc(c, c->rv, 3ull * n_f);
     rc |= dalloc(c, c->bv, 3ull * n_f);
     for (int i = 0; i < kMaxHist; ++i) { rc |= dalloc(c, c->hS[i], 3ull * n_f); rc |= dalloc(c, c->hY[i], 3ull * n_f); }
-    rc |= dalloc(c, c->v2e_ptr, vptr.size());
-    rc |= dalloc(c, c->cpos, cpos.size());
-    rc |= dalloc(c, c->a_cptr, ccnt.size());
-    rc |= dalloc(c, c->a_cidx, cidx.size());
     rc |= dalloc(c, c->Ax, c->nnzA);
     if (mu_e) { rc |= dalloc(c, c->mu_e, n_ep); } else if (c->mu_e) { (void)hipFree(c->mu_e); c->mu_e = nullptr; }
     if (lambda_e) { rc |= dalloc(c, c->lam_e, n_ep); } else if (c->lam_e) { (void)hipFree(c->lam_e); c->lam_e = nullptr; }
-    if (rc) return WRAPD_ERR_HIP;
-    rc |= h2d(c, c->vidx, vidx.data(), vidx.size());
-    rc |= h2d(c, c->v2e_ptr, vptr.data(), vptr.size());
-    rc |= h2d(c, c->cpos, cpos.data(), cpos.size());
-    rc |= h2d(c, c->a_cptr, ccnt.data(), ccnt.size());
-    rc |= h2d(c, c->a_cidx, cidx.data(), cidx.size());
+    if (rc) { free_tmp(); return WRAPD_ERR_HIP; }
     std::vector<double> me, le;
     if (mu_e) { me.assign(n_ep, mat->mu); std::copy(mu_e, mu_e + n_e, me.begin()); rc |= h2d(c, c->mu_e, me.data(), n_ep); }
     if (lambda_e) { le.assign(n_ep, mat->lambda); std::copy(lambda_e, lambda_e + n_e, le.begin()); rc |= h2d(c, c->lam_e, le.data(), n_ep); }
     c->hetero = mu_e || lambda_e;
     // setup kernel on the rest positions
-    int* d_tv = nullptr;
-    double* d_rest = nullptr;
-    int* d_bad = nullptr;
-    std::vector<int> tvp(4ll * n_ep, 0);
-    std::copy(tets, tets + 4ll * n_e, tvp.begin());
-    rc |= dalloc(c, d_tv, tvp.size());
-    rc |= dalloc(c, d_rest, 3ull * n_v);
-    rc |= dalloc(c, d_bad, 1);
-    if (rc) return WRAPD_ERR_HIP;
-    rc |= h2d(c, d_tv, tvp.data(), tvp.size());
-    rc |= h2d(c, d_rest, rest, 3ull * n_v);
     int zero = 0;
     rc |= h2d(c, d_bad, &zero, 1);
+    if (rc) { free_tmp(); return WRAPD_ERR_HIP; }
     launch_setup(c, d_tv, d_rest, d_bad);
     int bad = 0;
     rc |= launch_status(c);
     rc |= d2h(c, &bad, d_bad, 1);
-    (void)hipFree(d_tv);
-    (void)hipFree(d_rest);
-    (void)hipFree(d_bad);
+    free_tmp();
     if (rc) return WRAPD_ERR_HIP;
     // zero the dual state so that reads before init_duals are defined
     launch_fill(c, c->U, 0.0, 9ll * n_ep);

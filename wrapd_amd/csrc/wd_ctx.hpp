@@ -219,6 +219,8 @@ struct ProfScope {
 
 inline int blocks_for(long long n, int nt) { return static_cast<int>((n + nt - 1) / nt); }
 
+// (wd_setup.hip) vidx, A's pattern and contribution lists, the vertex -> corner lists
+int build_mesh_maps(wrapd_ctx* c, const int* d_tets, const int* d_comb, long long* n_corners);
 // launchers (wd_elem.hip)
 void launch_setup(wrapd_ctx* c, const int* d_tv, const double* d_rest, int* d_bad);
 void launch_init_duals(wrapd_ctx* c, int rot_aware);
