@@ -1,7 +1,7 @@
 """Writes the golden vectors under tests/golden/ from the reference build (oracle/_ref).
 
 Run in a container where the reference tree is present, after `make -C oracle ref`:
-    OMP_NUM_THREADS=1 python tests/golden/make_golden.py [all|base|samples|c3]
+    OMP_NUM_THREADS=1 python tests/golden/make_golden.py [all|base|samples|samples_pert|c3|voronoi]
 The reference runs single-threaded (its OpenMP reductions are order dependent).  Only
 inputs and outputs are stored; the mesh inputs of the sample case come from the
 reference's own data files (samples/data/*).
@@ -152,12 +152,18 @@ def sample_inputs(case):
     return rest, init, soln, tets.astype(np.int32), pins.astype(np.int32)
 
 
-def run_soln(case, threads):
+def run_soln(case, threads, init=None):
     """The reference with its defaults and -soln: stops at the first x-error below 1e-4
-    (src/ADMM.cpp:164, src/AlgorithmData.cpp:70)."""
+    (src/ADMM.cpp:164, src/AlgorithmData.cpp:70).  With `init` given, those initial
+    positions (on the case's tets) replace the case's own."""
     d = os.path.join(REF_TREE, "samples", "data", case)
     with tempfile.TemporaryDirectory() as out:
-        cmd = [REF, "solve", "-rest", os.path.join(d, f"{case}_rest"), "-init", os.path.join(d, f"{case}_init"),
+        init_path = os.path.join(d, f"{case}_init")
+        if init is not None:
+            _, tets = meshgen.load_elenode(init_path)
+            init_path = os.path.join(out, "init_perturbed")
+            meshgen.save_elenode(init_path, init, tets)
+        cmd = [REF, "solve", "-rest", os.path.join(d, f"{case}_rest"), "-init", init_path,
                "-soln", os.path.join(d, f"{case}_nh_soln"), "-handles", os.path.join(d, f"{case}_handles.txt"),
                "-p", "0", "-s", "0", "-out", out]
         subprocess.run(cmd, env=dict(ENV, OMP_NUM_THREADS=str(threads)), check=True, capture_output=True)
@@ -186,6 +192,28 @@ def samples():
                             spread_e=spread_e, spread_x=spread_x)
         print(case, "iters", len(t1["objectives"]) - 1, "(8 threads:", len(t8["objectives"]) - 1, ")",
               "spread_e %.2e spread_x %.2e" % (spread_e, spread_x), flush=True)
+
+
+def samples_pert():
+    """Adds to each sample fixture the reference's sensitivity to round-off: its run from
+    initial positions moved by ~1e-15 relative (exit iteration, energy and x-error
+    deviations, final-position deviation)."""
+    rng = np.random.default_rng(7)
+    for case in SAMPLES:
+        path = os.path.join(HERE, f"sample_{case}.npz")
+        z = dict(np.load(path))
+        if "pert_iters" in z and "--redo" not in sys.argv:
+            continue
+        init = z["init"] * (1.0 + 1e-15 * rng.uniform(-1.0, 1.0, z["init"].shape))
+        tp = run_soln(case, 1, init=init)
+        n = min(len(z["objectives"]), len(tp["objectives"]))
+        z["pert_iters"] = len(tp["objectives"]) - 1
+        z["pert_e"] = float(np.max(np.abs(z["objectives"][:n] - tp["objectives"][:n])) / abs(z["objectives"][0]))
+        z["pert_xe"] = float(np.max(np.abs(z["x_errors"][:n] - tp["x_errors"][:n])))
+        z["pert_x"] = float(np.linalg.norm(z["x"] - tp["x"]) / np.linalg.norm(z["x"]))
+        np.savez_compressed(path, **z)
+        print(case, "perturbed: iters", z["pert_iters"], "(unperturbed", int(z["iters"]), ")",
+              "pert_e %.2e pert_xe %.2e pert_x %.2e" % (z["pert_e"], z["pert_xe"], z["pert_x"]), flush=True)
 
 
 def c3():
@@ -267,6 +295,8 @@ if __name__ == "__main__":
         main()
     if what in ("all", "samples"):
         samples()
+    if what in ("all", "samples", "samples_pert"):
+        samples_pert()
     if what in ("all", "c3"):
         c3()
     if what in ("all", "voronoi"):

@@ -1,0 +1,103 @@
+"""`quasi` on the reference's sample data (samples/data/<case>) with -soln, against the
+reference's own runs of the same command (tests/golden/sample_<case>.npz, written by
+tests/golden/make_golden.py samples).
+
+-soln makes the run stop at the first iterate whose normalized x-error
+|x - x*| / |x0 - x*| over the free vertices falls below 1e-4 (src/ADMM.cpp:164,
+src/AlgorithmData.cpp:63-70), x* being the case's shipped neo-Hookean solution.
+
+Several of these runs are chaotic: the global step's stopping tolerance is the last local
+step's change of the augmented Lagrangian (src/GlobalStepBehavior.hpp:254-258), a
+difference of large sums that round-off moves by percents within a few iterations once
+elements invert (on cat it differs by 7% between this solver and the serial CPU oracle at
+iteration 20, with the energies still equal to 1e-10), and from the first iteration whose
+inner-iteration count or reweighting decision differs, two runs follow different branches.
+The reference shows the same: its single-thread, 8-thread and perturbed runs of cat end
+at iterations 254, 250 and 253, the oracle's at 414.  So the bars, from the reference's own
+round-off behaviour on the same case (s_E, s_x: the larger of its single- vs 8-thread
+deviation and its deviation under a 1e-15 relative perturbation of the initial positions):
+  * the run's logs through quasi and through the C ABI's solver agree (one code path);
+  * the exit criterion: the last x-error below 1e-4, every earlier one above, and the final
+    positions within 1e-4 |x0 - x*| of x* (the reference's too), the pins where they started;
+  * up to t*, the first iteration whose global-step inner iterations or reweighting decision
+    differ from the reference's (the whole run if none): energies within max(5e-9, 4 s_E) E_0
+    and x-errors within max(1e-8, 8 s_x) |x| / |x0 - x*| or 4 times the perturbed run's own
+    x-error deviation;
+  * without such a divergence, the exit iteration within the range of the reference's three
+    runs, widened by two iterations or 2%; after one, within twice the reference's.
+"""
+import glob
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from wrapd_amd import _lib, meshgen
+from wrapd_amd.solver import Solver
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.join(os.path.dirname(__file__), "golden")
+CASES = sorted(os.path.basename(p)[len("sample_"):-len(".npz")] for p in glob.glob(os.path.join(HERE, "sample_*.npz")))
+
+
+def _free(n_v, pins):
+    m = np.ones(n_v, bool)
+    m[pins] = False
+    return m
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_quasi_sample_to_solution(case, tmp_path):
+    z = np.load(os.path.join(HERE, f"sample_{case}.npz"))
+    rest, init, soln, tets, pins = z["rest"], z["init"], z["soln"], z["tets"], z["pins"]
+    meshgen.save_elenode(str(tmp_path / "rest"), rest, tets)
+    meshgen.save_elenode(str(tmp_path / "init"), init, tets)
+    meshgen.save_elenode(str(tmp_path / "soln"), soln, tets)
+    meshgen.save_handles(str(tmp_path / "handles.txt"), pins)
+    out = tmp_path / "out"
+    print(f"{case}: {tets.shape[0]} tets, {rest.shape[0]} vertices", flush=True)
+    r = subprocess.run([_lib.QUASI_PATH, "-rest", str(tmp_path / "rest.node"), "-init", str(tmp_path / "init.node"),
+                        "-soln", str(tmp_path / "soln.node"), "-handles", str(tmp_path / "handles.txt"),
+                        "-p", "0", "-s", "1", "-out", str(out)], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr
+    obj = np.loadtxt(out / "objectives.txt")[:, 1]
+    xe = np.loadtxt(out / "x_errors.txt")[:, 1]
+    n = len(obj) - 1
+    # the same run through the C ABI, for its per-iteration decisions
+    s = Solver(meshgen.Problem(case, rest, init, tets, pins), soln=soln)
+    assert s.run() == n
+    assert np.allclose(s.log("objectives"), obj, rtol=1e-13, atol=0.0)
+    inner, rw = s.log("inner_iters").astype(int), s.log("reweighted").astype(int)
+
+    iters = [int(z["iters"]), int(z["iters_8threads"]), int(z["pert_iters"])]
+    s_e = max(float(z["spread_e"]), float(z["pert_e"]))
+    s_x = max(float(z["spread_x"]), float(z["pert_x"]))
+    free = _free(rest.shape[0], pins)
+    d0 = np.linalg.norm(init[free] - soln[free])
+    e_ref, xe_ref = z["objectives"], z["x_errors"]
+    k = min(len(obj), len(e_ref))
+    flips = [t for t in range(min(len(inner), len(z["inner_iters"])))
+             if inner[t] != z["inner_iters"][t] or rw[t] != z["reweighted"][t]]
+    t_star = flips[0] if flips else k - 1
+    de = np.max(np.abs(obj[:t_star + 1] - e_ref[:t_star + 1])) / abs(e_ref[0])
+    dxe = np.max(np.abs(xe[:t_star + 1] - xe_ref[:t_star + 1]))
+    bar_e = max(5e-9, 4.0 * s_e)
+    bar_xe = max(max(1e-8, 8.0 * s_x) * np.linalg.norm(z["x"]) / d0, 4.0 * float(z["pert_xe"]))
+    slack = max(2, int(np.ceil(0.02 * iters[0])))
+    print(f"{case}: exit {n} (reference runs {iters}); first divergent decision "
+          f"{flips[0] if flips else None}; up to it max|dE|/E0 {de:.2e} (bar {bar_e:.1e}), "
+          f"max|d xerr| {dxe:.2e} (bar {bar_xe:.1e})")
+    assert de <= bar_e
+    assert dxe <= bar_xe
+    if flips:
+        assert n <= 2 * max(iters)
+    else:
+        assert min(iters) - slack <= n <= max(iters) + slack
+    assert xe[-1] < 1e-4 and np.all(xe[:-1] >= 1e-4)
+    X, T = meshgen.load_elenode(str(out / "final_mesh"))
+    assert np.array_equal(T, tets)
+    assert np.linalg.norm(X[free] - soln[free]) < 1e-4 * d0
+    assert np.linalg.norm(z["x"][free] - soln[free]) < 1e-4 * d0
+    assert np.array_equal(X[~free], init[~free])

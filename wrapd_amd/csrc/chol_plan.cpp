@@ -11,6 +11,59 @@ namespace wrapd {
 
 namespace {
 
+// The FINAL launch's task order.  Workgroups are dispatched in order, round-robin over the
+// 8 XCDs (block b to XCD b % 8), and each XCD has its own L2.  The super-tiles are cut into
+// units of up to 8 x 8 (row tile u, column tile v or c) of one front, which share 8 row
+// and 8 column operand blocks; units go round-robin to 8 queues, longest first, and the
+// queues are interleaved, so that one XCD runs one unit's tasks together (their operands
+// then come from its L2) while every round of 8 consecutive blocks holds tasks of about
+// the same length (an XCD whose turn comes while it is full stalls the dispatch of all).
+template <typename Push>
+void final_order(const std::vector<int>& L, const std::vector<int>& ns, const std::vector<int>& f, Push push) {
+    constexpr int kUnit = 8;
+    constexpr int kXcd = 8;
+    struct Unit {
+        int cost;                            // k range of its tasks (the longest)
+        std::vector<std::pair<int, int>> t;  // (s, y)
+    };
+    std::vector<Unit> units;
+    for (int s : L) {
+        const int U = final_row_tiles(ns[s], f[s]);
+        const int C = final_col_tiles(ns[s]);
+        for (int ub = 0; ub < U; ub += kUnit)
+            for (int vb = 0; vb <= ub; vb += kUnit) {   // SCHUR (u, v), v <= u
+                Unit un{ns[s], {}};
+                for (int u = ub; u < std::min(U, ub + kUnit); ++u)
+                    for (int v = vb; v < std::min(u + 1, vb + kUnit); ++v) un.t.push_back({s, (u << kTaskTileShift) | v});
+                units.push_back(std::move(un));
+            }
+        for (int ub = 0; ub < U; ub += kUnit)
+            for (int cb = 0; cb < C; cb += kUnit) {     // GBOT (u, c): k range ns - 128 c
+                Unit un{ns[s] - kFinalTile * cb, {}};
+                for (int c = cb; c < std::min(C, cb + kUnit); ++c)
+                    for (int u = ub; u < std::min(U, ub + kUnit); ++u)
+                        un.t.push_back({s, (1 << kTaskKindShift) | (u << kTaskTileShift) | c});
+                units.push_back(std::move(un));
+            }
+    }
+    std::stable_sort(units.begin(), units.end(), [](const Unit& a, const Unit& b) { return a.cost > b.cost; });
+    std::vector<std::vector<std::pair<int, int>>> q(kXcd);
+    for (size_t j = 0; j < units.size(); ++j) {
+        std::vector<std::pair<int, int>>& dst = q[j % kXcd];
+        dst.insert(dst.end(), units[j].t.begin(), units[j].t.end());
+    }
+    std::vector<size_t> at(kXcd, 0);
+    for (bool any = true; any;) {
+        any = false;
+        for (int x = 0; x < kXcd; ++x)
+            if (at[x] < q[x].size()) {
+                push(q[x][at[x]].first, q[x][at[x]].second);
+                ++at[x];
+                any = true;
+            }
+    }
+}
+
 // first-fit allocator over a growing pool, in doubles, with coalescing of free blocks
 struct PoolAlloc {
     std::map<int64_t, int64_t> free_blocks;   // offset -> size
@@ -207,16 +260,11 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P) {
             const int C = final_col_tiles(ns[s]);
             const double k32 = 32.0 * ((ns[s] + 31) / 32);   // the kernel's k-chunks are 32 deep
             for (int u = 0; u < U; ++u)
-                for (int v = 0; v <= u; ++v) {
-                    push(s, (u << kTaskTileShift) | v);
-                    P.flops += 2 * T2 * T2 * k32;
-                }
+                for (int v = 0; v <= u; ++v) P.flops += 2 * T2 * T2 * k32;
             for (int u = 0; u < U; ++u)
-                for (int c = 0; c < C; ++c) {
-                    push(s, (1 << kTaskKindShift) | (u << kTaskTileShift) | c);
-                    P.flops += 2 * T2 * T2 * (k32 - T2 * c);
-                }
+                for (int c = 0; c < C; ++c) P.flops += 2 * T2 * T2 * (k32 - T2 * c);
         }
+        final_order(L, ns, f, [&](int s, int y) { push(s, y); });
     }
     // drop empty launches (a level whose panel list is empty for a kind)
     std::vector<FactorLaunch> kept;
