@@ -11,57 +11,28 @@ namespace wrapd {
 
 namespace {
 
-// The FINAL launch's task order.  Workgroups are dispatched in order, round-robin over the
-// 8 XCDs (block b to XCD b % 8), and each XCD has its own L2.  The super-tiles are cut into
-// units of up to 8 x 8 (row tile u, column tile v or c) of one front, which share 8 row
-// and 8 column operand blocks; units go round-robin to 8 queues, longest first, and the
-// queues are interleaved, so that one XCD runs one unit's tasks together (their operands
-// then come from its L2) while every round of 8 consecutive blocks holds tasks of about
-// the same length (an XCD whose turn comes while it is full stalls the dispatch of all).
+// The FINAL launch's task order: longest first (k range ns for SCHUR, ns - 128 c for
+// GBOT), so that the launch's tail is made of short tasks.  (Grouping tasks that share
+// operand blocks onto one XCD's L2 measured slower on the C5 beam: the kernel is not
+// bound by L2 misses, and the groups unbalance the in-order round-robin dispatch.)
 template <typename Push>
 void final_order(const std::vector<int>& L, const std::vector<int>& ns, const std::vector<int>& f, Push push) {
-    constexpr int kUnit = 8;
-    constexpr int kXcd = 8;
-    struct Unit {
-        int cost;                            // k range of its tasks (the longest)
-        std::vector<std::pair<int, int>> t;  // (s, y)
-    };
-    std::vector<Unit> units;
+    std::vector<std::pair<int, int>> tk;   // (s, y)
     for (int s : L) {
         const int U = final_row_tiles(ns[s], f[s]);
         const int C = final_col_tiles(ns[s]);
-        for (int ub = 0; ub < U; ub += kUnit)
-            for (int vb = 0; vb <= ub; vb += kUnit) {   // SCHUR (u, v), v <= u
-                Unit un{ns[s], {}};
-                for (int u = ub; u < std::min(U, ub + kUnit); ++u)
-                    for (int v = vb; v < std::min(u + 1, vb + kUnit); ++v) un.t.push_back({s, (u << kTaskTileShift) | v});
-                units.push_back(std::move(un));
-            }
-        for (int ub = 0; ub < U; ub += kUnit)
-            for (int cb = 0; cb < C; cb += kUnit) {     // GBOT (u, c): k range ns - 128 c
-                Unit un{ns[s] - kFinalTile * cb, {}};
-                for (int c = cb; c < std::min(C, cb + kUnit); ++c)
-                    for (int u = ub; u < std::min(U, ub + kUnit); ++u)
-                        un.t.push_back({s, (1 << kTaskKindShift) | (u << kTaskTileShift) | c});
-                units.push_back(std::move(un));
-            }
+        for (int u = 0; u < U; ++u)
+            for (int v = 0; v <= u; ++v) tk.push_back({s, (u << kTaskTileShift) | v});
+        for (int u = 0; u < U; ++u)
+            for (int c = 0; c < C; ++c) tk.push_back({s, (1 << kTaskKindShift) | (u << kTaskTileShift) | c});
     }
-    std::stable_sort(units.begin(), units.end(), [](const Unit& a, const Unit& b) { return a.cost > b.cost; });
-    std::vector<std::vector<std::pair<int, int>>> q(kXcd);
-    for (size_t j = 0; j < units.size(); ++j) {
-        std::vector<std::pair<int, int>>& dst = q[j % kXcd];
-        dst.insert(dst.end(), units[j].t.begin(), units[j].t.end());
-    }
-    std::vector<size_t> at(kXcd, 0);
-    for (bool any = true; any;) {
-        any = false;
-        for (int x = 0; x < kXcd; ++x)
-            if (at[x] < q[x].size()) {
-                push(q[x][at[x]].first, q[x][at[x]].second);
-                ++at[x];
-                any = true;
-            }
-    }
+    auto cost = [&](const std::pair<int, int>& x) {
+        const int kind = x.second >> kTaskKindShift;
+        return kind == 0 ? ns[x.first] : ns[x.first] - kFinalTile * (x.second & kTaskTileMask);
+    };
+    std::stable_sort(tk.begin(), tk.end(),
+                     [&](const std::pair<int, int>& a, const std::pair<int, int>& b) { return cost(a) > cost(b); });
+    for (const auto& t : tk) push(t.first, t.second);
 }
 
 // first-fit allocator over a growing pool, in doubles, with coalescing of free blocks
