@@ -84,6 +84,32 @@ extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
             const int k = L.panel, c0 = kFactorTile * k, nb = std::min(kFactorTile, ns - c0);
             auto Fi = [&](int r, int c) { return fo + static_cast<int64_t>(c) * f + r; };
             auto Gi = [&](int r, int c) { return go + static_cast<int64_t>(c) * g_ld(f) + r; };
+            // Cholesky of panel kp's diagonal tile in place, and its inverse into G's tile (kp, kp)
+            auto potrf = [&](int kp) {
+                const int cp = kFactorTile * kp, nbp = std::min(kFactorTile, ns - cp);
+                for (int p = 0; p < nbp; ++p) {
+                    double d = rdF(Fi(cp + p, cp + p));
+                    if (!(d > 0)) { fail = 1; d = 1.0; }
+                    d = std::sqrt(d);
+                    wrF(Fi(cp + p, cp + p), d);
+                    for (int r = p + 1; r < nbp; ++r) wrF(Fi(cp + r, cp + p), rdF(Fi(cp + r, cp + p)) / d);
+                    for (int c = p + 1; c < nbp; ++c)
+                        for (int r = c; r < nbp; ++r)
+                            wrF(Fi(cp + r, cp + c), rdF(Fi(cp + r, cp + c)) - rdF(Fi(cp + r, cp + p)) * rdF(Fi(cp + c, cp + p)));
+                }
+                std::vector<double> X(kFactorTile * kFactorTile, 0.0);
+                for (int j = 0; j < nbp; ++j) X[j * 64 + j] = 1.0;
+                for (int p = 0; p < nbp; ++p) {
+                    const double d = rdF(Fi(cp + p, cp + p));
+                    for (int j = 0; j < nbp; ++j) X[j * 64 + p] /= d;
+                    for (int r = p + 1; r < nbp; ++r) {
+                        const double l = rdF(Fi(cp + r, cp + p));
+                        for (int j = 0; j < nbp; ++j) X[j * 64 + r] -= l * X[j * 64 + p];
+                    }
+                }
+                for (int j = 0; j < nbp; ++j)
+                    for (int r = 0; r < nbp; ++r) wrG(Gi(cp + r, cp + j), X[j * 64 + r]);
+            };
             if (L.kind == FK_ASSEMBLE) {
                 const int j = y;
                 const int64_t gc = PL.cbase[s] + j;
@@ -101,29 +127,7 @@ extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
                     }
                 }
             } else if (L.kind == FK_POTRF) {
-                for (int p = 0; p < nb; ++p) {
-                    double d = rdF(Fi(c0 + p, c0 + p));
-                    if (!(d > 0)) { fail = 1; d = 1.0; }
-                    d = std::sqrt(d);
-                    wrF(Fi(c0 + p, c0 + p), d);
-                    for (int r = p + 1; r < nb; ++r) wrF(Fi(c0 + r, c0 + p), rdF(Fi(c0 + r, c0 + p)) / d);
-                    for (int c = p + 1; c < nb; ++c)
-                        for (int r = c; r < nb; ++r)
-                            wrF(Fi(c0 + r, c0 + c), rdF(Fi(c0 + r, c0 + c)) - rdF(Fi(c0 + r, c0 + p)) * rdF(Fi(c0 + c, c0 + p)));
-                }
-                // the kernel also stores L_kk^{-1} in G's diagonal tile (k, k)
-                std::vector<double> X(kFactorTile * kFactorTile, 0.0);
-                for (int j = 0; j < nb; ++j) X[j * 64 + j] = 1.0;
-                for (int p = 0; p < nb; ++p) {
-                    const double d = rdF(Fi(c0 + p, c0 + p));
-                    for (int j = 0; j < nb; ++j) X[j * 64 + p] /= d;
-                    for (int r = p + 1; r < nb; ++r) {
-                        const double l = rdF(Fi(c0 + r, c0 + p));
-                        for (int j = 0; j < nb; ++j) X[j * 64 + r] -= l * X[j * 64 + p];
-                    }
-                }
-                for (int j = 0; j < nb; ++j)
-                    for (int r = 0; r < nb; ++r) wrG(Gi(c0 + r, c0 + j), X[j * 64 + r]);
+                potrf(k);
             } else if (L.kind == FK_TRSM) {
                 const int kind = y >> kTaskKindShift, t = y & ((1 << kTaskKindShift) - 1);
                 int t0, nt;
@@ -181,6 +185,7 @@ extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
                         if (kind == 0) wrF(Fi(a0 + i, b0 + j), rdF(Fi(a0 + i, b0 + j)) - acc);
                         else wrG(Gi(a0 + i, b0 + j), rdG(Gi(a0 + i, b0 + j)) - acc);
                     }
+                if (kind == 0 && a == b && a == k + 1) potrf(k + 1);   // the next panel's Cholesky
             }
         }
     }

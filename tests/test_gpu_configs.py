@@ -176,7 +176,11 @@ def test_split_reductions_match_in_launch(monkeypatch):
 def test_c3_fp64_and_mixed_against_reference_trace():
     """The pushed and twisted 1.47M-tet beam, 10 iterations: the plain FP64 path against the
     reference's own run (its L-BFGS memory 6; bars from its round-off spread), and the FP32
-    local step, whose FP64 fallback fires on the elements near inversion."""
+    local step, whose FP64 fallback fires on the elements near inversion.  The mixed run's
+    positions stay within 1e-6 of the reference's (FP32 rounding of the local step's Z,
+    ~6e-7 measured); its energies within 1e-3: these iterates are far from equilibrium (the
+    energy doubles between iterations 4 and 5), so the objective's gradient turns that
+    position difference into relative energy differences of up to ~7e-4."""
     z = load("trace_c3_it10.npz")
     prob = meshgen.config_problem(3)
     iters = len(z["objectives"]) - 1
@@ -199,7 +203,7 @@ def test_c3_fp64_and_mixed_against_reference_trace():
     print(f"mixed: FP64 fallback elements {rt['fallback_elems']:.0f}, x rel err vs reference {xerr:.2e}, "
           f"energy {eerr:.2e}", flush=True)
     assert rt["fallback_elems"] > 0
-    assert xerr < 1e-4 and eerr < 1e-4
+    assert xerr < 1e-6 and eerr < 1e-3
 
 
 # ---- the learned warm start on the 245,760-tet beam ----------------------------------------

@@ -197,9 +197,10 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P) {
         int pmax = 0;
         for (int s : L) pmax = std::max(pmax, tile_count_l11(ns[s]));
         for (int k = 0; k < pmax; ++k) {
-            begin(FK_POTRF, k);
-            for (int s : L)
-                if (k < tile_count_l11(ns[s])) push(s, 0);
+            if (k == 0) {   // later panels' diagonal tiles are factored by UPDATE(k - 1)'s task on them
+                begin(FK_POTRF, k);
+                for (int s : L) push(s, 0);
+            }
             begin(FK_TRSM, k);
             for (int s : L) {
                 if (k >= tile_count_l11(ns[s])) continue;
@@ -208,12 +209,20 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P) {
                 for (int c = 0; c <= k; ++c) push(s, (1 << kTaskKindShift) | c);
             }
             begin(FK_UPDATE, k);
+            // first the tasks on the next panels' diagonal tiles (k + 1, k + 1), which go on to
+            // factor them: dispatched first, their Cholesky overlaps the other updates
+            for (int s : L)
+                if (k + 1 < tile_count_l11(ns[s])) {
+                    push(s, ((k + 1) << kTaskTileShift) | (k + 1));
+                    P.flops += 2 * T * T * T;
+                }
             for (int s : L) {
                 const int p11 = tile_count_l11(ns[s]);
                 if (k >= p11) continue;
                 const int nt = tile_count(ns[s], f[s]);
                 for (int b = k + 1; b < p11; ++b)
                     for (int a = b; a < nt; ++a) {
+                        if (a == k + 1 && b == k + 1) continue;   // pushed above
                         push(s, (a << kTaskTileShift) | b);
                         P.flops += 2 * T * T * T;
                     }

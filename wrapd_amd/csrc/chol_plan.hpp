@@ -7,8 +7,10 @@
 //   1. assemble: one wave per front column zeroes the column, stores the A entries and
 //      adds the children's Schur-complement columns (children in index order, so every
 //      entry is summed in a fixed order, no atomics);
-//   2. per 64-column panel k of the supernode columns, three batched launches:
-//        POTRF         factor the panel's diagonal tile L_kk (and invert it)
+//   2. per 64-column panel k of the supernode columns, batched launches:
+//        POTRF         factor the panel's diagonal tile L_kk (and invert it); a launch of
+//                      its own for panel 0 only: the UPDATE task of panel k - 1 on the tile
+//                      (k, k) goes on to factor it
 //        TRSM + GDIAG  L[t, k] = F[t, k] L_kk^{-T} for row tiles t > k, and
 //                      X[k, c] = L_kk^{-1} X[k, c] for column tiles c <= k
 //        SYRK + GUPD   F[a, b] -= L[a, k] L[b, k]^T for tiles a >= b, k < b < P (the

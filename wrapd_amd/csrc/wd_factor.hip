@@ -13,7 +13,8 @@
 //   k_ftrsm      TRSM tiles F[t, k] <- F[t, k] L_kk^{-T}, or GDIAG tiles
 //                X[k, c] <- L_kk^{-1} X[k, c], both as products with the stored inverse
 //   k_fupdate    SYRK/GEMM updates of the supernode's own later columns, or GEMM
-//                updates of X = L11^{-1}
+//                updates of X = L11^{-1}; the task updating the next panel's diagonal
+//                tile then factors it (k_fpotrf's work), so only panel 0 has a POTRF launch
 //   k_ffinal     the Schur complement F[a, b] -= L21 L21^T and G's bottom -L21 L11^{-1},
 //                each 128x128 output super-tile accumulated over all ns columns in registers
 // Every product runs on the matrix cores in 64x64 output tiles (v_mfma_f64_16x16x4:
@@ -257,32 +258,14 @@ __device__ __forceinline__ dbl4 mma16(const double a[4], const double b[4], dbl4
     return acc;
 }
 
-__global__ void __launch_bounds__(kFactThreads)
-k_fpotrf(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ fail) {
-    __shared__ double D[kT * kT];
-    __shared__ double X[kT * kT];   // L^{-1}, column-major, lower blocks only
-    const int s = tasks[blockIdx.x].x;
-    const int f = A.fsize[s];
-    const int ns = A.col0[s + 1] - A.col0[s];
-    const int c0 = kT * k;
-    const int nb = min(kT, ns - c0);
+// The 64x64 tile D (LDS, column-major D[c * 64 + r], lower triangle; rows and columns from
+// nb on hold the identity) is factored in place, L^{-1} built in X (LDS, zeroed by the
+// caller); then L goes to the front's lower triangle at Fkk (leading dimension f) and L^{-1}
+// (zero above the diagonal) to Gkk (leading dimension lg).  All 256 threads take part; the
+// caller has synchronized after filling D and X.
+__device__ __forceinline__ void potrf_lds(double* D, double* X, int nb, double* __restrict__ Fkk, int64_t f,
+                          double* __restrict__ Gkk, int64_t lg, int* __restrict__ fail) {
     const int nblk = (nb + 15) >> 4;   // 16-blocks holding tile columns
-    double* F = A.pool + A.foff[s];
-    double* Fkk = F + static_cast<int64_t>(c0) * f + c0;
-    {
-        // the lower triangle; rows/columns beyond nb padded with the identity so every
-        // 16-block factors cleanly
-        double v[16];
-        tile_load(Fkk, f, nb, nb, v);
-        const int r = threadIdx.x & 63, cb = threadIdx.x >> 6;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int c = cb + 4 * q;
-            D[threadIdx.x + kFactThreads * q] = r < c ? 0.0 : (c < nb ? v[q] : (r == c ? 1.0 : 0.0));
-            X[threadIdx.x + kFactThreads * q] = 0.0;
-        }
-    }
-    __syncthreads();
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int li = l & 15, lk = l >> 4;
     int bad = 0;
@@ -300,18 +283,11 @@ k_fpotrf(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ fa
                     bad = 1;
                     d = 1.0;
                 }
-                // 1/sqrt(d) from the hardware estimate (error below 2^-22) and two Newton
-                // steps (below 2^-88 before rounding): about half the dependent chain of
-                // sqrt followed by a division, on the wave's serial path
-                double inv = __builtin_amdgcn_rsq(d);
-                {
-                    const double hd = 0.5 * d;
-                    double e = fma(-hd * inv, inv, 0.5);
-                    inv = fma(inv, e, inv);
-                    e = fma(-hd * inv, inv, 0.5);
-                    inv = fma(inv, e, inv);
-                }
-                const double sq = d * inv;
+                // correctly rounded: a hardware estimate with Newton steps saved 2.4 us per
+                // tile but changed L in the last bits, enough to move the sample runs past
+                // the reference's own round-off spread
+                const double sq = sqrt(d);
+                const double inv = 1.0 / sq;
                 rinv[p] = inv;
                 a[p] = li == p ? sq : (li > p ? a[p] * inv : a[p]);
 #pragma unroll
@@ -416,8 +392,6 @@ k_fpotrf(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ fa
     }
     // L_kk to the front's lower triangle, L_kk^{-1} (zero above the diagonal) to G's
     // diagonal tile (k, k), where the TRSM tasks read it and the solves use it
-    double* Gkk = A.G + A.goff[s] + static_cast<int64_t>(c0) * wrapd::g_ld(f) + c0;
-    const int64_t lg = wrapd::g_ld(f);
     {
         const int r = threadIdx.x & 63, cb = threadIdx.x >> 6;
         if (r < nb) {
@@ -431,6 +405,35 @@ k_fpotrf(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ fa
             }
         }
     }
+}
+
+__global__ void __launch_bounds__(kFactThreads)
+k_fpotrf(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ fail) {
+    __shared__ double D[kT * kT];
+    __shared__ double X[kT * kT];   // L^{-1}, column-major, lower blocks only
+    const int s = tasks[blockIdx.x].x;
+    const int f = A.fsize[s];
+    const int ns = A.col0[s + 1] - A.col0[s];
+    const int c0 = kT * k;
+    const int nb = min(kT, ns - c0);
+    double* F = A.pool + A.foff[s];
+    double* Fkk = F + static_cast<int64_t>(c0) * f + c0;
+    {
+        // the lower triangle; rows/columns beyond nb padded with the identity so every
+        // 16-block factors cleanly
+        double v[16];
+        tile_load(Fkk, f, nb, nb, v);
+        const int r = threadIdx.x & 63, cb = threadIdx.x >> 6;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int c = cb + 4 * q;
+            D[threadIdx.x + kFactThreads * q] = r < c ? 0.0 : (c < nb ? v[q] : (r == c ? 1.0 : 0.0));
+            X[threadIdx.x + kFactThreads * q] = 0.0;
+        }
+    }
+    __syncthreads();
+    const int64_t lg = wrapd::g_ld(f);
+    potrf_lds(D, X, nb, Fkk, f, A.G + A.goff[s] + static_cast<int64_t>(c0) * lg + c0, lg, fail);
 }
 
 // ---------------------------------------------------------------------------------
@@ -480,7 +483,7 @@ k_ftrsm(const int2* __restrict__ tasks, FactArgs A, int k) {
 // ---------------------------------------------------------------------------------
 // SYRK/GEMM (kind 0): F[a, b] -= L[a, k] L[b, k]^T;  GUPD (kind 1): X[i, c] -= L[i, k] X[k, c]
 __global__ void __launch_bounds__(kFactThreads)
-k_fupdate(const int2* __restrict__ tasks, FactArgs A, int k) {
+k_fupdate(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ fail) {
     __shared__ double As[kT * kT];   // As[kk * 64 + i]
     __shared__ double Bs[kT * kT];   // Bs[kk * 64 + j]
     const int2 task = tasks[blockIdx.x];
@@ -512,6 +515,40 @@ k_fupdate(const int2* __restrict__ tasks, FactArgs A, int k) {
     dbl4 acc[2][2];
     tile_zero(acc);
     tile_mma(As, Bs, (nb + 3) & ~3, acc);
+    if (kind == 0 && a == b && a == k + 1) {
+        // The next panel's diagonal tile: its update stays in registers and the tile is
+        // factored here (the POTRF of panel k + 1), overlapping the launch's other tasks.
+        // As and Bs become that factorization's D and X.
+        double* C = F + static_cast<int64_t>(b0) * f + a0;
+        const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+        const int j0 = 32 * (w >> 1), i0 = 32 * (w & 1);
+        double nv[16];
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int i = i0 + 16 * y + (l & 15), j = j0 + 16 * x + 4 * q + (l >> 4);
+                    nv[(x * 2 + y) * 4 + q] =
+                        C[static_cast<int64_t>(j < nbt ? j : 0) * f + (i < na ? i : 0)] - acc[x][y][q];
+                }
+        __syncthreads();   // every wave is done reading As and Bs
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int i = i0 + 16 * y + (l & 15), j = j0 + 16 * x + 4 * q + (l >> 4);
+                    As[j * 64 + i] = i < j ? 0.0 : ((i < nbt && j < nbt) ? nv[(x * 2 + y) * 4 + q] : (i == j ? 1.0 : 0.0));
+                }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Bs[threadIdx.x + kFactThreads * r] = 0.0;
+        __syncthreads();
+        potrf_lds(As, Bs, nbt, C, f, Gs + static_cast<int64_t>(b0) * lg + b0, lg, fail);
+        return;
+    }
     if (kind == 0) tile_write<true>(F + static_cast<int64_t>(b0) * f + a0, f, na, nbt, a == b, acc, 1.0);
     else tile_write<true>(Gs + static_cast<int64_t>(b0) * lg + a0, lg, na, nbt, false, acc, 1.0);
 }
@@ -709,7 +746,7 @@ void launch_factor(wrapd_ctx* c) {
             hipLaunchKernelGGL(k_ftrsm, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A, L.panel);
             break;
         case wrapd::FK_UPDATE:
-            hipLaunchKernelGGL(k_fupdate, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A, L.panel);
+            hipLaunchKernelGGL(k_fupdate, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A, L.panel, c->d_ffail);
             break;
         default:
             hipLaunchKernelGGL(k_ffinal, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A);
