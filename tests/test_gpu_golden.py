@@ -125,3 +125,45 @@ def test_soln_x_errors_and_early_exit():
     o.run(10)
     xo = o.log("x_errors")
     assert np.allclose(xe[:11], xo[:11], rtol=1e-7, atol=0.0)
+
+
+@pytest.mark.parametrize("prox", ["newton", "lbfgs"])
+def test_fp32_local_step_redoes_fallbacks_in_fp64(prox):
+    """An FP32 local step defers the elements whose FP32 prox result is unusable (near
+    inversion) to a second pass that redoes them in FP64, 64 per wave.  From one state (the
+    Voronoi beam after 60 iterations, elements inverted), an FP32 step and an FP64 step with
+    the fallback's prox settings: exactly the fallback elements come out bit for bit equal
+    to the FP64 step's (U, P, the prox solution, the candidate weight).  With the fallback
+    threshold above every singular value, every element is redone: all of them then equal
+    the FP64 step's, and the step's sums (now all the second pass's) equal the FP64 step's
+    up to summation order."""
+    prob = meshgen.small_voronoi_beam(64, 8, 8, regions=16, twist_deg=360.0, axial=-0.2)
+    s = Solver(prob, tol=0.0, precision=1, prox_mode=1, fp64_switch=0.0)
+    s.initialize()
+    s.setup()
+    for _ in range(60):
+        s.iterate()
+    d = s.device()
+    blob = d.state_save()
+    names = ("U", "P", "sigma_last", "cand_wsq")
+    kw = dict(pe_prev=1.0, prox=prox, newton_rtol=1e-12, update_cand=True, use_hessian=True)
+
+    def step(**over):
+        d.state_load(blob)
+        o = d.local_step(**dict(kw, **over))
+        return o, {k: d.elem(k).reshape(prob.n_elems, -1) for k in names}
+
+    o64, a64 = step(precision="fp64")
+    o32, a32 = step(precision="fp32")
+    same = np.all([np.all(a32[k] == a64[k], axis=1) for k in names], axis=0)
+    print(f"{prox}: {o32['fallback_count']:.0f} fallbacks, {same.sum()} elements equal to the FP64 step", flush=True)
+    assert o32["fallback_count"] > 0 and o64["fallback_count"] == 0
+    assert same.sum() == o32["fallback_count"]
+    assert o32["bef"] == o64["bef"]
+    assert o32["evals_total"] > o32["prox_iters_total"] > 0
+    oall, aall = step(precision="fp32", fallback_sigma=1e300)
+    assert oall["fallback_count"] == prob.n_elems
+    assert all(np.array_equal(aall[k], a64[k]) for k in names)
+    for k in ("aft", "pe", "prox_iters_total", "neg_sigma_count"):
+        assert oall[k] == pytest.approx(o64[k], rel=1e-12, abs=0.0), k
+    assert oall["max_wsq_ratio"] == o64["max_wsq_ratio"]

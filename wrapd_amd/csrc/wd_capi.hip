@@ -21,7 +21,7 @@ wrapd_ctx::~wrapd_ctx() {
     f(vidx); f(Di); f(vol); f(kst); f(mu_e); f(lam_e); f(wsq); f(cand); f(U); f(P); f(symF); f(sig); f(gstage);
     f(xall); f(grad); f(grad_old); f(qv); f(rv); f(bv); f(xref);
     for (int i = 0; i < wd::kMaxHist; ++i) { f(hS[i]); f(hY[i]); }
-    f(v2e_ptr); f(cpos); f(a_cptr); f(a_cidx); f(Ax);
+    f(v2e_ptr); f(cpos); f(a_cptr); f(a_cidx); f(Ax); f(fb_info); f(fb_list); f(fb_count); f(fb_tmp);
     f(G); f(ysol); f(xsol); f(ubuf); f(d_perm); f(d_iperm); f(d_col0); f(d_fsize); f(d_rowptr); f(d_rows);
     f(d_goff); f(d_uoff); f(d_asm_ptr); f(d_asm_idx); f(d_udest); f(d_leaf_ptr); f(d_leaf_sn); f(d_flow_f); f(d_flow_b);
     f(d_flow_abort); f(d_bcopy); f(bperm);
@@ -378,9 +378,12 @@ int wrapd_upload_mesh(wrapd_ctx* c, int n_v, const double* rest, int n_e, const 
     rc |= dalloc(c, c->bv, 3ull * n_f);
     for (int i = 0; i < kMaxHist; ++i) { rc |= dalloc(c, c->hS[i], 3ull * n_f); rc |= dalloc(c, c->hY[i], 3ull * n_f); }
     rc |= dalloc(c, c->Ax, c->nnzA);
+    rc |= dalloc(c, c->fb_info, n_ep);
+    rc |= dalloc(c, c->fb_list, n_ep);
+    rc |= dalloc(c, c->fb_count, 1);
     if (mu_e) { rc |= dalloc(c, c->mu_e, n_ep); } else if (c->mu_e) { (void)hipFree(c->mu_e); c->mu_e = nullptr; }
     if (lambda_e) { rc |= dalloc(c, c->lam_e, n_ep); } else if (c->lam_e) { (void)hipFree(c->lam_e); c->lam_e = nullptr; }
-    if (rc) { free_tmp(); return WRAPD_ERR_HIP; }
+    if (rc || prepare_local_fallback(c) != WRAPD_OK) { free_tmp(); return WRAPD_ERR_HIP; }
     std::vector<double> me, le;
     if (mu_e) { me.assign(n_ep, mat->mu); std::copy(mu_e, mu_e + n_e, me.begin()); rc |= h2d(c, c->mu_e, me.data(), n_ep); }
     if (lambda_e) { le.assign(n_ep, mat->lambda); std::copy(lambda_e, lambda_e + n_e, le.begin()); rc |= h2d(c, c->lam_e, le.data(), n_ep); }
@@ -543,8 +546,15 @@ int wrapd_local_step_result(wrapd_ctx* c, double pe_prev, wrapd_local_out* out) 
         return WRAPD_ERR_STATE;
     }
     c->local_pending = false;
-    const double* r = scal_read_epoch(c, S_LOCAL, c->local_epoch);
-    if (!r) return WRAPD_ERR_HIP;
+    const double* r0 = scal_read_epoch(c, S_LOCAL, c->local_epoch);
+    if (!r0) return WRAPD_ERR_HIP;
+    // an FP32 step's sums are those of its element kernel plus those of its FP64 pass over
+    // the deferred elements (S_LOCAL_FB, reduced after it: the epoch waited for is that one)
+    double r[11];
+    for (int k = 0; k < 11; ++k) {
+        const double fb = c->local_fb ? r0[S_LOCAL_FB - S_LOCAL + k] : 0.0;
+        r[k] = (k == 3 || k == 9) ? std::max(r0[k], fb) : r0[k] + fb;
+    }
     if (out) {
         out->bef = r[0];
         out->aft = r[1];

@@ -32,7 +32,8 @@ enum ScalarSlot {
     S_GRAM = 64,        // 64: s_i . y_j by slot (8 x 8)
     S_XD = 128,         // |x - x_ref|^2 of the last x-distance launch
     S_LOCAL = 136,      // 11: the last local step's sums and maxima
-    S_NUM = 152
+    S_LOCAL_FB = 152,   // 11: the same over its FP64 fallback elements (FP32 local steps)
+    S_NUM = 168
 };
 
 // defo-pass mode bits
@@ -176,6 +177,15 @@ struct wrapd_ctx {
     unsigned host_epoch = 0;           // the epoch of the last reduction with a host copy (what the host waits for)
     unsigned local_epoch = 0;          // the epoch of the last local step launched
     bool local_pending = false;        // a local step launched and its results not read yet
+    bool local_fb = false;             // ... and it had an FP64 fallback pass (S_LOCAL_FB)
+    // FP32 local steps defer their FP64 fallbacks: the element kernel marks them in fb_info
+    // (0: none, else 1 + the FP32 attempt's evaluations), a stable compaction lists them in
+    // fb_list (fb_count on the device) and a second kernel redoes them in FP64, 64 per wave
+    float* fb_info = nullptr;
+    int* fb_list = nullptr;
+    int* fb_count = nullptr;
+    void* fb_tmp = nullptr;
+    size_t fb_tmp_bytes = 0;
     bool gs_begun = false;             // the global step's first launches are queued (wrapd_global_step_begin)
     wd::RedTail pending_tail;          // armed by arm_tail, taken by the next reducing launch
     struct PendingFinalize {
@@ -225,6 +235,7 @@ int build_mesh_maps(wrapd_ctx* c, const int* d_tets, const int* d_comb, long lon
 void launch_setup(wrapd_ctx* c, const int* d_tv, const double* d_rest, int* d_bad);
 void launch_init_duals(wrapd_ctx* c, int rot_aware);
 void launch_defo(wrapd_ctx* c, int mode, int use_hessian);   // partial slots: 0 value, 1 energy, 2 flips
+int prepare_local_fallback(wrapd_ctx* c);   // the deferred-fallback list's compaction storage
 void launch_local(wrapd_ctx* c, const wrapd_local_args* a);  // slots: 0 bef,1 aft,2 pe,3 maxratio,4 iters,5 fallback,6 neg,7 accepted
 void launch_local_unaware(wrapd_ctx* c, const wrapd_local_args* a);
 void launch_update_weights(wrapd_ctx* c);

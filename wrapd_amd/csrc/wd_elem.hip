@@ -11,6 +11,8 @@
 //   k_assemble_A  update_A_coeffs (src/TetElements.cpp:498-510) as a per-entry gather
 #include <type_traits>
 
+#include <hipcub/hipcub.hpp>
+
 #include "wd_ctx.hpp"
 #include "device/wd_math.hpp"
 #include "device/wd_nh.hpp"
@@ -386,11 +388,12 @@ k_local(int n_e, int n_ep, const double* __restrict__ symF, double* __restrict__
         const double* __restrict__ wsq, const double* __restrict__ vol, const double* __restrict__ kst,
         double* __restrict__ cand, double* __restrict__ sig, MatArgs m, int prox_iters, double rtol,
         double fallback_sigma, int update_cand, int use_hessian, double* __restrict__ partials,
-        float* __restrict__ tlog, RedTail tail) {
+        float* __restrict__ tlog, float* __restrict__ fbinfo, RedTail tail) {
     __shared__ double shk[11][kLocalThreads / 64];   // the block reductions' wave results
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int e = lb * blockDim.x + threadIdx.x;
     double bef = 0.0, aft = 0.0, pe = 0.0, ratio = 0.0, iters = 0.0, fallback = 0.0, neg = 0.0, evals = 0.0;
+    bool defer = false;   // an FP32 result unusable here: redone in FP64 by k_local_fallback
     if (e < n_e) {
         M3<double> S, Uo, Po;
         ld9(symF, n_ep, e, S);
@@ -416,19 +419,16 @@ k_local(int n_e, int n_ep, const double* __restrict__ symF, double* __restrict__
             double smin, shat[3];
             elem_prox<typename std::conditional<PREC == WRAPD_FP32, float, double>::type, PROX>(
                 Sin, sl, pd, w2 / v, prox_iters, rtol, Zn, x, iters, ok, smin, evals, shat);
-            if (PREC == WRAPD_FP32) {
-                if (!ok || smin < fallback_sigma || x[0] < 0.0 || x[1] < 0.0 || x[2] < 0.0) {
-                    fallback = 1.0;
-                    elem_prox<double, PROX>(Sin, sl, pd, w2 / v, prox_iters, PROX == WRAPD_PROX_LBFGS ? rtol : 1e-12,
-                                            Zn, x, iters, ok, smin, evals, shat);
-                }
+            if (PREC == WRAPD_FP32)
+                defer = !ok || smin < fallback_sigma || x[0] < 0.0 || x[1] < 0.0 || x[2] < 0.0;
+            if (!defer) {
+                if (tlog) log_tuple(tlog, e, shat, w2 / v, pd, x);
+                pe = v * nh_psi(x, pd);
+                neg = (x[0] < 0.0 || x[1] < 0.0 || x[2] < 0.0) ? 1.0 : 0.0;
+                sig[e] = x[0];
+                sig[n_ep + e] = x[1];
+                sig[2 * n_ep + e] = x[2];
             }
-            if (tlog) log_tuple(tlog, e, shat, w2 / v, pd, x);
-            pe = v * nh_psi(x, pd);
-            neg = (x[0] < 0.0 || x[1] < 0.0 || x[2] < 0.0) ? 1.0 : 0.0;
-            sig[e] = x[0];
-            sig[n_ep + e] = x[1];
-            sig[2 * n_ep + e] = x[2];
         } else {
 #pragma unroll
             for (int i = 0; i < 9; ++i) Zn.m[i] = S.m[i] + Uo.m[i];
@@ -439,31 +439,107 @@ k_local(int n_e, int n_ep, const double* __restrict__ symF, double* __restrict__
             D.m[0] -= 1.0; D.m[4] -= 1.0; D.m[8] -= 1.0;
             pe = 0.5 * 1.0 * v * sqnorm9(D);
         }
-        M3<double> Un, Pn;
-        {
-            M3<double> D;
+        if (PREC == WRAPD_FP32) fbinfo[e] = defer ? static_cast<float>(evals + 1.0) : 0.0f;
+        if (defer) {   // U, P, sig and the candidate weight stay for the FP64 pass; its sums are that pass's
+            fallback = 1.0;
+            iters = 0.0;
+        } else {
+            M3<double> Un, Pn;
+            {
+                M3<double> D;
 #pragma unroll
-            for (int i = 0; i < 9; ++i) D.m[i] = S.m[i] - (Zn.m[i] - Uo.m[i]);
-            aft = 0.5 * w2 * sqnorm9(D);
+                for (int i = 0; i < 9; ++i) D.m[i] = S.m[i] - (Zn.m[i] - Uo.m[i]);
+                aft = 0.5 * w2 * sqnorm9(D);
+            }
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                Un.m[i] = Uo.m[i] + (S.m[i] - Zn.m[i]);
+                Pn.m[i] = Zn.m[i] - Un.m[i];
+            }
+            st9(U, n_ep, e, Un);
+            st9(P, n_ep, e, Pn);
+            if (update_cand) {
+                const double k = kst[e];
+                double cs = k;
+                if (use_hessian) cs = nh_candidate_stiffness(x, mat_of(m, e), k * m.beta_min, k * m.beta_max);
+                const double cw = v * cs;
+                cand[e] = cw;
+                const double a = w2 / cw, b = 1.0 / a;
+                ratio = a > b ? a : b;
+            }
         }
+    }
+    // slot 9 (max evaluations per element) of a deferred element is the FP64 pass's
+    store_local_partials(partials, lb, bef, aft, pe, ratio, iters, fallback, neg, 0.0, evals, defer ? 0.0 : evals, 0.0,
+                         shk, tail.nslots);
+    reduce_tail<kLocalThreads, 11>(tail, partials);
+}
+
+// The FP64 redo of the elements an FP32 local step deferred (fb_info nonzero), listed in
+// element order in `list` (count elements): the k_local FP64 prox on each, from the
+// unchanged U, P, sig, with the same outputs.  Its sums go to their own slots: aft, pe,
+// max ratio, FP64 iterations, inverted, FP64 evaluations (slot 8) and each element's
+// evaluations of both attempts (slot 9, by max); the FP32 kernel counted the fallbacks and
+// summed bef and the FP32 evaluations.  A fixed grid walks the list (i = block * 256 +
+// thread, + grid * 256, ...), so every thread's sums are taken in a fixed order.
+template <int PROX>
+__global__ void __launch_bounds__(kLocalThreads)
+k_local_fallback(int n_ep, const int* __restrict__ list, const int* __restrict__ count,
+                 const double* __restrict__ symF, double* __restrict__ U, double* __restrict__ P,
+                 const double* __restrict__ wsq, const double* __restrict__ vol, const double* __restrict__ kst,
+                 double* __restrict__ cand, double* __restrict__ sig, MatArgs m, int prox_iters, double rtol,
+                 int update_cand, int use_hessian, const float* __restrict__ fbinfo, double* __restrict__ partials,
+                 float* __restrict__ tlog, RedTail tail) {
+    __shared__ double shk[11][kLocalThreads / 64];
+    const int n = *count;
+    double aft = 0.0, pe = 0.0, ratio = 0.0, iters = 0.0, neg = 0.0, evals = 0.0, emax = 0.0;
+    for (int i = blockIdx.x * kLocalThreads + threadIdx.x; i < n; i += gridDim.x * kLocalThreads) {
+        const int e = list[i];
+        M3<double> S, Uo;
+        ld9(symF, n_ep, e, S);
+        ld9(U, n_ep, e, Uo);
+        const double w2 = wsq[e], v = vol[e];
+        M3<double> Sin;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            Un.m[i] = Uo.m[i] + (S.m[i] - Zn.m[i]);
-            Pn.m[i] = Zn.m[i] - Un.m[i];
+        for (int k = 0; k < 9; ++k) Sin.m[k] = S.m[k] + Uo.m[k];
+        Sin = transpose(Sin);
+        const NHParams<double> pd = mat_of(m, e);
+        const double sl[3] = {sig[e], sig[n_ep + e], sig[2 * n_ep + e]};
+        M3<double> Zn;
+        double x[3] = {1.0, 1.0, 1.0}, it = 0.0, smin, ev = 0.0, shat[3];
+        bool ok;
+        elem_prox<double, PROX>(Sin, sl, pd, w2 / v, prox_iters, rtol, Zn, x, it, ok, smin, ev, shat);
+        if (tlog) log_tuple(tlog, e, shat, w2 / v, pd, x);
+        pe += v * nh_psi(x, pd);
+        neg += (x[0] < 0.0 || x[1] < 0.0 || x[2] < 0.0) ? 1.0 : 0.0;
+        sig[e] = x[0];
+        sig[n_ep + e] = x[1];
+        sig[2 * n_ep + e] = x[2];
+        M3<double> D, Un, Pn;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) D.m[k] = S.m[k] - (Zn.m[k] - Uo.m[k]);
+        aft += 0.5 * w2 * sqnorm9(D);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            Un.m[k] = Uo.m[k] + (S.m[k] - Zn.m[k]);
+            Pn.m[k] = Zn.m[k] - Un.m[k];
         }
         st9(U, n_ep, e, Un);
         st9(P, n_ep, e, Pn);
         if (update_cand) {
             const double k = kst[e];
             double cs = k;
-            if (use_hessian) cs = nh_candidate_stiffness(x, mat_of(m, e), k * m.beta_min, k * m.beta_max);
+            if (use_hessian) cs = nh_candidate_stiffness(x, pd, k * m.beta_min, k * m.beta_max);
             const double cw = v * cs;
             cand[e] = cw;
             const double a = w2 / cw, b = 1.0 / a;
-            ratio = a > b ? a : b;
+            ratio = fmax(ratio, a > b ? a : b);
         }
+        iters += it;
+        evals += ev;
+        emax = fmax(emax, static_cast<double>(fbinfo[e]) - 1.0 + ev);
     }
-    store_local_partials(partials, lb, bef, aft, pe, ratio, iters, fallback, neg, 0.0, evals, evals, 0.0, shk,
+    store_local_partials(partials, blockIdx.x, 0.0, aft, pe, ratio, iters, 0.0, neg, 0.0, evals, emax, 0.0, shk,
                          tail.nslots);
     reduce_tail<kLocalThreads, 11>(tail, partials);
 }
@@ -494,7 +570,7 @@ k_local_learned(int n_e, int n_ep, const double* __restrict__ symF, double* __re
                 const double* __restrict__ wsq, const double* __restrict__ vol, const double* __restrict__ kst,
                 double* __restrict__ cand, double* __restrict__ sig, MatArgs m, int prox_iters, double rtol,
                 double fallback_sigma, int update_cand, int use_hessian, double* __restrict__ partials,
-                float* __restrict__ tlog, MlpDevice mlp, RedTail tail) {
+                float* __restrict__ tlog, float* __restrict__ fbinfo, MlpDevice mlp, RedTail tail) {
     using T = typename std::conditional<PREC == WRAPD_FP32, float, double>::type;
     __shared__ double shk[11][kLocalThreads / 64];   // the block reductions' wave results
     const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -503,6 +579,7 @@ k_local_learned(int n_e, int n_ep, const double* __restrict__ symF, double* __re
     const bool valid = e < n_e;
     double bef = 0.0, aft = 0.0, pe = 0.0, ratio = 0.0, iters = 0.0, fallback = 0.0, neg = 0.0, evals = 0.0;
     double accepted = 0.0, used = 0.0;
+    bool defer = false;   // an FP32 result unusable here: redone in FP64 by k_local_fallback
     M3<double> S, Uo, Po;
     double w2 = 1.0, v = 1.0;
     NHParams<double> pd{1.0, 1.0};
@@ -578,42 +655,44 @@ k_local_learned(int n_e, int n_ep, const double* __restrict__ symF, double* __re
         }
         double xd[3] = {static_cast<double>(x[0]), static_cast<double>(x[1]), static_cast<double>(x[2])};
         double shd[3] = {static_cast<double>(shat[0]), static_cast<double>(shat[1]), static_cast<double>(shat[2])};
-        if (PREC == WRAPD_FP32 &&
-            (!ok || static_cast<double>(shat[2]) < fallback_sigma || xd[0] < 0.0 || xd[1] < 0.0 || xd[2] < 0.0)) {
+        // an unusable FP32 result is redone in FP64 by k_local_fallback (Newton, 1e-12)
+        defer = PREC == WRAPD_FP32 &&
+                (!ok || static_cast<double>(shat[2]) < fallback_sigma || xd[0] < 0.0 || xd[1] < 0.0 || xd[2] < 0.0);
+        if (PREC == WRAPD_FP32) fbinfo[e] = defer ? static_cast<float>(evals + 1.0) : 0.0f;
+        if (defer) {
             fallback = 1.0;
-            double smin;
-            elem_prox<double, WRAPD_PROX_NEWTON>(Sin, sl, pd, w2 / v, prox_iters, 1e-12, Zn, xd, iters, ok, smin, evals,
-                                                 shd);
-        }
-        pe = v * nh_psi(xd, pd);
-        neg = (xd[0] < 0.0 || xd[1] < 0.0 || xd[2] < 0.0) ? 1.0 : 0.0;
-        sig[e] = xd[0];
-        sig[n_ep + e] = xd[1];
-        sig[2 * n_ep + e] = xd[2];
-        if (tlog) log_tuple(tlog, e, shd, w2 / v, pd, xd);
-        M3<double> D, Un, Pn;
+            iters = 0.0;
+        } else {
+            pe = v * nh_psi(xd, pd);
+            neg = (xd[0] < 0.0 || xd[1] < 0.0 || xd[2] < 0.0) ? 1.0 : 0.0;
+            sig[e] = xd[0];
+            sig[n_ep + e] = xd[1];
+            sig[2 * n_ep + e] = xd[2];
+            if (tlog) log_tuple(tlog, e, shd, w2 / v, pd, xd);
+            M3<double> D, Un, Pn;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) D.m[i] = S.m[i] - (Zn.m[i] - Uo.m[i]);
-        aft = 0.5 * w2 * sqnorm9(D);
+            for (int i = 0; i < 9; ++i) D.m[i] = S.m[i] - (Zn.m[i] - Uo.m[i]);
+            aft = 0.5 * w2 * sqnorm9(D);
 #pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            Un.m[i] = Uo.m[i] + (S.m[i] - Zn.m[i]);
-            Pn.m[i] = Zn.m[i] - Un.m[i];
-        }
-        st9(U, n_ep, e, Un);
-        st9(P, n_ep, e, Pn);
-        if (update_cand) {
-            const double k = kst[e];
-            double cs = k;
-            if (use_hessian) cs = nh_candidate_stiffness(xd, pd, k * m.beta_min, k * m.beta_max);
-            const double cw = v * cs;
-            cand[e] = cw;
-            const double a = w2 / cw, b = 1.0 / a;
-            ratio = a > b ? a : b;
+            for (int i = 0; i < 9; ++i) {
+                Un.m[i] = Uo.m[i] + (S.m[i] - Zn.m[i]);
+                Pn.m[i] = Zn.m[i] - Un.m[i];
+            }
+            st9(U, n_ep, e, Un);
+            st9(P, n_ep, e, Pn);
+            if (update_cand) {
+                const double k = kst[e];
+                double cs = k;
+                if (use_hessian) cs = nh_candidate_stiffness(xd, pd, k * m.beta_min, k * m.beta_max);
+                const double cw = v * cs;
+                cand[e] = cw;
+                const double a = w2 / cw, b = 1.0 / a;
+                ratio = a > b ? a : b;
+            }
         }
     }
-    store_local_partials(partials, lb, bef, aft, pe, ratio, iters, fallback, neg, accepted, evals, evals, used, shk,
-                         tail.nslots);
+    store_local_partials(partials, lb, bef, aft, pe, ratio, iters, fallback, neg, accepted, evals, defer ? 0.0 : evals,
+                         used, shk, tail.nslots);
     reduce_tail<kLocalThreads, 11>(tail, partials);
 }
 
@@ -817,7 +896,7 @@ static void local_inst(wrapd_ctx* c, const wrapd_local_args* a, float* tlog) {
     hipLaunchKernelGGL((k_local<MODEL, PREC, PROX>), dim3(nb), dim3(kLocalThreads), 0, c->stream, c->n_e, c->n_ep,
                        c->symF, c->U, c->P, c->wsq, c->vol, c->kst, c->cand, c->sig, mat_args(c), a->prox_iters,
                        a->newton_rtol, a->fallback_sigma, a->update_cand, a->cand_use_hessian, c->partials, tlog,
-                       take_tail(c, nb, true));
+                       PREC == WRAPD_FP32 ? c->fb_info : nullptr, take_tail(c, nb, true));
 }
 
 template <int PREC>
@@ -826,27 +905,50 @@ static void local_learned_inst(wrapd_ctx* c, const wrapd_local_args* a, float* t
     hipLaunchKernelGGL((k_local_learned<PREC>), dim3(nb), dim3(kLocalThreads), 0, c->stream, c->n_e, c->n_ep,
                        c->symF, c->U, c->P, c->wsq, c->vol, c->kst, c->cand, c->sig, mat_args(c), a->prox_iters,
                        a->newton_rtol, a->fallback_sigma, a->update_cand, a->cand_use_hessian, c->partials, tlog,
-                       mlp_device(c), take_tail(c, nb, true));
+                       PREC == WRAPD_FP32 ? c->fb_info : nullptr, mlp_device(c), take_tail(c, nb, true));
 }
 
-// The local-step phase times the element kernel; a wide-grid reduction launch that
-// finishes its sums (finish_reduction) follows outside it.
-static void launch_local_kernel(wrapd_ctx* c, const wrapd_local_args* a);
+constexpr int kFallbackBlocks = 256;   // the FP64 pass's fixed grid
+
+int prepare_local_fallback(wrapd_ctx* c) {
+    size_t bytes = 0;
+    if (check_hip(c, hipcub::DeviceSelect::Flagged(nullptr, bytes, hipcub::CountingInputIterator<int>(0), c->fb_info,
+                                                   c->fb_list, c->fb_count, c->n_e, c->stream),
+                  "fallback list size"))
+        return WRAPD_ERR_HIP;
+    if (c->fb_tmp) { (void)hipFree(c->fb_tmp); c->fb_tmp = nullptr; }
+    c->fb_tmp_bytes = std::max<size_t>(bytes, 1);
+    return check_hip(c, hipMalloc(&c->fb_tmp, c->fb_tmp_bytes), "hipMalloc(fallback list)") ? WRAPD_ERR_HIP : WRAPD_OK;
+}
+
+// The local step: the element kernel and, for FP32 steps, the compaction of the deferred
+// elements and their FP64 redo (its sums armed into S_LOCAL_FB), all in the local-step
+// phase.  A wide-grid element kernel finishes its sums in a separate launch (finish_reduction).
+static void launch_local_kernel(wrapd_ctx* c, const wrapd_local_args* a, float* tlog);
 void launch_local(wrapd_ctx* c, const wrapd_local_args* a) {
-    {
-        ProfScope ps(c, "local_step");
-        launch_local_kernel(c, a);
-    }
-    finish_reduction(c);
-}
-
-static void launch_local_kernel(wrapd_ctx* c, const wrapd_local_args* a) {
+    ProfScope ps(c, "local_step");
     // tuple log: the whole step is appended when it fits
     float* tlog = nullptr;
     if (c->mlp_logging && c->mat.model == 1 && c->mlp_log_count + c->n_e <= c->mlp_log_cap) {
         tlog = c->mlp_log + 9ll * c->mlp_log_count;
         c->mlp_log_count += c->n_e;
     }
+    launch_local_kernel(c, a, tlog);
+    finish_reduction(c);
+    c->local_fb = c->mat.model == 1 && a->precision == WRAPD_FP32;
+    if (!c->local_fb) return;
+    (void)hipcub::DeviceSelect::Flagged(c->fb_tmp, c->fb_tmp_bytes, hipcub::CountingInputIterator<int>(0), c->fb_info,
+                                        c->fb_list, c->fb_count, c->n_e, c->stream);
+    arm_tail(c, 11, kLocalMaxMask, S_LOCAL_FB);
+    const bool lbfgs = a->prox_mode == WRAPD_PROX_LBFGS;
+    auto k = lbfgs ? k_local_fallback<WRAPD_PROX_LBFGS> : k_local_fallback<WRAPD_PROX_NEWTON>;
+    hipLaunchKernelGGL(k, dim3(kFallbackBlocks), dim3(kLocalThreads), 0, c->stream, c->n_ep, c->fb_list, c->fb_count,
+                       c->symF, c->U, c->P, c->wsq, c->vol, c->kst, c->cand, c->sig, mat_args(c), a->prox_iters,
+                       lbfgs ? a->newton_rtol : 1e-12, a->update_cand, a->cand_use_hessian, c->fb_info, c->partials,
+                       tlog, take_tail(c, kFallbackBlocks, false));
+}
+
+static void launch_local_kernel(wrapd_ctx* c, const wrapd_local_args* a, float* tlog) {
     if (c->mat.model != 1) { local_inst<0, WRAPD_FP64, WRAPD_PROX_LBFGS>(c, a, nullptr); return; }
     const bool f32 = a->precision == WRAPD_FP32;
     if (a->prox_mode == WRAPD_PROX_LEARNED) {
