@@ -265,6 +265,10 @@ int wrapd_create(wrapd_ctx** out, int device) {
         const int v = std::atoi(e);
         if (v >= 0 && v <= c->asm_lds_max) c->asm_lds_max = v;
     }
+    if (const char* e = std::getenv("WRAPD_LEARNED_GATE")) {
+        const double v = std::atof(e);
+        if (v > 0.0) c->learned_gate = v;
+    }
     launch_fill(c, c->scal, 0.0, S_NUM);
     std::fill(c->scal_host, c->scal_host + S_NUM, 0.0);
     *c->host_seq = 0;

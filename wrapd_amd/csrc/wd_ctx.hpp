@@ -202,6 +202,13 @@ struct wrapd_ctx {
     // front assembly builds a column in LDS up to this many rows (64 KiB), larger fronts in
     // memory (WRAPD_ASM_LDS_MAX in the environment at wrapd_create lowers it, for tests)
     int asm_lds_max = 8192;
+    // the learned warm start runs the network for elements whose previous prox solution has
+    // a relative optimality residual above this (WRAPD_LEARNED_GATE at wrapd_create).  At
+    // 1e-3 the network ran for most 16-element blocks in steady state and cost more than the
+    // Newton iterations it saved (C2: 108 vs 75 us per local step for 21% fewer iterations;
+    // C5: 0.85 vs 0.58 ms); from 1e-1 on it runs where a start is far off (a first step,
+    // a jump after reweighting) and costs ~5% elsewhere
+    double learned_gate = 1e-1;
     double* scal_host = nullptr; // pinned host mirror
     unsigned* host_seq = nullptr; // pinned: the epoch of the last reduction whose host copy is stored
 

@@ -546,12 +546,11 @@ k_local_fallback(int n_ep, const int* __restrict__ list, const int* __restrict__
 
 // Learned warm start (WRAPD_PROX_LEARNED): after the SVD of S_in, a wave in which some
 // element's previous solution is a poor start (relative optimality residual above
-// kLearnedGate, where one Newton step no longer suffices) evaluates the MLP
+// the context's learned_gate) evaluates the MLP
 // (device/wd_mlp.hpp) on its 64 elements; each element's Newton prox then starts from
 // whichever of the prediction and the previous solution has the smaller residual |grad|,
 // and a prediction already within the Newton tolerance is taken as is.  Everything else
 // (reductions, FP64 fallback, candidate weights) is k_local's.
-constexpr double kLearnedGate = 1e-3;
 // |grad| of the prox objective at x (infinite outside x > 0 or when not finite); f and g
 // are kept for the Newton prox that may start there.
 template <typename T>
@@ -570,7 +569,7 @@ k_local_learned(int n_e, int n_ep, const double* __restrict__ symF, double* __re
                 const double* __restrict__ wsq, const double* __restrict__ vol, const double* __restrict__ kst,
                 double* __restrict__ cand, double* __restrict__ sig, MatArgs m, int prox_iters, double rtol,
                 double fallback_sigma, int update_cand, int use_hessian, double* __restrict__ partials,
-                float* __restrict__ tlog, float* __restrict__ fbinfo, MlpDevice mlp, RedTail tail) {
+                float* __restrict__ tlog, float* __restrict__ fbinfo, MlpDevice mlp, double gate, RedTail tail) {
     using T = typename std::conditional<PREC == WRAPD_FP32, float, double>::type;
     __shared__ double shk[11][kLocalThreads / 64];   // the block reductions' wave results
     const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -617,7 +616,7 @@ k_local_learned(int n_e, int n_ep, const double* __restrict__ symF, double* __re
     T fl, gl[3];
     const T rl = prox_residual(PP, xl, fl, gl);
     const T xln = wsqrt(dot3(xl, xl));
-    const bool poor = valid && !(rl <= static_cast<T>(kLearnedGate) * (PP.p.mu + PP.wov) * (xln > T(1) ? xln : T(1)));
+    const bool poor = valid && !(rl <= static_cast<T>(gate) * (PP.p.mu + PP.wov) * (xln > T(1) ? xln : T(1)));
     float dpred[kMlpOut] = {0.0f, 0.0f, 0.0f};
     // wave-uniform: the MLP is a whole-wave product, over the 16-element blocks with a poor start
     const unsigned long long poor_lanes = __ballot(poor);
@@ -905,7 +904,7 @@ static void local_learned_inst(wrapd_ctx* c, const wrapd_local_args* a, float* t
     hipLaunchKernelGGL((k_local_learned<PREC>), dim3(nb), dim3(kLocalThreads), 0, c->stream, c->n_e, c->n_ep,
                        c->symF, c->U, c->P, c->wsq, c->vol, c->kst, c->cand, c->sig, mat_args(c), a->prox_iters,
                        a->newton_rtol, a->fallback_sigma, a->update_cand, a->cand_use_hessian, c->partials, tlog,
-                       PREC == WRAPD_FP32 ? c->fb_info : nullptr, mlp_device(c), take_tail(c, nb, true));
+                       PREC == WRAPD_FP32 ? c->fb_info : nullptr, mlp_device(c), c->learned_gate, take_tail(c, nb, true));
 }
 
 constexpr int kFallbackBlocks = 256;   // the FP64 pass's fixed grid
