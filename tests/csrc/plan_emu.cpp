@@ -20,14 +20,23 @@ std::vector<double> pool, G;
 std::vector<int64_t> wF, rF, wG, rG;
 int64_t stamp = 0, cur_launch = 0, conflicts = 0;
 
+// Launches on the second stream (FactorLaunch::side) run concurrently with the main
+// stream's launches issued after them, until the next level's assembly: an access of such a
+// main launch to an entry a running side launch wrote (or a write to one it read) counts too.
+std::vector<char> side_active;   // by launch index + 1
+bool cur_main = true;
 inline int64_t launch_of(int64_t st) { return st >> 32; }
+inline bool concurrent(int64_t st) {
+    const int64_t l = launch_of(st);
+    return l == cur_launch ? st != stamp : (cur_main && l < static_cast<int64_t>(side_active.size()) && side_active[l]);
+}
 inline void on_read(std::vector<int64_t>& w, std::vector<int64_t>& r, int64_t e) {
-    if (w[e] && launch_of(w[e]) == cur_launch && w[e] != stamp) conflicts++;
+    if (w[e] && concurrent(w[e])) conflicts++;
     r[e] = stamp;
 }
 inline void on_write(std::vector<int64_t>& w, std::vector<int64_t>& r, int64_t e) {
-    if (w[e] && launch_of(w[e]) == cur_launch && w[e] != stamp) conflicts++;
-    if (r[e] && launch_of(r[e]) == cur_launch && r[e] != stamp) conflicts++;
+    if (w[e] && concurrent(w[e])) conflicts++;
+    if (r[e] && concurrent(r[e])) conflicts++;
     w[e] = stamp;
 }
 inline double rdF(int64_t e) { on_read(wF, rF, e); return pool[e]; }
@@ -52,6 +61,8 @@ extern "C" int pe_analyze(int n, const int* ap, const int* aj, const double* xyz
     st[0] = S.nsuper; st[1] = PL.n_levels; st[2] = static_cast<double>(PL.launches.size());
     st[3] = S.max_ns; st[4] = static_cast<double>(S.goff[S.nsuper]); st[5] = static_cast<double>(PL.pool_size);
     st[6] = static_cast<double>(PL.pool_no_reuse);
+    st[7] = static_cast<double>(std::count_if(PL.launches.begin(), PL.launches.end(),
+                                              [](const FactorLaunch& L) { return L.side; }));
     return 0;
 }
 
@@ -73,9 +84,13 @@ extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
     wG.assign(G.size(), 0); rG.assign(G.size(), 0);
     conflicts = 0;
     int fail = 0;
+    side_active.assign(PL.launches.size() + 1, 0);
     for (size_t li = 0; li < PL.launches.size(); ++li) {
         const FactorLaunch& L = PL.launches[li];
         cur_launch = static_cast<int64_t>(li) + 1;
+        if (L.kind == FK_ASSEMBLE) std::fill(side_active.begin(), side_active.end(), 0);   // the streams join
+        cur_main = !L.side;
+        if (L.side) side_active[cur_launch] = 1;
         for (int q = 0; q < L.count; ++q) {
             stamp = (cur_launch << 32) | (q + 1);
             const int s = PL.tasks[2 * (L.offset + q)], y = PL.tasks[2 * (L.offset + q) + 1];

@@ -48,7 +48,7 @@ def _upper_csr(prob):
 def _run(emu, prob, leaf_size, negate=False, coords=True):
     n, Ap, Aj, Ax = _upper_csr(prob)
     free = np.setdiff1d(np.arange(prob.n_verts), prob.pins)
-    st = np.zeros(7)
+    st = np.zeros(8)
     xyz = ol.dp(np.ascontiguousarray(prob.rest[free])) if coords else None
     emu.pe_analyze(n, ol.ip(Ap), ol.ip(Aj), xyz, leaf_size, ol.dp(st))
     nsup = int(st[0])
@@ -93,6 +93,8 @@ def test_plan_matches_dense_cholesky(emu, name):
     assert R["conflicts"] == 0
     if name == "two_panel_separators":
         assert R["st"][3] > 64   # a supernode spans more than one 64-column panel
+        assert R["st"][7] >= 1   # and a FINAL launch overlaps a wider supernode's panels
+    print(f"{name}: {R['st'][7]:.0f} FINAL launches on the second stream")
     p = R["perm"]
     L = np.linalg.cholesky(R["A"][np.ix_(p, p)])
     worst = 0.0

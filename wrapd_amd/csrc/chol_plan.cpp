@@ -187,8 +187,10 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P) {
         P.launches.back().count++;
     };
     const double T = kFactorTile;
+    std::vector<std::pair<int, size_t>> final_at;   // (TRSM panel, side FINAL launch index)
     for (int l = 0; l < nlev; ++l) {
         const std::vector<int>& L = by_level[l];
+        const size_t level_first = P.launches.size();
         begin(FK_ASSEMBLE, 0);
         for (int s : L) {
             for (int j = 0; j < f[s]; ++j) push(s, j);
@@ -233,7 +235,6 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P) {
                     }
             }
         }
-        begin(FK_FINAL, 0);
         const double T2 = kFinalTile;
         for (int s : L) {
             const int U = final_row_tiles(ns[s], f[s]);
@@ -244,7 +245,35 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P) {
             for (int u = 0; u < U; ++u)
                 for (int c = 0; c < C; ++c) P.flops += 2 * T2 * T2 * (k32 - T2 * c);
         }
-        final_order(L, ns, f, [&](int s, int y) { push(s, y); });
+        // the FINAL launches, by panel count p: those of p < pmax go right after TRSM(p - 1),
+        // the last panel step of their supernodes, on the second stream
+        for (int p = 1; p <= pmax; ++p) {
+            std::vector<int> grp;
+            for (int s : L)
+                if (tile_count_l11(ns[s]) == p) grp.push_back(s);
+            if (grp.empty()) continue;
+            begin(FK_FINAL, 0);
+            P.launches.back().side = p < pmax;
+            final_order(grp, ns, f, [&](int s, int y) { push(s, y); });
+            if (p < pmax) final_at.push_back({p - 1, P.launches.size() - 1});
+        }
+        // move each side FINAL launch to just after its TRSM(p - 1) launch
+        if (!final_at.empty()) {
+            std::vector<FactorLaunch> head(P.launches.begin(), P.launches.begin() + level_first), rest;
+            std::vector<FactorLaunch> lv(P.launches.begin() + level_first, P.launches.end());
+            std::vector<FactorLaunch> out;
+            std::vector<bool> placed(lv.size(), false);
+            for (size_t i = 0; i < lv.size(); ++i) {
+                if (lv[i].kind == FK_FINAL && lv[i].side) continue;
+                out.push_back(lv[i]);
+                if (lv[i].kind == FK_TRSM)
+                    for (const auto& fa : final_at)
+                        if (fa.first == lv[i].panel) out.push_back(P.launches[fa.second]);
+            }
+            head.insert(head.end(), out.begin(), out.end());
+            P.launches.swap(head);
+        }
+        final_at.clear();
     }
     // drop empty launches (a level whose panel list is empty for a kind)
     std::vector<FactorLaunch> kept;

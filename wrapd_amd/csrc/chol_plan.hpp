@@ -17,7 +17,9 @@
 //                      supernode's remaining columns only), and X[i, c] -= L[i, k] X[k, c]
 //                      for L11 row tiles k < i < P, c <= k
 //      after which X = L11^{-1} (the top of G_s);
-//   3. one launch for the rest, each 128x128 super-tile a single pass over all ns columns:
+//   3. FINAL launches for the rest, each 128x128 super-tile a single pass over all ns
+//      columns, one per panel count: the supernodes whose panels end early are finished on a
+//      second stream while the level's wider supernodes go on with their panels:
 //        SCHUR         F[a, b] -= sum_q L[a, q] L[b, q]^T for update super-tiles a >= b
 //        GBOT          G[a, c] = -sum_q L[a, q] X[q, c] for update rows a, L11 columns c
 //      so the Schur complement is read and written once (not once per panel).
@@ -52,6 +54,11 @@ struct FactorLaunch {
     int64_t offset;     // first task (int pairs) in tasks
     int count;          // tasks
     int maxf = 0;       // FK_ASSEMBLE: the largest front of the launch (its LDS column buffer)
+    // FK_FINAL of supernodes whose panels end before the level's last panel step: run on a
+    // second stream once the main stream has issued the launches before this one, so that
+    // it overlaps the remaining panel steps of the level's wider supernodes; the main
+    // stream waits for it before the next level's assembly
+    bool side = false;
 };
 
 struct FactorPlan {

@@ -56,6 +56,10 @@ struct Prof {
 struct wrapd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // the factorization's second stream (FINAL launches of supernodes whose panels end before
+    // their level's last panel step) and the events that order it with `stream`
+    hipStream_t fstream2 = nullptr;
+    hipEvent_t fev_fork = nullptr, fev_join = nullptr;
     std::string last_error;
     wd::Prof prof;
 

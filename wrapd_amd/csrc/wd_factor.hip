@@ -712,19 +712,41 @@ int factor_prepare_device(wrapd_ctx* c) {
     rc |= up_vec(c, c->d_ftasks, P.tasks);
     std::vector<int> zero1(1, 0);
     rc |= up_vec(c, c->d_ffail, zero1);
+    if (!c->fstream2) {
+        rc |= check_hip(c, hipStreamCreateWithFlags(&c->fstream2, hipStreamNonBlocking), "hipStreamCreate");
+        rc |= check_hip(c, hipEventCreateWithFlags(&c->fev_fork, hipEventDisableTiming), "hipEventCreate");
+        rc |= check_hip(c, hipEventCreateWithFlags(&c->fev_join, hipEventDisableTiming), "hipEventCreate");
+    }
     if (c->pool) { (void)hipFree(c->pool); c->pool = nullptr; }
     rc |= check_hip(c, hipMalloc(&c->pool, std::max<size_t>(8, P.pool_size * sizeof(double))), "hipMalloc(front pool)");
     return rc ? WRAPD_ERR_HIP : WRAPD_OK;
 }
 
-// queues the whole factorization on c->stream; the caller reads c->d_ffail afterwards
+// queues the whole factorization on c->stream (with the side FINAL launches on
+// c->fstream2, forked after the main-stream launch before them and joined before the next
+// level's assembly and at the end); the caller reads c->d_ffail afterwards
 void launch_factor(wrapd_ctx* c) {
     const wrapd::FactorPlan& P = c->fplan;
     FactArgs A{c->d_col0, c->d_fsize, c->d_foff, c->d_goff, c->pool, c->G};
     const int2* tasks = reinterpret_cast<const int2*>(c->d_ftasks);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->d_ffail, 1, 0);
+    bool side_busy = false;
+    auto join = [&] {
+        if (!side_busy) return;
+        (void)hipEventRecord(c->fev_join, c->fstream2);
+        (void)hipStreamWaitEvent(c->stream, c->fev_join, 0);
+        side_busy = false;
+    };
     for (const wrapd::FactorLaunch& L : P.launches) {
         const int2* t = tasks + L.offset;
+        if (L.kind == wrapd::FK_ASSEMBLE) join();
+        if (L.side) {
+            (void)hipEventRecord(c->fev_fork, c->stream);
+            (void)hipStreamWaitEvent(c->fstream2, c->fev_fork, 0);
+            hipLaunchKernelGGL(k_ffinal, dim3(L.count), dim3(kFactThreads), 0, c->fstream2, t, A);
+            side_busy = true;
+            continue;
+        }
         switch (L.kind) {
         case wrapd::FK_ASSEMBLE:
             // small fronts (the many leaf-level columns): one wave per column in memory is
@@ -753,6 +775,7 @@ void launch_factor(wrapd_ctx* c) {
             break;
         }
     }
+    join();
 }
 
 }  // namespace wd
