@@ -24,7 +24,7 @@ wrapd_ctx::~wrapd_ctx() {
     f(vidx); f(Di); f(vol); f(kst); f(mu_e); f(lam_e); f(wsq); f(cand); f(U); f(P); f(symF); f(sig); f(gstage);
     f(xall); f(grad); f(grad_old); f(qv); f(rv); f(bv); f(xref);
     for (int i = 0; i < wd::kMaxHist; ++i) { f(hS[i]); f(hY[i]); }
-    f(v2e_ptr); f(cpos); f(a_cptr); f(a_cidx); f(Ax); f(fb_info); f(fb_list); f(fb_count); f(fb_tmp);
+    f(v2e_ptr); f(cpos); f(a_cptr); f(a_cidx); f(Ax); f(fb_info); f(fb_list); f(fb_count); f(fb_tmp); f(d_fin_ticket);
     f(G); f(ysol); f(xsol); f(ubuf); f(d_perm); f(d_iperm); f(d_col0); f(d_fsize); f(d_rowptr); f(d_rows);
     f(d_goff); f(d_uoff); f(d_asm_ptr); f(d_asm_idx); f(d_udest); f(d_leaf_ptr); f(d_leaf_sn); f(d_flow_f); f(d_flow_b);
     f(d_flow_abort); f(d_bcopy); f(bperm);
@@ -241,7 +241,7 @@ int wrapd_create(wrapd_ctx** out, int device) {
     if (check_hip(c, hipSetDevice(device), "hipSetDevice") ||
         check_hip(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate") ||
         dalloc(c, c->partials, static_cast<size_t>(kMaxBlocks) * kRedStride) || dalloc(c, c->scal, S_NUM) ||
-        dalloc(c, c->d_red_abort, 1) ||
+        dalloc(c, c->d_red_abort, 1) || dalloc(c, c->d_fin_ticket, 1) ||
         check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&c->scal_host), S_NUM * sizeof(double)), "hipHostMalloc") ||
         check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&c->host_seq), 64), "hipHostMalloc")) {
         delete c;
@@ -251,7 +251,8 @@ int wrapd_create(wrapd_ctx** out, int device) {
     static_assert(wd::kPendingBits == 0xFFF40000FFF40000ull, "pending pattern: two equal halves");
     if (check_hip(c, hipMemsetD32(reinterpret_cast<hipDeviceptr_t>(c->partials), 0xFFF40000u,
                                   2 * static_cast<size_t>(kMaxBlocks) * kRedStride), "hipMemsetD32") ||
-        check_hip(c, hipMemset(c->d_red_abort, 0, sizeof(int)), "hipMemset")) {
+        check_hip(c, hipMemset(c->d_red_abort, 0, sizeof(int)), "hipMemset") ||
+        check_hip(c, hipMemset(c->d_fin_ticket, 0, sizeof(unsigned)), "hipMemset")) {
         delete c;
         return WRAPD_ERR_HIP;
     }

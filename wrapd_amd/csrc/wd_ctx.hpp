@@ -174,6 +174,7 @@ struct wrapd_ctx {
     double* partials = nullptr;
     double* scal = nullptr;      // device scalars
     int* d_red_abort = nullptr;        // sticky: a reduction's wait ran out
+    unsigned* d_fin_ticket = nullptr;  // k_finalize's count of finished slots (0 between launches)
     unsigned red_epoch = 0;
     unsigned xd_epoch = 0;             // the epoch of the last x-distance reduction (its host copy: S_XD)
     int alpha_n = -1;                  // S_ALPHA holds the alphas of this list for the gradient in grad (-1: none)

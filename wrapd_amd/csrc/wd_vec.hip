@@ -27,63 +27,54 @@ struct VecPtrs {
     const double* v[kMaxHist];
 };
 
-// The reduction of a launch with more than kTailMaxBlocks blocks, as a launch of its own:
-// the same order as reduce_tail (lane v of 256 takes blocks v, v + 256, ...).
-// KS: the most slots (nslots <= KS); BB blocks' slots are loaded per batch.
-template <int KS>
+// The reduction of a launch with more than kTailMaxBlocks blocks, as a launch of its own
+// with one block per slot: the same order as reduce_tail (lane v of 256 takes blocks v,
+// v + 256, ...), so the same bits as the one-block version it replaces, which took 103 us
+// for the C5 local step's 11 slots over 12,000 blocks.
 __global__ void __launch_bounds__(256)
 k_finalize(double* __restrict__ partials, int nblk, int nslots, unsigned maxmask, double* __restrict__ out,
-           double* __restrict__ hout, unsigned* __restrict__ hseq, unsigned epoch) {
-    constexpr int BB = KS > 4 ? 4 : 8;
-    __shared__ double sh[KS][4];
-    double acc[KS];
-#pragma unroll
-    for (int k = 0; k < KS; ++k) acc[k] = ((maxmask >> k) & 1u) ? -1e308 : 0.0;
+           double* __restrict__ hout, unsigned* __restrict__ hseq, unsigned epoch, unsigned* __restrict__ ticket) {
+    // block k reduces slot k; lane v takes blocks v, v + 256, ... in that order (the
+    // in-launch tail's order), 16 loads in flight per lane
+    constexpr int BB = 16;
+    __shared__ double sh[4];
+    const int k = blockIdx.x;
+    const bool ismax = (maxmask >> k) & 1u;
+    double* pk = partials + static_cast<long long>(k) * kTailBlocks;
+    double acc = ismax ? -1e308 : 0.0;
     for (int b0 = threadIdx.x; b0 < nblk; b0 += 256 * BB) {
-        double v[BB][KS];
+        double v[BB];
+#pragma unroll
+        for (int u = 0; u < BB; ++u) v[u] = b0 + 256 * u < nblk ? pk[b0 + 256 * u] : 0.0;
 #pragma unroll
         for (int u = 0; u < BB; ++u)
-#pragma unroll
-            for (int k = 0; k < KS; ++k) {
-                const int b = b0 + 256 * u;
-                v[u][k] = (b < nblk && k < nslots) ? partials[static_cast<long long>(k) * kTailBlocks + b] : 0.0;
+            if (b0 + 256 * u < nblk) {
+                acc = ismax ? fmax(acc, v[u]) : acc + v[u];
+                pk[b0 + 256 * u] = pending_value();   // pending again for the next in-launch reduction
             }
-#pragma unroll
-        for (int u = 0; u < BB; ++u)
-#pragma unroll
-            for (int k = 0; k < KS; ++k)
-                if (b0 + 256 * u < nblk && k < nslots) {
-                    acc[k] = ((maxmask >> k) & 1u) ? fmax(acc[k], v[u][k]) : acc[k] + v[u][k];
-                    // pending again for the next in-launch reduction
-                    partials[static_cast<long long>(k) * kTailBlocks + b0 + 256 * u] = pending_value();
-                }
     }
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 0; k < KS; ++k)
-        if (k < nslots) {
-            const double t = ((maxmask >> k) & 1u) ? wave_max(acc[k]) : wave_sum(acc[k]);
-            if (l == 0) sh[k][w] = t;
-        }
+    const double t = ismax ? wave_max(acc) : wave_sum(acc);
+    if (l == 0) sh[w] = t;
     __syncthreads();
-    if (static_cast<int>(threadIdx.x) < nslots) {
-        const int k = threadIdx.x;
+    if (threadIdx.x == 0) {
         double r;
-        if ((maxmask >> k) & 1u) {
-            r = sh[k][0];
-            for (int i = 1; i < 4; ++i) r = fmax(r, sh[k][i]);
+        if (ismax) {
+            r = sh[0];
+            for (int i = 1; i < 4; ++i) r = fmax(r, sh[i]);
         } else {
             r = 0.0;
-            for (int i = 0; i < 4; ++i) r += sh[k][i];
+            for (int i = 0; i < 4; ++i) r += sh[i];
         }
         out[k] = r;
-        if (hout) {
-            __hip_atomic_store(hout + k, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            __builtin_amdgcn_s_waitcnt(0);
+        if (hout) __hip_atomic_store(hout + k, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        // the block finishing last publishes the epoch, after every block's host copy
+        const unsigned prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (prev == static_cast<unsigned>(nslots) - 1u) {
+            __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (hseq) __hip_atomic_store(hseq, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0 && hseq) __hip_atomic_store(hseq, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Store the reduced Gram entries of the new pair (red: the reduction's results in LDS,
@@ -490,10 +481,9 @@ void finish_reduction(wrapd_ctx* c) {
     const wrapd_ctx::PendingFinalize f = c->pending_fin;
     if (f.nblk == 0) return;
     c->pending_fin = wrapd_ctx::PendingFinalize{};
-    auto k = f.nslots <= 1 ? k_finalize<1> : f.nslots <= 3 ? k_finalize<3> : f.nslots <= 11 ? k_finalize<11>
-                                                                                          : k_finalize<kTailSlots>;
-    hipLaunchKernelGGL(k, dim3(1), dim3(256), 0, c->stream, c->partials, f.nblk, f.nslots, f.maxmask, c->scal + f.dst,
-                       f.host ? c->scal_host + f.dst : nullptr, f.host ? c->host_seq : nullptr, c->red_epoch);
+    hipLaunchKernelGGL(k_finalize, dim3(f.nslots), dim3(256), 0, c->stream, c->partials, f.nblk, f.nslots, f.maxmask,
+                       c->scal + f.dst, f.host ? c->scal_host + f.dst : nullptr, f.host ? c->host_seq : nullptr,
+                       c->red_epoch, c->d_fin_ticket);
 }
 
 void launch_gather_grad(wrapd_ctx* c, double* out, int mode, int new_pos, const int* list_slots, int nlist,
