@@ -51,13 +51,16 @@ void tile(int t, int ns, int f, int& st, int& sz) {
 }
 }  // namespace
 
+double l21_right_max = kL21RightMaxTiles;
+extern "C" void pe_set_l21_right_max(double v) { l21_right_max = v; }
+
 extern "C" int pe_analyze(int n, const int* ap, const int* aj, const double* xyz, int leaf_size, double* st) {
     Ap.assign(ap, ap + n + 1);
     Aj.assign(aj, aj + ap[n]);
     SymbolicOptions o;
     o.leaf_size = leaf_size;
     chol_analyze(n, Ap, Aj, xyz, o, S);
-    chol_plan(S, PL);
+    chol_plan(S, PL, l21_right_max);
     st[0] = S.nsuper; st[1] = PL.n_levels; st[2] = static_cast<double>(PL.launches.size());
     st[3] = S.max_ns; st[4] = static_cast<double>(S.goff[S.nsuper]); st[5] = static_cast<double>(PL.pool_size);
     st[6] = static_cast<double>(PL.pool_no_reuse);
@@ -76,7 +79,8 @@ extern "C" void pe_layout(int* perm, int* col0, int* fsize, int64_t* goff, int64
     if (rows) std::copy(S.rows.begin(), S.rows.end(), rows);
 }
 
-// returns 0 on success, 1 on a non-positive pivot; G_out gets G; conflicts_out the count
+// returns 0 on success, 1 on a non-positive pivot, 2 on an L21 task outside the update rows;
+// G_out gets G; conflicts_out the count
 extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
     pool.assign(PL.pool_size, std::nan(""));
     G.assign(S.goff[S.nsuper], std::nan(""));
@@ -144,7 +148,7 @@ extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
             } else if (L.kind == FK_POTRF) {
                 potrf(k);
             } else if (L.kind == FK_TRSM) {
-                const int kind = y >> kTaskKindShift, t = y & ((1 << kTaskKindShift) - 1);
+                const int kind = task_kind(y), t = y & ((1 << kTaskKindShift) - 1);
                 int t0, nt;
                 tile(t, ns, f, t0, nt);
                 if (kind == 0) {
@@ -172,8 +176,32 @@ extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
                     for (int j = 0; j < nt; ++j)
                         for (int r = 0; r < nb; ++r) wrG(Gi(c0 + r, t0 + j), X[j * 64 + r]);
                 }
+            } else if (L.kind == FK_L21) {   // the 64-row tile t = y >= P at panel k
+                const int P = tile_count_l11(ns);
+                if (y < P || k >= P) return 2;
+                int t0, nt;
+                tile(y, ns, f, t0, nt);
+                for (int p = 0; p < k; ++p) {   // F[t, k] -= L[t, p] L[k, p]^T
+                    const int cp = kFactorTile * p;
+                    for (int j = 0; j < nb; ++j)
+                        for (int i = 0; i < nt; ++i) {
+                            double acc = 0;
+                            for (int kk = 0; kk < kFactorTile; ++kk)
+                                acc += rdF(Fi(t0 + i, cp + kk)) * rdF(Fi(c0 + j, cp + kk));
+                            wrF(Fi(t0 + i, c0 + j), rdF(Fi(t0 + i, c0 + j)) - acc);
+                        }
+                }
+                for (int p = 0; p < nb; ++p) {   // L[t, k] = F[t, k] L_kk^{-T}
+                    const double d = rdF(Fi(c0 + p, c0 + p));
+                    for (int r = 0; r < nt; ++r) wrF(Fi(t0 + r, c0 + p), rdF(Fi(t0 + r, c0 + p)) / d);
+                    for (int c = p + 1; c < nb; ++c) {
+                        const double l = rdF(Fi(c0 + c, c0 + p));
+                        for (int r = 0; r < nt; ++r)
+                            wrF(Fi(t0 + r, c0 + c), rdF(Fi(t0 + r, c0 + c)) - rdF(Fi(t0 + r, c0 + p)) * l);
+                    }
+                }
             } else if (L.kind == FK_FINAL) {   // 128x128 super-tiles (kFinalTile)
-                const int kind = y >> kTaskKindShift;
+                const int kind = task_kind(y);
                 const int u = (y >> kTaskTileShift) & kTaskTileMask, v = y & kTaskTileMask;
                 const int a0 = ns + kFinalTile * u, na = std::min(kFinalTile, f - a0);
                 const int b0 = kind == 0 ? ns + kFinalTile * v : kFinalTile * v;
@@ -187,7 +215,7 @@ extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
                         else wrG(Gi(a0 + i, b0 + j), -acc);
                     }
             } else {
-                const int kind = y >> kTaskKindShift;
+                const int kind = task_kind(y);
                 const int a = (y >> kTaskTileShift) & kTaskTileMask, b = y & kTaskTileMask;
                 int a0, na, b0, nbt;
                 tile(a, ns, f, a0, na);

@@ -28,6 +28,7 @@ def emu(tmp_path_factory):
     subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", out] + src, check=True)
     lib = C.CDLL(out)
     lib.pe_run.restype = C.c_int
+    lib.pe_set_l21_right_max.argtypes = [C.c_double]
     return lib
 
 
@@ -45,8 +46,9 @@ def _upper_csr(prob):
     return n, np.cumsum(Ap).astype(np.int32), np.ascontiguousarray(c.astype(np.int32)), np.ascontiguousarray(v)
 
 
-def _run(emu, prob, leaf_size, negate=False, coords=True):
+def _run(emu, prob, leaf_size, negate=False, coords=True, l21_right_max=256.0):
     n, Ap, Aj, Ax = _upper_csr(prob)
+    emu.pe_set_l21_right_max(C.c_double(l21_right_max))
     free = np.setdiff1d(np.arange(prob.n_verts), prob.pins)
     st = np.zeros(8)
     xyz = ol.dp(np.ascontiguousarray(prob.rest[free])) if coords else None
@@ -85,10 +87,16 @@ CASES = {
 }
 
 
+# where L21 is formed: the default rule, always by a left-looking L21 launch, always by the
+# right-looking panel steps
+L21_MODES = {"default": 256.0, "left": 0.0, "right": 1e30}
+
+
+@pytest.mark.parametrize("mode", sorted(L21_MODES))
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_plan_matches_dense_cholesky(emu, name):
+def test_plan_matches_dense_cholesky(emu, name, mode):
     make, leaf = CASES[name]
-    R = _run(emu, make(), leaf, coords=not name.startswith("graph_order"))
+    R = _run(emu, make(), leaf, coords=not name.startswith("graph_order"), l21_right_max=L21_MODES[mode])
     assert R["fail"] == 0
     assert R["conflicts"] == 0
     if name == "two_panel_separators":
@@ -116,6 +124,15 @@ def test_plan_matches_dense_cholesky(emu, name):
         rows = R["rows"][R["rowptr"][s]:R["rowptr"][s + 1]]
         Ls[np.ix_(rows, np.arange(c0, c1))] = L[np.ix_(rows, np.arange(c0, c1))]
     assert np.allclose(np.tril(L), np.tril(Ls), atol=1e-12 * np.abs(L).max())
+
+
+def test_l21_left_and_right_looking_same_bits(emu):
+    """The left-looking L21 launch applies the right-looking panel steps' products in their
+    order, so G comes out bit for bit the same whichever forms L21."""
+    make, leaf = CASES["two_panel_separators"]
+    prob = make()
+    Gs = [_run(emu, prob, leaf, l21_right_max=v)["G"] for v in (0.0, 1e30)]
+    assert np.array_equal(Gs[0], Gs[1], equal_nan=True)   # (G's row padding is never written)
 
 
 def test_plan_pool_reuse_and_indefinite(emu):

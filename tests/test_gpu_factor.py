@@ -92,3 +92,23 @@ def test_assembly_in_lds_and_in_memory_agree(monkeypatch):
         xs.append(d.solve3(b))
         d.close()
     assert np.array_equal(xs[0], xs[1])
+
+
+def test_l21_left_and_right_looking_agree(monkeypatch):
+    """L21 is formed either by the panel steps (right-looking) or by one left-looking launch
+    per level after them; WRAPD_L21_RIGHT_MAX forces either placement on every level.  Both
+    apply the same tile products in the same order, so the factorizations are the same bits."""
+    prob = meshgen.config_problem(3)
+    b = np.random.default_rng(11).standard_normal((prob.n_verts - len(prob.pins), 3))
+    xs = []
+    for lim in ("0", "1e30", None):
+        if lim is None:
+            monkeypatch.delenv("WRAPD_L21_RIGHT_MAX", raising=False)
+        else:
+            monkeypatch.setenv("WRAPD_L21_RIGHT_MAX", lim)
+        d = _device(prob)
+        d.refactor()
+        xs.append(d.solve3(b))
+        d.close()
+    assert np.array_equal(xs[0], xs[1])
+    assert np.array_equal(xs[0], xs[2])

@@ -27,7 +27,7 @@ void final_order(const std::vector<int>& L, const std::vector<int>& ns, const st
             for (int c = 0; c < C; ++c) tk.push_back({s, (1 << kTaskKindShift) | (u << kTaskTileShift) | c});
     }
     auto cost = [&](const std::pair<int, int>& x) {
-        const int kind = x.second >> kTaskKindShift;
+        const int kind = task_kind(x.second);
         return kind == 0 ? ns[x.first] : ns[x.first] - kFinalTile * (x.second & kTaskTileMask);
     };
     std::stable_sort(tk.begin(), tk.end(),
@@ -85,7 +85,7 @@ struct PoolAlloc {
 
 }  // namespace
 
-void chol_plan(const CholSymbolic& S, FactorPlan& P) {
+void chol_plan(const CholSymbolic& S, FactorPlan& P, double l21_right_max_tiles) {
     const int nsup = S.nsuper;
     P = FactorPlan();
     std::vector<int> ns(nsup), f(nsup);
@@ -187,10 +187,8 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P) {
         P.launches.back().count++;
     };
     const double T = kFactorTile;
-    std::vector<std::pair<int, size_t>> final_at;   // (TRSM panel, side FINAL launch index)
     for (int l = 0; l < nlev; ++l) {
         const std::vector<int>& L = by_level[l];
-        const size_t level_first = P.launches.size();
         begin(FK_ASSEMBLE, 0);
         for (int s : L) {
             for (int j = 0; j < f[s]; ++j) push(s, j);
@@ -198,18 +196,55 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P) {
         }
         int pmax = 0;
         for (int s : L) pmax = std::max(pmax, tile_count_l11(ns[s]));
+        // Where L21 is formed: by the panel steps (right-looking: each panel's TRSM and updates
+        // also cover the update rows), or by one L21 launch per panel k on the second stream,
+        // queued once the diagonal tile L_kk is factored: each update-row tile t then applies
+        // the updates of panels p < k to F[t, k] in order and solves with L_kk, overlapping the
+        // main stream's remaining panel steps (which touch L11 only).  Both apply the same
+        // products in the same order, so L21 is the same either way.
+        double rl_tiles = 0;
+        for (int s : L) {
+            const double p11 = tile_count_l11(ns[s]);
+            rl_tiles += (tile_count(ns[s], f[s]) - p11) * p11 * (p11 + 1) / 2;
+        }
+        const bool left = rl_tiles > l21_right_max_tiles * pmax;
+        auto l21_launch = [&](int k) {   // after L_kk and the L11 tiles (k, p < k) are final
+            if (!left) return;
+            begin(FK_L21, k);
+            P.launches.back().side = true;
+            for (int s : L) {
+                const int p11 = tile_count_l11(ns[s]);
+                if (k >= p11) continue;
+                for (int t = p11; t < tile_count(ns[s], f[s]); ++t) push(s, t);
+                P.flops += (tile_count(ns[s], f[s]) - p11) * (k + 1) * 2 * T * T * T;
+            }
+        };
+        // the FINAL launch of the supernodes with p panels: after TRSM(p - 1), their last panel
+        // step; on the second stream (after their L21 launches) unless it is the level's last
+        // and L21 comes from the panel steps
+        auto final_launch = [&](int p) {
+            std::vector<int> grp;
+            for (int s : L)
+                if (tile_count_l11(ns[s]) == p) grp.push_back(s);
+            if (grp.empty()) return;
+            begin(FK_FINAL, 0);
+            P.launches.back().side = left || p < pmax;
+            final_order(grp, ns, f, [&](int s, int y) { push(s, y); });
+        };
         for (int k = 0; k < pmax; ++k) {
             if (k == 0) {   // later panels' diagonal tiles are factored by UPDATE(k - 1)'s task on them
                 begin(FK_POTRF, k);
                 for (int s : L) push(s, 0);
+                l21_launch(0);
             }
             begin(FK_TRSM, k);
             for (int s : L) {
                 if (k >= tile_count_l11(ns[s])) continue;
-                const int nt = tile_count(ns[s], f[s]);
-                for (int t = k + 1; t < nt; ++t) push(s, t);
-                for (int c = 0; c <= k; ++c) push(s, (1 << kTaskKindShift) | c);
+                const int rows = left ? tile_count_l11(ns[s]) : tile_count(ns[s], f[s]);
+                for (int t = k + 1; t < rows; ++t) push(s, t);
+                for (int c = 0; c < k; ++c) push(s, (1 << kTaskKindShift) | c);   // X[k, k]: from POTRF
             }
+            final_launch(k + 1);
             begin(FK_UPDATE, k);
             // first the tasks on the next panels' diagonal tiles (k + 1, k + 1), which go on to
             // factor them: dispatched first, their Cholesky overlaps the other updates
@@ -221,9 +256,9 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P) {
             for (int s : L) {
                 const int p11 = tile_count_l11(ns[s]);
                 if (k >= p11) continue;
-                const int nt = tile_count(ns[s], f[s]);
+                const int rows = left ? p11 : tile_count(ns[s], f[s]);
                 for (int b = k + 1; b < p11; ++b)
-                    for (int a = b; a < nt; ++a) {
+                    for (int a = b; a < rows; ++a) {
                         if (a == k + 1 && b == k + 1) continue;   // pushed above
                         push(s, (a << kTaskTileShift) | b);
                         P.flops += 2 * T * T * T;
@@ -234,46 +269,18 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P) {
                         P.flops += 2 * T * T * T;
                     }
             }
+            if (k + 1 < pmax) l21_launch(k + 1);
         }
         const double T2 = kFinalTile;
         for (int s : L) {
             const int U = final_row_tiles(ns[s], f[s]);
             const int C = final_col_tiles(ns[s]);
-            const double k32 = 32.0 * ((ns[s] + 31) / 32);   // the kernel's k-chunks are 32 deep
+            const double kch = kFinalChunk * ((ns[s] + kFinalChunk - 1) / kFinalChunk);   // the kernel's k-chunks
             for (int u = 0; u < U; ++u)
-                for (int v = 0; v <= u; ++v) P.flops += 2 * T2 * T2 * k32;
+                for (int v = 0; v <= u; ++v) P.flops += 2 * T2 * T2 * kch;
             for (int u = 0; u < U; ++u)
-                for (int c = 0; c < C; ++c) P.flops += 2 * T2 * T2 * (k32 - T2 * c);
+                for (int c = 0; c < C; ++c) P.flops += 2 * T2 * T2 * (kch - T2 * c);
         }
-        // the FINAL launches, by panel count p: those of p < pmax go right after TRSM(p - 1),
-        // the last panel step of their supernodes, on the second stream
-        for (int p = 1; p <= pmax; ++p) {
-            std::vector<int> grp;
-            for (int s : L)
-                if (tile_count_l11(ns[s]) == p) grp.push_back(s);
-            if (grp.empty()) continue;
-            begin(FK_FINAL, 0);
-            P.launches.back().side = p < pmax;
-            final_order(grp, ns, f, [&](int s, int y) { push(s, y); });
-            if (p < pmax) final_at.push_back({p - 1, P.launches.size() - 1});
-        }
-        // move each side FINAL launch to just after its TRSM(p - 1) launch
-        if (!final_at.empty()) {
-            std::vector<FactorLaunch> head(P.launches.begin(), P.launches.begin() + level_first), rest;
-            std::vector<FactorLaunch> lv(P.launches.begin() + level_first, P.launches.end());
-            std::vector<FactorLaunch> out;
-            std::vector<bool> placed(lv.size(), false);
-            for (size_t i = 0; i < lv.size(); ++i) {
-                if (lv[i].kind == FK_FINAL && lv[i].side) continue;
-                out.push_back(lv[i]);
-                if (lv[i].kind == FK_TRSM)
-                    for (const auto& fa : final_at)
-                        if (fa.first == lv[i].panel) out.push_back(P.launches[fa.second]);
-            }
-            head.insert(head.end(), out.begin(), out.end());
-            P.launches.swap(head);
-        }
-        final_at.clear();
     }
     // drop empty launches (a level whose panel list is empty for a kind)
     std::vector<FactorLaunch> kept;

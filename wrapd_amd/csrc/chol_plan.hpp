@@ -12,11 +12,17 @@
 //                      its own for panel 0 only: the UPDATE task of panel k - 1 on the tile
 //                      (k, k) goes on to factor it
 //        TRSM + GDIAG  L[t, k] = F[t, k] L_kk^{-T} for row tiles t > k, and
-//                      X[k, c] = L_kk^{-1} X[k, c] for column tiles c <= k
+//                      X[k, c] = L_kk^{-1} X[k, c] for column tiles c < k
 //        SYRK + GUPD   F[a, b] -= L[a, k] L[b, k]^T for tiles a >= b, k < b < P (the
 //                      supernode's remaining columns only), and X[i, c] -= L[i, k] X[k, c]
 //                      for L11 row tiles k < i < P, c <= k
-//      after which X = L11^{-1} (the top of G_s);
+//      after which X = L11^{-1} (the top of G_s).  On levels with many update rows the panel
+//      steps cover L11's rows only, and one L21 launch then forms each 64-row tile t of the
+//      update rows left-looking, in place: for k < P, F[t, k] -= L[t, p] L[k, p]^T for
+//      p = 0, 1, .., k - 1 (each product from zero, subtracted in that order), then
+//      L[t, k] = F[t, k] L_kk^{-T}.  These are the right-looking steps' operations in the
+//      right-looking order, so L21 comes out bit for bit the same, with the update rows read
+//      and written once per row tile instead of once per panel step.
 //   3. FINAL launches for the rest, each 128x128 super-tile a single pass over all ns
 //      columns, one per panel count: the supernodes whose panels end early are finished on a
 //      second stream while the level's wider supernodes go on with their panels:
@@ -38,15 +44,17 @@
 
 namespace wrapd {
 
-enum FactorKind { FK_ASSEMBLE = 0, FK_POTRF = 1, FK_TRSM = 2, FK_UPDATE = 3, FK_FINAL = 4 };
+enum FactorKind { FK_ASSEMBLE = 0, FK_POTRF = 1, FK_TRSM = 2, FK_UPDATE = 3, FK_FINAL = 4, FK_L21 = 5 };
 
 // task pair (s, y): FK_ASSEMBLE y = front column j; FK_POTRF y = 0;
 // FK_TRSM y = (kind << 30) | t with kind 0 TRSM (row tile t), 1 GDIAG (column tile t);
 // FK_UPDATE / FK_FINAL y = (kind << 30) | (a << 15) | b with kind 0 SYRK / SCHUR
-// (tiles a, b), 1 GUPD / GBOT (row tile a, column tile b).
+// (tiles a, b), 1 GUPD / GBOT (row tile a, column tile b); FK_L21 y = row tile t >= P.
 constexpr int kTaskKindShift = 30;
 constexpr int kTaskTileShift = 15;
 constexpr int kTaskTileMask = (1 << 15) - 1;
+// the kind field of a task word (read unsigned: a kind of 2 or more sets the sign bit)
+constexpr int task_kind(int y) { return static_cast<int>(static_cast<unsigned>(y) >> kTaskKindShift); }
 
 struct FactorLaunch {
     int kind;
@@ -54,10 +62,10 @@ struct FactorLaunch {
     int64_t offset;     // first task (int pairs) in tasks
     int count;          // tasks
     int maxf = 0;       // FK_ASSEMBLE: the largest front of the launch (its LDS column buffer)
-    // FK_FINAL of supernodes whose panels end before the level's last panel step: run on a
-    // second stream once the main stream has issued the launches before this one, so that
-    // it overlaps the remaining panel steps of the level's wider supernodes; the main
-    // stream waits for it before the next level's assembly
+    // FK_L21 / FK_FINAL of supernodes whose panels end before the level's last panel step:
+    // run on a second stream once the main stream has issued the launches before them, so
+    // that they overlap the remaining panel steps of the level's wider supernodes; the main
+    // stream waits for them before the next level's assembly
     bool side = false;
 };
 
@@ -81,12 +89,18 @@ constexpr int kFactorTile = 64;
 // SCHUR (u, v), u >= v: rows ns + 128 u.., columns ns + 128 v.. of the update block;
 // GBOT (u, c): rows ns + 128 u.. of G's bottom, columns 128 c.. (< ns).
 constexpr int kFinalTile = 128;
+constexpr int kFinalChunk = 16;   // the FINAL kernel's k-chunk
+// a level whose right-looking panel steps would add at most this many update-row tiles per
+// step on average forms L21 in the panel steps (it is bound by the diagonal tiles' chain)
+constexpr double kL21RightMaxTiles = 256;
 inline int final_row_tiles(int ns, int f) { return (f - ns + kFinalTile - 1) / kFinalTile; }
 inline int final_col_tiles(int ns) { return (ns + kFinalTile - 1) / kFinalTile; }
 
 inline int tile_count_l11(int ns) { return (ns + kFactorTile - 1) / kFactorTile; }
 inline int tile_count(int ns, int f) { return tile_count_l11(ns) + (f - ns + kFactorTile - 1) / kFactorTile; }
 
-void chol_plan(const CholSymbolic& S, FactorPlan& P);
+// l21_right_max_tiles: the threshold above (a test can force either placement of L21 with 0
+// or a huge value)
+void chol_plan(const CholSymbolic& S, FactorPlan& P, double l21_right_max_tiles = kL21RightMaxTiles);
 
 }  // namespace wrapd

@@ -269,6 +269,10 @@ int wrapd_create(wrapd_ctx** out, int device) {
         const int v = std::atoi(e);
         if (v >= 0 && v <= c->asm_lds_max) c->asm_lds_max = v;
     }
+    if (const char* e = std::getenv("WRAPD_L21_RIGHT_MAX")) {
+        const double v = std::atof(e);
+        if (v >= 0.0) c->l21_right_max = v;
+    }
     if (const char* e = std::getenv("WRAPD_LEARNED_GATE")) {
         const double v = std::atof(e);
         if (v > 0.0) c->learned_gate = v;

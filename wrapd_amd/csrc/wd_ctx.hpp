@@ -207,6 +207,9 @@ struct wrapd_ctx {
     // front assembly builds a column in LDS up to this many rows (64 KiB), larger fronts in
     // memory (WRAPD_ASM_LDS_MAX in the environment at wrapd_create lowers it, for tests)
     int asm_lds_max = 8192;
+    // factorization plan: where L21 is formed (wrapd::kL21RightMaxTiles; WRAPD_L21_RIGHT_MAX
+    // in the environment at wrapd_create overrides it, so tests can force either placement)
+    double l21_right_max = wrapd::kL21RightMaxTiles;
     // the learned warm start runs the network for elements whose previous prox solution has
     // a relative optimality residual above this (WRAPD_LEARNED_GATE at wrapd_create).  At
     // 1e-3 the network ran for most 16-element blocks in steady state and cost more than the

@@ -4,22 +4,26 @@
 // src/GlobalStepBehavior.hpp:254-258).
 //
 // The schedule (levels, 64x64 tiles, task lists, pool offsets) comes from chol_plan on
-// the host.  Four kernels, one launch per (level, step):
+// the host.  Five kernels, one launch per (level, step):
 //   k_fassemble  one workgroup per front column, built in LDS: A entries, then children's
 //                Schur-complement columns in child order (deterministic, no atomics), stored once
 //   k_fpotrf     one workgroup per supernode: Cholesky L_kk of the panel's diagonal tile
 //                and its inverse in LDS; L_kk goes to the tile's lower half, L_kk^{-1}
 //                to G's diagonal tile (k, k), which is also the diagonal tile of L11^{-1}
-//   k_ftrsm      TRSM tiles F[t, k] <- F[t, k] L_kk^{-T}, or GDIAG tiles
+//   k_ftrsm      TRSM tiles F[t, k] <- F[t, k] L_kk^{-T} of L11, or GDIAG tiles
 //                X[k, c] <- L_kk^{-1} X[k, c], both as products with the stored inverse
-//   k_fupdate    SYRK/GEMM updates of the supernode's own later columns, or GEMM
-//                updates of X = L11^{-1}; the task updating the next panel's diagonal
-//                tile then factors it (k_fpotrf's work), so only panel 0 has a POTRF launch
-//   k_ffinal     the Schur complement F[a, b] -= L21 L21^T and G's bottom -L21 L11^{-1},
-//                each 128x128 output super-tile accumulated over all ns columns in registers
-// Every product runs on the matrix cores in 64x64 output tiles (v_mfma_f64_16x16x4:
-// each of the 4 waves owns a 32x32 quarter as 2x2 16x16 accumulators) over 64-deep
-// operand chunks staged in two 32 KB LDS tiles; the accumulator layout puts
+//   k_fupdate    SYRK/GEMM updates of L11's later columns, or GEMM updates of X = L11^{-1};
+//                the task updating the next panel's diagonal tile then factors it
+//                (k_fpotrf's work), so only panel 0 has a POTRF launch
+//   k_ffinal     the off-diagonal rows once X is complete: L21 = A21 X^T (into the front's
+//                free upper triangle), then the Schur complement F[a, b] -= L21 L21^T and
+//                G's bottom -L21 X, each 128x128 output super-tile accumulated over its
+//                whole k range in registers
+// The panel steps touch only the supernode's diagonal block (ns x ns), so the m update rows
+// are read and written once per pass instead of once per 64-column panel.
+// Every panel-step product runs on the matrix cores in 64x64 output tiles
+// (v_mfma_f64_16x16x4: each of the 4 waves owns a 32x32 quarter as 2x2 16x16 accumulators)
+// over 64-deep operand chunks staged in two 32 KB LDS tiles; the accumulator layout puts
 // consecutive lanes on consecutive rows, so tile stores are coalesced.  Tiles are loaded
 // with every thread's 16 loads in flight at once (tile_load).  All FP64.
 #include "wd_ctx.hpp"
@@ -445,7 +449,7 @@ k_ftrsm(const int2* __restrict__ tasks, FactArgs A, int k) {
     __shared__ double Bs[kT * kT];   // Bs[q * 64 + j]
     const int2 task = tasks[blockIdx.x];
     const int s = task.x;
-    const int kind = task.y >> wrapd::kTaskKindShift;
+    const int kind = wrapd::task_kind(task.y);
     const int t = task.y & ((1 << wrapd::kTaskKindShift) - 1);
     const int f = A.fsize[s];
     const int ns = A.col0[s + 1] - A.col0[s];
@@ -488,7 +492,7 @@ k_fupdate(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ f
     __shared__ double Bs[kT * kT];   // Bs[kk * 64 + j]
     const int2 task = tasks[blockIdx.x];
     const int s = task.x;
-    const int kind = task.y >> wrapd::kTaskKindShift;
+    const int kind = wrapd::task_kind(task.y);
     const int a = (task.y >> wrapd::kTaskTileShift) & wrapd::kTaskTileMask;
     const int b = task.y & wrapd::kTaskTileMask;
     const int f = A.fsize[s];
@@ -554,42 +558,152 @@ k_fupdate(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ f
 }
 
 // ---------------------------------------------------------------------------------
+// L21 at panel k (FK_L21): one workgroup per 64-row tile t of the update rows, in place,
+//   F[t, k] -= L[t, p] L[k, p]^T   for p = 0, 1, .., k - 1, each product 64 deep from zero
+//   L[t, k]  = F[t, k] L_kk^{-T}   (the product with the diagonal tile's stored inverse)
+// When the panel steps cover the update rows, k_fupdate and k_ftrsm apply exactly these
+// tile products (tile_mma) in this order, so L21 is the same to the last bit either way; here
+// F[t, k] stays in registers between the products, so each update-row tile is written once
+// instead of once per panel.  The next product's tiles are loaded while the current one runs
+// on the matrix cores.
+__global__ void __launch_bounds__(kFactThreads)
+k_fl21(const int2* __restrict__ tasks, FactArgs A, int k) {
+    __shared__ double As[kT * kT];   // As[q * 64 + i]
+    __shared__ double Bs[kT * kT];   // Bs[q * 64 + j]
+    const int2 task = tasks[blockIdx.x];
+    const int s = task.x;
+    const int f = A.fsize[s];
+    const int ns = A.col0[s + 1] - A.col0[s];
+    int a0, na;
+    front_tile(task.y, ns, f, a0, na);
+    double* F = A.pool + A.foff[s];
+    const double* Gs = A.G + A.goff[s];
+    const int lg = wrapd::g_ld(f);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int j0 = 32 * (w >> 1), i0 = 32 * (w & 1);
+    const int c0 = kT * k;
+    const int nbk = min(kT, ns - c0);
+    // the operand tiles of product p: L[t, p] and L[k, p]
+    double va[16], vb[16];
+    auto load_upd = [&](int p) {
+        tile_load(F + static_cast<int64_t>(kT * p) * f + a0, f, na, kT, va);
+        tile_load(F + static_cast<int64_t>(kT * p) * f + c0, f, nbk, kT, vb);
+    };
+    if (k > 0) load_upd(0);
+    // run = F[t, k] in the accumulator layout (output (i, j) = (i0 + 16 b + l % 16,
+    // j0 + 16 a + 4 q + l / 16)); positions outside the tile hold zero
+    dbl4 run[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = i0 + 16 * y + (l & 15), j = j0 + 16 * x + 4 * q + (l >> 4);
+                const double v = F[static_cast<int64_t>(c0 + (j < nbk ? j : 0)) * f + a0 + (i < na ? i : 0)];
+                run[x][y][q] = (i < na && j < nbk) ? v : 0.0;
+            }
+    for (int p = 0; p < k; ++p) {   // panels p < k < P are a full 64 columns
+        if (p > 0) __syncthreads();   // the previous product has read As and Bs
+        tile_to_lds(As, va);
+        tile_to_lds(Bs, vb);
+        __syncthreads();
+        if (p + 1 < k) load_upd(p + 1);
+        dbl4 acc[2][2];
+        tile_zero(acc);
+        tile_mma(As, Bs, kT, acc);
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) run[x][y][q] = run[x][y][q] - acc[x][y][q];
+    }
+    // L[t, k] = run L_kk^{-T}: run becomes the A operand (As[j * 64 + i] = run(i, j))
+    tile_load(Gs + static_cast<int64_t>(c0) * lg + c0, lg, nbk, nbk, vb);
+    if (k > 0) __syncthreads();
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = i0 + 16 * y + (l & 15), j = j0 + 16 * x + 4 * q + (l >> 4);
+                As[j * kT + i] = run[x][y][q];
+            }
+    tile_to_lds(Bs, vb);
+    __syncthreads();
+    dbl4 acc[2][2];
+    tile_zero(acc);
+    tile_mma(As, Bs, (nbk + 3) & ~3, acc);
+    tile_write<false>(F + static_cast<int64_t>(c0) * f + a0, f, na, nbk, false, acc, 1.0);
+}
+
+// ---------------------------------------------------------------------------------
 // SCHUR (kind 0): F[a, b] -= sum_q L[a, q] L[b, q]^T over all ns columns, update super-tiles
 // a >= b;  GBOT (kind 1): G[a, c] = -sum_{q >= c} L[a, q] X[q, c], X = L11^{-1}.
 // 128x128 output super-tiles: each wave owns a 64x64 quarter as 4x4 16x16 accumulators, so
 // every operand value staged in LDS feeds twice as many products as with 64x64 tiles.  The
-// k range is walked in 32-deep chunks, chunk q+1 fetched into registers while chunk q runs
-// on the matrix cores.  B is staged column by column (Bs[j * 33 + k]): the padding keeps the
-// transposed stores of G's columns and the matrix-core reads free of bank conflicts.
-constexpr int kFT = wrapd::kFinalTile;   // 128
-constexpr int kFK = 32;                  // k-chunk
-constexpr int kBsLd = kFK + 1;
+// k range is walked in kFK-deep chunks, chunk q+1 fetched into registers while chunk q runs
+// on the matrix cores.  Both operands are staged index-major (As[i * kLd + k],
+// Bs[j * kLd + k]): the padding keeps the stores of either load orientation and the
+// matrix-core reads free of bank conflicts.  Two workgroups fit a compute unit (registers
+// and LDS), so one's loads and barriers overlap the other's products.
+constexpr int kFT = wrapd::kFinalTile;     // 128
+constexpr int kFK = wrapd::kFinalChunk;    // k-chunk
+constexpr int kLd = kFK + 1;               // LDS operand row length (padded)
 
-// R x C block (R * C = 4096) of a column-major source into registers: thread t's entry r
-// is (row t % R, column t / R + (256 / R) r).  Rows >= nr and columns >= nc read as zero;
-// addresses are clamped into the block (nr, nc >= 1), so the 16 loads issue together.
+// Operand loads go through buffer resources (a 32-bit byte offset per lane, a uniform base
+// in scalar registers): addresses cost no 64-bit registers, and an offset past the block's
+// end reads as zero, so no load needs a clamped address.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t blk_rsrc(const double* base, int64_t n_doubles) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), static_cast<short>(0),
+                                             static_cast<int>(n_doubles * 8), 0x00020000);
+}
+__device__ __forceinline__ double blk_ld(__amdgpu_buffer_rsrc_t r, uint32_t elem) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, elem * 8u, 0, 0));
+}
+
+// R x C block (R * C = 256 * kNv) of a column-major source (element offset off0, leading
+// dimension ld) into registers: thread t's entry r is (row t % R, column t / R + (256 / R) r).
+// The k dimension (rows for R = kFK, columns for R = 128) is zero from nk on; entries past
+// the other dimension's end hold whatever the block holds there (they only reach outputs that
+// are not stored) or zero past the block's end.
+constexpr int kNv = kFT * kFK / kFactThreads;   // entries per thread
 template <int R>
-__device__ __forceinline__ void blk_load(const double* __restrict__ src, int64_t ld, int nr, int nc, double v[16]) {
+__device__ __forceinline__ void blk_load(__amdgpu_buffer_rsrc_t rs, uint32_t off0, uint32_t ld, int nk, double v[kNv]) {
     constexpr int CS = kFactThreads / R;
     const int row = threadIdx.x % R, col0 = threadIdx.x / R;
-    const bool rok = row < nr;
-    const double* p = src + (rok ? row : 0);
+    const uint32_t o = off0 + row + col0 * ld;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int col = col0 + CS * r;
-        const bool ok = rok && col < nc;
-        const double x = p[static_cast<int64_t>(col < nc ? col : 0) * ld];
+    for (int r = 0; r < kNv; ++r) {
+        const double x = blk_ld(rs, o + static_cast<uint32_t>(CS * r) * ld);
+        const bool ok = R == kFK ? row < nk : col0 + CS * r < nk;
         v[r] = ok ? x : 0.0;
     }
 }
 
-__global__ void __launch_bounds__(kFactThreads)
+// the registers of blk_load<R> into an index-major LDS operand T[idx * kLd + k]: with R =
+// 128 the block's rows are the index (entry r: idx t % 128, k t / 128 + 2 r), with R = kFK
+// its rows are k (entry r: k t % kFK, idx t / kFK + (256 / kFK) r)
+template <int R>
+__device__ __forceinline__ void blk_to_lds(double* T, const double v[kNv]) {
+    const int t = threadIdx.x;
+    constexpr int CS = kFactThreads / R;
+#pragma unroll
+    for (int r = 0; r < kNv; ++r) {
+        if (R == kFT) T[(t % kFT) * kLd + t / kFT + CS * r] = v[r];
+        else T[(t / kFK + CS * r) * kLd + t % kFK] = v[r];
+    }
+}
+
+__global__ void __launch_bounds__(kFactThreads, 2)
 k_ffinal(const int2* __restrict__ tasks, FactArgs A) {
-    __shared__ double As[kFK * kFT];      // As[k * 128 + i]
-    __shared__ double Bs[kFT * kBsLd];    // Bs[j * 33 + k]
+    __shared__ double As[kFT * kLd];    // As[i * kLd + k]
+    __shared__ double Bs[kFT * kLd];    // Bs[j * kLd + k]
     const int2 task = tasks[blockIdx.x];
     const int s = task.x;
-    const int kind = task.y >> wrapd::kTaskKindShift;
+    const int kind = wrapd::task_kind(task.y);
     const int u = (task.y >> wrapd::kTaskTileShift) & wrapd::kTaskTileMask;
     const int v = task.y & wrapd::kTaskTileMask;
     const int f = A.fsize[s];
@@ -600,37 +714,34 @@ k_ffinal(const int2* __restrict__ tasks, FactArgs A) {
     double* F = A.pool + A.foff[s];
     double* Gs = A.G + A.goff[s];
     const int lg = wrapd::g_ld(f);
+    // the front (f x f) and G_s (lg x ns) as buffers
+    const __amdgpu_buffer_rsrc_t rF = blk_rsrc(F, static_cast<int64_t>(f) * f);
+    const __amdgpu_buffer_rsrc_t rG = blk_rsrc(Gs, static_cast<int64_t>(lg) * ns);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int li = l & 15, lk = l >> 4;
     const int i0 = 64 * (w & 1), j0 = 64 * (w >> 1);
     // a diagonal SCHUR super-tile keeps its lower half: the quarter above the diagonal is idle
     const bool idle = kind == 0 && u == v && i0 < j0;
-    double ra[16], rb[16];
+    const int qfirst = kind == 0 ? 0 : b0;   // X[q, c] = 0 for q < c
+    double ra[kNv], rb[kNv];
     auto fetch = [&](int q0) {
         const int nq = min(kFK, ns - q0);
-        blk_load<kFT>(F + static_cast<int64_t>(q0) * f + a0, f, na, nq, ra);
-        if (kind == 0) blk_load<kFT>(F + static_cast<int64_t>(q0) * f + b0, f, nbt, nq, rb);
-        else blk_load<kFK>(Gs + static_cast<int64_t>(b0) * lg + q0, lg, nq, nbt, rb);
+        blk_load<kFT>(rF, q0 * f + a0, f, nq, ra);                      // L[a0 + i, q0 + k]
+        if (kind == 0) blk_load<kFT>(rF, q0 * f + b0, f, nq, rb);       // L[b0 + j, q0 + k]
+        else blk_load<kFK>(rG, b0 * lg + q0, lg, nq, rb);               // X[q0 + k, b0 + j]
     };
     dbl4 acc[4][4];
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int y = 0; y < 4; ++y) acc[x][y] = dbl4{0.0, 0.0, 0.0, 0.0};
-    const int qfirst = kind == 0 ? 0 : b0;   // X[q, c] = 0 for q < c
     fetch(qfirst);
     for (int q0 = qfirst; q0 < ns; q0 += kFK) {
         const int nq = min(kFK, ns - q0);
         __syncthreads();   // the previous chunk has been consumed
-#pragma unroll
-        for (int r = 0; r < 16; ++r) As[threadIdx.x + kFactThreads * r] = ra[r];
-        if (kind == 0) {   // rb entry r: (row j = t % 128, column k = t / 128 + 2 r)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) Bs[(threadIdx.x & 127) * kBsLd + (threadIdx.x >> 7) + 2 * r] = rb[r];
-        } else {           // rb entry r: (row k = t % 32, column j = t / 32 + 8 r)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) Bs[((threadIdx.x >> 5) + 8 * r) * kBsLd + (threadIdx.x & 31)] = rb[r];
-        }
+        blk_to_lds<kFT>(As, ra);
+        if (kind == 0) blk_to_lds<kFT>(Bs, rb);
+        else blk_to_lds<kFK>(Bs, rb);
         __syncthreads();
         if (q0 + kFK < ns) fetch(q0 + kFK);
         if (!idle) {
@@ -640,8 +751,8 @@ k_ffinal(const int2* __restrict__ tasks, FactArgs A) {
                 double bj[4], ai[4];
 #pragma unroll
                 for (int x = 0; x < 4; ++x) {
-                    bj[x] = Bs[(j0 + 16 * x + li) * kBsLd + k];
-                    ai[x] = As[k * kFT + i0 + 16 * x + li];
+                    bj[x] = Bs[(j0 + 16 * x + li) * kLd + k];
+                    ai[x] = As[(i0 + 16 * x + li) * kLd + k];
                 }
 #pragma unroll
                 for (int x = 0; x < 4; ++x)
@@ -694,7 +805,7 @@ static int up_vec(wrapd_ctx* c, T*& dptr, const std::vector<T>& h) {
 int factor_prepare_device(wrapd_ctx* c) {
     const wrapd::CholSymbolic& S = c->sym;
     try {
-        wrapd::chol_plan(S, c->fplan);
+        wrapd::chol_plan(S, c->fplan, c->l21_right_max);
     } catch (const std::exception& ex) {
         c->last_error = ex.what();
         return WRAPD_ERR_STATE;
@@ -743,7 +854,10 @@ void launch_factor(wrapd_ctx* c) {
         if (L.side) {
             (void)hipEventRecord(c->fev_fork, c->stream);
             (void)hipStreamWaitEvent(c->fstream2, c->fev_fork, 0);
-            hipLaunchKernelGGL(k_ffinal, dim3(L.count), dim3(kFactThreads), 0, c->fstream2, t, A);
+            if (L.kind == wrapd::FK_L21)
+                hipLaunchKernelGGL(k_fl21, dim3(L.count), dim3(kFactThreads), 0, c->fstream2, t, A, L.panel);
+            else
+                hipLaunchKernelGGL(k_ffinal, dim3(L.count), dim3(kFactThreads), 0, c->fstream2, t, A);
             side_busy = true;
             continue;
         }
@@ -769,6 +883,9 @@ void launch_factor(wrapd_ctx* c) {
             break;
         case wrapd::FK_UPDATE:
             hipLaunchKernelGGL(k_fupdate, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A, L.panel, c->d_ffail);
+            break;
+        case wrapd::FK_L21:
+            hipLaunchKernelGGL(k_fl21, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A, L.panel);
             break;
         default:
             hipLaunchKernelGGL(k_ffinal, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A);
