@@ -52,7 +52,9 @@ void tile(int t, int ns, int f, int& st, int& sz) {
 }  // namespace
 
 double l21_right_max = kL21RightMaxTiles;
+int asm_chunk = kAsmChunk;
 extern "C" void pe_set_l21_right_max(double v) { l21_right_max = v; }
+extern "C" void pe_set_asm_chunk(int v) { asm_chunk = v; }
 
 extern "C" int pe_analyze(int n, const int* ap, const int* aj, const double* xyz, int leaf_size, double* st) {
     Ap.assign(ap, ap + n + 1);
@@ -60,7 +62,7 @@ extern "C" int pe_analyze(int n, const int* ap, const int* aj, const double* xyz
     SymbolicOptions o;
     o.leaf_size = leaf_size;
     chol_analyze(n, Ap, Aj, xyz, o, S);
-    chol_plan(S, PL, l21_right_max);
+    chol_plan(S, PL, l21_right_max, asm_chunk);
     st[0] = S.nsuper; st[1] = PL.n_levels; st[2] = static_cast<double>(PL.launches.size());
     st[3] = S.max_ns; st[4] = static_cast<double>(S.goff[S.nsuper]); st[5] = static_cast<double>(PL.pool_size);
     st[6] = static_cast<double>(PL.pool_no_reuse);
@@ -79,7 +81,8 @@ extern "C" void pe_layout(int* perm, int* col0, int* fsize, int64_t* goff, int64
     if (rows) std::copy(S.rows.begin(), S.rows.end(), rows);
 }
 
-// returns 0 on success, 1 on a non-positive pivot, 2 on an L21 task outside the update rows;
+// returns 0 on success, 1 on a non-positive pivot, 2 on an L21 task outside the update rows,
+// 3 on a child row outside its assembly chunk;
 // G_out gets G; conflicts_out the count
 extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
     pool.assign(PL.pool_size, std::nan(""));
@@ -129,18 +132,21 @@ extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
                 for (int j = 0; j < nbp; ++j)
                     for (int r = 0; r < nbp; ++r) wrG(Gi(cp + r, cp + j), X[j * 64 + r]);
             };
-            if (L.kind == FK_ASSEMBLE) {
-                const int j = y;
+            if (L.kind == FK_ASSEMBLE) {   // rows [max(j, q C), min(f, (q + 1) C)) of column j
+                const int j = y & 0xFFFF, qc = y >> 16, C = PL.asm_chunk;
+                const int r0 = std::max(j, qc * C), r1 = std::min(f, (qc + 1) * C);
                 const int64_t gc = PL.cbase[s] + j;
-                for (int r = j; r < f; ++r) wrF(Fi(r, j), 0.0);
-                if (j < ns) for (int r = 0; r < ns; ++r) wrG(Gi(r, j), 0.0);   // L11^{-1} rows only
+                for (int r = r0; r < r1; ++r) wrF(Fi(r, j), 0.0);
+                if (j < ns && qc == j / C) for (int r = 0; r < ns; ++r) wrG(Gi(r, j), 0.0);   // L11^{-1} rows only
                 for (int64_t a = PL.acol_ptr[gc]; a < PL.acol_ptr[gc + 1]; ++a)
-                    wrF(Fi(PL.acol[2 * a], j), ax[PL.acol[2 * a + 1]]);
+                    if (PL.acol[2 * a] >= r0 && PL.acol[2 * a] < r1) wrF(Fi(PL.acol[2 * a], j), ax[PL.acol[2 * a + 1]]);
                 for (int64_t a = PL.ccol_ptr[gc]; a < PL.ccol_ptr[gc + 1]; ++a) {
                     const int ch = PL.ccol[2 * a], jj = PL.ccol[2 * a + 1];
-                    const int fc = S.fsize[ch], nsc = S.col0[ch + 1] - S.col0[ch], mc = fc - nsc;
+                    const int fc = S.fsize[ch], nsc = S.col0[ch + 1] - S.col0[ch];
                     const int* rl = &S.rel[S.relptr[ch]];
-                    for (int ii = jj; ii < mc; ++ii) {
+                    const int* cs = &PL.cchunk[PL.cchunk_ptr[ch]];
+                    for (int ii = std::max(jj, cs[qc]); ii < cs[qc + 1]; ++ii) {
+                        if (rl[ii] < r0 || rl[ii] >= r1) return 3;   // the chunk table is wrong
                         const double v = rdF(PL.foff[ch] + static_cast<int64_t>(nsc + jj) * fc + nsc + ii);
                         wrF(Fi(rl[ii], j), rdF(Fi(rl[ii], j)) + v);
                     }

@@ -29,6 +29,7 @@ def emu(tmp_path_factory):
     lib = C.CDLL(out)
     lib.pe_run.restype = C.c_int
     lib.pe_set_l21_right_max.argtypes = [C.c_double]
+    lib.pe_set_asm_chunk.argtypes = [C.c_int]
     return lib
 
 
@@ -46,9 +47,10 @@ def _upper_csr(prob):
     return n, np.cumsum(Ap).astype(np.int32), np.ascontiguousarray(c.astype(np.int32)), np.ascontiguousarray(v)
 
 
-def _run(emu, prob, leaf_size, negate=False, coords=True, l21_right_max=256.0):
+def _run(emu, prob, leaf_size, negate=False, coords=True, l21_right_max=256.0, asm_chunk=512):
     n, Ap, Aj, Ax = _upper_csr(prob)
     emu.pe_set_l21_right_max(C.c_double(l21_right_max))
+    emu.pe_set_asm_chunk(asm_chunk)
     free = np.setdiff1d(np.arange(prob.n_verts), prob.pins)
     st = np.zeros(8)
     xyz = ol.dp(np.ascontiguousarray(prob.rest[free])) if coords else None
@@ -87,16 +89,17 @@ CASES = {
 }
 
 
-# where L21 is formed: the default rule, always by a left-looking L21 launch, always by the
-# right-looking panel steps
-L21_MODES = {"default": 256.0, "left": 0.0, "right": 1e30}
+# where L21 is formed (the default rule, always by the left-looking L21 launches, always by
+# the right-looking panel steps) and rows per assembly task
+L21_MODES = {"default": (256.0, 512), "left": (0.0, 512), "right": (1e30, 512), "chunk64": (256.0, 64)}
 
 
 @pytest.mark.parametrize("mode", sorted(L21_MODES))
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_plan_matches_dense_cholesky(emu, name, mode):
     make, leaf = CASES[name]
-    R = _run(emu, make(), leaf, coords=not name.startswith("graph_order"), l21_right_max=L21_MODES[mode])
+    l21, chunk = L21_MODES[mode]
+    R = _run(emu, make(), leaf, coords=not name.startswith("graph_order"), l21_right_max=l21, asm_chunk=chunk)
     assert R["fail"] == 0
     assert R["conflicts"] == 0
     if name == "two_panel_separators":

@@ -75,23 +75,24 @@ def test_factor_reports_indefinite_matrix():
     _check_solve(d, 4)
 
 
-def test_assembly_in_lds_and_in_memory_agree(monkeypatch):
-    """Front columns of fronts with 2048 to 8192 rows are assembled in LDS, the others in
-    memory; with WRAPD_ASM_LDS_MAX=0 every front takes the memory path.  Both add A's entries and then the
-    children's columns in the same order, so the factorizations are the same bits."""
+def test_assembly_chunkings_agree(monkeypatch):
+    """Front columns are assembled in chunks of rows (512 by default, WRAPD_ASM_CHUNK sets
+    it); every chunking adds A's entries and then the children's columns in the same order, so
+    the factorizations are the same bits."""
     prob = meshgen.config_problem(3)   # fronts up to ~4000 rows
     b = np.random.default_rng(7).standard_normal((prob.n_verts - len(prob.pins), 3))
     xs = []
-    for lim in (None, "0"):
-        if lim is None:
-            monkeypatch.delenv("WRAPD_ASM_LDS_MAX", raising=False)
+    for chunk in (None, "64", "1024"):
+        if chunk is None:
+            monkeypatch.delenv("WRAPD_ASM_CHUNK", raising=False)
         else:
-            monkeypatch.setenv("WRAPD_ASM_LDS_MAX", lim)
+            monkeypatch.setenv("WRAPD_ASM_CHUNK", chunk)
         d = _device(prob)
         d.refactor()
         xs.append(d.solve3(b))
         d.close()
     assert np.array_equal(xs[0], xs[1])
+    assert np.array_equal(xs[0], xs[2])
 
 
 def test_l21_left_and_right_looking_agree(monkeypatch):

@@ -85,7 +85,7 @@ struct PoolAlloc {
 
 }  // namespace
 
-void chol_plan(const CholSymbolic& S, FactorPlan& P, double l21_right_max_tiles) {
+void chol_plan(const CholSymbolic& S, FactorPlan& P, double l21_right_max_tiles, int asm_chunk) {
     const int nsup = S.nsuper;
     P = FactorPlan();
     std::vector<int> ns(nsup), f(nsup);
@@ -93,7 +93,7 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P, double l21_right_max_tiles)
         ns[s] = S.col0[s + 1] - S.col0[s];
         f[s] = S.fsize[s];
         if (ns[s] <= 0 || f[s] < ns[s]) throw std::runtime_error("chol_plan: bad supernode");
-        if (tile_count(ns[s], f[s]) > kTaskTileMask) throw std::runtime_error("chol_plan: front too large");
+        if (tile_count(ns[s], f[s]) > kTaskTileMask || f[s] >= (1 << 16)) throw std::runtime_error("chol_plan: front too large");
     }
     // levels: leaves 0, parent above every child (parent index > child index)
     std::vector<int> lev(nsup, 0);
@@ -150,6 +150,27 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P, double l21_right_max_tiles)
         acnt[i + 1] += acnt[i];
         ccnt[i + 1] += ccnt[i];
     }
+    // the children's update rows by parent row chunk (rel is increasing along a child's rows)
+    if (asm_chunk < 64 || asm_chunk > 1024) throw std::runtime_error("chol_plan: bad assembly chunk");
+    P.asm_chunk = asm_chunk;
+    P.cchunk_ptr.assign(nsup + 1, 0);
+    for (int c = 0; c < nsup; ++c) {
+        const int p = S.parent[c];
+        P.cchunk_ptr[c + 1] = P.cchunk_ptr[c] + (p >= 0 ? (f[p] + asm_chunk - 1) / asm_chunk + 1 : 0);
+    }
+    P.cchunk.assign(P.cchunk_ptr[nsup], 0);
+    for (int c = 0; c < nsup; ++c) {
+        const int p = S.parent[c];
+        if (p < 0) continue;
+        const int mc = f[c] - ns[c];
+        const int nq = (f[p] + asm_chunk - 1) / asm_chunk;
+        const int* rl = &S.rel[S.relptr[c]];
+        int ii = 0;
+        for (int q = 0; q <= nq; ++q) {
+            while (ii < mc && rl[ii] < static_cast<int64_t>(asm_chunk) * q) ++ii;
+            P.cchunk[P.cchunk_ptr[c] + q] = q == nq ? mc : ii;
+        }
+    }
     P.acol_ptr = acnt;
     P.ccol_ptr = ccnt;
     P.acol.assign(2 * acnt[ncols], 0);
@@ -191,8 +212,9 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P, double l21_right_max_tiles)
         const std::vector<int>& L = by_level[l];
         begin(FK_ASSEMBLE, 0);
         for (int s : L) {
-            for (int j = 0; j < f[s]; ++j) push(s, j);
-            P.launches.back().maxf = std::max(P.launches.back().maxf, f[s]);
+            for (int j = 0; j < f[s]; ++j)
+                for (int q = j / asm_chunk; q * asm_chunk < f[s]; ++q) push(s, j | (q << 16));
+            P.launches.back().rows = std::max(P.launches.back().rows, std::min(f[s], asm_chunk));
         }
         int pmax = 0;
         for (int s : L) pmax = std::max(pmax, tile_count_l11(ns[s]));

@@ -4,9 +4,10 @@
 //
 // The elimination tree is processed level by level (a level holds supernodes whose
 // children all sit at lower levels).  Per level:
-//   1. assemble: one wave per front column zeroes the column, stores the A entries and
-//      adds the children's Schur-complement columns (children in index order, so every
-//      entry is summed in a fixed order, no atomics);
+//   1. assemble: one wave per chunk of rows (the plan's asm_chunk, kAsmChunk by default) of a
+//      front column builds it in LDS from A's entries and the children's Schur-complement
+//      columns (children in index order, so every entry is summed in a fixed order, no
+//      atomics) and stores it once;
 //   2. per 64-column panel k of the supernode columns, batched launches:
 //        POTRF         factor the panel's diagonal tile L_kk (and invert it); a launch of
 //                      its own for panel 0 only: the UPDATE task of panel k - 1 on the tile
@@ -46,7 +47,8 @@ namespace wrapd {
 
 enum FactorKind { FK_ASSEMBLE = 0, FK_POTRF = 1, FK_TRSM = 2, FK_UPDATE = 3, FK_FINAL = 4, FK_L21 = 5 };
 
-// task pair (s, y): FK_ASSEMBLE y = front column j; FK_POTRF y = 0;
+// task pair (s, y): FK_ASSEMBLE y = j | (q << 16), rows [q C, (q + 1) C) (C = asm_chunk) of
+// front column j (from row j on); FK_POTRF y = 0;
 // FK_TRSM y = (kind << 30) | t with kind 0 TRSM (row tile t), 1 GDIAG (column tile t);
 // FK_UPDATE / FK_FINAL y = (kind << 30) | (a << 15) | b with kind 0 SYRK / SCHUR
 // (tiles a, b), 1 GUPD / GBOT (row tile a, column tile b); FK_L21 y = row tile t >= P.
@@ -61,7 +63,7 @@ struct FactorLaunch {
     int panel;          // k (unused for FK_ASSEMBLE)
     int64_t offset;     // first task (int pairs) in tasks
     int count;          // tasks
-    int maxf = 0;       // FK_ASSEMBLE: the largest front of the launch (its LDS column buffer)
+    int rows = 0;       // FK_ASSEMBLE: the most rows a task of the launch builds (its LDS column)
     // FK_L21 / FK_FINAL of supernodes whose panels end before the level's last panel step:
     // run on a second stream once the main stream has issued the launches before them, so
     // that they overlap the remaining panel steps of the level's wider supernodes; the main
@@ -79,12 +81,20 @@ struct FactorPlan {
     std::vector<int> acol;          // pairs (row in front, A slot)
     std::vector<int64_t> ccol_ptr;  // [total columns + 1]
     std::vector<int> ccol;          // pairs (child supernode, child update column jj)
+    int asm_chunk = 0;              // rows per assembly task
+    // per child c, for each row chunk q of its parent's front (q = 0 .. nq, nq = the chunk
+    // count): the first update row ii of c with rel[ii] >= asm_chunk q (entry nq is m_c)
+    std::vector<int64_t> cchunk_ptr;   // [nsuper + 1]
+    std::vector<int> cchunk;
     std::vector<int> tasks;         // pairs, all launches concatenated
     std::vector<FactorLaunch> launches;   // in stream order
     double flops = 0;               // dense flops issued by the tile tasks (incl. G)
 };
 
 constexpr int kFactorTile = 64;
+// rows of a front column per assembly task (C5 refactorization: 1024 rows 63.7 ms, 512 rows
+// 62.1 ms, 256 rows 64.0 ms)
+constexpr int kAsmChunk = 512;
 // FK_FINAL tasks work on 128x128 output super-tiles (twice the operand reuse of 64x64):
 // SCHUR (u, v), u >= v: rows ns + 128 u.., columns ns + 128 v.. of the update block;
 // GBOT (u, c): rows ns + 128 u.. of G's bottom, columns 128 c.. (< ns).
@@ -100,7 +110,8 @@ inline int tile_count_l11(int ns) { return (ns + kFactorTile - 1) / kFactorTile;
 inline int tile_count(int ns, int f) { return tile_count_l11(ns) + (f - ns + kFactorTile - 1) / kFactorTile; }
 
 // l21_right_max_tiles: the threshold above (a test can force either placement of L21 with 0
-// or a huge value)
-void chol_plan(const CholSymbolic& S, FactorPlan& P, double l21_right_max_tiles = kL21RightMaxTiles);
+// or a huge value); asm_chunk: rows per assembly task, 64 .. 1024
+void chol_plan(const CholSymbolic& S, FactorPlan& P, double l21_right_max_tiles = kL21RightMaxTiles,
+               int asm_chunk = kAsmChunk);
 
 }  // namespace wrapd

@@ -29,6 +29,7 @@ wrapd_ctx::~wrapd_ctx() {
     f(d_goff); f(d_uoff); f(d_asm_ptr); f(d_asm_idx); f(d_udest); f(d_leaf_ptr); f(d_leaf_sn); f(d_flow_f); f(d_flow_b);
     f(d_flow_abort); f(d_bcopy); f(bperm);
     f(pool); f(d_foff); f(d_cbase); f(d_acol_ptr); f(d_acol); f(d_ccol_ptr); f(d_ccol); f(d_relptr); f(d_rel);
+    f(d_cchunk_ptr); f(d_cchunk);
     f(d_ftasks); f(d_ffail);
     f(mlp_master); f(mlp_m1); f(mlp_m2); f(mlp_gsum); f(mlp_norm); f(mlp_bf); f(mlp_loss); f(mlp_partial);
     f(mlp_lpart); f(mlp_log);
@@ -265,9 +266,9 @@ int wrapd_create(wrapd_ctx** out, int device) {
         const int v = std::atoi(e);
         if (v >= 1 && v <= kTailMaxBlocks) c->tail_max_blocks = v;
     }
-    if (const char* e = std::getenv("WRAPD_ASM_LDS_MAX")) {
+    if (const char* e = std::getenv("WRAPD_ASM_CHUNK")) {
         const int v = std::atoi(e);
-        if (v >= 0 && v <= c->asm_lds_max) c->asm_lds_max = v;
+        if (v >= 64 && v <= 1024) c->asm_chunk = v;
     }
     if (const char* e = std::getenv("WRAPD_L21_RIGHT_MAX")) {
         const double v = std::atof(e);

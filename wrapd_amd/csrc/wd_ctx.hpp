@@ -150,6 +150,8 @@ struct wrapd_ctx {
     int* d_ccol = nullptr;
     int64_t* d_relptr = nullptr;
     int* d_rel = nullptr;
+    int64_t* d_cchunk_ptr = nullptr;
+    int* d_cchunk = nullptr;
     int* d_ftasks = nullptr;
     int* d_ffail = nullptr;
 
@@ -204,9 +206,9 @@ struct wrapd_ctx {
     // launch (kTailMaxBlocks; WRAPD_TAIL_MAX_BLOCKS in the environment at wrapd_create
     // lowers it, so small meshes can exercise that path in tests)
     int tail_max_blocks = 1024;
-    // front assembly builds a column in LDS up to this many rows (64 KiB), larger fronts in
-    // memory (WRAPD_ASM_LDS_MAX in the environment at wrapd_create lowers it, for tests)
-    int asm_lds_max = 8192;
+    // rows of a front column per assembly task (wrapd::kAsmChunk; WRAPD_ASM_CHUNK in the
+    // environment at wrapd_create sets it, 64 .. 1024, so tests can compare chunkings)
+    int asm_chunk = wrapd::kAsmChunk;
     // factorization plan: where L21 is formed (wrapd::kL21RightMaxTiles; WRAPD_L21_RIGHT_MAX
     // in the environment at wrapd_create overrides it, so tests can force either placement)
     double l21_right_max = wrapd::kL21RightMaxTiles;

@@ -61,89 +61,65 @@ __global__ void k_fill_int(int* __restrict__ p, int n, int v) {
 }
 
 // ---------------------------------------------------------------------------------
-// One workgroup per front column j: the column's lower part [j, f) is built in LDS (A's
-// entries, then each child's Schur-complement column added in child order: deterministic,
-// no atomics) and written to the front once, so the front is neither zeroed in memory nor
-// read back.  Rows < ns of G's column j (L11^{-1}'s) are zeroed for the TRSM/update steps.
-constexpr int kAsmThreads = 256;
-constexpr int kAsmLdsMin = 2048;   // fronts from this size on are assembled in LDS
-__global__ void __launch_bounds__(kAsmThreads)
-k_fassemble(const int2* __restrict__ tasks, FactArgs A, const double* __restrict__ Ax,
-            const int64_t* __restrict__ cbase, const int64_t* __restrict__ acol_ptr, const int2* __restrict__ acol,
-            const int64_t* __restrict__ ccol_ptr, const int2* __restrict__ ccol, const int64_t* __restrict__ relptr,
-            const int* __restrict__ rel) {
-    extern __shared__ double col[];   // rows j .. f-1 of the column
-    const int2 task = tasks[blockIdx.x];
-    const int s = task.x, j = task.y;
-    const int t = threadIdx.x;
-    const int f = A.fsize[s];
-    const int ns = A.col0[s + 1] - A.col0[s];
-    const int len = f - j;
-    for (int r = t; r < len; r += kAsmThreads) col[r] = 0.0;
-    if (j < ns) {   // L11^{-1} rows only: G's bottom rows are assigned whole later
-        double* Gc = A.G + A.goff[s] + static_cast<int64_t>(j) * wrapd::g_ld(f);
-        for (int r = t; r < ns; r += kAsmThreads) Gc[r] = 0.0;
-    }
-    __syncthreads();
-    const int64_t gc = cbase[s] + j;
-    for (int64_t q = acol_ptr[gc] + t; q < acol_ptr[gc + 1]; q += kAsmThreads) {
-        const int2 e = acol[q];
-        col[e.x - j] = Ax[e.y];
-    }
-    const int64_t c1 = ccol_ptr[gc + 1];
-    for (int64_t q = ccol_ptr[gc]; q < c1; ++q) {
-        __syncthreads();   // the previous contribution (or A's entries) is in
-        const int2 e = ccol[q];
-        const int ch = e.x, jj = e.y;
-        const int fc = A.fsize[ch];
-        const int nsc = A.col0[ch + 1] - A.col0[ch];
-        const int mc = fc - nsc;
-        const double* src = A.pool + A.foff[ch] + static_cast<int64_t>(nsc + jj) * fc + nsc;
-        const int* rl = rel + relptr[ch];
-        for (int ii = jj + t; ii < mc; ii += kAsmThreads) col[rl[ii] - j] += src[ii];
-    }
-    __syncthreads();
-    double* F = A.pool + A.foff[s] + static_cast<int64_t>(j) * f + j;
-    for (int r = t; r < len; r += kAsmThreads) F[r] = col[r];
-}
+// One wave per chunk q of front column j: rows [max(j, q C), min(f, (q + 1) C)), C the plan's
+// chunk, are built in LDS (A's entries, then each child's Schur-complement column added in
+// child order: deterministic, no atomics) and written to the front once, so the front is
+// neither zeroed in memory nor read back.  The plan's per-child chunk table gives the child
+// rows that land in the chunk (rel is increasing), so each child value is read once.  The
+// column's first chunk also zeroes rows < ns of G's column j (L11^{-1}'s) for the TRSM and
+// update steps.  A task needs at most C doubles of LDS (the launch sizes it to its largest
+// chunk), so many waves share a compute unit and their global loads overlap.
+struct AsmArgs {
+    const double* Ax;
+    const int64_t* cbase;
+    const int64_t* acol_ptr;
+    const int2* acol;
+    const int64_t* ccol_ptr;
+    const int2* ccol;
+    const int64_t* relptr;
+    const int* rel;
+    const int64_t* cchunk_ptr;
+    const int* cchunk;
+    int chunk;
+};
 
-// The same in memory, one wave per column, zeroing the column and adding into it there: for
-// fronts below kAsmLdsMin rows (measured faster there: the leaf levels hold hundreds of
-// thousands of short columns) and above what a workgroup's LDS holds (64 KiB).
 __global__ void __launch_bounds__(64)
-k_fassemble_mem(const int2* __restrict__ tasks, FactArgs A, const double* __restrict__ Ax,
-                const int64_t* __restrict__ cbase, const int64_t* __restrict__ acol_ptr, const int2* __restrict__ acol,
-                const int64_t* __restrict__ ccol_ptr, const int2* __restrict__ ccol, const int64_t* __restrict__ relptr,
-                const int* __restrict__ rel) {
+k_fassemble(const int2* __restrict__ tasks, FactArgs A, AsmArgs M) {
+    extern __shared__ double col[];   // rows r0 .. r1 - 1 of the column
     const int2 task = tasks[blockIdx.x];
-    const int s = task.x, j = task.y;
+    const int s = task.x, j = task.y & 0xFFFF, q = task.y >> 16;
     const int lane = threadIdx.x;
     const int f = A.fsize[s];
     const int ns = A.col0[s + 1] - A.col0[s];
-    double* F = A.pool + A.foff[s] + static_cast<int64_t>(j) * f;
-    for (int r = j + lane; r < f; r += 64) F[r] = 0.0;
-    if (j < ns) {
+    const int r0 = max(j, q * M.chunk), r1 = min(f, (q + 1) * M.chunk);
+    const int len = r1 - r0;
+    for (int r = lane; r < len; r += 64) col[r] = 0.0;
+    if (j < ns && q == j / M.chunk) {   // L11^{-1}'s rows only: G's bottom rows are assigned whole later
         double* Gc = A.G + A.goff[s] + static_cast<int64_t>(j) * wrapd::g_ld(f);
         for (int r = lane; r < ns; r += 64) Gc[r] = 0.0;
     }
     __syncthreads();
-    const int64_t gc = cbase[s] + j;
-    for (int64_t q = acol_ptr[gc] + lane; q < acol_ptr[gc + 1]; q += 64) {
-        const int2 e = acol[q];
-        F[e.x] = Ax[e.y];
+    const int64_t gc = M.cbase[s] + j;
+    for (int64_t e = M.acol_ptr[gc] + lane; e < M.acol_ptr[gc + 1]; e += 64) {
+        const int2 a = M.acol[e];
+        if (a.x >= r0 && a.x < r1) col[a.x - r0] = M.Ax[a.y];
     }
-    __syncthreads();
-    for (int64_t q = ccol_ptr[gc]; q < ccol_ptr[gc + 1]; ++q) {
-        const int2 e = ccol[q];
+    const int64_t k1 = M.ccol_ptr[gc + 1];
+    for (int64_t k = M.ccol_ptr[gc]; k < k1; ++k) {
+        __syncthreads();   // the previous contribution (or A's entries) is in
+        const int2 e = M.ccol[k];
         const int ch = e.x, jj = e.y;
         const int fc = A.fsize[ch];
         const int nsc = A.col0[ch + 1] - A.col0[ch];
-        const int mc = fc - nsc;
         const double* src = A.pool + A.foff[ch] + static_cast<int64_t>(nsc + jj) * fc + nsc;
-        const int* rl = rel + relptr[ch];
-        for (int ii = jj + lane; ii < mc; ii += 64) F[rl[ii]] += src[ii];
-        __syncthreads();
+        const int* rl = M.rel + M.relptr[ch];
+        const int* cs = M.cchunk + M.cchunk_ptr[ch];
+        const int lo = max(jj, cs[q]), hi = cs[q + 1];
+        for (int ii = lo + lane; ii < hi; ii += 64) col[rl[ii] - r0] += src[ii];
     }
+    __syncthreads();
+    double* F = A.pool + A.foff[s] + static_cast<int64_t>(j) * f + r0;
+    for (int r = lane; r < len; r += 64) F[r] = col[r];
 }
 
 // ---------------------------------------------------------------------------------
@@ -805,7 +781,7 @@ static int up_vec(wrapd_ctx* c, T*& dptr, const std::vector<T>& h) {
 int factor_prepare_device(wrapd_ctx* c) {
     const wrapd::CholSymbolic& S = c->sym;
     try {
-        wrapd::chol_plan(S, c->fplan, c->l21_right_max);
+        wrapd::chol_plan(S, c->fplan, c->l21_right_max, c->asm_chunk);
     } catch (const std::exception& ex) {
         c->last_error = ex.what();
         return WRAPD_ERR_STATE;
@@ -820,6 +796,8 @@ int factor_prepare_device(wrapd_ctx* c) {
     rc |= up_vec(c, c->d_ccol, P.ccol);
     rc |= up_vec(c, c->d_relptr, S.relptr);
     rc |= up_vec(c, c->d_rel, S.rel);
+    rc |= up_vec(c, c->d_cchunk_ptr, P.cchunk_ptr);
+    rc |= up_vec(c, c->d_cchunk, P.cchunk);
     rc |= up_vec(c, c->d_ftasks, P.tasks);
     std::vector<int> zero1(1, 0);
     rc |= up_vec(c, c->d_ffail, zero1);
@@ -839,6 +817,9 @@ int factor_prepare_device(wrapd_ctx* c) {
 void launch_factor(wrapd_ctx* c) {
     const wrapd::FactorPlan& P = c->fplan;
     FactArgs A{c->d_col0, c->d_fsize, c->d_foff, c->d_goff, c->pool, c->G};
+    AsmArgs M{c->Ax, c->d_cbase, c->d_acol_ptr, reinterpret_cast<const int2*>(c->d_acol), c->d_ccol_ptr,
+              reinterpret_cast<const int2*>(c->d_ccol), c->d_relptr, c->d_rel, c->d_cchunk_ptr, c->d_cchunk,
+              P.asm_chunk};
     const int2* tasks = reinterpret_cast<const int2*>(c->d_ftasks);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->d_ffail, 1, 0);
     bool side_busy = false;
@@ -863,17 +844,8 @@ void launch_factor(wrapd_ctx* c) {
         }
         switch (L.kind) {
         case wrapd::FK_ASSEMBLE:
-            // small fronts (the many leaf-level columns): one wave per column in memory is
-            // faster than a workgroup per column; large ones are built in LDS
-            if (L.maxf >= kAsmLdsMin && L.maxf <= c->asm_lds_max)
-                hipLaunchKernelGGL(k_fassemble, dim3(L.count), dim3(kAsmThreads),
-                                   static_cast<size_t>(L.maxf) * sizeof(double), c->stream, t, A, c->Ax, c->d_cbase,
-                                   c->d_acol_ptr, reinterpret_cast<const int2*>(c->d_acol), c->d_ccol_ptr,
-                                   reinterpret_cast<const int2*>(c->d_ccol), c->d_relptr, c->d_rel);
-            else
-                hipLaunchKernelGGL(k_fassemble_mem, dim3(L.count), dim3(64), 0, c->stream, t, A, c->Ax, c->d_cbase,
-                                   c->d_acol_ptr, reinterpret_cast<const int2*>(c->d_acol), c->d_ccol_ptr,
-                                   reinterpret_cast<const int2*>(c->d_ccol), c->d_relptr, c->d_rel);
+            hipLaunchKernelGGL(k_fassemble, dim3(L.count), dim3(64), static_cast<size_t>(L.rows) * sizeof(double),
+                               c->stream, t, A, M);
             break;
         case wrapd::FK_POTRF:
             hipLaunchKernelGGL(k_fpotrf, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A, L.panel, c->d_ffail);
