@@ -51,10 +51,11 @@ void tile(int t, int ns, int f, int& st, int& sz) {
 }
 }  // namespace
 
-double l21_right_max = kL21RightMaxTiles;
-int asm_chunk = kAsmChunk;
-extern "C" void pe_set_l21_right_max(double v) { l21_right_max = v; }
-extern "C" void pe_set_asm_chunk(int v) { asm_chunk = v; }
+PlanOptions popt;
+extern "C" void pe_set_options(double l21_right_max_tiles, int asm_chunk) {
+    popt.l21_right_max_tiles = l21_right_max_tiles;
+    popt.asm_chunk = asm_chunk;
+}
 
 extern "C" int pe_analyze(int n, const int* ap, const int* aj, const double* xyz, int leaf_size, double* st) {
     Ap.assign(ap, ap + n + 1);
@@ -62,12 +63,15 @@ extern "C" int pe_analyze(int n, const int* ap, const int* aj, const double* xyz
     SymbolicOptions o;
     o.leaf_size = leaf_size;
     chol_analyze(n, Ap, Aj, xyz, o, S);
-    chol_plan(S, PL, l21_right_max, asm_chunk);
+    chol_plan(S, PL, popt);
     st[0] = S.nsuper; st[1] = PL.n_levels; st[2] = static_cast<double>(PL.launches.size());
     st[3] = S.max_ns; st[4] = static_cast<double>(S.goff[S.nsuper]); st[5] = static_cast<double>(PL.pool_size);
     st[6] = static_cast<double>(PL.pool_no_reuse);
     st[7] = static_cast<double>(std::count_if(PL.launches.begin(), PL.launches.end(),
                                               [](const FactorLaunch& L) { return L.side; }));
+    st[8] = 0;
+    st[9] = static_cast<double>(std::count_if(PL.launches.begin(), PL.launches.end(),
+                                              [](const FactorLaunch& L) { return L.kind == FK_L21; }));
     return 0;
 }
 
@@ -206,12 +210,12 @@ extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
                             wrF(Fi(t0 + r, c0 + c), rdF(Fi(t0 + r, c0 + c)) - rdF(Fi(t0 + r, c0 + p)) * l);
                     }
                 }
-            } else if (L.kind == FK_FINAL) {   // 128x128 super-tiles (kFinalTile)
-                const int kind = task_kind(y);
+            } else if (L.kind == FK_FINAL) {   // kFinalTile x kFinalTile tiles
+                const int kind = task_kind(y), T = kFinalTile;
                 const int u = (y >> kTaskTileShift) & kTaskTileMask, v = y & kTaskTileMask;
-                const int a0 = ns + kFinalTile * u, na = std::min(kFinalTile, f - a0);
-                const int b0 = kind == 0 ? ns + kFinalTile * v : kFinalTile * v;
-                const int nbt = std::min(kFinalTile, (kind == 0 ? f : ns) - b0);
+                const int a0 = ns + T * u, na = std::min(T, f - a0);
+                const int b0 = kind == 0 ? ns + T * v : T * v;
+                const int nbt = std::min(T, (kind == 0 ? f : ns) - b0);
                 for (int j = 0; j < nbt; ++j)
                     for (int i = (kind == 0 && u == v) ? j : 0; i < na; ++i) {
                         double acc = 0;

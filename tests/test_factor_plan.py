@@ -28,8 +28,7 @@ def emu(tmp_path_factory):
     subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", out] + src, check=True)
     lib = C.CDLL(out)
     lib.pe_run.restype = C.c_int
-    lib.pe_set_l21_right_max.argtypes = [C.c_double]
-    lib.pe_set_asm_chunk.argtypes = [C.c_int]
+    lib.pe_set_options.argtypes = [C.c_double, C.c_int]
     return lib
 
 
@@ -47,12 +46,11 @@ def _upper_csr(prob):
     return n, np.cumsum(Ap).astype(np.int32), np.ascontiguousarray(c.astype(np.int32)), np.ascontiguousarray(v)
 
 
-def _run(emu, prob, leaf_size, negate=False, coords=True, l21_right_max=256.0, asm_chunk=512):
+def _run(emu, prob, leaf_size, negate=False, coords=True, options=(256.0, 512)):
     n, Ap, Aj, Ax = _upper_csr(prob)
-    emu.pe_set_l21_right_max(C.c_double(l21_right_max))
-    emu.pe_set_asm_chunk(asm_chunk)
+    emu.pe_set_options(*options)
     free = np.setdiff1d(np.arange(prob.n_verts), prob.pins)
-    st = np.zeros(8)
+    st = np.zeros(10)
     xyz = ol.dp(np.ascontiguousarray(prob.rest[free])) if coords else None
     emu.pe_analyze(n, ol.ip(Ap), ol.ip(Aj), xyz, leaf_size, ol.dp(st))
     nsup = int(st[0])
@@ -89,17 +87,18 @@ CASES = {
 }
 
 
-# where L21 is formed (the default rule, always by the left-looking L21 launches, always by
-# the right-looking panel steps) and rows per assembly task
-L21_MODES = {"default": (256.0, 512), "left": (0.0, 512), "right": (1e30, 512), "chunk64": (256.0, 64)}
+# the plan's choices (L21 placement threshold, rows per assembly task): the defaults, L21
+# launches on every level, L21 in the panel steps everywhere, 64-row assembly chunks
+MODES = {"default": (256.0, 512), "left": (0.0, 512), "right": (1e30, 512), "chunk64": (256.0, 64)}
 
 
-@pytest.mark.parametrize("mode", sorted(L21_MODES))
+@pytest.mark.parametrize("mode", sorted(MODES))
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_plan_matches_dense_cholesky(emu, name, mode):
     make, leaf = CASES[name]
-    l21, chunk = L21_MODES[mode]
-    R = _run(emu, make(), leaf, coords=not name.startswith("graph_order"), l21_right_max=l21, asm_chunk=chunk)
+    R = _run(emu, make(), leaf, coords=not name.startswith("graph_order"), options=MODES[mode])
+    if mode == "left":
+        assert R["st"][9] > 0   # L21 launches
     assert R["fail"] == 0
     assert R["conflicts"] == 0
     if name == "two_panel_separators":
@@ -129,13 +128,15 @@ def test_plan_matches_dense_cholesky(emu, name, mode):
     assert np.allclose(np.tril(L), np.tril(Ls), atol=1e-12 * np.abs(L).max())
 
 
-def test_l21_left_and_right_looking_same_bits(emu):
-    """The left-looking L21 launch applies the right-looking panel steps' products in their
-    order, so G comes out bit for bit the same whichever forms L21."""
+def test_plan_choices_same_bits(emu):
+    """The left-looking L21 launches apply the right-looking panel steps' products in their
+    order and assembly chunks add the same values in the same order, so G comes out bit for bit
+    the same whichever the plan chooses."""
     make, leaf = CASES["two_panel_separators"]
     prob = make()
-    Gs = [_run(emu, prob, leaf, l21_right_max=v)["G"] for v in (0.0, 1e30)]
-    assert np.array_equal(Gs[0], Gs[1], equal_nan=True)   # (G's row padding is never written)
+    Gs = [_run(emu, prob, leaf, options=MODES[m])["G"] for m in ("left", "right", "chunk64")]
+    for g in Gs[1:]:
+        assert np.array_equal(Gs[0], g, equal_nan=True)   # (G's row padding is never written)
 
 
 def test_plan_pool_reuse_and_indefinite(emu):

@@ -75,41 +75,36 @@ def test_factor_reports_indefinite_matrix():
     _check_solve(d, 4)
 
 
-def test_assembly_chunkings_agree(monkeypatch):
-    """Front columns are assembled in chunks of rows (512 by default, WRAPD_ASM_CHUNK sets
-    it); every chunking adds A's entries and then the children's columns in the same order, so
-    the factorizations are the same bits."""
-    prob = meshgen.config_problem(3)   # fronts up to ~4000 rows
+# the factorization plan's choices, each forced through the environment at context creation
+PLAN_CHOICES = {
+    "l21_launches_everywhere": {"WRAPD_L21_RIGHT_MAX": "0"},
+    "l21_in_panel_steps": {"WRAPD_L21_RIGHT_MAX": "1e30"},
+    "assembly_chunk_64": {"WRAPD_ASM_CHUNK": "64"},
+    "assembly_chunk_1024": {"WRAPD_ASM_CHUNK": "1024"},
+}
+
+
+def test_plan_choices_agree(monkeypatch):
+    """Where L21 is formed (by the panel steps or by per-panel L21 launches) and how many rows
+    an assembly task builds are the plan's choices.  L21 launches apply the panel steps'
+    products in their order and every assembly chunking adds A's entries and then the
+    children's columns in the same order, so every choice gives the same factorization, bit
+    for bit."""
+    prob = meshgen.config_problem(3)   # fronts up to ~4000 rows, levels of both kinds
     b = np.random.default_rng(7).standard_normal((prob.n_verts - len(prob.pins), 3))
-    xs = []
-    for chunk in (None, "64", "1024"):
-        if chunk is None:
-            monkeypatch.delenv("WRAPD_ASM_CHUNK", raising=False)
-        else:
-            monkeypatch.setenv("WRAPD_ASM_CHUNK", chunk)
+    env_names = sorted({k for v in PLAN_CHOICES.values() for k in v})
+
+    def solve(env):
+        for k in env_names:
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
         d = _device(prob)
         d.refactor()
-        xs.append(d.solve3(b))
+        x = d.solve3(b)
         d.close()
-    assert np.array_equal(xs[0], xs[1])
-    assert np.array_equal(xs[0], xs[2])
+        return x
 
-
-def test_l21_left_and_right_looking_agree(monkeypatch):
-    """L21 is formed either by the panel steps (right-looking) or by one left-looking launch
-    per level after them; WRAPD_L21_RIGHT_MAX forces either placement on every level.  Both
-    apply the same tile products in the same order, so the factorizations are the same bits."""
-    prob = meshgen.config_problem(3)
-    b = np.random.default_rng(11).standard_normal((prob.n_verts - len(prob.pins), 3))
-    xs = []
-    for lim in ("0", "1e30", None):
-        if lim is None:
-            monkeypatch.delenv("WRAPD_L21_RIGHT_MAX", raising=False)
-        else:
-            monkeypatch.setenv("WRAPD_L21_RIGHT_MAX", lim)
-        d = _device(prob)
-        d.refactor()
-        xs.append(d.solve3(b))
-        d.close()
-    assert np.array_equal(xs[0], xs[1])
-    assert np.array_equal(xs[0], xs[2])
+    ref = solve({})
+    for name, env in PLAN_CHOICES.items():
+        assert np.array_equal(solve(env), ref), name

@@ -17,6 +17,7 @@ namespace {
 // bound by L2 misses, and the groups unbalance the in-order round-robin dispatch.)
 template <typename Push>
 void final_order(const std::vector<int>& L, const std::vector<int>& ns, const std::vector<int>& f, Push push) {
+    constexpr int T = kFinalTile;
     std::vector<std::pair<int, int>> tk;   // (s, y)
     for (int s : L) {
         const int U = final_row_tiles(ns[s], f[s]);
@@ -28,7 +29,7 @@ void final_order(const std::vector<int>& L, const std::vector<int>& ns, const st
     }
     auto cost = [&](const std::pair<int, int>& x) {
         const int kind = task_kind(x.second);
-        return kind == 0 ? ns[x.first] : ns[x.first] - kFinalTile * (x.second & kTaskTileMask);
+        return kind == 0 ? ns[x.first] : ns[x.first] - T * (x.second & kTaskTileMask);
     };
     std::stable_sort(tk.begin(), tk.end(),
                      [&](const std::pair<int, int>& a, const std::pair<int, int>& b) { return cost(a) > cost(b); });
@@ -85,7 +86,8 @@ struct PoolAlloc {
 
 }  // namespace
 
-void chol_plan(const CholSymbolic& S, FactorPlan& P, double l21_right_max_tiles, int asm_chunk) {
+void chol_plan(const CholSymbolic& S, FactorPlan& P, const PlanOptions& opt) {
+    const int asm_chunk = opt.asm_chunk;
     const int nsup = S.nsuper;
     P = FactorPlan();
     std::vector<int> ns(nsup), f(nsup);
@@ -229,7 +231,7 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P, double l21_right_max_tiles,
             const double p11 = tile_count_l11(ns[s]);
             rl_tiles += (tile_count(ns[s], f[s]) - p11) * p11 * (p11 + 1) / 2;
         }
-        const bool left = rl_tiles > l21_right_max_tiles * pmax;
+        const bool left = rl_tiles > opt.l21_right_max_tiles * pmax;
         auto l21_launch = [&](int k) {   // after L_kk and the L11 tiles (k, p < k) are final
             if (!left) return;
             begin(FK_L21, k);
@@ -252,6 +254,16 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P, double l21_right_max_tiles,
             begin(FK_FINAL, 0);
             P.launches.back().side = left || p < pmax;
             final_order(grp, ns, f, [&](int s, int y) { push(s, y); });
+            constexpr double T = kFinalTile;
+            for (int s : grp) {   // flops issued (the kernel's k-chunks)
+                const int U = final_row_tiles(ns[s], f[s]);
+                const int C = final_col_tiles(ns[s]);
+                const double kch = kFinalChunk * ((ns[s] + kFinalChunk - 1) / kFinalChunk);
+                for (int u = 0; u < U; ++u) {
+                    P.flops += (u + 1) * 2.0 * T * T * kch;
+                    for (int c = 0; c < C; ++c) P.flops += 2.0 * T * T * (kch - T * c);
+                }
+            }
         };
         for (int k = 0; k < pmax; ++k) {
             if (k == 0) {   // later panels' diagonal tiles are factored by UPDATE(k - 1)'s task on them
@@ -292,16 +304,6 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P, double l21_right_max_tiles,
                     }
             }
             if (k + 1 < pmax) l21_launch(k + 1);
-        }
-        const double T2 = kFinalTile;
-        for (int s : L) {
-            const int U = final_row_tiles(ns[s], f[s]);
-            const int C = final_col_tiles(ns[s]);
-            const double kch = kFinalChunk * ((ns[s] + kFinalChunk - 1) / kFinalChunk);   // the kernel's k-chunks
-            for (int u = 0; u < U; ++u)
-                for (int v = 0; v <= u; ++v) P.flops += 2 * T2 * T2 * kch;
-            for (int u = 0; u < U; ++u)
-                for (int c = 0; c < C; ++c) P.flops += 2 * T2 * T2 * (kch - T2 * c);
         }
     }
     // drop empty launches (a level whose panel list is empty for a kind)

@@ -95,10 +95,11 @@ constexpr int kFactorTile = 64;
 // rows of a front column per assembly task (C5 refactorization: 1024 rows 63.7 ms, 512 rows
 // 62.1 ms, 256 rows 64.0 ms)
 constexpr int kAsmChunk = 512;
-// FK_FINAL tasks work on 128x128 output super-tiles (twice the operand reuse of 64x64):
-// SCHUR (u, v), u >= v: rows ns + 128 u.., columns ns + 128 v.. of the update block;
-// GBOT (u, c): rows ns + 128 u.. of G's bottom, columns 128 c.. (< ns).
-constexpr int kFinalTile = 128;
+// FK_FINAL tasks work on T x T output tiles, T = kFinalTile: SCHUR (u, v), u >= v: rows
+// ns + T u.., columns ns + T v.. of the update block; GBOT (u, c): rows ns + T u.. of G's
+// bottom, columns T c.. (< ns).  (128x128 tiles, with twice the operand reuse, measured
+// slower on the C5 refactorization at every workgroup shape tried: 61.2-62.1 vs 58.7 ms.)
+constexpr int kFinalTile = 64;
 constexpr int kFinalChunk = 16;   // the FINAL kernel's k-chunk
 // a level whose right-looking panel steps would add at most this many update-row tiles per
 // step on average forms L21 in the panel steps (it is bound by the diagonal tiles' chain)
@@ -109,9 +110,13 @@ inline int final_col_tiles(int ns) { return (ns + kFinalTile - 1) / kFinalTile; 
 inline int tile_count_l11(int ns) { return (ns + kFactorTile - 1) / kFactorTile; }
 inline int tile_count(int ns, int f) { return tile_count_l11(ns) + (f - ns + kFactorTile - 1) / kFactorTile; }
 
-// l21_right_max_tiles: the threshold above (a test can force either placement of L21 with 0
-// or a huge value); asm_chunk: rows per assembly task, 64 .. 1024
-void chol_plan(const CholSymbolic& S, FactorPlan& P, double l21_right_max_tiles = kL21RightMaxTiles,
-               int asm_chunk = kAsmChunk);
+// The plan's choices (the defaults are the measured best; tests set others to exercise
+// every path, which give the same factor bit for bit)
+struct PlanOptions {
+    double l21_right_max_tiles = kL21RightMaxTiles;   // 0: L21 launches everywhere; huge: never
+    int asm_chunk = kAsmChunk;                         // rows per assembly task, 64 .. 1024
+};
+
+void chol_plan(const CholSymbolic& S, FactorPlan& P, const PlanOptions& opt = PlanOptions());
 
 }  // namespace wrapd

@@ -268,11 +268,12 @@ int wrapd_create(wrapd_ctx** out, int device) {
     }
     if (const char* e = std::getenv("WRAPD_ASM_CHUNK")) {
         const int v = std::atoi(e);
-        if (v >= 64 && v <= 1024) c->asm_chunk = v;
+        if (v >= 64 && v <= 1024) c->plan_opt.asm_chunk = v;
     }
+
     if (const char* e = std::getenv("WRAPD_L21_RIGHT_MAX")) {
         const double v = std::atof(e);
-        if (v >= 0.0) c->l21_right_max = v;
+        if (v >= 0.0) c->plan_opt.l21_right_max_tiles = v;
     }
     if (const char* e = std::getenv("WRAPD_LEARNED_GATE")) {
         const double v = std::atof(e);

@@ -206,12 +206,10 @@ struct wrapd_ctx {
     // launch (kTailMaxBlocks; WRAPD_TAIL_MAX_BLOCKS in the environment at wrapd_create
     // lowers it, so small meshes can exercise that path in tests)
     int tail_max_blocks = 1024;
-    // rows of a front column per assembly task (wrapd::kAsmChunk; WRAPD_ASM_CHUNK in the
-    // environment at wrapd_create sets it, 64 .. 1024, so tests can compare chunkings)
-    int asm_chunk = wrapd::kAsmChunk;
-    // factorization plan: where L21 is formed (wrapd::kL21RightMaxTiles; WRAPD_L21_RIGHT_MAX
-    // in the environment at wrapd_create overrides it, so tests can force either placement)
-    double l21_right_max = wrapd::kL21RightMaxTiles;
+    // the factorization plan's choices: WRAPD_L21_RIGHT_MAX and WRAPD_ASM_CHUNK in the
+    // environment at wrapd_create override the defaults, so tests can exercise every path
+    // (all give the same factor)
+    wrapd::PlanOptions plan_opt;
     // the learned warm start runs the network for elements whose previous prox solution has
     // a relative optimality residual above this (WRAPD_LEARNED_GATE at wrapd_create).  At
     // 1e-3 the network ran for most 16-element blocks in steady state and cost more than the

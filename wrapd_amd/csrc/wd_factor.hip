@@ -616,18 +616,18 @@ k_fl21(const int2* __restrict__ tasks, FactArgs A, int k) {
 }
 
 // ---------------------------------------------------------------------------------
-// SCHUR (kind 0): F[a, b] -= sum_q L[a, q] L[b, q]^T over all ns columns, update super-tiles
-// a >= b;  GBOT (kind 1): G[a, c] = -sum_{q >= c} L[a, q] X[q, c], X = L11^{-1}.
-// 128x128 output super-tiles: each wave owns a 64x64 quarter as 4x4 16x16 accumulators, so
-// every operand value staged in LDS feeds twice as many products as with 64x64 tiles.  The
-// k range is walked in kFK-deep chunks, chunk q+1 fetched into registers while chunk q runs
-// on the matrix cores.  Both operands are staged index-major (As[i * kLd + k],
-// Bs[j * kLd + k]): the padding keeps the stores of either load orientation and the
-// matrix-core reads free of bank conflicts.  Two workgroups fit a compute unit (registers
-// and LDS), so one's loads and barriers overlap the other's products.
-constexpr int kFT = wrapd::kFinalTile;     // 128
+// SCHUR (kind 0): F[a, b] -= sum_q L[a, q] L[b, q]^T over all ns columns, update tiles a >= b;
+// GBOT (kind 1): G[a, c] = -sum_{q >= c} L[a, q] X[q, c], X = L11^{-1}.
+// TILE x TILE output tiles over the whole k range: the NT / 64 waves split the tile into
+// parts of 16x16 accumulators.  The k range is walked in KC-deep chunks, chunk q+1 fetched
+// into registers while chunk q runs on the matrix cores; each output's products are
+// accumulated in k order whatever the tiling, so the tile shape does not change the bits.
+// Both operands are staged index-major (As[i * (KC + 1) + k], Bs[j * (KC + 1) + k]): the
+// padding keeps the stores of either load orientation and the matrix-core reads free of bank
+// conflicts.  At 64x64 with 4 waves a kernel needs 80 registers, so six waves share a SIMD and
+// one's loads and barriers hide behind the others' products (C5: 58.7 ms per refactorization,
+// against 61.2-62.1 ms for 128x128 tiles at two or four waves per SIMD).
 constexpr int kFK = wrapd::kFinalChunk;    // k-chunk
-constexpr int kLd = kFK + 1;               // LDS operand row length (padded)
 
 // Operand loads go through buffer resources (a 32-bit byte offset per lane, a uniform base
 // in scalar registers): addresses cost no 64-bit registers, and an offset past the block's
@@ -640,43 +640,50 @@ __device__ __forceinline__ double blk_ld(__amdgpu_buffer_rsrc_t r, uint32_t elem
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, elem * 8u, 0, 0));
 }
 
-// R x C block (R * C = 256 * kNv) of a column-major source (element offset off0, leading
-// dimension ld) into registers: thread t's entry r is (row t % R, column t / R + (256 / R) r).
-// The k dimension (rows for R = kFK, columns for R = 128) is zero from nk on; entries past
-// the other dimension's end hold whatever the block holds there (they only reach outputs that
-// are not stored) or zero past the block's end.
-constexpr int kNv = kFT * kFK / kFactThreads;   // entries per thread
-template <int R>
-__device__ __forceinline__ void blk_load(__amdgpu_buffer_rsrc_t rs, uint32_t off0, uint32_t ld, int nk, double v[kNv]) {
-    constexpr int CS = kFactThreads / R;
+// R x C block (R * C = NT * NV, NT threads) of a column-major source (element offset off0,
+// leading dimension ld) into registers: thread t's entry r is (row t % R, column t / R +
+// (NT / R) r).
+// The k dimension (rows for R = kFK, columns otherwise) is zero from nk on; entries past the
+// other dimension's end hold whatever the block holds there (they only reach outputs that are
+// not stored) or zero past the block's end.
+template <int R, int KC, int NV, int NT>
+__device__ __forceinline__ void blk_load(__amdgpu_buffer_rsrc_t rs, uint32_t off0, uint32_t ld, int nk, double v[NV]) {
+    constexpr int CS = NT / R;
     const int row = threadIdx.x % R, col0 = threadIdx.x / R;
     const uint32_t o = off0 + row + col0 * ld;
 #pragma unroll
-    for (int r = 0; r < kNv; ++r) {
+    for (int r = 0; r < NV; ++r) {
         const double x = blk_ld(rs, o + static_cast<uint32_t>(CS * r) * ld);
-        const bool ok = R == kFK ? row < nk : col0 + CS * r < nk;
+        const bool ok = R == KC ? row < nk : col0 + CS * r < nk;
         v[r] = ok ? x : 0.0;
     }
 }
 
-// the registers of blk_load<R> into an index-major LDS operand T[idx * kLd + k]: with R =
-// 128 the block's rows are the index (entry r: idx t % 128, k t / 128 + 2 r), with R = kFK
-// its rows are k (entry r: k t % kFK, idx t / kFK + (256 / kFK) r)
-template <int R>
-__device__ __forceinline__ void blk_to_lds(double* T, const double v[kNv]) {
+// the registers of blk_load<R> into an index-major LDS operand T[idx * (KC + 1) + k]: with
+// R = TILE the block's rows are the index (entry r: idx t % TILE, k t / TILE + (NT / TILE) r),
+// with R = KC its rows are k (entry r: k t % KC, idx t / KC + (NT / KC) r)
+template <int R, int TILE, int KC, int NV, int NT>
+__device__ __forceinline__ void blk_to_lds(double* T, const double v[NV]) {
     const int t = threadIdx.x;
-    constexpr int CS = kFactThreads / R;
+    constexpr int CS = NT / R;
+    constexpr int LD = KC + 1;
 #pragma unroll
-    for (int r = 0; r < kNv; ++r) {
-        if (R == kFT) T[(t % kFT) * kLd + t / kFT + CS * r] = v[r];
-        else T[(t / kFK + CS * r) * kLd + t % kFK] = v[r];
+    for (int r = 0; r < NV; ++r) {
+        if (R == TILE) T[(t % TILE) * LD + t / TILE + CS * r] = v[r];
+        else T[(t / KC + CS * r) * LD + t % KC] = v[r];
     }
 }
 
-__global__ void __launch_bounds__(kFactThreads, 2)
+template <int TILE, int KC, int NT>
+__global__ void __launch_bounds__(NT, 2)
 k_ffinal(const int2* __restrict__ tasks, FactArgs A) {
-    __shared__ double As[kFT * kLd];    // As[i * kLd + k]
-    __shared__ double Bs[kFT * kLd];    // Bs[j * kLd + k]
+    constexpr int kLd = KC + 1;          // LDS operand row length (padded)
+    constexpr int NV = TILE * KC / NT;   // operand entries per thread
+    constexpr int WI = 2, WJ = NT / 128; // waves along the output rows i and columns j
+    constexpr int SI = TILE / WI, SJ = TILE / WJ;   // a wave's part of the tile
+    constexpr int NBI = SI / 16, NBJ = SJ / 16;     // its 16x16 blocks
+    __shared__ double As[TILE * kLd];    // As[i * kLd + k]
+    __shared__ double Bs[TILE * kLd];    // Bs[j * kLd + k]
     const int2 task = tasks[blockIdx.x];
     const int s = task.x;
     const int kind = wrapd::task_kind(task.y);
@@ -684,9 +691,9 @@ k_ffinal(const int2* __restrict__ tasks, FactArgs A) {
     const int v = task.y & wrapd::kTaskTileMask;
     const int f = A.fsize[s];
     const int ns = A.col0[s + 1] - A.col0[s];
-    const int a0 = ns + kFT * u, na = min(kFT, f - a0);
-    const int b0 = kind == 0 ? ns + kFT * v : kFT * v;
-    const int nbt = min(kFT, (kind == 0 ? f : ns) - b0);
+    const int a0 = ns + TILE * u, na = min(TILE, f - a0);
+    const int b0 = kind == 0 ? ns + TILE * v : TILE * v;
+    const int nbt = min(TILE, (kind == 0 ? f : ns) - b0);
     double* F = A.pool + A.foff[s];
     double* Gs = A.G + A.goff[s];
     const int lg = wrapd::g_ld(f);
@@ -695,45 +702,45 @@ k_ffinal(const int2* __restrict__ tasks, FactArgs A) {
     const __amdgpu_buffer_rsrc_t rG = blk_rsrc(Gs, static_cast<int64_t>(lg) * ns);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int li = l & 15, lk = l >> 4;
-    const int i0 = 64 * (w & 1), j0 = 64 * (w >> 1);
-    // a diagonal SCHUR super-tile keeps its lower half: the quarter above the diagonal is idle
-    const bool idle = kind == 0 && u == v && i0 < j0;
+    const int i0 = SI * (w % WI), j0 = SJ * (w / WI);
+    // a diagonal SCHUR super-tile keeps its lower half: a wave whose part lies wholly above
+    // the diagonal is idle
+    const bool idle = kind == 0 && u == v && i0 + SI <= j0;
     const int qfirst = kind == 0 ? 0 : b0;   // X[q, c] = 0 for q < c
-    double ra[kNv], rb[kNv];
+    double ra[NV], rb[NV];
     auto fetch = [&](int q0) {
-        const int nq = min(kFK, ns - q0);
-        blk_load<kFT>(rF, q0 * f + a0, f, nq, ra);                      // L[a0 + i, q0 + k]
-        if (kind == 0) blk_load<kFT>(rF, q0 * f + b0, f, nq, rb);       // L[b0 + j, q0 + k]
-        else blk_load<kFK>(rG, b0 * lg + q0, lg, nq, rb);               // X[q0 + k, b0 + j]
+        const int nq = min(KC, ns - q0);
+        blk_load<TILE, KC, NV, NT>(rF, q0 * f + a0, f, nq, ra);                  // L[a0 + i, q0 + k]
+        if (kind == 0) blk_load<TILE, KC, NV, NT>(rF, q0 * f + b0, f, nq, rb);   // L[b0 + j, q0 + k]
+        else blk_load<KC, KC, NV, NT>(rG, b0 * lg + q0, lg, nq, rb);             // X[q0 + k, b0 + j]
     };
-    dbl4 acc[4][4];
+    dbl4 acc[NBJ][NBI];
 #pragma unroll
-    for (int x = 0; x < 4; ++x)
+    for (int x = 0; x < NBJ; ++x)
 #pragma unroll
-        for (int y = 0; y < 4; ++y) acc[x][y] = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int y = 0; y < NBI; ++y) acc[x][y] = dbl4{0.0, 0.0, 0.0, 0.0};
     fetch(qfirst);
-    for (int q0 = qfirst; q0 < ns; q0 += kFK) {
-        const int nq = min(kFK, ns - q0);
+    for (int q0 = qfirst; q0 < ns; q0 += KC) {
+        const int nq = min(KC, ns - q0);
         __syncthreads();   // the previous chunk has been consumed
-        blk_to_lds<kFT>(As, ra);
-        if (kind == 0) blk_to_lds<kFT>(Bs, rb);
-        else blk_to_lds<kFK>(Bs, rb);
+        blk_to_lds<TILE, TILE, KC, NV, NT>(As, ra);
+        if (kind == 0) blk_to_lds<TILE, TILE, KC, NV, NT>(Bs, rb);
+        else blk_to_lds<KC, TILE, KC, NV, NT>(Bs, rb);
         __syncthreads();
-        if (q0 + kFK < ns) fetch(q0 + kFK);
+        if (q0 + KC < ns) fetch(q0 + KC);
         if (!idle) {
             const int nk = (nq + 3) & ~3;
             for (int kk = 0; kk < nk; kk += 4) {
                 const int k = kk + lk;
-                double bj[4], ai[4];
+                double bj[NBJ], ai[NBI];
 #pragma unroll
-                for (int x = 0; x < 4; ++x) {
-                    bj[x] = Bs[(j0 + 16 * x + li) * kLd + k];
-                    ai[x] = As[(i0 + 16 * x + li) * kLd + k];
-                }
+                for (int x = 0; x < NBJ; ++x) bj[x] = Bs[(j0 + 16 * x + li) * kLd + k];
 #pragma unroll
-                for (int x = 0; x < 4; ++x)
+                for (int y = 0; y < NBI; ++y) ai[y] = As[(i0 + 16 * y + li) * kLd + k];
 #pragma unroll
-                    for (int y = 0; y < 4; ++y)
+                for (int x = 0; x < NBJ; ++x)
+#pragma unroll
+                    for (int y = 0; y < NBI; ++y)
                         acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(bj[x], ai[y], acc[x][y], 0, 0, 0);
             }
         }
@@ -744,11 +751,11 @@ k_ffinal(const int2* __restrict__ tasks, FactArgs A) {
     const int64_t ldc = kind == 0 ? f : lg;
     const bool lower = kind == 0 && u == v;
 #pragma unroll
-    for (int x = 0; x < 4; ++x) {
-        double old[16];
+    for (int x = 0; x < NBJ; ++x) {
+        double old[NBI * 4];
         if (kind == 0) {
 #pragma unroll
-            for (int y = 0; y < 4; ++y)
+            for (int y = 0; y < NBI; ++y)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int i = i0 + 16 * y + li, j = j0 + 16 * x + 4 * q + lk;
@@ -756,7 +763,7 @@ k_ffinal(const int2* __restrict__ tasks, FactArgs A) {
                 }
         }
 #pragma unroll
-        for (int y = 0; y < 4; ++y)
+        for (int y = 0; y < NBI; ++y)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int i = i0 + 16 * y + li, j = j0 + 16 * x + 4 * q + lk;
@@ -781,7 +788,7 @@ static int up_vec(wrapd_ctx* c, T*& dptr, const std::vector<T>& h) {
 int factor_prepare_device(wrapd_ctx* c) {
     const wrapd::CholSymbolic& S = c->sym;
     try {
-        wrapd::chol_plan(S, c->fplan, c->l21_right_max, c->asm_chunk);
+        wrapd::chol_plan(S, c->fplan, c->plan_opt);
     } catch (const std::exception& ex) {
         c->last_error = ex.what();
         return WRAPD_ERR_STATE;
@@ -811,6 +818,11 @@ int factor_prepare_device(wrapd_ctx* c) {
     return rc ? WRAPD_ERR_HIP : WRAPD_OK;
 }
 
+static void launch_final(const wrapd::FactorLaunch& L, const int2* t, const FactArgs& A, hipStream_t st) {
+    hipLaunchKernelGGL((k_ffinal<wrapd::kFinalTile, kFK, kFactThreads>), dim3(L.count), dim3(kFactThreads), 0, st, t,
+                       A);
+}
+
 // queues the whole factorization on c->stream (with the side FINAL launches on
 // c->fstream2, forked after the main-stream launch before them and joined before the next
 // level's assembly and at the end); the caller reads c->d_ffail afterwards
@@ -838,7 +850,7 @@ void launch_factor(wrapd_ctx* c) {
             if (L.kind == wrapd::FK_L21)
                 hipLaunchKernelGGL(k_fl21, dim3(L.count), dim3(kFactThreads), 0, c->fstream2, t, A, L.panel);
             else
-                hipLaunchKernelGGL(k_ffinal, dim3(L.count), dim3(kFactThreads), 0, c->fstream2, t, A);
+                launch_final(L, t, A, c->fstream2);
             side_busy = true;
             continue;
         }
@@ -860,7 +872,7 @@ void launch_factor(wrapd_ctx* c) {
             hipLaunchKernelGGL(k_fl21, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A, L.panel);
             break;
         default:
-            hipLaunchKernelGGL(k_ffinal, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A);
+            launch_final(L, t, A, c->stream);
             break;
         }
     }
