@@ -168,6 +168,28 @@ __device__ __forceinline__ void tile_load(const double* __restrict__ src, int64_
         v[r] = ok ? x : 0.0;
     }
 }
+// Operand loads go through buffer resources (a 32-bit byte offset per lane, a uniform base
+// in scalar registers): addresses cost no 64-bit registers, and an offset past the block's
+// end reads as zero, so no load needs a clamped address.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t blk_rsrc(const double* base, int64_t n_doubles) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), static_cast<short>(0),
+                                             static_cast<int>(n_doubles * 8), 0x00020000);
+}
+__device__ __forceinline__ double blk_ld(__amdgpu_buffer_rsrc_t r, uint32_t elem) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, elem * 8u, 0, 0));
+}
+
+// tile_load_h through a buffer resource (rows and columns past nr, nc read as zero)
+__device__ __forceinline__ void tile_ldb_h(__amdgpu_buffer_rsrc_t rs, uint32_t off0, uint32_t ld, int nr, int nc,
+                                           double v[16], int h) {
+    const int row = threadIdx.x & 63, col0 = threadIdx.x >> 6;
+    const uint32_t o = off0 + row + col0 * ld;
+#pragma unroll
+    for (int r = 8 * h; r < 8 * h + 8; ++r) {
+        const double x = blk_ld(rs, o + static_cast<uint32_t>(4 * r) * ld);
+        v[r] = (row < nr && col0 + 4 * r < nc) ? x : 0.0;
+    }
+}
 // the registers of tile_load into an LDS tile, as loaded (T[col * 64 + row]) or transposed
 __device__ __forceinline__ void tile_to_lds(double* T, const double v[16]) {
 #pragma unroll
@@ -177,6 +199,28 @@ __device__ __forceinline__ void tile_to_lds_t(double* T, const double v[16]) {
     const int row = threadIdx.x & 63, col0 = threadIdx.x >> 6;
 #pragma unroll
     for (int r = 0; r < 16; ++r) T[row * 64 + col0 + 4 * r] = v[r];
+}
+
+// Half-depth staging: the k range 32 h .. 32 h + 31 of a 64-deep product into a 32 x 64 LDS
+// operand (T[(k - 32 h) * 64 + i]), so a product takes 32 KB of LDS instead of 64 KB and is
+// run as tile_mma over each half in turn: the same products in the same k order.
+// tile_to_lds_h: the k index is the tile's column (entries r = 8 h .. 8 h + 7 of tile_load);
+// tile_to_lds_t_h: the k index is the tile's row (threads of rows 32 h .. 32 h + 31).
+constexpr int kHalf = kT / 2;
+__device__ __forceinline__ void tile_to_lds_h(double* T, const double v[16], int h) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) T[threadIdx.x + kFactThreads * r] = v[8 * h + r];
+}
+__device__ __forceinline__ void tile_to_lds_t_h(double* T, const double v[16], int h) {
+    const int row = threadIdx.x & 63, col0 = threadIdx.x >> 6;
+    if ((row >> 5) == h) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) T[(row - kHalf * h) * 64 + col0 + 4 * r] = v[r];
+    }
+}
+__device__ __forceinline__ int half_depth(int nk, int h) {   // the half's k count, a multiple of 4
+    const int n = min(kHalf, nk - kHalf * h);
+    return n > 0 ? (n + 3) & ~3 : 0;
 }
 
 // accumulator tile into a column-major block C (leading dimension ldc): C[i][j] -= acc
@@ -421,8 +465,8 @@ k_fpotrf(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ fa
 // both as products with L_kk^{-1}, which k_fpotrf left in G's diagonal tile (k, k)
 __global__ void __launch_bounds__(kFactThreads)
 k_ftrsm(const int2* __restrict__ tasks, FactArgs A, int k) {
-    __shared__ double As[kT * kT];   // As[q * 64 + i]
-    __shared__ double Bs[kT * kT];   // Bs[q * 64 + j]
+    __shared__ double As[kHalf * kT];   // As[(q - 32 h) * 64 + i], half h of the k range
+    __shared__ double Bs[kHalf * kT];   // Bs[(q - 32 h) * 64 + j]
     const int2 task = tasks[blockIdx.x];
     const int s = task.x;
     const int kind = wrapd::task_kind(task.y);
@@ -443,19 +487,23 @@ k_ftrsm(const int2* __restrict__ tasks, FactArgs A, int k) {
         // C[i][j] = sum_q B[i][q] Linv[j][q]: As[q][i] = B[i][q], Bs[q][j] = Linv[j][q]
         tile_load(F + static_cast<int64_t>(c0) * f + t0, f, nt, nb, va);
         tile_load(Linv, lg, nb, nb, vb);
-        tile_to_lds(As, va);
-        tile_to_lds(Bs, vb);
     } else {
         // C[i][j] = sum_q Linv[i][q] X[q][j], X = G rows [c0, c0+nb), columns [t0, t0+nt)
         tile_load(Linv, lg, nb, nb, va);
         tile_load(Gs + static_cast<int64_t>(t0) * lg + c0, lg, nb, nt, vb);
-        tile_to_lds(As, va);
-        tile_to_lds_t(Bs, vb);
     }
-    __syncthreads();
     dbl4 acc[2][2];
     tile_zero(acc);
-    tile_mma(As, Bs, (nb + 3) & ~3, acc);
+    for (int h = 0; h < 2; ++h) {
+        const int nh = half_depth(nb, h);
+        if (nh == 0) break;
+        if (h) __syncthreads();   // the first half has been read
+        tile_to_lds_h(As, va, h);
+        if (kind == 0) tile_to_lds_h(Bs, vb, h);
+        else tile_to_lds_t_h(Bs, vb, h);
+        __syncthreads();
+        tile_mma(As, Bs, nh, acc);
+    }
     if (kind == 0) tile_write<false>(F + static_cast<int64_t>(c0) * f + t0, f, nt, nb, false, acc, 1.0);
     else tile_write<false>(Gs + static_cast<int64_t>(t0) * lg + c0, lg, nb, nt, false, acc, 1.0);
 }
@@ -544,8 +592,8 @@ k_fupdate(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ f
 // on the matrix cores.
 __global__ void __launch_bounds__(kFactThreads)
 k_fl21(const int2* __restrict__ tasks, FactArgs A, int k) {
-    __shared__ double As[kT * kT];   // As[q * 64 + i]
-    __shared__ double Bs[kT * kT];   // Bs[q * 64 + j]
+    __shared__ double As[kHalf * kT];   // As[(q - 32 h) * 64 + i], half h of the k range
+    __shared__ double Bs[kHalf * kT];   // Bs[(q - 32 h) * 64 + j]
     const int2 task = tasks[blockIdx.x];
     const int s = task.x;
     const int f = A.fsize[s];
@@ -559,13 +607,17 @@ k_fl21(const int2* __restrict__ tasks, FactArgs A, int k) {
     const int j0 = 32 * (w >> 1), i0 = 32 * (w & 1);
     const int c0 = kT * k;
     const int nbk = min(kT, ns - c0);
-    // the operand tiles of product p: L[t, p] and L[k, p]
+    // the operand tiles of product p: L[t, p] and L[k, p], from the front as a buffer
+    const __amdgpu_buffer_rsrc_t rF = blk_rsrc(F, static_cast<int64_t>(f) * f);
     double va[16], vb[16];
-    auto load_upd = [&](int p) {
-        tile_load(F + static_cast<int64_t>(kT * p) * f + a0, f, na, kT, va);
-        tile_load(F + static_cast<int64_t>(kT * p) * f + c0, f, nbk, kT, vb);
+    auto load_upd_h = [&](int p, int h) {
+        tile_ldb_h(rF, kT * p * f + a0, f, na, kT, va, h);
+        tile_ldb_h(rF, kT * p * f + c0, f, nbk, kT, vb, h);
     };
-    if (k > 0) load_upd(0);
+    if (k > 0) {
+        load_upd_h(0, 0);
+        load_upd_h(0, 1);
+    }
     // run = F[t, k] in the accumulator layout (output (i, j) = (i0 + 16 b + l % 16,
     // j0 + 16 a + 4 q + l / 16)); positions outside the tile hold zero
     dbl4 run[2][2];
@@ -576,18 +628,20 @@ k_fl21(const int2* __restrict__ tasks, FactArgs A, int k) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int i = i0 + 16 * y + (l & 15), j = j0 + 16 * x + 4 * q + (l >> 4);
-                const double v = F[static_cast<int64_t>(c0 + (j < nbk ? j : 0)) * f + a0 + (i < na ? i : 0)];
+                const double v = blk_ld(rF, (c0 + j) * f + a0 + i);
                 run[x][y][q] = (i < na && j < nbk) ? v : 0.0;
             }
     for (int p = 0; p < k; ++p) {   // panels p < k < P are a full 64 columns
-        if (p > 0) __syncthreads();   // the previous product has read As and Bs
-        tile_to_lds(As, va);
-        tile_to_lds(Bs, vb);
-        __syncthreads();
-        if (p + 1 < k) load_upd(p + 1);
         dbl4 acc[2][2];
         tile_zero(acc);
-        tile_mma(As, Bs, kT, acc);
+        for (int h = 0; h < 2; ++h) {
+            if (p > 0 || h) __syncthreads();   // the previous half has been read
+            tile_to_lds_h(As, va, h);
+            tile_to_lds_h(Bs, vb, h);
+            __syncthreads();
+            if (p + 1 < k) load_upd_h(p + 1, h);   // this half is staged: load the next product's
+            tile_mma(As, Bs, kHalf, acc);
+        }
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -595,23 +649,28 @@ k_fl21(const int2* __restrict__ tasks, FactArgs A, int k) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) run[x][y][q] = run[x][y][q] - acc[x][y][q];
     }
-    // L[t, k] = run L_kk^{-T}: run becomes the A operand (As[j * 64 + i] = run(i, j))
+    // L[t, k] = run L_kk^{-T}: run becomes the A operand (As[j * 64 + i] = run(i, j), j the
+    // k index of this product), staged half by half
     tile_load(Gs + static_cast<int64_t>(c0) * lg + c0, lg, nbk, nbk, vb);
-    if (k > 0) __syncthreads();
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int i = i0 + 16 * y + (l & 15), j = j0 + 16 * x + 4 * q + (l >> 4);
-                As[j * kT + i] = run[x][y][q];
-            }
-    tile_to_lds(Bs, vb);
-    __syncthreads();
     dbl4 acc[2][2];
     tile_zero(acc);
-    tile_mma(As, Bs, (nbk + 3) & ~3, acc);
+    for (int h = 0; h < 2; ++h) {
+        const int nh = half_depth(nbk, h);
+        if (nh == 0) break;
+        if (k > 0 || h) __syncthreads();
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int i = i0 + 16 * y + (l & 15), j = j0 + 16 * x + 4 * q + (l >> 4);
+                    if ((j >> 5) == h) As[(j - kHalf * h) * kT + i] = run[x][y][q];
+                }
+        tile_to_lds_h(Bs, vb, h);
+        __syncthreads();
+        tile_mma(As, Bs, nh, acc);
+    }
     tile_write<false>(F + static_cast<int64_t>(c0) * f + a0, f, na, nbk, false, acc, 1.0);
 }
 
@@ -628,17 +687,6 @@ k_fl21(const int2* __restrict__ tasks, FactArgs A, int k) {
 // one's loads and barriers hide behind the others' products (C5: 58.7 ms per refactorization,
 // against 61.2-62.1 ms for 128x128 tiles at two or four waves per SIMD).
 constexpr int kFK = wrapd::kFinalChunk;    // k-chunk
-
-// Operand loads go through buffer resources (a 32-bit byte offset per lane, a uniform base
-// in scalar registers): addresses cost no 64-bit registers, and an offset past the block's
-// end reads as zero, so no load needs a clamped address.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t blk_rsrc(const double* base, int64_t n_doubles) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), static_cast<short>(0),
-                                             static_cast<int>(n_doubles * 8), 0x00020000);
-}
-__device__ __forceinline__ double blk_ld(__amdgpu_buffer_rsrc_t r, uint32_t elem) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, elem * 8u, 0, 0));
-}
 
 // R x C block (R * C = NT * NV, NT threads) of a column-major source (element offset off0,
 // leading dimension ld) into registers: thread t's entry r is (row t % R, column t / R +
