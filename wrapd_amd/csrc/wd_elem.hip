@@ -382,8 +382,12 @@ __device__ __forceinline__ void store_local_partials(double* partials, int lb, d
     if (static_cast<int>(threadIdx.x) < min(nslots, 11)) put_partial(partials, lb, threadIdx.x, t);
 }
 
+// The FP32 Newton local steps fit 168 registers (a few spilled to scratch), which gives three
+// waves per SIMD instead of two; the FP64 and L-BFGS variants need more and keep one or two.
+constexpr int local_min_blocks(int prec, int prox) { return prec == WRAPD_FP32 && prox != WRAPD_PROX_LBFGS ? 3 : 1; }
+
 template <int MODEL, int PREC, int PROX>
-__global__ void __launch_bounds__(kLocalThreads)
+__global__ void __launch_bounds__(kLocalThreads, local_min_blocks(PREC, PROX))
 k_local(int n_e, int n_ep, const double* __restrict__ symF, double* __restrict__ U, double* __restrict__ P,
         const double* __restrict__ wsq, const double* __restrict__ vol, const double* __restrict__ kst,
         double* __restrict__ cand, double* __restrict__ sig, MatArgs m, int prox_iters, double rtol,
@@ -564,7 +568,7 @@ __device__ __forceinline__ T prox_residual(const ProxProblem<T>& P, const T x[3]
 }
 
 template <int PREC>
-__global__ void __launch_bounds__(kLocalThreads)
+__global__ void __launch_bounds__(kLocalThreads, local_min_blocks(PREC, WRAPD_PROX_LEARNED))
 k_local_learned(int n_e, int n_ep, const double* __restrict__ symF, double* __restrict__ U, double* __restrict__ P,
                 const double* __restrict__ wsq, const double* __restrict__ vol, const double* __restrict__ kst,
                 double* __restrict__ cand, double* __restrict__ sig, MatArgs m, int prox_iters, double rtol,

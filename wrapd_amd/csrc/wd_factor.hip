@@ -300,6 +300,11 @@ __device__ __forceinline__ void potrf_lds(double* D, double* X, int nb, double* 
             double a[16], x[16], rinv[16];   // rinv[p] = 1 / L[p][p] (uniform)
 #pragma unroll
             for (int c = 0; c < 16; ++c) a[c] = D[(j0 + c) * 64 + j0 + li];
+            // X_JJ = L_JJ^{-1} by substitution on the identity (row li in x), its step p right
+            // after the Cholesky's column p, which is final then: the substitution's broadcasts
+            // and products fill the latency of the next column's square root and division
+#pragma unroll
+            for (int c = 0; c < 16; ++c) x[c] = c == li ? 1.0 : 0.0;
 #pragma unroll
             for (int p = 0; p < 16; ++p) {
                 double d = lane_bcast(a[p], p);
@@ -319,12 +324,6 @@ __device__ __forceinline__ void potrf_lds(double* D, double* X, int nb, double* 
                     const double lc = lane_bcast(a[p], c);   // L[c][p]
                     if (li >= c) a[c] -= a[p] * lc;
                 }
-            }
-            // X_JJ = L_JJ^{-1} by substitution on the identity (row li in x)
-#pragma unroll
-            for (int c = 0; c < 16; ++c) x[c] = c == li ? 1.0 : 0.0;
-#pragma unroll
-            for (int p = 0; p < 16; ++p) {
 #pragma unroll
                 for (int c = 0; c <= p; ++c) x[c] = li == p ? x[c] * rinv[p] : x[c];
 #pragma unroll
