@@ -16,10 +16,11 @@ Metric: time to a 1e-6 relative energy gap, and local-step elements/s, learned v
     158-169, at tol 1e-12).  On the 3.07M-tet beam the reference algorithm does not converge
     in any horizon a benchmark can afford (after 4,069 iterations the energy is still 2.8x
     its initial value with ~540 elements inverted; with static weights it runs away to
-    ~1e15), so no E* exists there and the line reports time_to_gap as not reached.  The
-    16x16x160 beam converges in ~150 iterations; there each path runs until its gap is
-    <= 1e-6 (whatever --steps says), against E* of a separate converged FP64 run, and that
-    is reported under "c2_time_to_gap".
+    ~1e15), so no E* exists there and the line reports time_to_gap as not reached (the
+    cause is the 100x stiffness contrast: the same load on a homogeneous beam converges).
+    The 16x16x160 beam (~130 iterations) and the 1.47M-tet 32x32x240 beam (~5,300) converge;
+    there each path runs until its gap is <= 1e-6 (whatever --steps says), against E* of a
+    separate converged FP64 run, reported under "time_to_gap_converging".
   * `value` is therefore the metric's measurable part on the metric's workload: the
     learned path's local-step throughput (elements/s), with every path's figure beside it.
 
@@ -143,23 +144,30 @@ VARIANTS = (("plain_fp64", {}), ("fp32_newton", dict(precision=1, prox_mode=1, f
             ("learned_fp32", dict(precision=1, prox_mode=2, fp64_switch=1e-8)))
 
 
-def c2_time_to_gap(max_iters=1500):
-    """Time to a 1e-6 energy gap on the 16x16x160 beam, where the reference algorithm
-    converges.  E*: a plain FP64 run stopped by ADMM::converged at tol 1e-12.  Each path
-    then runs from the start until (E_k - E*)/|E*| <= 1e-6, timed from its first iteration
-    (refactorizations included), at most max_iters iterations."""
+def time_to_gap(cfg, max_iters, data_iters, e_star_max_iters):
+    """Time to a 1e-6 energy gap on a beam where the reference algorithm converges.  E*: a
+    plain FP64 run stopped by ADMM::converged at tol 1e-12 (untimed).  Each path then runs from
+    the start until (E_k - E*)/|E*| <= 1e-6, timed from its first iteration (refactorizations
+    included), at most max_iters iterations."""
     from wrapd_amd import meshgen
     from wrapd_amd.solver import Solver
-    prob = meshgen.config_problem(2)
-    ref = Solver(prob, tol=1e-12, max_admm_iters=5000)
+    prob = meshgen.config_problem(cfg)
+    t0 = time.perf_counter()
+    ref = Solver(prob, tol=1e-12, max_admm_iters=e_star_max_iters)
     iters = ref.run()
     e_ref = ref.log("objectives")
     e_star = float(e_ref[-1])
     x_star = ref.x_free()
     ref.close()
-    mlp, minfo = train_mlp(prob, 200, 3000, 16384, 2e-3)
-    out = dict(workload=prob.name, e_star=e_star, e_star_source=f"plain FP64 run, ADMM::converged at tol 1e-12 "
-               f"after {iters} iterations", gap=GAP, mlp_tuples=minfo["tuples"], paths={})
+    converged = iters < e_star_max_iters
+    mlp, minfo = train_mlp(prob, data_iters, 3000, 16384, 2e-3, capacity=min(prob.n_elems * data_iters, 50_000_000))
+    out = dict(workload=prob.name, tets=prob.n_elems, e_star=e_star,
+               e_star_source=f"plain FP64 run, ADMM::converged at tol 1e-12 after {iters} iterations "
+                             f"({time.perf_counter() - t0:.1f} s, untimed)" if converged else
+                             f"plain FP64 run NOT converged in {iters} iterations", gap=GAP,
+               mlp_tuples=minfo["tuples"], paths={})
+    if not converged:
+        return out
     for name, over in VARIANTS:
         s = Solver(prob, tol=0.0, max_admm_iters=max_iters, **over)
         dev = s.device()
@@ -183,7 +191,8 @@ def c2_time_to_gap(max_iters=1500):
         out["paths"][name] = dict(time_to_gap_s=t if reached else None, iters_to_gap=k if reached else None,
                                   ms_per_iter=1e3 * t / k, reweights=int(np.sum(s.log("reweighted")[1:])),
                                   x_rel_err_vs_converged_fp64=float(np.linalg.norm(x - x_star) /
-                                                                    np.linalg.norm(x_star)))
+                                                                    np.linalg.norm(x_star)),
+                                  energy_rel_err_vs_converged_fp64=float((e - e_star) / abs(e_star)))
         s.close()
     p, l_ = out["paths"]["plain_fp64"], out["paths"]["learned_fp32"]
     out["speedup_learned_vs_plain"] = (p["time_to_gap_s"] / l_["time_to_gap_s"]
@@ -231,7 +240,8 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-c2", action="store_true", help="skip the 16x16x160 time-to-gap runs")
+    ap.add_argument("--no-gap", "--no-c2", action="store_true", help="skip the time-to-gap runs on the 16x16x160 and "
+                    "32x32x240 beams")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus must be 1).", file=sys.stderr)
@@ -325,13 +335,17 @@ def main():
         roofline = dict(roofline_local, traffic=None, largest_phase_per_iter=dom,
                         note="no refactorization in the timed iterations")
 
-    c2 = None
-    if not args.no_c2 and args.config != 2:
-        try:
-            c2 = c2_time_to_gap()
-            log(f"c2 time to gap: " + ", ".join(f"{k} {v['time_to_gap_s']}" for k, v in c2["paths"].items()))
-        except Exception as ex:   # the sub-result must never sink the line
-            c2 = dict(error=str(ex)[:300])
+    gaps = {}
+    if not args.no_gap and args.config == 5:
+        # the beams where the reference algorithm converges: 16x16x160 (~130 iterations) and
+        # the 1.47M-tet 32x32x240 beam (~5,300 iterations; the gap is reached near 3,000)
+        for cfg, max_iters, data_iters, e_iters in ((2, 1500, 200, 5000), (3, 8000, 16, 12000)):
+            try:
+                g = time_to_gap(cfg, max_iters, data_iters, e_iters)
+                log(f"c{cfg} time to gap: " + ", ".join(f"{k} {v['time_to_gap_s']}" for k, v in g["paths"].items()))
+            except Exception as ex:   # the sub-result must never sink the line
+                g = dict(error=str(ex)[:300])
+            gaps[f"c{cfg}"] = g
 
     cpu = None
     if not args.no_cpu:
@@ -366,7 +380,8 @@ def main():
                         "reason": "no converged FP64 run exists for this workload: the reference algorithm's energy "
                                   "stays above its initial value for thousands of iterations with hundreds of "
                                   "inverted elements (reweighting every iteration), and diverges with static "
-                                  "weights; the gap is measured on the 16x16x160 beam instead (c2_time_to_gap)"}
+                                  "weights; the gap is measured on the 16x16x160 and 32x32x240 beams instead "
+                                  "(time_to_gap_converging)"}
         if args.config == 5 else None,
         "local_step_elems_per_s": {k: v["local_step_elems_per_s"] for k, v in out_paths.items()},
         "local_step_speedup_learned_vs_plain": (lp["local_step_elems_per_s"] /
@@ -376,7 +391,7 @@ def main():
         "learned": learned_info,
         "roofline": roofline,
         "roofline_local_step": roofline_local,
-        "c2_time_to_gap": c2,
+        "time_to_gap_converging": gaps or None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
