@@ -262,15 +262,22 @@ __device__ __forceinline__ void tile_write(double* __restrict__ C, int64_t ldc, 
 // Cholesky of the diagonal tile and the inverse of its factor, blocked by 16.
 //
 // The 64x64 tile D (LDS, column-major D[c * 64 + r]) is factored in four block steps:
-// wave 0 factors the 16x16 diagonal block in registers (lane r holds row r; columns are
-// broadcast with v_readlane, no barrier inside the block) and inverts it; the panel below
-// becomes L_IJ = A_IJ L_JJ^{-T} and the trailing blocks get A_IK -= L_IJ L_KJ^T, both as
-// 16x16x16 products on the matrix cores, one block per wave.  The inverse of the whole
-// factor is then assembled block row by block row, X_IJ = -X_II sum_K L_IK X_KJ.  Three
-// barriers per block step instead of three per column.
+// wave 0 factors the 16x16 diagonal block in registers and inverts it (lane (r, g) holds row
+// r of columns 4 m + g; columns move between lanes by ds_bpermute, no barrier inside the
+// block); the panel below becomes L_IJ = A_IJ L_JJ^{-T} and the trailing blocks get
+// A_IK -= L_IJ L_KJ^T, both as 16x16x16 products on the matrix cores, one block per wave.
+// Block row J of the inverse, X_JI = -X_JJ sum_K L_JK X_KI, is formed by the waves the
+// panel step leaves idle.  Three barriers per block step instead of three per column.
 __device__ __forceinline__ double lane_bcast(double v, int lane) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// lane src's v, for each lane (src may differ between lanes; every lane of the wave active)
+__device__ __forceinline__ double lane_fetch(double v, int src) {
+    const int lo = __builtin_amdgcn_ds_bpermute(src << 2, __double2loint(v));
+    const int hi = __builtin_amdgcn_ds_bpermute(src << 2, __double2hiint(v));
     return __hiloint2double(hi, lo);
 }
 
@@ -282,11 +289,43 @@ __device__ __forceinline__ dbl4 mma16(const double a[4], const double b[4], dbl4
     return acc;
 }
 
+// The inverse X in LDS: entry (r, c) at c * 64 + (r ^ (c % 16)), so that the 16 lanes
+// reading one row across 16 columns hit 16 different banks.
+__device__ __forceinline__ int xs_at(int r, int c) { return c * 64 + (r ^ (c & 15)); }
+
+// X_IJ = -X_II (sum_{K=J}^{I-1} L_IK X_KJ), by one wave: L from D, X_II and X_KJ (K < I) final
+__device__ __forceinline__ void inverse_block(const double* D, double* X, int I, int J) {
+    const int l = threadIdx.x & 63;
+    const int li = l & 15, lk = l >> 4;
+    dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};   // T = sum_K L_IK X_KJ, acc[q] = T[4q + lk][li]
+    for (int K = J; K < I; ++K) {
+        double a[4], b[4];
+#pragma unroll
+        for (int kc = 0; kc < 4; ++kc) {
+            const int kk = 4 * kc + lk;
+            a[kc] = D[(16 * K + kk) * 64 + 16 * I + li];   // L_IK[li][kk]
+            b[kc] = X[xs_at(16 * K + kk, 16 * J + li)];     // X_KJ[kk][li]
+        }
+        acc = mma16(a, b, acc);
+    }
+    // acc[q] = T[4q + lk][li] is the B operand layout for k-step q
+    double a[4], b[4];
+#pragma unroll
+    for (int kc = 0; kc < 4; ++kc) {
+        a[kc] = X[xs_at(16 * I + li, 16 * I + 4 * kc + lk)];   // X_II[li][4kc + lk]
+        b[kc] = acc[kc];
+    }
+    dbl4 r = dbl4{0.0, 0.0, 0.0, 0.0};
+    r = mma16(a, b, r);   // r[q] = (X_II T)[4q + lk][li]
+#pragma unroll
+    for (int q = 0; q < 4; ++q) X[xs_at(16 * I + 4 * q + lk, 16 * J + li)] = -r[q];
+}
+
 // The 64x64 tile D (LDS, column-major D[c * 64 + r], lower triangle; rows and columns from
 // nb on hold the identity) is factored in place, L^{-1} built in X (LDS, zeroed by the
-// caller); then L goes to the front's lower triangle at Fkk (leading dimension f) and L^{-1}
-// (zero above the diagonal) to Gkk (leading dimension lg).  All 256 threads take part; the
-// caller has synchronized after filling D and X.
+// caller, entry (r, c) at xs_at(r, c)); then L goes to the front's lower triangle at Fkk
+// (leading dimension f) and L^{-1} (zero above the diagonal) to Gkk (leading dimension lg).
+// All 256 threads take part; the caller has synchronized after filling D and X.
 __device__ __forceinline__ void potrf_lds(double* D, double* X, int nb, double* __restrict__ Fkk, int64_t f,
                           double* __restrict__ Gkk, int64_t lg, int* __restrict__ fail) {
     const int nblk = (nb + 15) >> 4;   // 16-blocks holding tile columns
@@ -296,53 +335,68 @@ __device__ __forceinline__ void potrf_lds(double* D, double* X, int nb, double* 
     for (int J = 0; J < nblk; ++J) {
         const int j0 = 16 * J;
         if (w == 0) {
-            // Cholesky of D_JJ: lane li holds row li (lanes 16..63 repeat rows, write nothing)
-            double a[16], x[16], rinv[16];   // rinv[p] = 1 / L[p][p] (uniform)
+            // Cholesky of D_JJ, L_JJ^{-1} by substitution on the identity (its step p right
+            // after the Cholesky's column p, which is final then).  Lane (li, g = lk) holds row
+            // li of columns 4 m + g in a[m] (D_JJ, then L_JJ) and x[m] (X_JJ).  Every entry
+            // gets the operations of the row-per-lane loop, in the same order.
+            const int g = lk;
+            double a[4], x[4];
 #pragma unroll
-            for (int c = 0; c < 16; ++c) a[c] = D[(j0 + c) * 64 + j0 + li];
-            // X_JJ = L_JJ^{-1} by substitution on the identity (row li in x), its step p right
-            // after the Cholesky's column p, which is final then: the substitution's broadcasts
-            // and products fill the latency of the next column's square root and division
-#pragma unroll
-            for (int c = 0; c < 16; ++c) x[c] = c == li ? 1.0 : 0.0;
+            for (int m = 0; m < 4; ++m) {
+                a[m] = D[(j0 + 4 * m + g) * 64 + j0 + li];
+                x[m] = 4 * m + g == li ? 1.0 : 0.0;
+            }
+            double d = lane_bcast(a[0], 0);   // A[0][0]
 #pragma unroll
             for (int p = 0; p < 16; ++p) {
-                double d = lane_bcast(a[p], p);
+                const int gp = p & 3, mp = p >> 2;   // column p: a[mp] of the lanes of group gp
                 if (!(d > 0.0)) {
                     bad = 1;
                     d = 1.0;
                 }
+                // A[p+1][p+1] with the updates of the columns before p; column p's update of it
+                // is the operation its own lane applies below, done here as well so the next
+                // pivot does not wait for the column exchange
+                double dn = p < 15 ? lane_bcast(a[(p + 1) >> 2], (p + 1) + 16 * ((p + 1) & 3)) : 0.0;
                 // correctly rounded: a hardware estimate with Newton steps saved 2.4 us per
                 // tile but changed L in the last bits, enough to move the sample runs past
                 // the reference's own round-off spread
                 const double sq = sqrt(d);
                 const double inv = 1.0 / sq;
-                rinv[p] = inv;
-                a[p] = li == p ? sq : (li > p ? a[p] * inv : a[p]);
-#pragma unroll
-                for (int c = p + 1; c < 16; ++c) {
-                    const double lc = lane_bcast(a[p], c);   // L[c][p]
-                    if (li >= c) a[c] -= a[p] * lc;
+                if (g == gp) a[mp] = li == p ? sq : a[mp] * inv;   // rows above p: never read
+                if (p < 15) {
+                    const double l1 = lane_bcast(a[mp], (p + 1) + 16 * gp);   // L[p+1][p]
+                    dn -= l1 * l1;
                 }
+                const double lp = lane_fetch(a[mp], li + 16 * gp);   // L[li][p]
 #pragma unroll
-                for (int c = 0; c <= p; ++c) x[c] = li == p ? x[c] * rinv[p] : x[c];
-#pragma unroll
-                for (int c = 0; c <= p; ++c) {
-                    const double xpc = lane_bcast(x[c], p);   // X[p][c]
-                    if (li > p) x[c] -= a[p] * xpc;
+                for (int m = mp; m < 4; ++m) {
+                    const double lc = lane_fetch(a[mp], 4 * m + g + 16 * gp);   // L[4m+g][p]
+                    if (m > mp || g > gp) a[m] -= lp * lc;   // columns after p (rows above: never read)
                 }
+                const double xs = li == p ? inv : 1.0;   // row p of X is final after this
+#pragma unroll
+                for (int m = 0; m <= mp; ++m) x[m] *= xs;
+#pragma unroll
+                for (int m = 0; m <= mp; ++m) {
+                    const double xpc = lane_fetch(x[m], p + 16 * g);   // X[p][4m+g]
+                    if (li > p && (m < mp || g <= gp)) x[m] -= lp * xpc;
+                }
+                d = dn;
             }
-            if (l < 16) {
 #pragma unroll
-                for (int c = 0; c < 16; ++c) {
-                    if (li >= c) {
-                        D[(j0 + c) * 64 + j0 + li] = a[c];
-                        X[(j0 + c) * 64 + j0 + li] = x[c];
-                    }
+            for (int m = 0; m < 4; ++m) {
+                const int c = 4 * m + g;
+                if (li >= c) {
+                    D[(j0 + c) * 64 + j0 + li] = a[m];
+                    X[xs_at(j0 + li, j0 + c)] = x[m];
                 }
             }
         }
         __syncthreads();
+        // block row J of the inverse, block J2 < J on wave 3 - J2: it reads L_JK (K < J), X_JJ
+        // and the earlier block rows, none of which the panel below writes
+        if (w > 2 - J && 3 - w < J) inverse_block(D, X, J, 3 - w);
         // panel: L_IJ = A_IJ X_JJ^T for I > J, one block per wave.  Computed transposed,
         // C^T = X_JJ A_IJ^T, so consecutive lanes hold consecutive rows of L_IJ.
         {
@@ -352,7 +406,7 @@ __device__ __forceinline__ void potrf_lds(double* D, double* X, int nb, double* 
 #pragma unroll
                 for (int kc = 0; kc < 4; ++kc) {
                     const int kk = 4 * kc + lk;
-                    a[kc] = X[(j0 + kk) * 64 + j0 + li];       // X_JJ[li][kk]
+                    a[kc] = X[xs_at(j0 + li, j0 + kk)];       // X_JJ[li][kk]
                     b[kc] = D[(j0 + kk) * 64 + 16 * I + li];   // A_IJ[li][kk]
                 }
                 dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
@@ -384,35 +438,6 @@ __device__ __forceinline__ void potrf_lds(double* D, double* X, int nb, double* 
         __syncthreads();
     }
     if (w == 0 && l == 0 && bad) *fail = 1;
-    // off-diagonal blocks of the inverse: X_IJ = -X_II (sum_{K=J}^{I-1} L_IK X_KJ), by block rows
-    for (int I = 1; I < nblk; ++I) {
-        const int J = w;
-        if (J < I) {
-            dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};   // T = sum_K L_IK X_KJ, acc[q] = T[4q + lk][li]
-            for (int K = J; K < I; ++K) {
-                double a[4], b[4];
-#pragma unroll
-                for (int kc = 0; kc < 4; ++kc) {
-                    const int kk = 4 * kc + lk;
-                    a[kc] = D[(16 * K + kk) * 64 + 16 * I + li];   // L_IK[li][kk]
-                    b[kc] = X[(16 * J + li) * 64 + 16 * K + kk];   // X_KJ[kk][li]
-                }
-                acc = mma16(a, b, acc);
-            }
-            // acc[q] = T[4q + lk][li] is the B operand layout for k-step q
-            double a[4], b[4];
-#pragma unroll
-            for (int kc = 0; kc < 4; ++kc) {
-                a[kc] = X[(16 * I + 4 * kc + lk) * 64 + 16 * I + li];   // X_II[li][4kc + lk]
-                b[kc] = acc[kc];
-            }
-            dbl4 r = dbl4{0.0, 0.0, 0.0, 0.0};
-            r = mma16(a, b, r);   // r[q] = (X_II T)[4q + lk][li]
-#pragma unroll
-            for (int q = 0; q < 4; ++q) X[(16 * J + li) * 64 + 16 * I + 4 * q + lk] = -r[q];
-        }
-        __syncthreads();
-    }
     // L_kk to the front's lower triangle, L_kk^{-1} (zero above the diagonal) to G's
     // diagonal tile (k, k), where the TRSM tasks read it and the solves use it
     {
@@ -423,7 +448,7 @@ __device__ __forceinline__ void potrf_lds(double* D, double* X, int nb, double* 
                 const int c = cb + 4 * q;
                 if (c < nb) {
                     if (r >= c) Fkk[static_cast<int64_t>(c) * f + r] = D[threadIdx.x + kFactThreads * q];
-                    Gkk[c * lg + r] = X[threadIdx.x + kFactThreads * q];
+                    Gkk[c * lg + r] = X[xs_at(r, c)];
                 }
             }
         }
