@@ -107,6 +107,25 @@ __device__ bool poll3(const double* p, double v[3], int* abortf) {
     return true;
 }
 
+// poll3 for one word
+__device__ bool poll1(const double* p, double& v, int* abortf) {
+    for (int polls = 1; pending(v); ++polls) {
+        if ((polls & 63) == 0) {
+            if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+            if (polls >= kSpinLimit) {
+                __hip_atomic_store(abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return false;
+            }
+        }
+        if (polls < 16)
+            __builtin_amdgcn_s_sleep(2);
+        else
+            __builtin_amdgcn_s_sleep(40);
+        v = ld_agent(p);
+    }
+    return true;
+}
+
 // Leaf subtrees: one workgroup walks a whole subtree in postorder.  A subtree is a
 // contiguous range of supernodes [first, root] (postorder, checked at set-up), and every
 // update row it produces except its root's is consumed inside it, so those rows stay in
@@ -352,7 +371,10 @@ static_assert(kBwdChunk % kSolveThreads == 0 && kNsMax % kSolveThreads == 0, "st
 // out before the previous round is summed).  The pivot rows'
 // right-hand side comes from bperm, b in the solve's numbering (pending until the leaf
 // workgroups of the launch have copied it), so every load of the task depends on the task
-// record alone.  B: G terms per thread loaded before the wait.  LDS (dyn): cv, then sred.
+// record alone.  Slots and b rows are loaded and polled word by word, consecutive threads
+// on consecutive words: these loads go past the L2, so a wave loading word c of 64 rows
+// 24 bytes apart would fetch every line the rows span once per word.
+// B: G terms per thread loaded before the wait.  LDS (dyn): cv, then sred.
 constexpr int kFwdStage = 2 * kSolveThreads;
 constexpr int kFwdFlowLds = (kNsMax + kFwdStage) * 3 * static_cast<int>(sizeof(double));
 template <int B>
@@ -362,7 +384,8 @@ __device__ __forceinline__ void fwd_flow_body(int ti, const SolveTask* __restric
                                               double* __restrict__ xsol, double* __restrict__ slots, double* dyn,
                                               int& sh_ok) {
     constexpr int kRowsPerThread = kNsMax / kSolveThreads;
-    constexpr int kSlotsPerThread = kFwdStage / kSolveThreads;
+    constexpr int kWordsPerThread = 3 * kFwdStage / kSolveThreads;   // slot words per window
+    constexpr int kBWordsPerThread = 3 * kRowsPerThread;              // b words
     double* cv = dyn;                    // kNsMax x 3
     double* sred = dyn + kNsMax * 3;     // staged slots, then the column-group partial sums
     static_assert(kFwdStage >= kSolveThreads, "the partial sums reuse the staging buffer");
@@ -373,19 +396,16 @@ __device__ __forceinline__ void fwd_flow_body(int ti, const SolveTask* __restric
     const int g = threadIdx.x / R, rl = threadIdx.x - g * R;
     const int r = T.first + rl;
     const int np = T.np, ntot = T.np + T.nu;
-    auto slot_ptr = [&](int i) { return slots + 3 * (i < np ? T.qa + i : T.uq0 + (i - np)); };
-    double v[kSlotsPerThread][3];
-    auto load_window = [&](int base) {
+    // word wi of the staged sequence: 3 words per slot, pivot slots first
+    auto word_ptr = [&](int wi) {
+        return wi < 3 * np ? slots + 3 * T.qa + wi : slots + 3 * T.uq0 + (wi - 3 * np);
+    };
+    double v[kWordsPerThread];
+    auto load_window = [&](int base) {   // the words of slots [base, base + kFwdStage)
 #pragma unroll
-        for (int k = 0; k < kSlotsPerThread; ++k) {
-            const int i = base + threadIdx.x + k * kSolveThreads;
-            v[k][0] = v[k][1] = v[k][2] = 0.0;
-            if (i < ntot) {
-                const double* sp = slot_ptr(i);
-                v[k][0] = ld_agent(sp + 0);
-                v[k][1] = ld_agent(sp + 1);
-                v[k][2] = ld_agent(sp + 2);
-            }
+        for (int k = 0; k < kWordsPerThread; ++k) {
+            const int wi = 3 * base + threadIdx.x + k * kSolveThreads;
+            v[k] = wi < 3 * ntot ? ld_agent(word_ptr(wi)) : 0.0;
         }
     };
     // Before the wait: the first window of inputs, this thread's first G terms, b of the
@@ -400,20 +420,22 @@ __device__ __forceinline__ void fwd_flow_body(int ti, const SolveTask* __restric
 #pragma unroll
     for (int u = 0; u < B; ++u) gv[u] = u < nterm ? gp[u * gstride] : 0.0;
     int qlo[kRowsPerThread], qhi[kRowsPerThread];   // relative to the pivot slots' start
-    unsigned bpend = 0;                              // this thread's b rows still pending
 #pragma unroll
     for (int k = 0; k < kRowsPerThread; ++k) {
         const int j = threadIdx.x + k * kSolveThreads;
         const bool in = j < ns;
         qlo[k] = in ? static_cast<int>(M.asm_ptr[T.rowptr + j] - T.qa) : 0;
         qhi[k] = in ? static_cast<int>(M.asm_ptr[T.rowptr + j + 1] - T.qa) : 0;
-        if (in) {
-            const double* bp = bperm + 3ll * (c0 + j);
-            const double b0 = ld_agent(bp + 0), b1 = ld_agent(bp + 1), b2 = ld_agent(bp + 2);
-            cv[3 * j + 0] = b0;
-            cv[3 * j + 1] = b1;
-            cv[3 * j + 2] = b2;
-            if (pending(b0) || pending(b1) || pending(b2)) bpend |= 1u << k;
+    }
+    const double* bp = bperm + 3ll * c0;
+    unsigned bpend = 0;   // this thread's b words still pending
+#pragma unroll
+    for (int k = 0; k < kBWordsPerThread; ++k) {
+        const int wi = threadIdx.x + k * kSolveThreads;
+        if (wi < 3 * ns) {
+            const double x = ld_agent(bp + wi);
+            cv[wi] = x;
+            if (pending(x)) bpend |= 1u << k;
         }
     }
     const bool urow = g == 0 && r >= ns && r < f;
@@ -427,36 +449,27 @@ __device__ __forceinline__ void fwd_flow_body(int ti, const SolveTask* __restric
     __syncthreads();   // sh_ok is set
     const bool live = sh_ok != 0;
 #pragma unroll
-    for (int k = 0; k < kRowsPerThread; ++k)
+    for (int k = 0; k < kBWordsPerThread; ++k)
         if ((bpend >> k) & 1u) {
-            const int j = threadIdx.x + k * kSolveThreads;
-            double t3[3] = {cv[3 * j + 0], cv[3 * j + 1], cv[3 * j + 2]};
-            if (live && !poll3(bperm + 3ll * (c0 + j), t3, abortf)) sh_ok = 0;
-            cv[3 * j + 0] = t3[0];
-            cv[3 * j + 1] = t3[1];
-            cv[3 * j + 2] = t3[2];
+            const int wi = threadIdx.x + k * kSolveThreads;
+            double x = cv[wi];
+            if (live && !poll1(bp + wi, x, abortf)) sh_ok = 0;
+            cv[wi] = x;
         }
     __syncthreads();   // cv holds b
     // The children's updates, polled, staged and summed in slot order.
     double ua0 = 0.0, ua1 = 0.0, ua2 = 0.0;
     for (int base = 0; base < ntot; base += kFwdStage) {
 #pragma unroll
-        for (int k = 0; k < kSlotsPerThread; ++k) {
-            const int i = base + threadIdx.x + k * kSolveThreads;
-            if (i < ntot) {
-                double* sp = slot_ptr(i);
-                if (live && !poll3(sp, v[k], abortf)) sh_ok = 0;
-                if (i >= np) {   // this task is the update rows' only reader
-                    const double pv = pending_value();
-                    sp[0] = pv;
-                    sp[1] = pv;
-                    sp[2] = pv;
-                }
+        for (int k = 0; k < kWordsPerThread; ++k) {
+            const int wl = threadIdx.x + k * kSolveThreads;   // word of the window
+            const int wi = 3 * base + wl;
+            if (wi < 3 * ntot) {
+                double* wp = word_ptr(wi);
+                if (live && !poll1(wp, v[k], abortf)) sh_ok = 0;
+                if (wi >= 3 * np) *wp = pending_value();   // this task is the update rows' only reader
             }
-            const int il = threadIdx.x + k * kSolveThreads;
-            sred[3 * il + 0] = v[k][0];
-            sred[3 * il + 1] = v[k][1];
-            sred[3 * il + 2] = v[k][2];
+            sred[wl] = v[k];
         }
         if (base + kFwdStage < ntot) load_window(base + kFwdStage);   // in flight during this window's sums
         __syncthreads();
