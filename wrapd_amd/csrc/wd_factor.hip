@@ -178,6 +178,10 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t blk_rsrc(const double* base, i
 __device__ __forceinline__ double blk_ld(__amdgpu_buffer_rsrc_t r, uint32_t elem) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, elem * 8u, 0, 0));
 }
+// An element index past the end of every buffer here (they are under 2 GB): the load
+// returns zero.  Entries that must read as zero are loaded from it rather than masked after
+// the load, so a prefetch does not wait for its data where it is issued.
+constexpr uint32_t kZeroElem = 0x1FFFFFFFu;
 
 // tile_load_h through a buffer resource (rows and columns past nr, nc read as zero)
 __device__ __forceinline__ void tile_ldb_h(__amdgpu_buffer_rsrc_t rs, uint32_t off0, uint32_t ld, int nr, int nc,
@@ -185,10 +189,8 @@ __device__ __forceinline__ void tile_ldb_h(__amdgpu_buffer_rsrc_t rs, uint32_t o
     const int row = threadIdx.x & 63, col0 = threadIdx.x >> 6;
     const uint32_t o = off0 + row + col0 * ld;
 #pragma unroll
-    for (int r = 8 * h; r < 8 * h + 8; ++r) {
-        const double x = blk_ld(rs, o + static_cast<uint32_t>(4 * r) * ld);
-        v[r] = (row < nr && col0 + 4 * r < nc) ? x : 0.0;
-    }
+    for (int r = 8 * h; r < 8 * h + 8; ++r)
+        v[r] = blk_ld(rs, (row < nr && col0 + 4 * r < nc) ? o + static_cast<uint32_t>(4 * r) * ld : kZeroElem);
 }
 // the registers of tile_load into an LDS tile, as loaded (T[col * 64 + row]) or transposed
 __device__ __forceinline__ void tile_to_lds(double* T, const double v[16]) {
@@ -725,9 +727,8 @@ __device__ __forceinline__ void blk_load(__amdgpu_buffer_rsrc_t rs, uint32_t off
     const uint32_t o = off0 + row + col0 * ld;
 #pragma unroll
     for (int r = 0; r < NV; ++r) {
-        const double x = blk_ld(rs, o + static_cast<uint32_t>(CS * r) * ld);
         const bool ok = R == KC ? row < nk : col0 + CS * r < nk;
-        v[r] = ok ? x : 0.0;
+        v[r] = blk_ld(rs, ok ? o + static_cast<uint32_t>(CS * r) * ld : kZeroElem);
     }
 }
 
