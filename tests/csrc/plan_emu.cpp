@@ -21,7 +21,8 @@ std::vector<int64_t> wF, rF, wG, rG;
 int64_t stamp = 0, cur_launch = 0, conflicts = 0;
 
 // Launches on the second stream (FactorLaunch::side) run concurrently with the main
-// stream's launches issued after them, until the next level's assembly: an access of such a
+// stream's launches issued after them, until an assembly launch waits for them (all of them,
+// or those up to the previous level's FINAL launch FactorLaunch::wait): an access of such a
 // main launch to an entry a running side launch wrote (or a write to one it read) counts too.
 std::vector<char> side_active;   // by launch index + 1
 bool cur_main = true;
@@ -69,7 +70,8 @@ extern "C" int pe_analyze(int n, const int* ap, const int* aj, const double* xyz
     st[6] = static_cast<double>(PL.pool_no_reuse);
     st[7] = static_cast<double>(std::count_if(PL.launches.begin(), PL.launches.end(),
                                               [](const FactorLaunch& L) { return L.side; }));
-    st[8] = 0;
+    st[8] = static_cast<double>(std::count_if(PL.launches.begin(), PL.launches.end(),
+                                              [](const FactorLaunch& L) { return L.kind == FK_ASSEMBLE && L.wait >= 0; }));
     st[9] = static_cast<double>(std::count_if(PL.launches.begin(), PL.launches.end(),
                                               [](const FactorLaunch& L) { return L.kind == FK_L21; }));
     return 0;
@@ -96,10 +98,26 @@ extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
     conflicts = 0;
     int fail = 0;
     side_active.assign(PL.launches.size() + 1, 0);
+    std::vector<int64_t> fin_prev, fin_cur;   // launch index + 1 of the previous / this level's FINAL launch k
     for (size_t li = 0; li < PL.launches.size(); ++li) {
         const FactorLaunch& L = PL.launches[li];
         cur_launch = static_cast<int64_t>(li) + 1;
-        if (L.kind == FK_ASSEMBLE) std::fill(side_active.begin(), side_active.end(), 0);   // the streams join
+        if (L.kind == FK_ASSEMBLE) {
+            if (li > 0 && PL.launches[li - 1].kind != FK_ASSEMBLE) {   // a new level
+                fin_prev = fin_cur;
+                fin_cur.clear();
+            }
+            // the second stream runs in order: waiting for its launch w waits for all before it
+            const int64_t upto = L.wait < 0 ? cur_launch
+                                 : L.wait < static_cast<int>(fin_prev.size()) ? fin_prev[L.wait] : -1;
+            if (upto < 0 || (L.wait >= 0 && !PL.launches[upto - 1].side)) return 3;   // not a second-stream launch
+            std::fill(side_active.begin(), side_active.begin() + upto + 1, 0);
+        }
+        if (L.kind == FK_FINAL) {
+            if (L.ord < 0) return 3;
+            if (static_cast<int>(fin_cur.size()) <= L.ord) fin_cur.resize(L.ord + 1, -1);
+            fin_cur[L.ord] = cur_launch;
+        }
         cur_main = !L.side;
         if (L.side) side_active[cur_launch] = 1;
         for (int q = 0; q < L.count; ++q) {

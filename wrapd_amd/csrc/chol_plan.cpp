@@ -210,14 +210,32 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P, const PlanOptions& opt) {
         P.launches.back().count++;
     };
     const double T = kFactorTile;
+    std::vector<int> fin_launch(nsup, -1);   // the FINAL launch holding a supernode's tasks
     for (int l = 0; l < nlev; ++l) {
         const std::vector<int>& L = by_level[l];
-        begin(FK_ASSEMBLE, 0);
+        // The level's fronts, grouped by the last FINAL launch of the previous level that
+        // their children there are in (every front of level l has one): a group whose launch
+        // is on the second stream and not the level's last waits for that launch only, so
+        // its assembly overlaps the later FINAL launches; the last group waits for all.
+        std::map<int, std::vector<int>> groups;
         for (int s : L) {
-            for (int j = 0; j < f[s]; ++j)
-                for (int q = j / asm_chunk; q * asm_chunk < f[s]; ++q) push(s, j | (q << 16));
-            P.launches.back().rows = std::max(P.launches.back().rows, std::min(f[s], asm_chunk));
+            int key = -1;
+            for (int c : children[s])
+                if (lev[c] == l - 1) key = std::max(key, fin_launch[c]);
+            groups[key].push_back(s);
         }
+        int gi = 0;
+        for (const auto& [key, grp] : groups) {
+            begin(FK_ASSEMBLE, 0);
+            const bool last = ++gi == static_cast<int>(groups.size());
+            if (!last && key >= 0 && P.launches[key].side) P.launches.back().wait = P.launches[key].ord;
+            for (int s : grp) {
+                for (int j = 0; j < f[s]; ++j)
+                    for (int q = j / asm_chunk; q * asm_chunk < f[s]; ++q) push(s, j | (q << 16));
+                P.launches.back().rows = std::max(P.launches.back().rows, std::min(f[s], asm_chunk));
+            }
+        }
+        int nfinal = 0;   // the level's FINAL launches so far
         int pmax = 0;
         for (int s : L) pmax = std::max(pmax, tile_count_l11(ns[s]));
         // Where L21 is formed: by the panel steps (right-looking: each panel's TRSM and updates
@@ -253,6 +271,8 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P, const PlanOptions& opt) {
             if (grp.empty()) return;
             begin(FK_FINAL, 0);
             P.launches.back().side = left || p < pmax;
+            P.launches.back().ord = nfinal++;
+            for (int s : grp) fin_launch[s] = static_cast<int>(P.launches.size()) - 1;
             final_order(grp, ns, f, [&](int s, int y) { push(s, y); });
             constexpr double T = kFinalTile;
             for (int s : grp) {   // flops issued (the kernel's k-chunks)

@@ -66,9 +66,14 @@ struct FactorLaunch {
     int rows = 0;       // FK_ASSEMBLE: the most rows a task of the launch builds (its LDS column)
     // FK_L21 / FK_FINAL of supernodes whose panels end before the level's last panel step:
     // run on a second stream once the main stream has issued the launches before them, so
-    // that they overlap the remaining panel steps of the level's wider supernodes; the main
-    // stream waits for them before the next level's assembly
+    // that they overlap the remaining panel steps of the level's wider supernodes
     bool side = false;
+    // FK_FINAL: its place among the level's FINAL launches.  FK_ASSEMBLE: the second
+    // stream's work it waits for: -1 all of it; k >= 0 that stream's launches up to and
+    // including the previous level's FINAL launch k (the last its fronts' children are in),
+    // so that it overlaps the later ones
+    int ord = -1;
+    int wait = -1;
 };
 
 struct FactorPlan {

@@ -19,6 +19,7 @@ using wd::check_hip;
 wrapd_ctx::~wrapd_ctx() {
     if (fev_fork) (void)hipEventDestroy(fev_fork);
     if (fev_join) (void)hipEventDestroy(fev_join);
+    for (hipEvent_t e : fev_final) (void)hipEventDestroy(e);
     if (fstream2) (void)hipStreamDestroy(fstream2);
     auto f = [](void* p) { if (p) (void)hipFree(p); };
     f(vidx); f(Di); f(vol); f(kst); f(mu_e); f(lam_e); f(wsq); f(cand); f(U); f(P); f(symF); f(sig); f(gstage);

@@ -60,6 +60,7 @@ struct wrapd_ctx {
     // their level's last panel step) and the events that order it with `stream`
     hipStream_t fstream2 = nullptr;
     hipEvent_t fev_fork = nullptr, fev_join = nullptr;
+    std::vector<hipEvent_t> fev_final;   // after a level's FINAL launch k on fstream2
     std::string last_error;
     wd::Prof prof;
 

@@ -886,6 +886,14 @@ int factor_prepare_device(wrapd_ctx* c) {
         rc |= check_hip(c, hipEventCreateWithFlags(&c->fev_fork, hipEventDisableTiming), "hipEventCreate");
         rc |= check_hip(c, hipEventCreateWithFlags(&c->fev_join, hipEventDisableTiming), "hipEventCreate");
     }
+    size_t nfinal = 0;   // one event per FINAL launch of a level
+    for (const wrapd::FactorLaunch& L : P.launches) nfinal = std::max(nfinal, static_cast<size_t>(L.ord + 1));
+    while (c->fev_final.size() < nfinal) {
+        hipEvent_t e = nullptr;
+        rc |= check_hip(c, hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        if (!e) break;
+        c->fev_final.push_back(e);
+    }
     if (c->pool) { (void)hipFree(c->pool); c->pool = nullptr; }
     rc |= check_hip(c, hipMalloc(&c->pool, std::max<size_t>(8, P.pool_size * sizeof(double))), "hipMalloc(front pool)");
     return rc ? WRAPD_ERR_HIP : WRAPD_OK;
@@ -896,9 +904,10 @@ static void launch_final(const wrapd::FactorLaunch& L, const int2* t, const Fact
                        A);
 }
 
-// queues the whole factorization on c->stream (with the side FINAL launches on
-// c->fstream2, forked after the main-stream launch before them and joined before the next
-// level's assembly and at the end); the caller reads c->d_ffail afterwards
+// queues the whole factorization on c->stream (with the side launches on c->fstream2, forked
+// after the main-stream launch before them; an assembly launch waits for the second stream's
+// launches up to the previous level's FINAL launch its fronts need (FactorLaunch::wait), the
+// level's last one for all of them, and so does the end); the caller reads c->d_ffail afterwards
 void launch_factor(wrapd_ctx* c) {
     const wrapd::FactorPlan& P = c->fplan;
     FactArgs A{c->d_col0, c->d_fsize, c->d_foff, c->d_goff, c->pool, c->G};
@@ -916,14 +925,22 @@ void launch_factor(wrapd_ctx* c) {
     };
     for (const wrapd::FactorLaunch& L : P.launches) {
         const int2* t = tasks + L.offset;
-        if (L.kind == wrapd::FK_ASSEMBLE) join();
+        if (L.kind == wrapd::FK_ASSEMBLE) {
+            if (L.wait < 0 || L.wait >= static_cast<int>(c->fev_final.size()))
+                join();
+            else if (side_busy)
+                (void)hipStreamWaitEvent(c->stream, c->fev_final[L.wait], 0);
+        }
         if (L.side) {
             (void)hipEventRecord(c->fev_fork, c->stream);
             (void)hipStreamWaitEvent(c->fstream2, c->fev_fork, 0);
-            if (L.kind == wrapd::FK_L21)
+            if (L.kind == wrapd::FK_L21) {
                 hipLaunchKernelGGL(k_fl21, dim3(L.count), dim3(kFactThreads), 0, c->fstream2, t, A, L.panel);
-            else
+            } else {
                 launch_final(L, t, A, c->fstream2);
+                if (L.ord >= 0 && L.ord < static_cast<int>(c->fev_final.size()))
+                    (void)hipEventRecord(c->fev_final[L.ord], c->fstream2);
+            }
             side_busy = true;
             continue;
         }
