@@ -178,10 +178,14 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t blk_rsrc(const double* base, i
 __device__ __forceinline__ double blk_ld(__amdgpu_buffer_rsrc_t r, uint32_t elem) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, elem * 8u, 0, 0));
 }
-// An element index past the end of every buffer here (they are under 2 GB): the load
-// returns zero.  Entries that must read as zero are loaded from it rather than masked after
-// the load, so a prefetch does not wait for its data where it is issued.
-constexpr uint32_t kZeroElem = 0x1FFFFFFFu;
+// Entries that must read as zero are loaded from past the buffer's end (the load returns
+// zero) rather than masked after the load, so a prefetch does not wait for its data where it
+// is issued: element indexes are below 2^28 (every buffer here is under 2 GB), and setting
+// bit 28 puts the byte offset past 2^31.  Arithmetic, not a select, so the compiler does
+// not split the load into two branches (whose common destination costs a full wait).
+__device__ __forceinline__ uint32_t zero_unless(bool ok, uint32_t elem) {
+    return elem | (static_cast<uint32_t>(!ok) << 28);
+}
 
 // tile_load_h through a buffer resource (rows and columns past nr, nc read as zero)
 __device__ __forceinline__ void tile_ldb_h(__amdgpu_buffer_rsrc_t rs, uint32_t off0, uint32_t ld, int nr, int nc,
@@ -190,7 +194,7 @@ __device__ __forceinline__ void tile_ldb_h(__amdgpu_buffer_rsrc_t rs, uint32_t o
     const uint32_t o = off0 + row + col0 * ld;
 #pragma unroll
     for (int r = 8 * h; r < 8 * h + 8; ++r)
-        v[r] = blk_ld(rs, (row < nr && col0 + 4 * r < nc) ? o + static_cast<uint32_t>(4 * r) * ld : kZeroElem);
+        v[r] = blk_ld(rs, zero_unless(row < nr && col0 + 4 * r < nc, o + static_cast<uint32_t>(4 * r) * ld));
 }
 // the registers of tile_load into an LDS tile, as loaded (T[col * 64 + row]) or transposed
 __device__ __forceinline__ void tile_to_lds(double* T, const double v[16]) {
@@ -728,7 +732,7 @@ __device__ __forceinline__ void blk_load(__amdgpu_buffer_rsrc_t rs, uint32_t off
 #pragma unroll
     for (int r = 0; r < NV; ++r) {
         const bool ok = R == KC ? row < nk : col0 + CS * r < nk;
-        v[r] = blk_ld(rs, ok ? o + static_cast<uint32_t>(CS * r) * ld : kZeroElem);
+        v[r] = blk_ld(rs, zero_unless(ok, o + static_cast<uint32_t>(CS * r) * ld));
     }
 }
 
