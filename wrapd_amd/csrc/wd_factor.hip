@@ -152,22 +152,6 @@ __device__ __forceinline__ void tile_mma(const double* __restrict__ As, const do
     }
 }
 
-// 64x64 tile of a column-major block (leading dimension ld) into registers: thread t's
-// entry r (0..15) is (row t % 64, column t / 64 + 4 r), that is LDS index t + 256 r of a
-// column-major tile.  Rows >= nr and columns >= nc read as zero.  Every address is
-// clamped into the block (nr, nc >= 1), so the 16 loads carry no branch and issue together.
-__device__ __forceinline__ void tile_load(const double* __restrict__ src, int64_t ld, int nr, int nc, double v[16]) {
-    const int row = threadIdx.x & 63, col0 = threadIdx.x >> 6;
-    const bool rok = row < nr;
-    const double* p = src + (rok ? row : 0);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int col = col0 + 4 * r;
-        const bool ok = rok && col < nc;
-        const double x = p[static_cast<int64_t>(col < nc ? col : 0) * ld];
-        v[r] = ok ? x : 0.0;
-    }
-}
 // Operand loads go through buffer resources (a 32-bit byte offset per lane, a uniform base
 // in scalar registers): addresses cost no 64-bit registers, and an offset past the block's
 // end reads as zero, so no load needs a clamped address.
@@ -185,6 +169,22 @@ __device__ __forceinline__ double blk_ld(__amdgpu_buffer_rsrc_t r, uint32_t elem
 // not split the load into two branches (whose common destination costs a full wait).
 __device__ __forceinline__ uint32_t zero_unless(bool ok, uint32_t elem) {
     return elem | (static_cast<uint32_t>(!ok) << 28);
+}
+// a buffer window for a tile given by its first entry: 2^28 - 1 doubles (the tile's entries
+// lie within 64 leading dimensions of it)
+constexpr int64_t kTileWindow = (int64_t{1} << 28) - 1;
+
+// 64x64 tile of a column-major block (leading dimension ld) into registers: thread t's
+// entry r (0..15) is (row t % 64, column t / 64 + 4 r), that is LDS index t + 256 r of a
+// column-major tile.  Rows >= nr and columns >= nc read as zero (from past the buffer's
+// end), so the 16 loads carry no branch or select and stay in flight together.
+__device__ __forceinline__ void tile_load(const double* __restrict__ src, int64_t ld, int nr, int nc, double v[16]) {
+    const __amdgpu_buffer_rsrc_t rs = blk_rsrc(src, kTileWindow);
+    const int row = threadIdx.x & 63, col0 = threadIdx.x >> 6;
+    const uint32_t o = row + col0 * static_cast<uint32_t>(ld);
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+        v[r] = blk_ld(rs, zero_unless(row < nr && col0 + 4 * r < nc, o + static_cast<uint32_t>(4 * r * ld)));
 }
 
 // tile_load_h through a buffer resource (rows and columns past nr, nc read as zero)
