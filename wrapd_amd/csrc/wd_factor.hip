@@ -83,6 +83,13 @@ struct AsmArgs {
     int chunk;
 };
 
+// lane k's 64-bit value, for every lane
+__device__ __forceinline__ int64_t bcast64(int64_t v, int k) {
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(v), k);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(v >> 32), k);
+    return (static_cast<int64_t>(hi) << 32) | static_cast<uint32_t>(lo);
+}
+
 __global__ void __launch_bounds__(64)
 k_fassemble(const int2* __restrict__ tasks, FactArgs A, AsmArgs M) {
     extern __shared__ double col[];   // rows r0 .. r1 - 1 of the column
@@ -100,22 +107,49 @@ k_fassemble(const int2* __restrict__ tasks, FactArgs A, AsmArgs M) {
     }
     __syncthreads();
     const int64_t gc = M.cbase[s] + j;
+    const int64_t k0 = M.ccol_ptr[gc], nkid = M.ccol_ptr[gc + 1] - k0;
     for (int64_t e = M.acol_ptr[gc] + lane; e < M.acol_ptr[gc + 1]; e += 64) {
         const int2 a = M.acol[e];
         if (a.x >= r0 && a.x < r1) col[a.x - r0] = M.Ax[a.y];
     }
-    const int64_t k1 = M.ccol_ptr[gc + 1];
-    for (int64_t k = M.ccol_ptr[gc]; k < k1; ++k) {
-        __syncthreads();   // the previous contribution (or A's entries) is in
-        const int2 e = M.ccol[k];
-        const int ch = e.x, jj = e.y;
-        const int fc = A.fsize[ch];
-        const int nsc = A.col0[ch + 1] - A.col0[ch];
-        const double* src = A.pool + A.foff[ch] + static_cast<int64_t>(nsc + jj) * fc + nsc;
-        const int* rl = M.rel + M.relptr[ch];
-        const int* cs = M.cchunk + M.cchunk_ptr[ch];
-        const int lo = max(jj, cs[q]), hi = cs[q + 1];
-        for (int ii = lo + lane; ii < hi; ii += 64) col[rl[ii] - r0] += src[ii];
+    // The children's contributions, 64 children at a time: lane i loads child i's sizes,
+    // offsets and chunk bounds (all children's loads in flight together), and the loop over
+    // the children, in order, takes them from that lane.
+    for (int64_t kb = 0; kb < nkid; kb += 64) {
+        const int nb = static_cast<int>(min<int64_t>(64, nkid - kb));
+        int64_t msrc = 0, mrel = 0;
+        int mlo = 0, mhi = 0;
+        if (lane < nb) {
+            const int2 e = M.ccol[k0 + kb + lane];
+            const int ch = e.x, jj = e.y;
+            const int fc = A.fsize[ch];
+            const int nsc = A.col0[ch + 1] - A.col0[ch];
+            msrc = A.foff[ch] + static_cast<int64_t>(nsc + jj) * fc + nsc;
+            mrel = M.relptr[ch];
+            const int* cs = M.cchunk + M.cchunk_ptr[ch];
+            mlo = max(jj, cs[q]);
+            mhi = cs[q + 1];
+        }
+        for (int k = 0; k < nb; ++k) {
+            __syncthreads();   // the previous contribution (or A's entries) is in
+            const double* src = A.pool + bcast64(msrc, k);
+            const int* rl = M.rel + bcast64(mrel, k);
+            const int lo = __builtin_amdgcn_readlane(mlo, k), hi = __builtin_amdgcn_readlane(mhi, k);
+            // four rows per lane per round, their loads issued together (clamped in range)
+            for (int i0 = lo + lane; i0 < hi; i0 += 4 * 64) {
+                int rr[4];
+                double vv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int ii = min(i0 + 64 * u, hi - 1);
+                    rr[u] = rl[ii];
+                    vv[u] = src[ii];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (i0 + 64 * u < hi) col[rr[u] - r0] += vv[u];
+            }
+        }
     }
     __syncthreads();
     double* F = A.pool + A.foff[s] + static_cast<int64_t>(j) * f + r0;
