@@ -173,7 +173,8 @@ extern "C" int pe_run(const double* ax, double* G_out, int64_t* conflicts_out) {
                         wrF(Fi(rl[ii], j), rdF(Fi(rl[ii], j)) + v);
                     }
                 }
-            } else if (L.kind == FK_POTRF) {
+            } else if (L.kind == FK_POTRF || L.kind == FK_POTRF16) {
+                if (L.kind == FK_POTRF16 && S.col0[s + 1] - S.col0[s] > kPotrf16Cols) return 3;
                 potrf(k);
             } else if (L.kind == FK_TRSM) {
                 const int kind = task_kind(y), t = y & ((1 << kTaskKindShift) - 1);

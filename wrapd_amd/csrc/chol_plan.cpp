@@ -287,8 +287,16 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P, const PlanOptions& opt) {
         };
         for (int k = 0; k < pmax; ++k) {
             if (k == 0) {   // later panels' diagonal tiles are factored by UPDATE(k - 1)'s task on them
-                begin(FK_POTRF, k);
-                for (int s : L) push(s, 0);
+                // one wave per supernode of at most 16 columns, a workgroup per wider one
+                for (const int kind : {FK_POTRF16, FK_POTRF}) {
+                    bool any = false;
+                    for (int s : L) {
+                        if ((ns[s] <= kPotrf16Cols) != (kind == FK_POTRF16)) continue;
+                        if (!any) begin(kind, k);
+                        any = true;
+                        push(s, 0);
+                    }
+                }
                 l21_launch(0);
             }
             begin(FK_TRSM, k);

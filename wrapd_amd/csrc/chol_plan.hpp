@@ -45,10 +45,12 @@
 
 namespace wrapd {
 
-enum FactorKind { FK_ASSEMBLE = 0, FK_POTRF = 1, FK_TRSM = 2, FK_UPDATE = 3, FK_FINAL = 4, FK_L21 = 5 };
+// FK_POTRF16: FK_POTRF of supernodes with at most kPotrf16Cols columns (one 16x16 block)
+enum FactorKind { FK_ASSEMBLE = 0, FK_POTRF = 1, FK_TRSM = 2, FK_UPDATE = 3, FK_FINAL = 4, FK_L21 = 5, FK_POTRF16 = 6 };
+constexpr int kPotrf16Cols = 16;
 
 // task pair (s, y): FK_ASSEMBLE y = j | (q << 16), rows [q C, (q + 1) C) (C = asm_chunk) of
-// front column j (from row j on); FK_POTRF y = 0;
+// front column j (from row j on); FK_POTRF, FK_POTRF16 y = 0;
 // FK_TRSM y = (kind << 30) | t with kind 0 TRSM (row tile t), 1 GDIAG (column tile t);
 // FK_UPDATE / FK_FINAL y = (kind << 30) | (a << 15) | b with kind 0 SYRK / SCHUR
 // (tiles a, b), 1 GUPD / GBOT (row tile a, column tile b); FK_L21 y = row tile t >= P.

@@ -329,6 +329,53 @@ __device__ __forceinline__ dbl4 mma16(const double a[4], const double b[4], dbl4
     return acc;
 }
 
+// The Cholesky of a 16x16 block and its inverse by one wave, in registers: lane (li, g = lane
+// / 16) holds row li of columns 4 m + g (m < 4) in a[m] (the block, then L) and x[m] (the
+// identity, then L^{-1}); columns move between lanes by ds_bpermute.  L^{-1} is formed by
+// substitution on the identity, its step p right after the Cholesky's column p, which is
+// final then.  Every entry gets the operations of the row-per-lane loop, in the same order.
+// A pivot that is not positive is taken as 1 and sets bad.  All 64 lanes take part.
+__device__ __forceinline__ void block16_factor(double a[4], double x[4], int li, int g, int& bad) {
+    double d = lane_bcast(a[0], 0);   // A[0][0]
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        const int gp = p & 3, mp = p >> 2;   // column p: a[mp] of the lanes of group gp
+        if (!(d > 0.0)) {
+            bad = 1;
+            d = 1.0;
+        }
+        // A[p+1][p+1] with the updates of the columns before p; column p's update of it
+        // is the operation its own lane applies below, done here as well so the next
+        // pivot does not wait for the column exchange
+        double dn = p < 15 ? lane_bcast(a[(p + 1) >> 2], (p + 1) + 16 * ((p + 1) & 3)) : 0.0;
+        // correctly rounded: a hardware estimate with Newton steps saved 2.4 us per
+        // tile but changed L in the last bits, enough to move the sample runs past
+        // the reference's own round-off spread
+        const double sq = sqrt(d);
+        const double inv = 1.0 / sq;
+        if (g == gp) a[mp] = li == p ? sq : a[mp] * inv;   // rows above p: never read
+        if (p < 15) {
+            const double l1 = lane_bcast(a[mp], (p + 1) + 16 * gp);   // L[p+1][p]
+            dn -= l1 * l1;
+        }
+        const double lp = lane_fetch(a[mp], li + 16 * gp);   // L[li][p]
+#pragma unroll
+        for (int m = mp; m < 4; ++m) {
+            const double lc = lane_fetch(a[mp], 4 * m + g + 16 * gp);   // L[4m+g][p]
+            if (m > mp || g > gp) a[m] -= lp * lc;   // columns after p (rows above: never read)
+        }
+        const double xs = li == p ? inv : 1.0;   // row p of X is final after this
+#pragma unroll
+        for (int m = 0; m <= mp; ++m) x[m] *= xs;
+#pragma unroll
+        for (int m = 0; m <= mp; ++m) {
+            const double xpc = lane_fetch(x[m], p + 16 * g);   // X[p][4m+g]
+            if (li > p && (m < mp || g <= gp)) x[m] -= lp * xpc;
+        }
+        d = dn;
+    }
+}
+
 // The inverse X in LDS: entry (r, c) at c * 64 + (r ^ (c % 16)), so that the 16 lanes
 // reading one row across 16 columns hit 16 different banks.
 __device__ __forceinline__ int xs_at(int r, int c) { return c * 64 + (r ^ (c & 15)); }
@@ -375,10 +422,7 @@ __device__ __forceinline__ void potrf_lds(double* D, double* X, int nb, double* 
     for (int J = 0; J < nblk; ++J) {
         const int j0 = 16 * J;
         if (w == 0) {
-            // Cholesky of D_JJ, L_JJ^{-1} by substitution on the identity (its step p right
-            // after the Cholesky's column p, which is final then).  Lane (li, g = lk) holds row
-            // li of columns 4 m + g in a[m] (D_JJ, then L_JJ) and x[m] (X_JJ).  Every entry
-            // gets the operations of the row-per-lane loop, in the same order.
+            // Cholesky of D_JJ and L_JJ^{-1} (block16_factor's lane layout)
             const int g = lk;
             double a[4], x[4];
 #pragma unroll
@@ -386,44 +430,7 @@ __device__ __forceinline__ void potrf_lds(double* D, double* X, int nb, double* 
                 a[m] = D[(j0 + 4 * m + g) * 64 + j0 + li];
                 x[m] = 4 * m + g == li ? 1.0 : 0.0;
             }
-            double d = lane_bcast(a[0], 0);   // A[0][0]
-#pragma unroll
-            for (int p = 0; p < 16; ++p) {
-                const int gp = p & 3, mp = p >> 2;   // column p: a[mp] of the lanes of group gp
-                if (!(d > 0.0)) {
-                    bad = 1;
-                    d = 1.0;
-                }
-                // A[p+1][p+1] with the updates of the columns before p; column p's update of it
-                // is the operation its own lane applies below, done here as well so the next
-                // pivot does not wait for the column exchange
-                double dn = p < 15 ? lane_bcast(a[(p + 1) >> 2], (p + 1) + 16 * ((p + 1) & 3)) : 0.0;
-                // correctly rounded: a hardware estimate with Newton steps saved 2.4 us per
-                // tile but changed L in the last bits, enough to move the sample runs past
-                // the reference's own round-off spread
-                const double sq = sqrt(d);
-                const double inv = 1.0 / sq;
-                if (g == gp) a[mp] = li == p ? sq : a[mp] * inv;   // rows above p: never read
-                if (p < 15) {
-                    const double l1 = lane_bcast(a[mp], (p + 1) + 16 * gp);   // L[p+1][p]
-                    dn -= l1 * l1;
-                }
-                const double lp = lane_fetch(a[mp], li + 16 * gp);   // L[li][p]
-#pragma unroll
-                for (int m = mp; m < 4; ++m) {
-                    const double lc = lane_fetch(a[mp], 4 * m + g + 16 * gp);   // L[4m+g][p]
-                    if (m > mp || g > gp) a[m] -= lp * lc;   // columns after p (rows above: never read)
-                }
-                const double xs = li == p ? inv : 1.0;   // row p of X is final after this
-#pragma unroll
-                for (int m = 0; m <= mp; ++m) x[m] *= xs;
-#pragma unroll
-                for (int m = 0; m <= mp; ++m) {
-                    const double xpc = lane_fetch(x[m], p + 16 * g);   // X[p][4m+g]
-                    if (li > p && (m < mp || g <= gp)) x[m] -= lp * xpc;
-                }
-                d = dn;
-            }
+            block16_factor(a, x, li, g, bad);
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
                 const int c = 4 * m + g;
@@ -522,6 +529,38 @@ k_fpotrf(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ fa
     __syncthreads();
     const int64_t lg = wrapd::g_ld(f);
     potrf_lds(D, X, nb, Fkk, f, A.G + A.goff[s] + static_cast<int64_t>(c0) * lg + c0, lg, fail);
+}
+
+// Panel 0's diagonal tile of a supernode of at most 16 columns (FK_POTRF16): potrf_lds's
+// only block step for such a tile (rows and columns from nb on are the identity; nothing of
+// them is stored) by one wave, in registers, one supernode per 64-thread workgroup.
+__global__ void __launch_bounds__(64)
+k_fpotrf16(const int2* __restrict__ tasks, FactArgs A, int* __restrict__ fail) {
+    const int s = tasks[blockIdx.x].x;
+    const int f = A.fsize[s];
+    const int nb = A.col0[s + 1] - A.col0[s];
+    double* Fkk = A.pool + A.foff[s];
+    double* Gkk = A.G + A.goff[s];
+    const int64_t lg = wrapd::g_ld(f);
+    const int li = threadIdx.x & 15, g = threadIdx.x >> 4;
+    double a[4], x[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int c = 4 * m + g;
+        a[m] = li < c ? 0.0 : (c < nb ? (li < nb ? Fkk[static_cast<int64_t>(c) * f + li] : 0.0) : (li == c ? 1.0 : 0.0));
+        x[m] = c == li ? 1.0 : 0.0;
+    }
+    int bad = 0;
+    block16_factor(a, x, li, g, bad);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int c = 4 * m + g;
+        if (li < nb && c < nb) {
+            if (li >= c) Fkk[static_cast<int64_t>(c) * f + li] = a[m];
+            Gkk[c * lg + li] = li >= c ? x[m] : 0.0;
+        }
+    }
+    if (threadIdx.x == 0 && bad) *fail = 1;
 }
 
 // ---------------------------------------------------------------------------------
@@ -989,6 +1028,9 @@ void launch_factor(wrapd_ctx* c) {
             break;
         case wrapd::FK_POTRF:
             hipLaunchKernelGGL(k_fpotrf, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A, L.panel, c->d_ffail);
+            break;
+        case wrapd::FK_POTRF16:
+            hipLaunchKernelGGL(k_fpotrf16, dim3(L.count), dim3(64), 0, c->stream, t, A, c->d_ffail);
             break;
         case wrapd::FK_TRSM:
             hipLaunchKernelGGL(k_ftrsm, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A, L.panel);
