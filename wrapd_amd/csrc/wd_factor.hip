@@ -4,28 +4,30 @@
 // src/GlobalStepBehavior.hpp:254-258).
 //
 // The schedule (levels, 64x64 tiles, task lists, pool offsets) comes from chol_plan on
-// the host.  Five kernels, one launch per (level, step):
-//   k_fassemble  one workgroup per front column, built in LDS: A entries, then children's
-//                Schur-complement columns in child order (deterministic, no atomics), stored once
-//   k_fpotrf     one workgroup per supernode: Cholesky L_kk of the panel's diagonal tile
-//                and its inverse in LDS; L_kk goes to the tile's lower half, L_kk^{-1}
-//                to G's diagonal tile (k, k), which is also the diagonal tile of L11^{-1}
+// the host.  One launch per (level, step) and kernel:
+//   k_fassemble  one wave per chunk of a front column, built in LDS: A entries, then the
+//                children's Schur-complement columns in child order (deterministic, no
+//                atomics), stored once; a level's fronts in groups, each after the previous
+//                level's FINAL launch that its children are in
+//   k_fpotrf     one workgroup per supernode: Cholesky L_kk of panel 0's diagonal tile and
+//                its inverse in LDS; L_kk goes to the tile's lower half, L_kk^{-1} to G's
+//                diagonal tile (k, k), which is also the diagonal tile of L11^{-1}
+//   k_fpotrf16   the same for supernodes of at most 16 columns, one wave in registers
 //   k_ftrsm      TRSM tiles F[t, k] <- F[t, k] L_kk^{-T} of L11, or GDIAG tiles
 //                X[k, c] <- L_kk^{-1} X[k, c], both as products with the stored inverse
 //   k_fupdate    SYRK/GEMM updates of L11's later columns, or GEMM updates of X = L11^{-1};
 //                the task updating the next panel's diagonal tile then factors it
 //                (k_fpotrf's work), so only panel 0 has a POTRF launch
-//   k_ffinal     the off-diagonal rows once X is complete: L21 = A21 X^T (into the front's
-//                free upper triangle), then the Schur complement F[a, b] -= L21 L21^T and
-//                G's bottom -L21 X, each 128x128 output super-tile accumulated over its
-//                whole k range in registers
+//   k_fl21       L21 by panel, left-looking, on levels where the panel steps cover L11 only
+//   k_ffinal     once X is complete: the Schur complement F[a, b] -= L21 L21^T and G's
+//                bottom -L21 X, each 64x64 output tile accumulated over its whole k range
 // The panel steps touch only the supernode's diagonal block (ns x ns), so the m update rows
 // are read and written once per pass instead of once per 64-column panel.
 // Every panel-step product runs on the matrix cores in 64x64 output tiles
 // (v_mfma_f64_16x16x4: each of the 4 waves owns a 32x32 quarter as 2x2 16x16 accumulators)
-// over 64-deep operand chunks staged in two 32 KB LDS tiles; the accumulator layout puts
-// consecutive lanes on consecutive rows, so tile stores are coalesced.  Tiles are loaded
-// with every thread's 16 loads in flight at once (tile_load).  All FP64.
+// over 64-deep operand chunks staged in LDS; the accumulator layout puts consecutive lanes
+// on consecutive rows, so tile stores are coalesced.  Tiles are loaded with every thread's
+// 16 loads in flight at once (tile_load).  All FP64.
 #include "wd_ctx.hpp"
 #include "chol_plan.hpp"
 #include "device/wd_reduce.hpp"
