@@ -53,9 +53,11 @@ void tile(int t, int ns, int f, int& st, int& sz) {
 }  // namespace
 
 PlanOptions popt;
-extern "C" void pe_set_options(double l21_right_max_tiles, int asm_chunk) {
+extern "C" void pe_set_options(double l21_right_max_tiles, int asm_chunk, int asm_groups, int potrf16) {
     popt.l21_right_max_tiles = l21_right_max_tiles;
     popt.asm_chunk = asm_chunk;
+    popt.asm_groups = asm_groups != 0;
+    popt.potrf16 = potrf16 != 0;
 }
 
 extern "C" int pe_analyze(int n, const int* ap, const int* aj, const double* xyz, int leaf_size, double* st) {

@@ -28,7 +28,7 @@ def emu(tmp_path_factory):
     subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", out] + src, check=True)
     lib = C.CDLL(out)
     lib.pe_run.restype = C.c_int
-    lib.pe_set_options.argtypes = [C.c_double, C.c_int]
+    lib.pe_set_options.argtypes = [C.c_double, C.c_int, C.c_int, C.c_int]
     return lib
 
 
@@ -46,7 +46,7 @@ def _upper_csr(prob):
     return n, np.cumsum(Ap).astype(np.int32), np.ascontiguousarray(c.astype(np.int32)), np.ascontiguousarray(v)
 
 
-def _run(emu, prob, leaf_size, negate=False, coords=True, options=(256.0, 512)):
+def _run(emu, prob, leaf_size, negate=False, coords=True, options=(256.0, 512, 1, 1)):
     n, Ap, Aj, Ax = _upper_csr(prob)
     emu.pe_set_options(*options)
     free = np.setdiff1d(np.arange(prob.n_verts), prob.pins)
@@ -89,7 +89,8 @@ CASES = {
 
 # the plan's choices (L21 placement threshold, rows per assembly task): the defaults, L21
 # launches on every level, L21 in the panel steps everywhere, 64-row assembly chunks
-MODES = {"default": (256.0, 512), "left": (0.0, 512), "right": (1e30, 512), "chunk64": (256.0, 64)}
+MODES = {"default": (256.0, 512, 1, 1), "left": (0.0, 512, 1, 1), "right": (1e30, 512, 1, 1),
+         "chunk64": (256.0, 64, 1, 1), "joined": (256.0, 512, 0, 0)}
 
 
 @pytest.mark.parametrize("mode", sorted(MODES))
@@ -104,7 +105,7 @@ def test_plan_matches_dense_cholesky(emu, name, mode):
     if name == "two_panel_separators":
         assert R["st"][3] > 64   # a supernode spans more than one 64-column panel
         assert R["st"][7] >= 1   # and a FINAL launch overlaps a wider supernode's panels
-    if name.startswith("graph_order"):
+    if name.startswith("graph_order") and mode != "joined":
         assert R["st"][8] >= 1   # an assembly launch overlaps the previous level's later FINAL launches
     print(f"{name}: {R['st'][7]:.0f} launches on the second stream, {R['st'][8]:.0f} assembly launches "
           "waiting for part of it")
@@ -137,7 +138,7 @@ def test_plan_choices_same_bits(emu):
     the same whichever the plan chooses."""
     make, leaf = CASES["two_panel_separators"]
     prob = make()
-    Gs = [_run(emu, prob, leaf, options=MODES[m])["G"] for m in ("left", "right", "chunk64")]
+    Gs = [_run(emu, prob, leaf, options=MODES[m])["G"] for m in ("left", "right", "chunk64", "joined")]
     for g in Gs[1:]:
         assert np.array_equal(Gs[0], g, equal_nan=True)   # (G's row padding is never written)
 

@@ -81,15 +81,20 @@ PLAN_CHOICES = {
     "l21_in_panel_steps": {"WRAPD_L21_RIGHT_MAX": "1e30"},
     "assembly_chunk_64": {"WRAPD_ASM_CHUNK": "64"},
     "assembly_chunk_1024": {"WRAPD_ASM_CHUNK": "1024"},
+    "assembly_after_full_join": {"WRAPD_ASM_GROUPS": "0"},
+    "diagonal_tiles_all_64x64": {"WRAPD_POTRF16": "0"},
 }
 
 
 def test_plan_choices_agree(monkeypatch):
-    """Where L21 is formed (by the panel steps or by per-panel L21 launches) and how many rows
-    an assembly task builds are the plan's choices.  L21 launches apply the panel steps'
-    products in their order and every assembly chunking adds A's entries and then the
-    children's columns in the same order, so every choice gives the same factorization, bit
-    for bit."""
+    """Where L21 is formed (by the panel steps or by per-panel L21 launches), how many rows
+    an assembly task builds, whether a level's fronts are assembled in groups overlapping the
+    previous level's last FINAL launches, and whether narrow supernodes' diagonal tiles are
+    factored by one wave are the plan's choices.  L21 launches apply the panel steps'
+    products in their order, every assembly chunking and grouping adds A's entries and then
+    the children's columns in the same order, and both diagonal-tile kernels run the same
+    16x16 block step, so every choice gives the same factorization, bit for bit (and a race
+    between an early assembly and the previous level's work would show as a difference)."""
     prob = meshgen.config_problem(3)   # fronts up to ~4000 rows, levels of both kinds
     b = np.random.default_rng(7).standard_normal((prob.n_verts - len(prob.pins), 3))
     env_names = sorted({k for v in PLAN_CHOICES.values() for k in v})

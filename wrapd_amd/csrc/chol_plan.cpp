@@ -220,8 +220,9 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P, const PlanOptions& opt) {
         std::map<int, std::vector<int>> groups;
         for (int s : L) {
             int key = -1;
-            for (int c : children[s])
-                if (lev[c] == l - 1) key = std::max(key, fin_launch[c]);
+            if (opt.asm_groups)
+                for (int c : children[s])
+                    if (lev[c] == l - 1) key = std::max(key, fin_launch[c]);
             groups[key].push_back(s);
         }
         int gi = 0;
@@ -291,7 +292,7 @@ void chol_plan(const CholSymbolic& S, FactorPlan& P, const PlanOptions& opt) {
                 for (const int kind : {FK_POTRF16, FK_POTRF}) {
                     bool any = false;
                     for (int s : L) {
-                        if ((ns[s] <= kPotrf16Cols) != (kind == FK_POTRF16)) continue;
+                        if ((opt.potrf16 && ns[s] <= kPotrf16Cols) != (kind == FK_POTRF16)) continue;
                         if (!any) begin(kind, k);
                         any = true;
                         push(s, 0);

@@ -122,6 +122,8 @@ inline int tile_count(int ns, int f) { return tile_count_l11(ns) + (f - ns + kFa
 struct PlanOptions {
     double l21_right_max_tiles = kL21RightMaxTiles;   // 0: L21 launches everywhere; huge: never
     int asm_chunk = kAsmChunk;                         // rows per assembly task, 64 .. 1024
+    bool asm_groups = true;    // false: one assembly launch per level, after all of the previous level's work
+    bool potrf16 = true;       // false: every panel-0 diagonal tile by the 64x64 tile kernel
 };
 
 void chol_plan(const CholSymbolic& S, FactorPlan& P, const PlanOptions& opt = PlanOptions());

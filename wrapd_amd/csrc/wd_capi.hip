@@ -276,6 +276,8 @@ int wrapd_create(wrapd_ctx** out, int device) {
         const double v = std::atof(e);
         if (v >= 0.0) c->plan_opt.l21_right_max_tiles = v;
     }
+    if (const char* e = std::getenv("WRAPD_ASM_GROUPS")) c->plan_opt.asm_groups = std::atoi(e) != 0;
+    if (const char* e = std::getenv("WRAPD_POTRF16")) c->plan_opt.potrf16 = std::atoi(e) != 0;
     if (const char* e = std::getenv("WRAPD_LEARNED_GATE")) {
         const double v = std::atof(e);
         if (v > 0.0) c->learned_gate = v;
