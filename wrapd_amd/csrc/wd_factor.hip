@@ -36,6 +36,9 @@ namespace wd {
 
 constexpr int kT = wrapd::kFactorTile;   // 64
 constexpr int kFactThreads = 256;
+// the L21 kernel's workgroup: eight waves, each a 32x16 part of the 64x64 tile, so each
+// thread holds half the operand and accumulator registers of a four-wave one
+constexpr int kL21Threads = 512;
 
 struct FactArgs {
     const int* col0;
@@ -165,26 +168,43 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // v_mfma_f64_16x16x4 lane layout: A[row l%16][k l/16], B[k l/16][col l%16], D register q
 // of lane l = D[4q + l/16][l%16].  The product is formed as D = Bs^T-side x As-side, so
 // that D's column index (consecutive lanes) is the output row i.
-__device__ __forceinline__ void tile_zero(dbl4 acc[2][2]) {
+// How NT threads split a 64x64 tile: NT / 64 waves, two along the output rows i (32 rows
+// each) and the rest along the columns j (SJ columns, NBJ 16-column blocks each); tile_load
+// gives each thread R entries.
+template <int NT>
+struct TileWaves {
+    static constexpr int NW = NT / 64;
+    static constexpr int SJ = 64 / (NW / 2);
+    static constexpr int NBJ = SJ / 16;
+    static constexpr int R = 64 / NW;
+};
+
+template <int NBJ>
+__device__ __forceinline__ void tile_zero(dbl4 (&acc)[NBJ][2]) {
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < NBJ; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
 }
 
+template <int NT = kFactThreads>
 __device__ __forceinline__ void tile_mma(const double* __restrict__ As, const double* __restrict__ Bs, int nk,
-                                         dbl4 acc[2][2]) {
+                                         dbl4 (&acc)[TileWaves<NT>::NBJ][2]) {
+    constexpr int NBJ = TileWaves<NT>::NBJ;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int j0 = 32 * (w >> 1), i0 = 32 * (w & 1);
+    const int j0 = TileWaves<NT>::SJ * (w >> 1), i0 = 32 * (w & 1);
     const int li = l & 15, lk = l >> 4;
     for (int kk = 0; kk < nk; kk += 4) {
         const int k = (kk + lk) * kT;
-        const double b0 = Bs[k + j0 + li], b1 = Bs[k + j0 + 16 + li];
+        double b[NBJ];
+#pragma unroll
+        for (int x = 0; x < NBJ; ++x) b[x] = Bs[k + j0 + 16 * x + li];
         const double a0 = As[k + i0 + li], a1 = As[k + i0 + 16 + li];
-        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
+#pragma unroll
+        for (int x = 0; x < NBJ; ++x) {
+            acc[x][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[x], a0, acc[x][0], 0, 0, 0);
+            acc[x][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[x], a1, acc[x][1], 0, 0, 0);
+        }
     }
 }
 
@@ -214,23 +234,29 @@ constexpr int64_t kTileWindow = (int64_t{1} << 28) - 1;
 // entry r (0..15) is (row t % 64, column t / 64 + 4 r), that is LDS index t + 256 r of a
 // column-major tile.  Rows >= nr and columns >= nc read as zero (from past the buffer's
 // end), so the 16 loads carry no branch or select and stay in flight together.
-__device__ __forceinline__ void tile_load(const double* __restrict__ src, int64_t ld, int nr, int nc, double v[16]) {
+template <int NT = kFactThreads>
+__device__ __forceinline__ void tile_load(const double* __restrict__ src, int64_t ld, int nr, int nc,
+                                          double (&v)[TileWaves<NT>::R]) {
+    constexpr int NW = TileWaves<NT>::NW;
     const __amdgpu_buffer_rsrc_t rs = blk_rsrc(src, kTileWindow);
     const int row = threadIdx.x & 63, col0 = threadIdx.x >> 6;
     const uint32_t o = row + col0 * static_cast<uint32_t>(ld);
 #pragma unroll
-    for (int r = 0; r < 16; ++r)
-        v[r] = blk_ld(rs, zero_unless(row < nr && col0 + 4 * r < nc, o + static_cast<uint32_t>(4 * r * ld)));
+    for (int r = 0; r < TileWaves<NT>::R; ++r)
+        v[r] = blk_ld(rs, zero_unless(row < nr && col0 + NW * r < nc, o + static_cast<uint32_t>(NW * r * ld)));
 }
 
-// tile_load_h through a buffer resource (rows and columns past nr, nc read as zero)
+// tile_load_h through a buffer resource (rows and columns past nr, nc read as zero): the
+// entries of half h (columns 32 h .. 32 h + 31) of tile_load's
+template <int NT = kFactThreads>
 __device__ __forceinline__ void tile_ldb_h(__amdgpu_buffer_rsrc_t rs, uint32_t off0, uint32_t ld, int nr, int nc,
-                                           double v[16], int h) {
+                                           double (&v)[TileWaves<NT>::R], int h) {
+    constexpr int NW = TileWaves<NT>::NW, H = TileWaves<NT>::R / 2;
     const int row = threadIdx.x & 63, col0 = threadIdx.x >> 6;
     const uint32_t o = off0 + row + col0 * ld;
 #pragma unroll
-    for (int r = 8 * h; r < 8 * h + 8; ++r)
-        v[r] = blk_ld(rs, zero_unless(row < nr && col0 + 4 * r < nc, o + static_cast<uint32_t>(4 * r) * ld));
+    for (int r = H * h; r < H * h + H; ++r)
+        v[r] = blk_ld(rs, zero_unless(row < nr && col0 + NW * r < nc, o + static_cast<uint32_t>(NW * r) * ld));
 }
 // the registers of tile_load into an LDS tile, as loaded (T[col * 64 + row]) or transposed
 __device__ __forceinline__ void tile_to_lds(double* T, const double v[16]) {
@@ -249,9 +275,11 @@ __device__ __forceinline__ void tile_to_lds_t(double* T, const double v[16]) {
 // tile_to_lds_h: the k index is the tile's column (entries r = 8 h .. 8 h + 7 of tile_load);
 // tile_to_lds_t_h: the k index is the tile's row (threads of rows 32 h .. 32 h + 31).
 constexpr int kHalf = kT / 2;
-__device__ __forceinline__ void tile_to_lds_h(double* T, const double v[16], int h) {
+template <int NT = kFactThreads>
+__device__ __forceinline__ void tile_to_lds_h(double* T, const double (&v)[TileWaves<NT>::R], int h) {
+    constexpr int H = TileWaves<NT>::R / 2;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) T[threadIdx.x + kFactThreads * r] = v[8 * h + r];
+    for (int r = 0; r < H; ++r) T[threadIdx.x + NT * r] = v[H * h + r];
 }
 __device__ __forceinline__ void tile_to_lds_t_h(double* T, const double v[16], int h) {
     const int row = threadIdx.x & 63, col0 = threadIdx.x >> 6;
@@ -268,15 +296,16 @@ __device__ __forceinline__ int half_depth(int nk, int h) {   // the half's k cou
 // accumulator tile into a column-major block C (leading dimension ldc): C[i][j] -= acc
 // (kSub) or C[i][j] = scale * acc, for i < nr, j < nc, and i >= j when lower_only.  With
 // kSub the 16 old values are loaded first, all at once (clamped addresses).
-template <bool kSub>
+template <bool kSub, int NT = kFactThreads>
 __device__ __forceinline__ void tile_write(double* __restrict__ C, int64_t ldc, int nr, int nc, bool lower_only,
-                                           const dbl4 acc[2][2], double scale) {
+                                           const dbl4 (&acc)[TileWaves<NT>::NBJ][2], double scale) {
+    constexpr int NBJ = TileWaves<NT>::NBJ;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int j0 = 32 * (w >> 1), i0 = 32 * (w & 1);
-    double old[16];
+    const int j0 = TileWaves<NT>::SJ * (w >> 1), i0 = 32 * (w & 1);
+    double old[NBJ * 8];
     if (kSub) {
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < NBJ; ++a)
 #pragma unroll
             for (int b = 0; b < 2; ++b)
 #pragma unroll
@@ -287,7 +316,7 @@ __device__ __forceinline__ void tile_write(double* __restrict__ C, int64_t ldc, 
                 }
     }
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < NBJ; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
@@ -695,8 +724,10 @@ k_fupdate(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ f
 // F[t, k] stays in registers between the products, so each update-row tile is written once
 // instead of once per panel.  The next product's tiles are loaded while the current one runs
 // on the matrix cores.
-__global__ void __launch_bounds__(kFactThreads)
+template <int NT>
+__global__ void __launch_bounds__(NT)
 k_fl21(const int2* __restrict__ tasks, FactArgs A, int k) {
+    constexpr int NBJ = TileWaves<NT>::NBJ, R = TileWaves<NT>::R;
     __shared__ double As[kHalf * kT];   // As[(q - 32 h) * 64 + i], half h of the k range
     __shared__ double Bs[kHalf * kT];   // Bs[(q - 32 h) * 64 + j]
     const int2 task = tasks[blockIdx.x];
@@ -709,15 +740,15 @@ k_fl21(const int2* __restrict__ tasks, FactArgs A, int k) {
     const double* Gs = A.G + A.goff[s];
     const int lg = wrapd::g_ld(f);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int j0 = 32 * (w >> 1), i0 = 32 * (w & 1);
+    const int j0 = TileWaves<NT>::SJ * (w >> 1), i0 = 32 * (w & 1);
     const int c0 = kT * k;
     const int nbk = min(kT, ns - c0);
     // the operand tiles of product p: L[t, p] and L[k, p], from the front as a buffer
     const __amdgpu_buffer_rsrc_t rF = blk_rsrc(F, static_cast<int64_t>(f) * f);
-    double va[16], vb[16];
+    double va[R], vb[R];
     auto load_upd_h = [&](int p, int h) {
-        tile_ldb_h(rF, kT * p * f + a0, f, na, kT, va, h);
-        tile_ldb_h(rF, kT * p * f + c0, f, nbk, kT, vb, h);
+        tile_ldb_h<NT>(rF, kT * p * f + a0, f, na, kT, va, h);
+        tile_ldb_h<NT>(rF, kT * p * f + c0, f, nbk, kT, vb, h);
     };
     if (k > 0) {
         load_upd_h(0, 0);
@@ -725,9 +756,9 @@ k_fl21(const int2* __restrict__ tasks, FactArgs A, int k) {
     }
     // run = F[t, k] in the accumulator layout (output (i, j) = (i0 + 16 b + l % 16,
     // j0 + 16 a + 4 q + l / 16)); positions outside the tile hold zero
-    dbl4 run[2][2];
+    dbl4 run[NBJ][2];
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+    for (int x = 0; x < NBJ; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y)
 #pragma unroll
@@ -737,18 +768,18 @@ k_fl21(const int2* __restrict__ tasks, FactArgs A, int k) {
                 run[x][y][q] = (i < na && j < nbk) ? v : 0.0;
             }
     for (int p = 0; p < k; ++p) {   // panels p < k < P are a full 64 columns
-        dbl4 acc[2][2];
+        dbl4 acc[NBJ][2];
         tile_zero(acc);
         for (int h = 0; h < 2; ++h) {
             if (p > 0 || h) __syncthreads();   // the previous half has been read
-            tile_to_lds_h(As, va, h);
-            tile_to_lds_h(Bs, vb, h);
+            tile_to_lds_h<NT>(As, va, h);
+            tile_to_lds_h<NT>(Bs, vb, h);
             __syncthreads();
             if (p + 1 < k) load_upd_h(p + 1, h);   // this half is staged: load the next product's
-            tile_mma(As, Bs, kHalf, acc);
+            tile_mma<NT>(As, Bs, kHalf, acc);
         }
 #pragma unroll
-        for (int x = 0; x < 2; ++x)
+        for (int x = 0; x < NBJ; ++x)
 #pragma unroll
             for (int y = 0; y < 2; ++y)
 #pragma unroll
@@ -756,15 +787,15 @@ k_fl21(const int2* __restrict__ tasks, FactArgs A, int k) {
     }
     // L[t, k] = run L_kk^{-T}: run becomes the A operand (As[j * 64 + i] = run(i, j), j the
     // k index of this product), staged half by half
-    tile_load(Gs + static_cast<int64_t>(c0) * lg + c0, lg, nbk, nbk, vb);
-    dbl4 acc[2][2];
+    tile_load<NT>(Gs + static_cast<int64_t>(c0) * lg + c0, lg, nbk, nbk, vb);
+    dbl4 acc[NBJ][2];
     tile_zero(acc);
     for (int h = 0; h < 2; ++h) {
         const int nh = half_depth(nbk, h);
         if (nh == 0) break;
         if (k > 0 || h) __syncthreads();
 #pragma unroll
-        for (int x = 0; x < 2; ++x)
+        for (int x = 0; x < NBJ; ++x)
 #pragma unroll
             for (int y = 0; y < 2; ++y)
 #pragma unroll
@@ -772,11 +803,11 @@ k_fl21(const int2* __restrict__ tasks, FactArgs A, int k) {
                     const int i = i0 + 16 * y + (l & 15), j = j0 + 16 * x + 4 * q + (l >> 4);
                     if ((j >> 5) == h) As[(j - kHalf * h) * kT + i] = run[x][y][q];
                 }
-        tile_to_lds_h(Bs, vb, h);
+        tile_to_lds_h<NT>(Bs, vb, h);
         __syncthreads();
-        tile_mma(As, Bs, nh, acc);
+        tile_mma<NT>(As, Bs, nh, acc);
     }
-    tile_write<false>(F + static_cast<int64_t>(c0) * f + a0, f, na, nbk, false, acc, 1.0);
+    tile_write<false, NT>(F + static_cast<int64_t>(c0) * f + a0, f, na, nbk, false, acc, 1.0);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1014,7 +1045,7 @@ void launch_factor(wrapd_ctx* c) {
             (void)hipEventRecord(c->fev_fork, c->stream);
             (void)hipStreamWaitEvent(c->fstream2, c->fev_fork, 0);
             if (L.kind == wrapd::FK_L21) {
-                hipLaunchKernelGGL(k_fl21, dim3(L.count), dim3(kFactThreads), 0, c->fstream2, t, A, L.panel);
+                hipLaunchKernelGGL(k_fl21<kL21Threads>, dim3(L.count), dim3(kL21Threads), 0, c->fstream2, t, A, L.panel);
             } else {
                 launch_final(L, t, A, c->fstream2);
                 if (L.ord >= 0 && L.ord < static_cast<int>(c->fev_final.size()))
@@ -1041,7 +1072,7 @@ void launch_factor(wrapd_ctx* c) {
             hipLaunchKernelGGL(k_fupdate, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A, L.panel, c->d_ffail);
             break;
         case wrapd::FK_L21:
-            hipLaunchKernelGGL(k_fl21, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A, L.panel);
+            hipLaunchKernelGGL(k_fl21<kL21Threads>, dim3(L.count), dim3(kL21Threads), 0, c->stream, t, A, L.panel);
             break;
         default:
             launch_final(L, t, A, c->stream);
