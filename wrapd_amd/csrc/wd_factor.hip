@@ -39,6 +39,7 @@ constexpr int kFactThreads = 256;
 // the L21 kernel's workgroup: eight waves, each a 32x16 part of the 64x64 tile, so each
 // thread holds half the operand and accumulator registers of a four-wave one
 constexpr int kL21Threads = 512;
+constexpr int kTrsmThreads = 512;   // the TRSM kernel's workgroup, split the same way
 
 struct FactArgs {
     const int* col0;
@@ -281,11 +282,13 @@ __device__ __forceinline__ void tile_to_lds_h(double* T, const double (&v)[TileW
 #pragma unroll
     for (int r = 0; r < H; ++r) T[threadIdx.x + NT * r] = v[H * h + r];
 }
-__device__ __forceinline__ void tile_to_lds_t_h(double* T, const double v[16], int h) {
+template <int NT = kFactThreads>
+__device__ __forceinline__ void tile_to_lds_t_h(double* T, const double (&v)[TileWaves<NT>::R], int h) {
+    constexpr int NW = TileWaves<NT>::NW;
     const int row = threadIdx.x & 63, col0 = threadIdx.x >> 6;
     if ((row >> 5) == h) {
 #pragma unroll
-        for (int r = 0; r < 16; ++r) T[(row - kHalf * h) * 64 + col0 + 4 * r] = v[r];
+        for (int r = 0; r < TileWaves<NT>::R; ++r) T[(row - kHalf * h) * 64 + col0 + NW * r] = v[r];
     }
 }
 __device__ __forceinline__ int half_depth(int nk, int h) {   // the half's k count, a multiple of 4
@@ -597,8 +600,10 @@ k_fpotrf16(const int2* __restrict__ tasks, FactArgs A, int* __restrict__ fail) {
 // ---------------------------------------------------------------------------------
 // TRSM (kind 0): F[t, k] <- F[t, k] L_kk^{-T};  GDIAG (kind 1): X[k, c] <- L_kk^{-1} X[k, c],
 // both as products with L_kk^{-1}, which k_fpotrf left in G's diagonal tile (k, k)
-__global__ void __launch_bounds__(kFactThreads)
+template <int NT>
+__global__ void __launch_bounds__(NT)
 k_ftrsm(const int2* __restrict__ tasks, FactArgs A, int k) {
+    constexpr int NBJ = TileWaves<NT>::NBJ, R = TileWaves<NT>::R;
     __shared__ double As[kHalf * kT];   // As[(q - 32 h) * 64 + i], half h of the k range
     __shared__ double Bs[kHalf * kT];   // Bs[(q - 32 h) * 64 + j]
     const int2 task = tasks[blockIdx.x];
@@ -616,30 +621,30 @@ k_ftrsm(const int2* __restrict__ tasks, FactArgs A, int k) {
     int t0, nt;
     front_tile(t, ns, f, t0, nt);
     const double* Linv = Gs + static_cast<int64_t>(c0) * lg + c0;   // column-major, leading dimension lg
-    double va[16], vb[16];
+    double va[R], vb[R];
     if (kind == 0) {
         // C[i][j] = sum_q B[i][q] Linv[j][q]: As[q][i] = B[i][q], Bs[q][j] = Linv[j][q]
-        tile_load(F + static_cast<int64_t>(c0) * f + t0, f, nt, nb, va);
-        tile_load(Linv, lg, nb, nb, vb);
+        tile_load<NT>(F + static_cast<int64_t>(c0) * f + t0, f, nt, nb, va);
+        tile_load<NT>(Linv, lg, nb, nb, vb);
     } else {
         // C[i][j] = sum_q Linv[i][q] X[q][j], X = G rows [c0, c0+nb), columns [t0, t0+nt)
-        tile_load(Linv, lg, nb, nb, va);
-        tile_load(Gs + static_cast<int64_t>(t0) * lg + c0, lg, nb, nt, vb);
+        tile_load<NT>(Linv, lg, nb, nb, va);
+        tile_load<NT>(Gs + static_cast<int64_t>(t0) * lg + c0, lg, nb, nt, vb);
     }
-    dbl4 acc[2][2];
+    dbl4 acc[NBJ][2];
     tile_zero(acc);
     for (int h = 0; h < 2; ++h) {
         const int nh = half_depth(nb, h);
         if (nh == 0) break;
         if (h) __syncthreads();   // the first half has been read
-        tile_to_lds_h(As, va, h);
-        if (kind == 0) tile_to_lds_h(Bs, vb, h);
-        else tile_to_lds_t_h(Bs, vb, h);
+        tile_to_lds_h<NT>(As, va, h);
+        if (kind == 0) tile_to_lds_h<NT>(Bs, vb, h);
+        else tile_to_lds_t_h<NT>(Bs, vb, h);
         __syncthreads();
-        tile_mma(As, Bs, nh, acc);
+        tile_mma<NT>(As, Bs, nh, acc);
     }
-    if (kind == 0) tile_write<false>(F + static_cast<int64_t>(c0) * f + t0, f, nt, nb, false, acc, 1.0);
-    else tile_write<false>(Gs + static_cast<int64_t>(t0) * lg + c0, lg, nb, nt, false, acc, 1.0);
+    if (kind == 0) tile_write<false, NT>(F + static_cast<int64_t>(c0) * f + t0, f, nt, nb, false, acc, 1.0);
+    else tile_write<false, NT>(Gs + static_cast<int64_t>(t0) * lg + c0, lg, nb, nt, false, acc, 1.0);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1066,7 +1071,7 @@ void launch_factor(wrapd_ctx* c) {
             hipLaunchKernelGGL(k_fpotrf16, dim3(L.count), dim3(64), 0, c->stream, t, A, c->d_ffail);
             break;
         case wrapd::FK_TRSM:
-            hipLaunchKernelGGL(k_ftrsm, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A, L.panel);
+            hipLaunchKernelGGL(k_ftrsm<kTrsmThreads>, dim3(L.count), dim3(kTrsmThreads), 0, c->stream, t, A, L.panel);
             break;
         case wrapd::FK_UPDATE:
             hipLaunchKernelGGL(k_fupdate, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A, L.panel, c->d_ffail);
