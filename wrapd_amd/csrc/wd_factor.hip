@@ -40,6 +40,9 @@ constexpr int kFactThreads = 256;
 // thread holds half the operand and accumulator registers of a four-wave one
 constexpr int kL21Threads = 512;
 constexpr int kTrsmThreads = 512;   // the TRSM kernel's workgroup, split the same way
+// the update kernel's: the same split; its diagonal task's factorization runs on the first
+// four waves (the others only take part in its barriers)
+constexpr int kUpdThreads = 512;
 
 struct FactArgs {
     const int* col0;
@@ -260,14 +263,17 @@ __device__ __forceinline__ void tile_ldb_h(__amdgpu_buffer_rsrc_t rs, uint32_t o
         v[r] = blk_ld(rs, zero_unless(row < nr && col0 + NW * r < nc, o + static_cast<uint32_t>(NW * r) * ld));
 }
 // the registers of tile_load into an LDS tile, as loaded (T[col * 64 + row]) or transposed
-__device__ __forceinline__ void tile_to_lds(double* T, const double v[16]) {
+template <int NT = kFactThreads>
+__device__ __forceinline__ void tile_to_lds(double* T, const double (&v)[TileWaves<NT>::R]) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) T[threadIdx.x + kFactThreads * r] = v[r];
+    for (int r = 0; r < TileWaves<NT>::R; ++r) T[threadIdx.x + NT * r] = v[r];
 }
-__device__ __forceinline__ void tile_to_lds_t(double* T, const double v[16]) {
+template <int NT = kFactThreads>
+__device__ __forceinline__ void tile_to_lds_t(double* T, const double (&v)[TileWaves<NT>::R]) {
+    constexpr int NW = TileWaves<NT>::NW;
     const int row = threadIdx.x & 63, col0 = threadIdx.x >> 6;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) T[row * 64 + col0 + 4 * r] = v[r];
+    for (int r = 0; r < TileWaves<NT>::R; ++r) T[row * 64 + col0 + NW * r] = v[r];
 }
 
 // Half-depth staging: the k range 32 h .. 32 h + 31 of a 64-deep product into a 32 x 64 LDS
@@ -446,7 +452,8 @@ __device__ __forceinline__ void inverse_block(const double* D, double* X, int I,
 // nb on hold the identity) is factored in place, L^{-1} built in X (LDS, zeroed by the
 // caller, entry (r, c) at xs_at(r, c)); then L goes to the front's lower triangle at Fkk
 // (leading dimension f) and L^{-1} (zero above the diagonal) to Gkk (leading dimension lg).
-// All 256 threads take part; the caller has synchronized after filling D and X.
+// The first 256 threads do the work; a caller with more waves brings them to every barrier
+// (they do nothing else).  The caller has synchronized after filling D and X.
 __device__ __forceinline__ void potrf_lds(double* D, double* X, int nb, double* __restrict__ Fkk, int64_t f,
                           double* __restrict__ Gkk, int64_t lg, int* __restrict__ fail) {
     const int nblk = (nb + 15) >> 4;   // 16-blocks holding tile columns
@@ -477,7 +484,7 @@ __device__ __forceinline__ void potrf_lds(double* D, double* X, int nb, double* 
         __syncthreads();
         // block row J of the inverse, block J2 < J on wave 3 - J2: it reads L_JK (K < J), X_JJ
         // and the earlier block rows, none of which the panel below writes
-        if (w > 2 - J && 3 - w < J) inverse_block(D, X, J, 3 - w);
+        if (w < 4 && w > 2 - J && 3 - w < J) inverse_block(D, X, J, 3 - w);
         // panel: L_IJ = A_IJ X_JJ^T for I > J, one block per wave.  Computed transposed,
         // C^T = X_JJ A_IJ^T, so consecutive lanes hold consecutive rows of L_IJ.
         {
@@ -523,7 +530,7 @@ __device__ __forceinline__ void potrf_lds(double* D, double* X, int nb, double* 
     // diagonal tile (k, k), where the TRSM tasks read it and the solves use it
     {
         const int r = threadIdx.x & 63, cb = threadIdx.x >> 6;
-        if (r < nb) {
+        if (r < nb && threadIdx.x < kFactThreads) {
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
                 const int c = cb + 4 * q;
@@ -649,8 +656,10 @@ k_ftrsm(const int2* __restrict__ tasks, FactArgs A, int k) {
 
 // ---------------------------------------------------------------------------------
 // SYRK/GEMM (kind 0): F[a, b] -= L[a, k] L[b, k]^T;  GUPD (kind 1): X[i, c] -= L[i, k] X[k, c]
-__global__ void __launch_bounds__(kFactThreads)
+template <int NT>
+__global__ void __launch_bounds__(NT)
 k_fupdate(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ fail) {
+    constexpr int NBJ = TileWaves<NT>::NBJ, R = TileWaves<NT>::R;
     __shared__ double As[kT * kT];   // As[kk * 64 + i]
     __shared__ double Bs[kT * kT];   // Bs[kk * 64 + j]
     const int2 task = tasks[blockIdx.x];
@@ -668,30 +677,30 @@ k_fupdate(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ f
     double* F = A.pool + A.foff[s];
     double* Gs = A.G + A.goff[s];
     const int lg = wrapd::g_ld(f);
-    double va[16], vb[16];
-    tile_load(F + static_cast<int64_t>(c0) * f + a0, f, na, nb, va);
+    double va[R], vb[R];
+    tile_load<NT>(F + static_cast<int64_t>(c0) * f + a0, f, na, nb, va);
     if (kind == 0) {
-        tile_load(F + static_cast<int64_t>(c0) * f + b0, f, nbt, nb, vb);
-        tile_to_lds(Bs, vb);
+        tile_load<NT>(F + static_cast<int64_t>(c0) * f + b0, f, nbt, nb, vb);
+        tile_to_lds<NT>(Bs, vb);
     } else {
-        tile_load(Gs + static_cast<int64_t>(b0) * lg + c0, lg, nb, nbt, vb);
-        tile_to_lds_t(Bs, vb);
+        tile_load<NT>(Gs + static_cast<int64_t>(b0) * lg + c0, lg, nb, nbt, vb);
+        tile_to_lds_t<NT>(Bs, vb);
     }
-    tile_to_lds(As, va);
+    tile_to_lds<NT>(As, va);
     __syncthreads();
-    dbl4 acc[2][2];
+    dbl4 acc[NBJ][2];
     tile_zero(acc);
-    tile_mma(As, Bs, (nb + 3) & ~3, acc);
+    tile_mma<NT>(As, Bs, (nb + 3) & ~3, acc);
     if (kind == 0 && a == b && a == k + 1) {
         // The next panel's diagonal tile: its update stays in registers and the tile is
         // factored here (the POTRF of panel k + 1), overlapping the launch's other tasks.
         // As and Bs become that factorization's D and X.
         double* C = F + static_cast<int64_t>(b0) * f + a0;
         const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-        const int j0 = 32 * (w >> 1), i0 = 32 * (w & 1);
-        double nv[16];
+        const int j0 = TileWaves<NT>::SJ * (w >> 1), i0 = 32 * (w & 1);
+        double nv[NBJ * 8];
 #pragma unroll
-        for (int x = 0; x < 2; ++x)
+        for (int x = 0; x < NBJ; ++x)
 #pragma unroll
             for (int y = 0; y < 2; ++y)
 #pragma unroll
@@ -702,7 +711,7 @@ k_fupdate(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ f
                 }
         __syncthreads();   // every wave is done reading As and Bs
 #pragma unroll
-        for (int x = 0; x < 2; ++x)
+        for (int x = 0; x < NBJ; ++x)
 #pragma unroll
             for (int y = 0; y < 2; ++y)
 #pragma unroll
@@ -711,13 +720,13 @@ k_fupdate(const int2* __restrict__ tasks, FactArgs A, int k, int* __restrict__ f
                     As[j * 64 + i] = i < j ? 0.0 : ((i < nbt && j < nbt) ? nv[(x * 2 + y) * 4 + q] : (i == j ? 1.0 : 0.0));
                 }
 #pragma unroll
-        for (int r = 0; r < 16; ++r) Bs[threadIdx.x + kFactThreads * r] = 0.0;
+        for (int r = 0; r < kT * kT / NT; ++r) Bs[threadIdx.x + NT * r] = 0.0;
         __syncthreads();
         potrf_lds(As, Bs, nbt, C, f, Gs + static_cast<int64_t>(b0) * lg + b0, lg, fail);
         return;
     }
-    if (kind == 0) tile_write<true>(F + static_cast<int64_t>(b0) * f + a0, f, na, nbt, a == b, acc, 1.0);
-    else tile_write<true>(Gs + static_cast<int64_t>(b0) * lg + a0, lg, na, nbt, false, acc, 1.0);
+    if (kind == 0) tile_write<true, NT>(F + static_cast<int64_t>(b0) * f + a0, f, na, nbt, a == b, acc, 1.0);
+    else tile_write<true, NT>(Gs + static_cast<int64_t>(b0) * lg + a0, lg, na, nbt, false, acc, 1.0);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1074,7 +1083,7 @@ void launch_factor(wrapd_ctx* c) {
             hipLaunchKernelGGL(k_ftrsm<kTrsmThreads>, dim3(L.count), dim3(kTrsmThreads), 0, c->stream, t, A, L.panel);
             break;
         case wrapd::FK_UPDATE:
-            hipLaunchKernelGGL(k_fupdate, dim3(L.count), dim3(kFactThreads), 0, c->stream, t, A, L.panel, c->d_ffail);
+            hipLaunchKernelGGL(k_fupdate<kUpdThreads>, dim3(L.count), dim3(kUpdThreads), 0, c->stream, t, A, L.panel, c->d_ffail);
             break;
         case wrapd::FK_L21:
             hipLaunchKernelGGL(k_fl21<kL21Threads>, dim3(L.count), dim3(kL21Threads), 0, c->stream, t, A, L.panel);
