@@ -833,9 +833,10 @@ k_fl21(const int2* __restrict__ tasks, FactArgs A, int k) {
 // accumulated in k order whatever the tiling, so the tile shape does not change the bits.
 // Both operands are staged index-major (As[i * (KC + 1) + k], Bs[j * (KC + 1) + k]): the
 // padding keeps the stores of either load orientation and the matrix-core reads free of bank
-// conflicts.  At 64x64 with 4 waves a kernel needs 80 registers, so six waves share a SIMD and
-// one's loads and barriers hide behind the others' products (C5: 58.7 ms per refactorization,
-// against 61.2-62.1 ms for 128x128 tiles at two or four waves per SIMD).
+// conflicts.  At 64x64 with 4 waves a kernel needs 82 registers, so five waves share a SIMD
+// and one's loads and barriers hide behind the others' products (128x128 tiles at two or four
+// waves per SIMD were 2.5-3.4 ms slower per C5 refactorization; held to 80 registers, six waves
+// per SIMD measured the same as five).
 constexpr int kFK = wrapd::kFinalChunk;    // k-chunk
 
 // R x C block (R * C = NT * NV, NT threads) of a column-major source (element offset off0,
