@@ -24,10 +24,11 @@
 // The panel steps touch only the supernode's diagonal block (ns x ns), so the m update rows
 // are read and written once per pass instead of once per 64-column panel.
 // Every panel-step product runs on the matrix cores in 64x64 output tiles
-// (v_mfma_f64_16x16x4: each of the 4 waves owns a 32x32 quarter as 2x2 16x16 accumulators)
-// over 64-deep operand chunks staged in LDS; the accumulator layout puts consecutive lanes
-// on consecutive rows, so tile stores are coalesced.  Tiles are loaded with every thread's
-// 16 loads in flight at once (tile_load).  All FP64.
+// (v_mfma_f64_16x16x4: with 4 waves each owns a 32x32 quarter as 2x2 16x16 accumulators,
+// with 8 waves (the update, TRSM and L21 kernels) a 32x16 part as 2x1) over 64-deep operand
+// chunks staged in LDS; the accumulator layout puts consecutive lanes on consecutive rows, so
+// tile stores are coalesced.  Tiles are loaded with every thread's loads (16 with 4 waves, 8
+// with 8) in flight at once (tile_load).  All FP64.
 #include "wd_ctx.hpp"
 #include "chol_plan.hpp"
 #include "device/wd_reduce.hpp"
